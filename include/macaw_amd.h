@@ -1,0 +1,283 @@
+/*
+ * macaw_amd.h -- C ABI of the MI355X particle-tracking library.
+ *
+ * The library replaces the hot path of a MaCaw transport study: the MOOSE
+ * TraceRay element walk, MaCaw's CollisionKernel::onSegment and the OpenMC
+ * routines it calls (cross sections, random numbers, collisions, tallies).
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, returns
+ * 0 on success and non-zero on error, and leaves a message readable with
+ * mcw_last_error() -- the contract of OpenMC's C API
+ * (`int openmc_init(...)` + `openmc_err_msg`, used at
+ * src/userobjects/OpenMCStudy.C:127-131,213-215).
+ *
+ * Host arrays passed in are caller-owned and copied; the context owns every
+ * device allocation.  One context per host thread; calls on one context are
+ * serialised by the caller.
+ *
+ * Conventions that are part of the interface:
+ *   - element node order and side numbering are libMesh's (Tet4: sides
+ *     {0,2,1},{0,1,3},{1,2,3},{2,0,3}; Hex8: sides {0,3,2,1},{0,1,5,4},
+ *     {1,2,6,5},{2,3,7,6},{3,0,4,7},{4,5,6,7}); exit-face ties go to the
+ *     lowest side index;
+ *   - material indices follow CollisionKernel: block -> material *id*, and
+ *     the material used is the one whose id matches
+ *     (src/raykernels/CollisionKernel.C:63-69,110);
+ *   - random numbers follow OpenMC 0.13: 64-bit LCG with PCG-RXS-M-XS output,
+ *     per-particle streams (0 tracking, 1 source, 2 URR), particle seed
+ *     (batch-1)*n_particles + id, stride 152917;
+ *   - tally results are laid out as OpenMC's results_(filter, score, kind)
+ *     with kind VALUE=0, SUM=1, SUM_SQ=2 (src/auxkernels/OpenMCTallyAux.C:110-203).
+ */
+#ifndef MACAW_AMD_H
+#define MACAW_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums --------------------------------------------------------------- */
+
+enum { MCW_TET4 = 4, MCW_HEX8 = 8 };
+
+/* RayBC kinds: KillRayBC (leakage) and ReflectRayBC (specular). */
+enum { MCW_BC_VACUUM = 0, MCW_BC_REFLECT = 1 };
+
+/* Estimators, numbered as OpenMC's TallyEstimator / OpenMCTally's MooseEnum
+ * "ANALOG TRACKLENGTH COLLISION" (src/userobjects/OpenMCTally.C:39). */
+enum { MCW_EST_ANALOG = 0, MCW_EST_TRACKLENGTH = 1, MCW_EST_COLLISION = 2 };
+
+/* Tally result kinds (OpenMC TallyResult). */
+enum { MCW_RESULT_VALUE = 0, MCW_RESULT_SUM = 1, MCW_RESULT_SUM_SQ = 2 };
+
+/* Tally filters (OpenMCTally.C:125-191) and scores. */
+enum { MCW_FILTER_UNIVERSE = 1, MCW_FILTER_CELL = 2, MCW_FILTER_ENERGY = 3, MCW_FILTER_PARTICLE = 4 };
+enum { MCW_SCORE_FLUX = 0, MCW_SCORE_TOTAL = 1, MCW_SCORE_ABSORPTION = 2, MCW_SCORE_SCATTER = 3,
+       MCW_SCORE_FISSION = 4 };
+
+/* Source distributions. */
+enum { MCW_SPACE_POINT = 0, MCW_SPACE_BOX = 1, MCW_SPACE_MESH_VOLUME = 2 };
+enum { MCW_ANGLE_ISOTROPIC = 0, MCW_ANGLE_LINEAR_MU = 1 };
+enum { MCW_ENERGY_MONO = 0, MCW_ENERGY_WATT = 1 };
+
+/* Transport algorithms.  PLAIN: event kernels (xs, segment, collide,
+ * compact) driven from the host.  BATCHED: one persistent fused kernel per
+ * bank chunk.  Both give bit-identical histories. */
+enum { MCW_MODE_PLAIN = 0, MCW_MODE_BATCHED = 1 };
+
+/* Exit-face distance arithmetic.  FP64 everywhere, or FP32 candidate
+ * distances with an FP64 recheck of near ties and an FP64 final distance. */
+enum { MCW_PREC_FP64 = 0, MCW_PREC_MIXED = 1 };
+
+/* Per-history fates. */
+enum { MCW_FATE_ALIVE = 0, MCW_FATE_LEAKED = 1, MCW_FATE_ABSORBED = 2, MCW_FATE_CUTOFF = 3,
+       MCW_FATE_LOST = 4 };
+
+/* ---- plain-data inputs ---------------------------------------------------- */
+
+typedef struct mcw_ctx mcw_ctx;
+
+typedef struct {
+  int32_t device;           /* HIP device ordinal */
+  int32_t mode;             /* MCW_MODE_* */
+  int32_t precision;        /* MCW_PREC_* */
+  int32_t seed_roundtrip;   /* 1: round the tracking seed through a double at the end of
+                               every segment, as MaCaw's Real ray aux data does
+                               (CollisionKernel.C:132-134,157-159,217-219) */
+  int32_t record_histories; /* 1: keep per-history records (mcw_history_get) */
+  int32_t block_size;       /* threads per workgroup, 0 = default */
+  int64_t bank_capacity;    /* particles resident per chunk, 0 = default */
+  int64_t max_segments;     /* per-history guard; beyond it the history is lost; 0 = default */
+  uint64_t master_seed;     /* OpenMC settings::seed, default 1 */
+  int32_t sort_cadence;     /* BATCHED: refills between element sorts of the bank, 0 = never */
+  int32_t reserved[7];
+} mcw_options;
+
+/* One synthetic continuous-energy nuclide: pointwise lin-lin tables on an
+ * increasing energy grid [eV]; microscopic cross sections in barns; elastic
+ * scattering isotropic in the centre-of-mass frame off a target at rest of
+ * mass ratio awr. */
+typedef struct {
+  int32_t n_points;
+  double awr;
+  const double *energy;
+  const double *total;
+  const double *absorption;
+  const double *scatter;
+  const double *fission; /* may be NULL (no fission) */
+} mcw_nuclide;
+
+/* A material: id (as in materials.xml) and atom densities [1/(b cm)]. */
+typedef struct {
+  int32_t id;
+  int32_t n_nuclides;
+  const int32_t *nuclide; /* indices into the nuclide list */
+  const double *density;
+} mcw_material;
+
+typedef struct {
+  int32_t space;  /* MCW_SPACE_* */
+  int32_t angle;  /* MCW_ANGLE_* */
+  int32_t energy; /* MCW_ENERGY_* */
+  int32_t pad;
+  double point[3];
+  double box_lo[3];
+  double box_hi[3];
+  double axis[3]; /* LINEAR_MU reference direction (unit) */
+  double beta;    /* LINEAR_MU: p(mu) ~ 1 + beta*mu, |beta| <= 1 */
+  double E0;      /* MONO energy [eV] */
+  double watt_a;  /* WATT a [eV] */
+  double watt_b;  /* WATT b [1/eV] */
+} mcw_source;
+
+typedef struct {
+  int32_t type;         /* MCW_FILTER_* */
+  int32_t n_bins;       /* CELL/UNIVERSE: number of ids (0 = all); ENERGY: number of edges */
+  const int64_t *ids;   /* CELL: element ids; UNIVERSE: block ids */
+  const double *edges;  /* ENERGY: increasing edges [eV] */
+} mcw_filter;
+
+typedef struct {
+  int64_t histories;
+  int64_t segments;
+  int64_t collisions;
+  int64_t absorptions;
+  int64_t leaks;
+  int64_t cutoffs;
+  int64_t lost;
+  int64_t reflections;
+  double track_length;  /* total weighted track length [cm] */
+  double time_ms;       /* device time of the batch, transport only */
+  double kernel_ms;     /* device time of the dominant kernel(s) */
+  int64_t kernel_launches;
+} mcw_batch_stats;
+
+/* Per-history record, for parity checks against a history-based reference. */
+typedef struct {
+  int64_t id;          /* particle seed id (batch-1)*n + i + 1 */
+  uint64_t digest;     /* hash chain over (element, event) of every segment */
+  uint64_t seed;       /* final tracking seed */
+  double E;            /* final energy */
+  double x, y, z;      /* final position */
+  int32_t n_segments;
+  int32_t n_collisions;
+  int32_t elem;        /* final element */
+  int32_t fate;        /* MCW_FATE_* */
+} mcw_history;
+
+/* ---- context -------------------------------------------------------------- */
+
+/* Replaces openmc_init + openmc_simulation_init (OpenMCStudy.C:127-147). */
+int mcw_create(mcw_ctx **ctx, const mcw_options *opts);
+/* Replaces openmc_finalize (OpenMCStudy.C:213-215). */
+int mcw_destroy(mcw_ctx *ctx);
+/* Replaces openmc_err_msg. */
+const char *mcw_last_error(const mcw_ctx *ctx);
+void mcw_default_options(mcw_options *opts);
+
+/* ---- problem definition -------------------------------------------------- */
+
+/* The mesh the study traces (libMesh mesh of OpenMCStudy / TraceRay;
+ * OpenMCStudy.C:152-187 maps each element to a cell and each subdomain to a
+ * universe).  conn_offsets has n_elem+1 entries.  face_boundary, optional,
+ * gives for every side slot (element-major, sides of the element's type) the
+ * boundary id of an external side; NULL puts every external side on
+ * boundary 0.  Neighbours and outward face planes are derived here. */
+int mcw_mesh_set(mcw_ctx *ctx, int64_t n_nodes, const double *xyz, int64_t n_elem,
+                 const uint8_t *elem_type, const int64_t *conn_offsets, const int32_t *conn,
+                 const int32_t *subdomain, const int32_t *face_boundary);
+
+/* RayBCs: ReflectRayBC / KillRayBC on a boundary id (openmc_study.i:49-54). */
+int mcw_bc_set(mcw_ctx *ctx, int32_t boundary_id, int32_t kind);
+
+/* Nuclear data (replaces the HDF5 library OpenMC loads in openmc_init). */
+int mcw_xs_set(mcw_ctx *ctx, int32_t n_nuclides, const mcw_nuclide *nuclides);
+int mcw_materials_set(mcw_ctx *ctx, int32_t n_materials, const mcw_material *materials);
+
+/* CollisionKernel `blocks` / `materials` parameters (CollisionKernel.C:63-69). */
+int mcw_block_material_map(mcw_ctx *ctx, int32_t n, const int32_t *blocks,
+                           const int32_t *materials);
+
+/* External source (settings.xml <source>; sampled as OpenMC's
+ * sample_external_source on stream 1). */
+int mcw_source_set(mcw_ctx *ctx, const mcw_source *src);
+
+/* OpenMCTally::initialize (OpenMCTally.C:103-225): a tally with filters,
+ * scores and nuclide bins (-1 = "total").  Filters are ordered as
+ * OpenMCTally sorts them (universe < cell < energy). */
+int mcw_tally_add(mcw_ctx *ctx, int32_t id, int32_t estimator, int32_t n_filters,
+                  const mcw_filter *filters, int32_t n_scores, const int32_t *scores,
+                  int32_t n_nuclides, const int32_t *nuclides);
+
+/* Shape of a tally's results array: n_filter_bins x n_score_bins. */
+int mcw_tally_shape(mcw_ctx *ctx, int32_t id, int64_t *n_filter_bins, int64_t *n_score_bins);
+
+/* ---- execution ------------------------------------------------------------ */
+
+/* One batch: generateRays + execute + postExecuteStudy (OpenMCStudy.C:218-402)
+ * for particles 1..n_particles of `batch` (1-based), then VALUE -> SUM,
+ * SUM_SQ as OpenMC's finalize_batch. */
+int mcw_run_batch(mcw_ctx *ctx, int64_t batch, int64_t n_particles, mcw_batch_stats *out);
+
+/* Read a tally's results kind (MCW_RESULT_*) into out[n_filter_bins*n_score_bins]. */
+int mcw_tally_get(mcw_ctx *ctx, int32_t id, int32_t kind, double *out, size_t n);
+
+/* Zero every tally (VALUE, SUM, SUM_SQ). */
+int mcw_tally_reset(mcw_ctx *ctx);
+
+/* Per-history records of the last batch (options.record_histories = 1). */
+int mcw_history_get(mcw_ctx *ctx, mcw_history *out, size_t n);
+
+/* Diagnostics (no reference counterpart): evaluate the device log,
+ * sin/cos(2 pi frac(x)), sqrt, one random draw from each seed and a seed
+ * skip-ahead future_seed(i*152917 + 7, seed_i) on n host inputs, so tests can
+ * compare the device's arithmetic with the CPU's bit for bit. */
+int mcw_debug_math_probe(mcw_ctx *ctx, const double *x, int64_t n, double *log_out, double *sin_out,
+                         double *cos_out, double *sqrt_out, const uint64_t *seeds, double *prn_out,
+                         uint64_t *seed_out);
+
+/* ---- serving (continuous batching of fixed-source requests) -------------- */
+
+/* A request: a source, a particle count and a per-element flux tally
+ * (track-length) on one resident mesh (mesh_slot from mcw_mesh_register). */
+typedef struct {
+  int64_t request_id;
+  int32_t mesh_slot;
+  int32_t pad;
+  int64_t n_particles;
+  mcw_source source;
+} mcw_request;
+
+typedef struct {
+  int64_t request_id;
+  int64_t histories;
+  int64_t segments;
+  int64_t collisions;
+  double t_submit_ms;
+  double t_admit_ms;
+  double t_done_ms;
+} mcw_request_status;
+
+/* Build a server over `n_meshes` resident contexts (each a mesh + materials +
+ * block map); every context shares one XS set of its own.  max_inflight
+ * bounds concurrently admitted requests; bank_bytes bounds particle-bank
+ * memory across them. */
+typedef struct mcw_server mcw_server;
+int mcw_server_create(mcw_server **srv, mcw_ctx *const *meshes, int32_t n_meshes,
+                      int32_t max_inflight, int64_t bank_bytes);
+int mcw_server_destroy(mcw_server *srv);
+const char *mcw_server_last_error(const mcw_server *srv);
+int mcw_submit(mcw_server *srv, const mcw_request *req);
+/* Advance the device work; returns the number of finished requests waiting. */
+int mcw_poll(mcw_server *srv, int32_t *n_done);
+/* Pop one finished request; flux receives n_elem values of its mesh
+ * (track-length flux per source particle). */
+int mcw_result(mcw_server *srv, mcw_request_status *status, double *flux, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MACAW_AMD_H */

@@ -1,0 +1,111 @@
+"""ctypes mirror of the plain-data structs and enums in include/macaw_amd.h.
+
+These are the types of the C ABI; the library loader (`macaw_amd.lib`) and
+the test-side oracle loader share them so both are fed the same bytes.
+"""
+import ctypes as C
+
+TET4, HEX8 = 4, 8
+BC_VACUUM, BC_REFLECT = 0, 1
+EST_ANALOG, EST_TRACKLENGTH, EST_COLLISION = 0, 1, 2
+RESULT_VALUE, RESULT_SUM, RESULT_SUM_SQ = 0, 1, 2
+FILTER_UNIVERSE, FILTER_CELL, FILTER_ENERGY, FILTER_PARTICLE = 1, 2, 3, 4
+SCORE_FLUX, SCORE_TOTAL, SCORE_ABSORPTION, SCORE_SCATTER, SCORE_FISSION = 0, 1, 2, 3, 4
+SPACE_POINT, SPACE_BOX, SPACE_MESH_VOLUME = 0, 1, 2
+ANGLE_ISOTROPIC, ANGLE_LINEAR_MU = 0, 1
+ENERGY_MONO, ENERGY_WATT = 0, 1
+MODE_PLAIN, MODE_BATCHED = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+FATE_ALIVE, FATE_LEAKED, FATE_ABSORBED, FATE_CUTOFF, FATE_LOST = 0, 1, 2, 3, 4
+
+SCORE_NAMES = {"flux": SCORE_FLUX, "total": SCORE_TOTAL, "absorption": SCORE_ABSORPTION,
+               "scatter": SCORE_SCATTER, "fission": SCORE_FISSION}
+FILTER_NAMES = {"universe": FILTER_UNIVERSE, "cell": FILTER_CELL, "energy": FILTER_ENERGY,
+                "particle": FILTER_PARTICLE}
+ESTIMATOR_NAMES = {"ANALOG": EST_ANALOG, "TRACKLENGTH": EST_TRACKLENGTH,
+                   "COLLISION": EST_COLLISION}
+
+_dp = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+
+
+class Options(C.Structure):
+    _fields_ = [("device", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
+                ("seed_roundtrip", C.c_int32), ("record_histories", C.c_int32),
+                ("block_size", C.c_int32), ("bank_capacity", C.c_int64),
+                ("max_segments", C.c_int64), ("master_seed", C.c_uint64),
+                ("sort_cadence", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
+class Nuclide(C.Structure):
+    _fields_ = [("n_points", C.c_int32), ("awr", C.c_double), ("energy", _dp), ("total", _dp),
+                ("absorption", _dp), ("scatter", _dp), ("fission", _dp)]
+
+
+class Material(C.Structure):
+    _fields_ = [("id", C.c_int32), ("n_nuclides", C.c_int32), ("nuclide", _i32p),
+                ("density", _dp)]
+
+
+class Source(C.Structure):
+    _fields_ = [("space", C.c_int32), ("angle", C.c_int32), ("energy", C.c_int32),
+                ("pad", C.c_int32), ("point", C.c_double * 3), ("box_lo", C.c_double * 3),
+                ("box_hi", C.c_double * 3), ("axis", C.c_double * 3), ("beta", C.c_double),
+                ("E0", C.c_double), ("watt_a", C.c_double), ("watt_b", C.c_double)]
+
+
+class Filter(C.Structure):
+    _fields_ = [("type", C.c_int32), ("n_bins", C.c_int32), ("ids", _i64p), ("edges", _dp)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("histories", C.c_int64), ("segments", C.c_int64), ("collisions", C.c_int64),
+                ("absorptions", C.c_int64), ("leaks", C.c_int64), ("cutoffs", C.c_int64),
+                ("lost", C.c_int64), ("reflections", C.c_int64), ("track_length", C.c_double),
+                ("time_ms", C.c_double), ("kernel_ms", C.c_double),
+                ("kernel_launches", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class History(C.Structure):
+    _fields_ = [("id", C.c_int64), ("digest", C.c_uint64), ("seed", C.c_uint64),
+                ("E", C.c_double), ("x", C.c_double), ("y", C.c_double), ("z", C.c_double),
+                ("n_segments", C.c_int32), ("n_collisions", C.c_int32), ("elem", C.c_int32),
+                ("fate", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("request_id", C.c_int64), ("mesh_slot", C.c_int32), ("pad", C.c_int32),
+                ("n_particles", C.c_int64), ("source", Source)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("request_id", C.c_int64), ("histories", C.c_int64), ("segments", C.c_int64),
+                ("collisions", C.c_int64), ("t_submit_ms", C.c_double),
+                ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double)]
+
+
+HISTORY_DTYPE = None
+
+
+def history_dtype():
+    """numpy dtype matching mcw_history (for bulk copies)."""
+    import numpy as np
+    global HISTORY_DTYPE
+    if HISTORY_DTYPE is None:
+        HISTORY_DTYPE = np.dtype([("id", "<i8"), ("digest", "<u8"), ("seed", "<u8"), ("E", "<f8"),
+                                  ("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
+                                  ("n_segments", "<i4"), ("n_collisions", "<i4"),
+                                  ("elem", "<i4"), ("fate", "<i4")])
+        assert HISTORY_DTYPE.itemsize == C.sizeof(History)
+    return HISTORY_DTYPE
+
+
+def ptr(a, ctype):
+    """Pointer to a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,916 @@
+// mcw_api.hip -- the C ABI (include/macaw_amd.h): context, device memory,
+// problem upload and the batch driver.
+//
+// A context plays the part of OpenMCStudy + OpenMC's global state for one
+// GPU: mcw_create/mcw_destroy bracket openmc_init/openmc_finalize
+// (src/userobjects/OpenMCStudy.C:127-147,213-215), mcw_run_batch is one
+// Transient step (generateRays + execute + postExecuteStudy,
+// OpenMCStudy.C:218-402) and mcw_tally_get is what OpenMCTallyAux reads
+// (src/auxkernels/OpenMCTallyAux.C:180-203).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/macaw_amd.h"
+#include "mcw_mesh.h"
+#include "mcw_types.h"
+
+namespace mcw {
+hipError_t launch_segment(int ns, dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B,
+                          const int32_t *live_in, int qin, int32_t *live_out, int qout, int32_t *coll_q, int qcoll);
+hipError_t launch_persistent(int ns, dim3 g, dim3 b, hipStream_t s, const Params &P);
+hipError_t launch_source(dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B, int32_t *live, int64_t n);
+hipError_t launch_collide(dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B, const int32_t *coll_q,
+                          int qcoll, int32_t *live_out, int qout);
+hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1);
+hipError_t launch_reset_work(hipStream_t s, Counters *C);
+hipError_t launch_fill(hipStream_t s, double *p, int64_t n, double v);
+hipError_t launch_zero_counters(hipStream_t s, Counters *C);
+hipError_t launch_accumulate(hipStream_t s, double *value, double *sum, double *sum_sq, int64_t n, double norm);
+hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *lo, double *so, double *co,
+                             double *sq, const uint64_t *seeds, double *po, uint64_t *seo);
+} // namespace mcw
+
+using namespace mcw;
+
+namespace {
+
+struct McwError {
+  std::string msg;
+};
+
+#define HIPCHK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess)                                                                    \
+      throw McwError{std::string(#expr) + ": " + hipGetErrorString(_e)};                     \
+  } while (0)
+
+// Device allocation owned by a context.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release()
+  {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n)
+  {
+    if (n == 0) n = 8;
+    if (p && bytes >= n) return;
+    release();
+    HIPCHK(hipMalloc(&p, n));
+    bytes = n;
+  }
+  template <class T> void upload(const std::vector<T> &v, hipStream_t s)
+  {
+    alloc(v.size() * sizeof(T));
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  template <class T> T *as() const { return (T *)p; }
+};
+
+struct HostNuclide {
+  double awr;
+  std::vector<double> E;
+  std::vector<double> t, a, s, f;
+};
+
+struct HostMaterial {
+  int32_t id;
+  std::vector<int32_t> nuc;
+  std::vector<double> dens;
+};
+
+struct HostTally {
+  int32_t id, estimator;
+  std::vector<int32_t> ftype;
+  std::vector<std::vector<int64_t>> fids;
+  std::vector<std::vector<double>> fedges;
+  std::vector<int32_t> scores, nuclides;
+  int64_t n_filter_bins = 1, n_score_bins = 1;
+  std::vector<int64_t> strides, nbins;
+  // device
+  DevBuf value, sum, sum_sq, elem_base, edges, d_scores, d_nuclides;
+  bool identity = false;
+  int32_t n_edges = 0;
+  int64_t energy_stride = 0;
+};
+
+} // namespace
+
+struct mcw_ctx {
+  mcw_options opt;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int n_cu = 256;
+
+  HostMesh mesh;
+  bool have_mesh = false;
+  std::map<int32_t, int32_t> bc;
+  std::vector<HostNuclide> nucs;
+  std::vector<HostMaterial> mats;
+  std::map<int32_t, int32_t> block_mat;
+  bool have_block_map = false;
+  mcw_source src;
+  bool have_src = false;
+  int64_t src_elem = -1;
+  std::vector<std::unique_ptr<HostTally>> tallies;
+  bool dirty = true;
+
+  // device problem
+  DevBuf plane, nbr, elem_mat, face_off, n_sides, bc_kind, cdf, xyz, conn, conn_off, type;
+  DevBuf grid_off, npts, energy, values, awr, mat_off, mat_nuc, mat_dens;
+  DevBuf counters;
+  Counters *h_counters = nullptr; // pinned
+  // bank (plain) and records
+  DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_sigt, b_seed, b_elem, b_inface, b_mat, b_hid, b_digest, b_nseg,
+      b_ncoll, q_live0, q_live1, q_coll;
+  int64_t bank_cap = 0;
+  DevBuf r_digest, r_seed, r_E, r_x, r_y, r_z, r_nseg, r_ncoll, r_elem, r_fate;
+  int64_t rec_n = 0;
+  int64_t last_batch_n = 0;
+  int64_t last_batch = 0;
+  double e_min = 0, e_max = 0;
+  int32_t n_bc = 0;
+
+  Params base{};
+};
+
+namespace {
+
+int fail(mcw_ctx *c, const std::string &m)
+{
+  if (c) c->err = m;
+  return 1;
+}
+
+template <class F> int guard(mcw_ctx *c, F &&f)
+{
+  if (!c) return 1;
+  try {
+    HIPCHK(hipSetDevice(c->opt.device));
+    f();
+    return 0;
+  } catch (const McwError &e) {
+    return fail(c, e.msg);
+  } catch (const std::exception &e) {
+    return fail(c, e.what());
+  }
+}
+
+void require(bool ok, const std::string &m)
+{
+  if (!ok) throw McwError{m};
+}
+
+// Upload everything the kernels read; validate the problem as
+// CollisionKernel::initialSetup does (CollisionKernel.C:78-96).
+void prepare(mcw_ctx *c)
+{
+  if (!c->dirty) return;
+  require(c->have_mesh, "no mesh: call mcw_mesh_set");
+  require(!c->nucs.empty(), "no nuclear data: call mcw_xs_set");
+  require(!c->mats.empty(), "no materials: call mcw_materials_set");
+  require(c->have_block_map, "no block to material map: call mcw_block_material_map");
+  require(c->have_src, "no source: call mcw_source_set");
+  hipStream_t s = c->stream;
+  HostMesh &m = c->mesh;
+
+  // boundary conditions: every external boundary needs a RayBC
+  int32_t max_bid = -1;
+  for (int32_t b : m.boundary_ids) max_bid = std::max(max_bid, b);
+  std::vector<int8_t> kinds(max_bid + 1, 0);
+  for (int32_t b : m.boundary_ids) {
+    auto it = c->bc.find(b);
+    require(it != c->bc.end(), "boundary " + std::to_string(b) + " has external sides but no RayBC");
+    kinds[b] = (int8_t)it->second;
+  }
+  c->n_bc = max_bid + 1;
+
+  // materials by id (CollisionKernel: block -> OpenMC material id)
+  std::map<int32_t, int32_t> id_to_idx;
+  for (size_t i = 0; i < c->mats.size(); ++i) id_to_idx[c->mats[i].id] = (int32_t)i;
+  for (auto &bm : c->block_mat)
+    require(id_to_idx.count(bm.second), "Could not find material " + std::to_string(bm.second) + " in OpenMC materials.");
+  std::vector<int32_t> emat(m.n_elem, -1);
+  for (int64_t e = 0; e < m.n_elem; ++e) {
+    auto b = c->block_mat.find(m.subdomain[e]);
+    if (b != c->block_mat.end()) emat[e] = id_to_idx[b->second];
+  }
+
+  // mesh
+  std::vector<double4> planes(m.plane.size() / 4);
+  for (size_t i = 0; i < planes.size(); ++i)
+    planes[i] = make_double4(m.plane[4 * i], m.plane[4 * i + 1], m.plane[4 * i + 2], m.plane[4 * i + 3]);
+  c->plane.upload(planes, s);
+  c->nbr.upload(m.nbr, s);
+  c->elem_mat.upload(emat, s);
+  c->bc_kind.upload(kinds, s);
+  c->cdf.upload(m.cdf, s);
+  c->xyz.upload(m.xyz, s);
+  c->conn.upload(m.conn, s);
+  c->conn_off.upload(m.conn_off, s);
+  c->type.upload(m.type, s);
+  if (m.uniform_sides == 0) {
+    std::vector<int32_t> fo(m.n_elem);
+    std::vector<uint8_t> ns(m.n_elem);
+    for (int64_t e = 0; e < m.n_elem; ++e) {
+      fo[e] = (int32_t)m.face_off[e];
+      ns[e] = (uint8_t)(m.face_off[e + 1] - m.face_off[e]);
+    }
+    c->face_off.upload(fo, s);
+    c->n_sides.upload(ns, s);
+  }
+
+  // nuclear data
+  std::vector<int64_t> goff;
+  std::vector<int32_t> np;
+  std::vector<double> en, aw;
+  std::vector<double4> vals;
+  c->e_min = INFINITY;
+  c->e_max = -INFINITY;
+  for (auto &n : c->nucs) {
+    goff.push_back((int64_t)en.size());
+    np.push_back((int32_t)n.E.size());
+    aw.push_back(n.awr);
+    for (size_t j = 0; j < n.E.size(); ++j) {
+      en.push_back(n.E[j]);
+      vals.push_back(make_double4(n.t[j], n.a[j], n.s[j], n.f.empty() ? 0.0 : n.f[j]));
+    }
+    c->e_min = std::min(c->e_min, n.E.front());
+    c->e_max = std::max(c->e_max, n.E.back());
+  }
+  c->grid_off.upload(goff, s);
+  c->npts.upload(np, s);
+  c->energy.upload(en, s);
+  c->values.upload(vals, s);
+  c->awr.upload(aw, s);
+  std::vector<int32_t> moff{0}, mnuc;
+  std::vector<double> mden;
+  for (auto &mt : c->mats) {
+    for (size_t q = 0; q < mt.nuc.size(); ++q) {
+      require(mt.nuc[q] >= 0 && mt.nuc[q] < (int32_t)c->nucs.size(),
+              "material " + std::to_string(mt.id) + " refers to an unknown nuclide");
+      mnuc.push_back(mt.nuc[q]);
+      mden.push_back(mt.dens[q]);
+    }
+    moff.push_back((int32_t)mnuc.size());
+  }
+  c->mat_off.upload(moff, s);
+  c->mat_nuc.upload(mnuc, s);
+  c->mat_dens.upload(mden, s);
+
+  // source
+  c->src_elem = -1;
+  if (c->src.space == MCW_SPACE_POINT) {
+    c->src_elem = locate_point(m, c->src.point, 1e-12);
+    require(c->src_elem >= 0, "point source outside the mesh");
+  } else {
+    require(c->src.space == MCW_SPACE_MESH_VOLUME, "source space type not supported by this path");
+  }
+  require(c->src.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
+  require(c->src.energy == MCW_ENERGY_MONO || c->src.energy == MCW_ENERGY_WATT, "source energy type not supported");
+
+  c->counters.alloc(sizeof(Counters));
+  if (!c->h_counters) HIPCHK(hipHostMalloc((void **)&c->h_counters, sizeof(Counters), hipHostMallocDefault));
+
+  // views
+  Params &P = c->base;
+  P = Params{};
+  P.mesh.plane = c->plane.as<double4>();
+  P.mesh.nbr = c->nbr.as<int32_t>();
+  P.mesh.elem_mat = c->elem_mat.as<int32_t>();
+  P.mesh.face_off = m.uniform_sides ? nullptr : c->face_off.as<int32_t>();
+  P.mesh.n_sides = m.uniform_sides ? nullptr : c->n_sides.as<uint8_t>();
+  P.mesh.stride = m.uniform_sides;
+  P.mesh.n_bc = c->n_bc;
+  P.mesh.bc_kind = c->bc_kind.as<int8_t>();
+  P.mesh.n_elem = m.n_elem;
+  P.mesh.cdf = c->cdf.as<double>();
+  P.mesh.xyz = c->xyz.as<double>();
+  P.mesh.conn = c->conn.as<int32_t>();
+  P.mesh.conn_off = c->conn_off.as<int64_t>();
+  P.mesh.type = c->type.as<uint8_t>();
+  P.mesh.orig_id = nullptr;
+  P.xs.grid_off = c->grid_off.as<int64_t>();
+  P.xs.n_points = c->npts.as<int32_t>();
+  P.xs.energy = c->energy.as<double>();
+  P.xs.values = c->values.as<double4>();
+  P.xs.awr = c->awr.as<double>();
+  P.xs.mat_off = c->mat_off.as<int32_t>();
+  P.xs.mat_nuc = c->mat_nuc.as<int32_t>();
+  P.xs.mat_dens = c->mat_dens.as<double>();
+  P.xs.n_nuc = (int32_t)c->nucs.size();
+  P.xs.n_mat = (int32_t)c->mats.size();
+  P.xs.e_min = c->e_min;
+  P.xs.e_max = c->e_max;
+  P.src.space = c->src.space;
+  P.src.angle = c->src.angle;
+  P.src.energy = c->src.energy;
+  for (int k = 0; k < 3; ++k) {
+    P.src.point[k] = c->src.point[k];
+    P.src.box_lo[k] = c->src.box_lo[k];
+    P.src.box_hi[k] = c->src.box_hi[k];
+    P.src.axis[k] = c->src.axis[k];
+  }
+  P.src.beta = c->src.beta;
+  P.src.E0 = c->src.E0;
+  P.src.watt_a = c->src.watt_a;
+  P.src.watt_b = c->src.watt_b;
+  P.src.point_elem = c->src_elem;
+
+  // tallies: track-length ones first, then collision ones
+  std::vector<HostTally *> order;
+  for (auto &t : c->tallies)
+    if (t->estimator == MCW_EST_TRACKLENGTH) order.push_back(t.get());
+  int n_tl = (int)order.size();
+  for (auto &t : c->tallies)
+    if (t->estimator == MCW_EST_COLLISION) order.push_back(t.get());
+  require((int)order.size() <= MAX_TALLIES, "too many tallies (max 8)");
+  P.n_tallies = (int32_t)order.size();
+  P.n_tl = n_tl;
+  P.n_coll = (int32_t)order.size() - n_tl;
+  for (size_t k = 0; k < order.size(); ++k) {
+    HostTally &t = *order[k];
+    TallyView &V = P.tally[k];
+    V.estimator = t.estimator;
+    V.n_scores = (int32_t)t.scores.size();
+    V.n_nuclides = (int32_t)t.nuclides.size();
+    V.n_score_bins = (int32_t)t.n_score_bins;
+    V.simple_flux = (t.scores.size() == 1 && t.scores[0] == MCW_SCORE_FLUX && t.nuclides.size() == 1) ? 1 : 0;
+    V.needs_xs = 0;
+    for (int32_t sc : t.scores) V.needs_xs |= sc != MCW_SCORE_FLUX;
+    V.n_edges = t.n_edges;
+    V.energy_stride = t.energy_stride;
+    V.elem_base = t.identity ? nullptr : t.elem_base.as<int64_t>();
+    V.edges = t.n_edges ? t.edges.as<double>() : nullptr;
+    V.scores = t.d_scores.as<int32_t>();
+    V.nuclides = t.d_nuclides.as<int32_t>();
+    V.value = t.value.as<double>();
+  }
+  P.master_seed = c->opt.master_seed ? c->opt.master_seed : 1;
+  P.roundtrip = c->opt.seed_roundtrip;
+  P.max_seg = c->opt.max_segments > 0 ? c->opt.max_segments : ((int64_t)1 << 20);
+  P.cnt = c->counters.as<Counters>();
+  HIPCHK(hipStreamSynchronize(s));
+  c->dirty = false;
+}
+
+// Build a tally's device layout (OpenMC Tally::set_strides / init_results).
+void build_tally(mcw_ctx *c, HostTally &t)
+{
+  const HostMesh &m = c->mesh;
+  const size_t nf = t.ftype.size();
+  t.nbins.assign(nf, 1);
+  std::vector<std::vector<int64_t>> elem_bin(nf);
+  for (size_t k = 0; k < nf; ++k) {
+    int32_t ty = t.ftype[k];
+    if (ty == MCW_FILTER_CELL || ty == MCW_FILTER_UNIVERSE) {
+      std::vector<int64_t> ids = t.fids[k];
+      if (ids.empty()) {
+        if (ty == MCW_FILTER_CELL) {
+          ids.resize(m.n_elem);
+          for (int64_t e = 0; e < m.n_elem; ++e) ids[e] = e;
+        } else {
+          std::vector<int64_t> b(m.subdomain.begin(), m.subdomain.end());
+          std::sort(b.begin(), b.end());
+          b.erase(std::unique(b.begin(), b.end()), b.end());
+          ids = b;
+        }
+      }
+      t.nbins[k] = (int64_t)ids.size();
+      std::map<int64_t, int64_t> pos;
+      for (int64_t b = 0; b < (int64_t)ids.size(); ++b) pos.emplace(ids[b], b);
+      elem_bin[k].assign(m.n_elem, -1);
+      for (int64_t e = 0; e < m.n_elem; ++e) {
+        auto it = pos.find(ty == MCW_FILTER_CELL ? e : (int64_t)m.subdomain[e]);
+        if (it != pos.end()) elem_bin[k][e] = it->second;
+      }
+    } else if (ty == MCW_FILTER_ENERGY) {
+      require(t.fedges[k].size() >= 2, "energy filter needs at least two edges");
+      for (size_t q = 1; q < t.fedges[k].size(); ++q)
+        require(t.fedges[k][q] > t.fedges[k][q - 1], "energy filter edges must increase");
+      t.nbins[k] = (int64_t)t.fedges[k].size() - 1;
+    } else if (ty == MCW_FILTER_PARTICLE) {
+      t.nbins[k] = 1;
+    } else {
+      throw McwError{"Unrecognized filter"};
+    }
+  }
+  t.strides.assign(nf, 0);
+  int64_t stride = 1;
+  for (int k = (int)nf - 1; k >= 0; --k) {
+    t.strides[k] = stride;
+    stride *= t.nbins[k];
+  }
+  t.n_filter_bins = stride;
+  t.n_score_bins = (int64_t)t.scores.size() * (int64_t)t.nuclides.size();
+  // spatial base per element
+  std::vector<int64_t> base(m.n_elem, 0);
+  bool any_spatial = false;
+  t.n_edges = 0;
+  t.energy_stride = 0;
+  for (size_t k = 0; k < nf; ++k) {
+    if (t.ftype[k] == MCW_FILTER_CELL || t.ftype[k] == MCW_FILTER_UNIVERSE) {
+      any_spatial = true;
+      for (int64_t e = 0; e < m.n_elem; ++e) {
+        if (base[e] < 0) continue;
+        base[e] = elem_bin[k][e] < 0 ? -1 : base[e] + elem_bin[k][e] * t.strides[k];
+      }
+    } else if (t.ftype[k] == MCW_FILTER_ENERGY) {
+      require(t.n_edges == 0, "at most one energy filter per tally");
+      t.n_edges = (int32_t)t.fedges[k].size();
+      t.energy_stride = t.strides[k];
+      t.edges.upload(t.fedges[k], c->stream);
+    }
+  }
+  t.identity = any_spatial;
+  if (any_spatial)
+    for (int64_t e = 0; e < m.n_elem && t.identity; ++e) t.identity = base[e] == e;
+  if (!any_spatial) {
+    // no spatial filter: every element scores into spatial bin 0
+    t.identity = false;
+  }
+  t.elem_base.upload(base, c->stream);
+  t.d_scores.upload(t.scores, c->stream);
+  t.d_nuclides.upload(t.nuclides, c->stream);
+  const size_t n = (size_t)(t.n_filter_bins * t.n_score_bins);
+  t.value.alloc(n * 8);
+  t.sum.alloc(n * 8);
+  t.sum_sq.alloc(n * 8);
+  HIPCHK(launch_fill(c->stream, t.value.as<double>(), (int64_t)n, 0.0));
+  HIPCHK(launch_fill(c->stream, t.sum.as<double>(), (int64_t)n, 0.0));
+  HIPCHK(launch_fill(c->stream, t.sum_sq.as<double>(), (int64_t)n, 0.0));
+}
+
+void ensure_bank(mcw_ctx *c, int64_t n)
+{
+  if (c->bank_cap >= n) return;
+  size_t d = (size_t)n * 8, i = (size_t)n * 4;
+  c->b_x.alloc(d); c->b_y.alloc(d); c->b_z.alloc(d);
+  c->b_u.alloc(d); c->b_v.alloc(d); c->b_w.alloc(d);
+  c->b_E.alloc(d); c->b_sigt.alloc(d); c->b_seed.alloc(d); c->b_hid.alloc(d); c->b_digest.alloc(d);
+  c->b_elem.alloc(i); c->b_inface.alloc(i); c->b_mat.alloc(i); c->b_nseg.alloc(i); c->b_ncoll.alloc(i);
+  c->q_live0.alloc(i); c->q_live1.alloc(i); c->q_coll.alloc(i);
+  c->bank_cap = n;
+}
+
+void ensure_records(mcw_ctx *c, int64_t n)
+{
+  if (c->rec_n >= n) return;
+  size_t d = (size_t)n * 8, i = (size_t)n * 4;
+  c->r_digest.alloc(d); c->r_seed.alloc(d); c->r_E.alloc(d); c->r_x.alloc(d); c->r_y.alloc(d); c->r_z.alloc(d);
+  c->r_nseg.alloc(i); c->r_ncoll.alloc(i); c->r_elem.alloc(i); c->r_fate.alloc(i);
+  c->rec_n = n;
+}
+
+BankView bank_view(mcw_ctx *c)
+{
+  BankView B;
+  B.x = c->b_x.as<double>(); B.y = c->b_y.as<double>(); B.z = c->b_z.as<double>();
+  B.u = c->b_u.as<double>(); B.v = c->b_v.as<double>(); B.w = c->b_w.as<double>();
+  B.E = c->b_E.as<double>(); B.sigt = c->b_sigt.as<double>();
+  B.seed = c->b_seed.as<uint64_t>();
+  B.elem = c->b_elem.as<int32_t>(); B.inface = c->b_inface.as<int32_t>(); B.mat = c->b_mat.as<int32_t>();
+  B.hid = c->b_hid.as<int64_t>();
+  B.digest = c->b_digest.as<uint64_t>();
+  B.nseg = c->b_nseg.as<int32_t>(); B.ncoll = c->b_ncoll.as<int32_t>();
+  return B;
+}
+
+int persistent_blocks(mcw_ctx *c, int ns, int block)
+{
+  (void)ns;
+  // A wavefront's register file is the limit; 8 workgroups of 256 per CU keep
+  // 32 wavefronts resident when the kernel fits, fewer otherwise (the
+  // hardware queues the rest).
+  return c->n_cu * 8 * 256 / block;
+}
+
+void run_plain(mcw_ctx *c, Params &P, int64_t n)
+{
+  hipStream_t s = c->stream;
+  const int ns = c->mesh.uniform_sides;
+  const int block = 256;
+  ensure_bank(c, n);
+  BankView B = bank_view(c);
+  int32_t *q[2] = {c->q_live0.as<int32_t>(), c->q_live1.as<int32_t>()};
+  int32_t *qc = c->q_coll.as<int32_t>();
+  HIPCHK(launch_reset_queue(s, P.cnt, 1, 2));
+  HIPCHK(launch_source(dim3((unsigned)((n + block - 1) / block)), dim3(block), s, P, B, q[0], n));
+  const int64_t max_blocks = (int64_t)c->n_cu * 8;
+  int64_t bound = n;
+  int cur = 0;
+  for (int64_t it = 0;; ++it) {
+    const int nxt = 1 - cur;
+    int64_t blocks = std::min<int64_t>((bound + block - 1) / block, max_blocks);
+    if (blocks < 1) blocks = 1;
+    HIPCHK(launch_reset_queue(s, P.cnt, nxt, 2));
+    HIPCHK(launch_segment(ns, dim3((unsigned)blocks), dim3(block), s, P, B, q[cur], cur, q[nxt], nxt, qc, 2));
+    HIPCHK(launch_collide(dim3((unsigned)blocks), dim3(block), s, P, B, qc, 2, q[nxt], nxt));
+    cur = nxt;
+    if ((it & 7) == 7) {
+      HIPCHK(hipMemcpyAsync(&c->h_counters->q_count[cur], &P.cnt->q_count[cur], 8, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      bound = (int64_t)c->h_counters->q_count[cur];
+      if (bound == 0) break;
+    }
+  }
+}
+
+} // namespace
+
+// ======================================================================== C ABI
+extern "C" {
+
+void mcw_default_options(mcw_options *o)
+{
+  if (!o) return;
+  memset(o, 0, sizeof *o);
+  o->mode = MCW_MODE_BATCHED;
+  o->precision = MCW_PREC_FP64;
+  o->master_seed = 1;
+}
+
+int mcw_create(mcw_ctx **out, const mcw_options *opts)
+{
+  if (!out) return 1;
+  *out = nullptr;
+  mcw_ctx *c = new mcw_ctx();
+  mcw_default_options(&c->opt);
+  if (opts) c->opt = *opts;
+  if (c->opt.master_seed == 0) c->opt.master_seed = 1;
+  int rc = guard(c, [&] {
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    require(c->opt.device >= 0 && c->opt.device < ndev, "no such HIP device " + std::to_string(c->opt.device));
+    HIPCHK(hipSetDevice(c->opt.device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->opt.device));
+    c->n_cu = prop.multiProcessorCount;
+  });
+  *out = c; // returned even on failure so mcw_last_error can be read
+  return rc;
+}
+
+int mcw_destroy(mcw_ctx *c)
+{
+  if (!c) return 0;
+  (void)hipSetDevice(c->opt.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->tallies.clear();
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->h_counters) (void)hipHostFree(c->h_counters);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+const char *mcw_last_error(const mcw_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int mcw_mesh_set(mcw_ctx *c, int64_t n_nodes, const double *xyz, int64_t n_elem, const uint8_t *elem_type,
+                 const int64_t *conn_offsets, const int32_t *conn, const int32_t *subdomain,
+                 const int32_t *face_boundary)
+{
+  return guard(c, [&] {
+    require(n_nodes > 0 && n_elem > 0 && xyz && elem_type && conn_offsets && conn && subdomain,
+            "mcw_mesh_set: empty mesh or null array");
+    HostMesh &m = c->mesh;
+    m = HostMesh();
+    m.n_nodes = n_nodes;
+    m.n_elem = n_elem;
+    m.xyz.assign(xyz, xyz + 3 * n_nodes);
+    m.type.assign(elem_type, elem_type + n_elem);
+    m.conn_off.assign(conn_offsets, conn_offsets + n_elem + 1);
+    require(m.conn_off[0] == 0, "conn_offsets must start at 0");
+    for (int64_t e = 0; e < n_elem; ++e) require(m.conn_off[e + 1] >= m.conn_off[e], "conn_offsets must not decrease");
+    m.conn.assign(conn, conn + m.conn_off[n_elem]);
+    m.subdomain.assign(subdomain, subdomain + n_elem);
+    std::string e = derive_mesh(m, face_boundary);
+    require(e.empty(), e);
+    c->have_mesh = true;
+    c->dirty = true;
+    // tally layouts depend on the mesh
+    for (auto &t : c->tallies) build_tally(c, *t);
+  });
+}
+
+int mcw_bc_set(mcw_ctx *c, int32_t boundary_id, int32_t kind)
+{
+  return guard(c, [&] {
+    require(boundary_id >= 0, "boundary ids must be non-negative");
+    require(kind == MCW_BC_VACUUM || kind == MCW_BC_REFLECT, "unknown RayBC kind");
+    c->bc[boundary_id] = kind;
+    c->dirty = true;
+  });
+}
+
+int mcw_xs_set(mcw_ctx *c, int32_t n, const mcw_nuclide *nuc)
+{
+  return guard(c, [&] {
+    require(n > 0 && nuc, "mcw_xs_set: no nuclides");
+    c->nucs.clear();
+    for (int i = 0; i < n; ++i) {
+      const mcw_nuclide &q = nuc[i];
+      require(q.n_points >= 2 && q.energy && q.total && q.absorption && q.scatter,
+              "nuclide " + std::to_string(i) + ": need >= 2 points and total/absorption/scatter tables");
+      require(q.awr > 0.0, "nuclide " + std::to_string(i) + ": awr must be positive");
+      HostNuclide h;
+      h.awr = q.awr;
+      h.E.assign(q.energy, q.energy + q.n_points);
+      for (int j = 1; j < q.n_points; ++j)
+        require(h.E[j] > h.E[j - 1], "nuclide " + std::to_string(i) + ": energy grid must increase");
+      h.t.assign(q.total, q.total + q.n_points);
+      h.a.assign(q.absorption, q.absorption + q.n_points);
+      h.s.assign(q.scatter, q.scatter + q.n_points);
+      if (q.fission) h.f.assign(q.fission, q.fission + q.n_points);
+      c->nucs.push_back(std::move(h));
+    }
+    c->dirty = true;
+  });
+}
+
+int mcw_materials_set(mcw_ctx *c, int32_t n, const mcw_material *m)
+{
+  return guard(c, [&] {
+    require(n > 0 && m, "mcw_materials_set: no materials");
+    c->mats.clear();
+    for (int i = 0; i < n; ++i) {
+      HostMaterial h;
+      h.id = m[i].id;
+      require(m[i].n_nuclides > 0, "material " + std::to_string(h.id) + " has no nuclides");
+      h.nuc.assign(m[i].nuclide, m[i].nuclide + m[i].n_nuclides);
+      h.dens.assign(m[i].density, m[i].density + m[i].n_nuclides);
+      c->mats.push_back(std::move(h));
+    }
+    c->dirty = true;
+  });
+}
+
+int mcw_block_material_map(mcw_ctx *c, int32_t n, const int32_t *blocks, const int32_t *materials)
+{
+  return guard(c, [&] {
+    require(n > 0 && blocks && materials, "The blocks parameter must be the same size as the materials parameter.");
+    c->block_mat.clear();
+    for (int i = 0; i < n; ++i) c->block_mat[blocks[i]] = materials[i];
+    c->have_block_map = true;
+    c->dirty = true;
+  });
+}
+
+int mcw_source_set(mcw_ctx *c, const mcw_source *s)
+{
+  return guard(c, [&] {
+    require(s != nullptr, "null source");
+    c->src = *s;
+    c->have_src = true;
+    c->dirty = true;
+  });
+}
+
+int mcw_tally_add(mcw_ctx *c, int32_t id, int32_t estimator, int32_t n_filters, const mcw_filter *filters,
+                  int32_t n_scores, const int32_t *scores, int32_t n_nuclides, const int32_t *nuclides)
+{
+  return guard(c, [&] {
+    require(estimator == MCW_EST_TRACKLENGTH || estimator == MCW_EST_COLLISION,
+            "only track-length and collision estimators are supported");
+    require(n_scores > 0 && scores, "a tally needs at least one score");
+    for (auto &t : c->tallies) require(t->id != id, "tally id " + std::to_string(id) + " already exists");
+    auto t = std::make_unique<HostTally>();
+    t->id = id;
+    t->estimator = estimator;
+    for (int k = 0; k < n_filters; ++k) {
+      t->ftype.push_back(filters[k].type);
+      std::vector<int64_t> ids;
+      std::vector<double> edges;
+      if (filters[k].type == MCW_FILTER_ENERGY) {
+        require(filters[k].edges != nullptr, "energy filter without edges");
+        edges.assign(filters[k].edges, filters[k].edges + filters[k].n_bins);
+      } else if (filters[k].n_bins > 0) {
+        require(filters[k].ids != nullptr, "filter bins without ids");
+        ids.assign(filters[k].ids, filters[k].ids + filters[k].n_bins);
+      }
+      t->fids.push_back(ids);
+      t->fedges.push_back(edges);
+    }
+    for (int k = 0; k < n_scores; ++k)
+      require(scores[k] >= MCW_SCORE_FLUX && scores[k] <= MCW_SCORE_FISSION, "unknown score");
+    t->scores.assign(scores, scores + n_scores);
+    if (n_nuclides > 0) t->nuclides.assign(nuclides, nuclides + n_nuclides);
+    else t->nuclides.push_back(-1);
+    if (c->have_mesh) build_tally(c, *t);
+    c->tallies.push_back(std::move(t));
+    c->dirty = true;
+  });
+}
+
+int mcw_tally_shape(mcw_ctx *c, int32_t id, int64_t *nf, int64_t *ns)
+{
+  return guard(c, [&] {
+    for (auto &t : c->tallies)
+      if (t->id == id) {
+        require(c->have_mesh, "tally shape needs the mesh");
+        if (nf) *nf = t->n_filter_bins;
+        if (ns) *ns = t->n_score_bins;
+        return;
+      }
+    throw McwError{"no tally with id " + std::to_string(id)};
+  });
+}
+
+int mcw_tally_reset(mcw_ctx *c)
+{
+  return guard(c, [&] {
+    for (auto &t : c->tallies) {
+      const int64_t n = t->n_filter_bins * t->n_score_bins;
+      HIPCHK(launch_fill(c->stream, t->value.as<double>(), n, 0.0));
+      HIPCHK(launch_fill(c->stream, t->sum.as<double>(), n, 0.0));
+      HIPCHK(launch_fill(c->stream, t->sum_sq.as<double>(), n, 0.0));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stats *out)
+{
+  return guard(c, [&] {
+    require(batch >= 1, "batch numbers start at 1");
+    require(n_particles >= 1, "No Rays were moved to _rays in defineRays()");
+    prepare(c);
+    hipStream_t s = c->stream;
+    Params P = c->base;
+    P.batch = batch;
+    P.n_particles = n_particles;
+    P.record = c->opt.record_histories ? 1 : 0;
+    if (P.record) {
+      ensure_records(c, n_particles);
+      P.rec.digest = c->r_digest.as<uint64_t>();
+      P.rec.seed = c->r_seed.as<uint64_t>();
+      P.rec.E = c->r_E.as<double>();
+      P.rec.x = c->r_x.as<double>();
+      P.rec.y = c->r_y.as<double>();
+      P.rec.z = c->r_z.as<double>();
+      P.rec.nseg = c->r_nseg.as<int32_t>();
+      P.rec.ncoll = c->r_ncoll.as<int32_t>();
+      P.rec.elem = c->r_elem.as<int32_t>();
+      P.rec.fate = c->r_fate.as<int32_t>();
+    }
+    for (auto &t : c->tallies) HIPCHK(launch_fill(s, t->value.as<double>(), t->n_filter_bins * t->n_score_bins, 0.0));
+    HIPCHK(launch_zero_counters(s, P.cnt));
+    const int64_t cap_default = c->opt.mode == MCW_MODE_PLAIN ? ((int64_t)1 << 24) : ((int64_t)1 << 30);
+    const int64_t cap = c->opt.bank_capacity > 0 ? c->opt.bank_capacity : cap_default;
+    const int ns = c->mesh.uniform_sides;
+    float kernel_ms = 0.0f;
+    int64_t launches = 0;
+    HIPCHK(hipEventRecord(c->ev[0], s));
+    for (int64_t h0 = 0; h0 < n_particles; h0 += cap) {
+      const int64_t h1 = std::min(n_particles, h0 + cap);
+      P.hist_begin = h0;
+      P.hist_end = h1;
+      if (c->opt.mode == MCW_MODE_PLAIN) {
+        run_plain(c, P, h1 - h0);
+      } else {
+        const int block = c->opt.block_size > 0 ? c->opt.block_size : 256;
+        require(block % 64 == 0 && block <= 1024, "block_size must be a multiple of 64, at most 1024");
+        int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, block), (h1 - h0 + block - 1) / block);
+        if (blocks < 1) blocks = 1;
+        // the work counter restarts for every chunk
+        HIPCHK(launch_reset_work(s, P.cnt));
+        HIPCHK(hipEventRecord(c->ev[2], s));
+        HIPCHK(launch_persistent(ns, dim3((unsigned)blocks), dim3(block), s, P));
+        HIPCHK(hipEventRecord(c->ev[3], s));
+        HIPCHK(hipEventSynchronize(c->ev[3]));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        kernel_ms += ms;
+        ++launches;
+      }
+    }
+    HIPCHK(hipEventRecord(c->ev[1], s));
+    // batch close: VALUE / total source weight -> SUM, SUM_SQ
+    for (auto &t : c->tallies)
+      HIPCHK(launch_accumulate(s, t->value.as<double>(), t->sum.as<double>(), t->sum_sq.as<double>(),
+                               t->n_filter_bins * t->n_score_bins, (double)n_particles));
+    HIPCHK(hipMemcpyAsync(c->h_counters, P.cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float total_ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&total_ms, c->ev[0], c->ev[1]));
+    c->last_batch_n = n_particles;
+    c->last_batch = batch;
+    if (out) {
+      const Counters &k = *c->h_counters;
+      memset(out, 0, sizeof *out);
+      out->histories = n_particles;
+      out->segments = (int64_t)k.segments;
+      out->collisions = (int64_t)k.collisions;
+      out->absorptions = (int64_t)k.absorptions;
+      out->leaks = (int64_t)k.leaks;
+      out->cutoffs = (int64_t)k.cutoffs;
+      out->lost = (int64_t)k.lost;
+      out->reflections = (int64_t)k.reflections;
+      out->track_length = k.track;
+      out->time_ms = total_ms;
+      out->kernel_ms = c->opt.mode == MCW_MODE_PLAIN ? total_ms : kernel_ms;
+      out->kernel_launches = launches;
+    }
+  });
+}
+
+int mcw_tally_get(mcw_ctx *c, int32_t id, int32_t kind, double *out, size_t n)
+{
+  return guard(c, [&] {
+    for (auto &t : c->tallies)
+      if (t->id == id) {
+        const size_t need = (size_t)(t->n_filter_bins * t->n_score_bins);
+        require(n >= need, "output buffer holds " + std::to_string(n) + " values, tally has " + std::to_string(need));
+        const DevBuf &b = kind == MCW_RESULT_VALUE ? t->value : kind == MCW_RESULT_SUM ? t->sum : t->sum_sq;
+        require(kind >= MCW_RESULT_VALUE && kind <= MCW_RESULT_SUM_SQ, "unknown result kind");
+        HIPCHK(hipMemcpyAsync(out, b.p, need * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return;
+      }
+    throw McwError{"no tally with id " + std::to_string(id)};
+  });
+}
+
+int mcw_history_get(mcw_ctx *c, mcw_history *out, size_t n)
+{
+  return guard(c, [&] {
+    require(c->opt.record_histories, "histories are recorded only with options.record_histories = 1");
+    const int64_t N = c->last_batch_n;
+    require((int64_t)n >= N, "output buffer too small for the batch's histories");
+    std::vector<uint64_t> dg(N), sd(N);
+    std::vector<double> E(N), x(N), y(N), z(N);
+    std::vector<int32_t> ns(N), nc(N), el(N), fa(N);
+    hipStream_t s = c->stream;
+    HIPCHK(hipMemcpyAsync(dg.data(), c->r_digest.p, N * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(sd.data(), c->r_seed.p, N * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(E.data(), c->r_E.p, N * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(x.data(), c->r_x.p, N * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(y.data(), c->r_y.p, N * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(z.data(), c->r_z.p, N * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ns.data(), c->r_nseg.p, N * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(nc.data(), c->r_ncoll.p, N * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(el.data(), c->r_elem.p, N * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(fa.data(), c->r_fate.p, N * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < N; ++i) {
+      mcw_history &h = out[i];
+      h.id = (c->last_batch - 1) * N + i + 1;
+      h.digest = dg[i];
+      h.seed = sd[i];
+      h.E = E[i];
+      h.x = x[i];
+      h.y = y[i];
+      h.z = z[i];
+      h.n_segments = ns[i];
+      h.n_collisions = nc[i];
+      h.elem = el[i];
+      h.fate = fa[i];
+    }
+  });
+}
+
+/* Diagnostics: evaluate the device log, sin/cos(2 pi frac(x)), sqrt, one
+ * random draw and a seed skip-ahead on n inputs (host arrays). */
+int mcw_debug_math_probe(mcw_ctx *c, const double *x, int64_t n, double *log_out, double *sin_out,
+                         double *cos_out, double *sqrt_out, const uint64_t *seeds, double *prn_out,
+                         uint64_t *seed_out)
+{
+  return guard(c, [&] {
+    hipStream_t s = c->stream;
+    DevBuf dx, dl, ds, dc, dq, dse, dp, dso;
+    size_t b = (size_t)n * 8;
+    dx.alloc(b); dl.alloc(b); ds.alloc(b); dc.alloc(b); dq.alloc(b); dse.alloc(b); dp.alloc(b); dso.alloc(b);
+    HIPCHK(hipMemcpyAsync(dx.p, x, b, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dse.p, seeds, b, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_math_probe(s, dx.as<double>(), n, dl.as<double>(), ds.as<double>(), dc.as<double>(),
+                             dq.as<double>(), dse.as<uint64_t>(), dp.as<double>(), dso.as<uint64_t>()));
+    HIPCHK(hipMemcpyAsync(log_out, dl.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(sin_out, ds.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(cos_out, dc.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(sqrt_out, dq.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(prn_out, dp.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(seed_out, dso.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  });
+}
+
+} // extern "C"

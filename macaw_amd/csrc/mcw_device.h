@@ -1,0 +1,407 @@
+// mcw_device.h -- device-side building blocks of the transport kernels.
+//
+// One segment of a history is the body of MaCaw's CollisionKernel::onSegment
+// (src/raykernels/CollisionKernel.C:98-235) fused with the MOOSE TraceRay step
+// that produced it: exit side of the current element, one collision-distance
+// draw, track-length tally, then cross / reflect / leak or collide.
+//
+// Arithmetic contract (what makes GPU histories bit-identical to a
+// history-based FP64 CPU run of the same algorithm): compiled with
+// -ffp-contract=off, every expression is written with explicit association,
+// elementary functions below use only IEEE +,-,*,/ and sqrt, and integer
+// RNG arithmetic is exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mcw_types.h"
+
+namespace mcw {
+
+// ------------------------------------------------------------------ bits
+
+__device__ __forceinline__ uint64_t dbits(double x) { return (uint64_t)__double_as_longlong(x); }
+__device__ __forceinline__ double dfrom(uint64_t u) { return __longlong_as_double((long long)u); }
+
+// ------------------------------------------------------------------ elementary functions
+//
+// log: fdlibm 5.3 e_log.c method (Sun Microsystems, 1993): x = 2^k (1+f),
+// s = f/(2+f), log(1+f) = f - (hfsq - s (hfsq + R(s^2))), R a degree-14
+// minimax polynomial.  sin/cos(2 pi xi): exact quadrant reduction of 4 xi and
+// fdlibm's __kernel_sin / __kernel_cos polynomials on [-pi/4, pi/4].
+
+__device__ __forceinline__ double log_d(double x)
+{
+  const double ln2_hi = 6.93147180369123816490e-01;
+  const double ln2_lo = 1.90821492927058770002e-10;
+  const double two54 = 1.80143985094819840000e+16;
+  const double Lg1 = 6.666666666666735130e-01;
+  const double Lg2 = 3.999999999940941908e-01;
+  const double Lg3 = 2.857142874366239149e-01;
+  const double Lg4 = 2.222219843214978396e-01;
+  const double Lg5 = 1.818357216161805012e-01;
+  const double Lg6 = 1.531383769920937332e-01;
+  const double Lg7 = 1.479819860511658591e-01;
+
+  uint64_t u = dbits(x);
+  int32_t hx = (int32_t)(u >> 32);
+  uint32_t lx = (uint32_t)u;
+  int32_t k = 0;
+  if (hx < 0x00100000) {
+    if (((hx & 0x7fffffff) | lx) == 0) return -two54 / 0.0;
+    if (hx < 0) return (x - x) / 0.0;
+    k -= 54;
+    x *= two54;
+    u = dbits(x);
+    hx = (int32_t)(u >> 32);
+  }
+  if (hx >= 0x7ff00000) return x + x;
+  k += (hx >> 20) - 1023;
+  hx &= 0x000fffff;
+  int32_t i = (hx + 0x95f64) & 0x100000;
+  u = ((uint64_t)(uint32_t)(hx | (i ^ 0x3ff00000)) << 32) | (u & 0xffffffffull);
+  x = dfrom(u);
+  k += (i >> 20);
+  double f = x - 1.0;
+  double dk = (double)k;
+  if ((0x000fffff & (2 + hx)) < 3) {
+    if (f == 0.0) {
+      if (k == 0) return 0.0;
+      return dk * ln2_hi + dk * ln2_lo;
+    }
+    double R = f * f * (0.5 - 0.33333333333333333 * f);
+    if (k == 0) return f - R;
+    return dk * ln2_hi - ((R - dk * ln2_lo) - f);
+  }
+  double s = f / (2.0 + f);
+  double z = s * s;
+  i = hx - 0x6147a;
+  double w = z * z;
+  int32_t j = 0x6b851 - hx;
+  double t1 = w * (Lg2 + w * (Lg4 + w * Lg6));
+  double t2 = z * (Lg1 + w * (Lg3 + w * (Lg5 + w * Lg7)));
+  i |= j;
+  double R = t2 + t1;
+  if (i > 0) {
+    double hfsq = 0.5 * f * f;
+    if (k == 0) return f - (hfsq - s * (hfsq + R));
+    return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+  }
+  if (k == 0) return f - s * (f - R);
+  return dk * ln2_hi - ((s * (f - R) - dk * ln2_lo) - f);
+}
+
+__device__ __forceinline__ double ksin_d(double x)
+{
+  const double S1 = -1.66666666666666324348e-01;
+  const double S2 = 8.33333333332248946124e-03;
+  const double S3 = -1.98412698298579493134e-04;
+  const double S4 = 2.75573137070700676789e-06;
+  const double S5 = -2.50507602534068634195e-08;
+  const double S6 = 1.58969099521155010221e-10;
+  double z = x * x;
+  double v = z * x;
+  double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
+  return x + v * (S1 + z * r);
+}
+
+__device__ __forceinline__ double kcos_d(double x)
+{
+  const double C1 = 4.16666666666666019037e-02;
+  const double C2 = -1.38888888888741095749e-03;
+  const double C3 = 2.48015872894767294178e-05;
+  const double C4 = -2.75573143513906633035e-07;
+  const double C5 = 2.08757232129817482790e-09;
+  const double C6 = -1.13596475577881948265e-11;
+  int32_t ix = (int32_t)(dbits(x) >> 32) & 0x7fffffff;
+  double z = x * x;
+  double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
+  if (ix < 0x3FD33333) return 1.0 - (0.5 * z - z * r);
+  double qx;
+  if (ix > 0x3fe90000) qx = 0.28125;
+  else qx = dfrom((uint64_t)(uint32_t)(ix - 0x00200000) << 32);
+  double hz = 0.5 * z - qx;
+  double a = 1.0 - qx;
+  return a - (hz - z * r);
+}
+
+__device__ __forceinline__ void sincos_2pi_d(double xi, double *s, double *c)
+{
+  const double pio2 = 1.57079632679489661923;
+  double t = 4.0 * xi;
+  double q = (double)(int64_t)(t + 0.5);
+  double r = t - q;
+  double a = r * pio2;
+  double sa = ksin_d(a), ca = kcos_d(a);
+  int iq = ((int)q) & 3;
+  if (iq == 0) { *s = sa; *c = ca; }
+  else if (iq == 1) { *s = ca; *c = -sa; }
+  else if (iq == 2) { *s = -sa; *c = -ca; }
+  else { *s = -ca; *c = sa; }
+}
+
+__device__ __forceinline__ double cos_halfpi_d(double xi)
+{
+  double s, c;
+  sincos_2pi_d(0.25 * xi, &s, &c);
+  return c;
+}
+
+// ------------------------------------------------------------------ OpenMC 0.13 RNG
+// 64-bit LCG (M = 6364136223846793005, C = 1442695040888963407), PCG-RXS-M-XS
+// output permutation (O'Neill 2014), unit interval by 2^-64; skip-ahead by
+// Brown (1994).  Streams: 0 tracking, 1 source, 2 URR.
+
+constexpr uint64_t PRN_MULT = 6364136223846793005ull;
+constexpr uint64_t PRN_ADD = 1442695040888963407ull;
+constexpr uint64_t PRN_STRIDE = 152917ull;
+
+__device__ __forceinline__ double prn(uint64_t &s)
+{
+  s = PRN_MULT * s + PRN_ADD;
+  uint64_t word = ((s >> ((s >> 59u) + 5u)) ^ s) * 12605985483714917081ull;
+  uint64_t out = (word >> 43u) ^ word;
+  return (double)out * 0x1p-64;
+}
+
+__device__ __forceinline__ uint64_t future_seed(uint64_t n, uint64_t seed)
+{
+  uint64_t g = PRN_MULT, c = PRN_ADD, g_new = 1, c_new = 0;
+  while (n > 0) {
+    if (n & 1u) {
+      g_new *= g;
+      c_new = c_new * g + c;
+    }
+    c = (g + 1) * c;
+    g *= g;
+    n >>= 1;
+  }
+  return g_new * seed + c_new;
+}
+
+__device__ __forceinline__ uint64_t init_seed(int64_t id, int stream, uint64_t master)
+{
+  return future_seed((uint64_t)id * PRN_STRIDE, master + (uint64_t)stream);
+}
+
+// MaCaw stores seeds in Real ray aux data (CollisionKernel.C:132-134,157-159).
+__device__ __forceinline__ uint64_t seed_roundtrip(uint64_t s)
+{
+  double d = (double)s;
+  if (d >= 18446744073709551616.0) return 0;
+  return (uint64_t)d;
+}
+
+__device__ __forceinline__ uint64_t digest_step(uint64_t h, int64_t elem, int ev)
+{
+  h ^= (uint64_t)(elem * 8 + ev);
+  return h * 0x100000001b3ull;
+}
+
+// ------------------------------------------------------------------ cross sections
+
+struct Micro {
+  double t, a, s, f;
+};
+
+// Largest j with E_j <= E, clamped to [0, n-2], then lin-lin interpolation
+// (OpenMC Nuclide::calculate_xs on a pointwise grid).  lo/hi give an initial
+// bracket E_lo <= E < E_hi (0, n-1 when unknown).
+__device__ __forceinline__ void micro_xs(const XSView &X, int nuc, double E, Micro &m)
+{
+  const int64_t base = X.grid_off[nuc];
+  const int32_t np = X.n_points[nuc];
+  const double *__restrict__ Eg = X.energy + base;
+  int64_t j;
+  double f;
+  if (!(E > Eg[0])) {
+    j = 0;
+    f = 0.0;
+  } else if (E >= Eg[np - 1]) {
+    j = np - 2;
+    f = 1.0;
+  } else {
+    int64_t lo = 0, hi = np - 1;
+    while (hi - lo > 1) {
+      int64_t mid = lo + (hi - lo) / 2;
+      if (Eg[mid] <= E) lo = mid;
+      else hi = mid;
+    }
+    j = lo;
+    f = (E - Eg[j]) / (Eg[j + 1] - Eg[j]);
+  }
+  const double4 v0 = X.values[base + j];
+  const double4 v1 = X.values[base + j + 1];
+  double g = 1.0 - f;
+  m.t = g * v0.x + f * v1.x;
+  m.a = g * v0.y + f * v1.y;
+  m.s = g * v0.z + f * v1.z;
+  m.f = g * v0.w + f * v1.w;
+}
+
+__device__ __forceinline__ double macro_total(const XSView &X, int mat, double E)
+{
+  if (mat < 0) return 0.0;
+  double t = 0.0;
+  const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
+  for (int32_t q = b; q < e; ++q) {
+    Micro u;
+    micro_xs(X, X.mat_nuc[q], E, u);
+    t = t + X.mat_dens[q] * u.t;
+  }
+  return t;
+}
+
+struct Macro {
+  double t, a, s, f;
+};
+
+__device__ __forceinline__ void macro_all(const XSView &X, int mat, double E, Macro &M)
+{
+  M.t = M.a = M.s = M.f = 0.0;
+  if (mat < 0) return;
+  const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
+  for (int32_t q = b; q < e; ++q) {
+    Micro u;
+    micro_xs(X, X.mat_nuc[q], E, u);
+    double dn = X.mat_dens[q];
+    M.t = M.t + dn * u.t;
+    M.a = M.a + dn * u.a;
+    M.s = M.s + dn * u.s;
+    M.f = M.f + dn * u.f;
+  }
+}
+
+// ------------------------------------------------------------------ geometry
+
+// Exit side of element `e` for a ray at p along u: smallest non-negative
+// distance over sides whose outward normal has n.u > 0, the incoming side
+// excluded, ties to the lowest side index.  Returns -1 when no side qualifies.
+template <int NS>
+__device__ __forceinline__ int exit_side(const double4 *__restrict__ pl, double x, double y, double z,
+                                         double u, double v, double w, int inface, double &L)
+{
+  L = __builtin_inf();
+  int fx = -1;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double4 q = pl[s];
+    double den = (q.x * u + q.y * v) + q.z * w;
+    bool ok = (s != inface) && (den > 0.0);
+    double num = q.w - ((q.x * x + q.y * y) + q.z * z);
+    double t = num / den;
+    if (!(t > 0.0)) t = 0.0;
+    if (ok && t < L) {
+      L = t;
+      fx = s;
+    }
+  }
+  return fx;
+}
+
+__device__ __forceinline__ int exit_side_dyn(const double4 *__restrict__ pl, int ns, double x, double y,
+                                             double z, double u, double v, double w, int inface,
+                                             double &L)
+{
+  L = __builtin_inf();
+  int fx = -1;
+  for (int s = 0; s < ns; ++s) {
+    const double4 q = pl[s];
+    double den = (q.x * u + q.y * v) + q.z * w;
+    if (s == inface || !(den > 0.0)) continue;
+    double num = q.w - ((q.x * x + q.y * y) + q.z * z);
+    double t = num / den;
+    if (!(t > 0.0)) t = 0.0;
+    if (t < L) {
+      L = t;
+      fx = s;
+    }
+  }
+  return fx;
+}
+
+// OpenMC rotate_angle with phi = 2 pi xi.
+__device__ __forceinline__ void rotate_angle(double &u, double &v, double &w, double mu, double xi)
+{
+  double sinphi, cosphi;
+  sincos_2pi_d(xi, &sinphi, &cosphi);
+  double u0 = u, v0 = v, w0 = w;
+  double a = sqrt(fmax(0.0, 1.0 - mu * mu));
+  double b = sqrt(fmax(0.0, 1.0 - w0 * w0));
+  if (b > 1e-10) {
+    u = mu * u0 + a * (u0 * w0 * cosphi - v0 * sinphi) / b;
+    v = mu * v0 + a * (v0 * w0 * cosphi + u0 * sinphi) / b;
+    w = mu * w0 - a * b * cosphi;
+  } else {
+    b = sqrt(1.0 - v0 * v0);
+    u = mu * u0 + a * (u0 * v0 * cosphi + w0 * sinphi) / b;
+    v = mu * v0 - a * b * cosphi;
+    w = mu * w0 + a * (v0 * w0 * cosphi - u0 * sinphi) / b;
+  }
+}
+
+// ------------------------------------------------------------------ tallies
+
+__device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X, int64_t elem, double E,
+                                            double wfl, int mat)
+{
+  int64_t fi;
+  if (T.elem_base) {
+    fi = T.elem_base[elem];
+    if (fi < 0) return;
+  } else {
+    fi = elem;
+  }
+  if (T.n_edges > 0) {
+    const double *ed = T.edges;
+    const int ne = T.n_edges;
+    if (E < ed[0] || E > ed[ne - 1]) return;
+    int lo = 0, hi = ne - 1; // ed[lo] <= E, bin = largest lo <= ne-2
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (ed[mid] <= E) lo = mid;
+      else hi = mid;
+    }
+    fi += (int64_t)lo * T.energy_stride;
+  }
+  double *row = T.value + fi * T.n_score_bins;
+  if (T.simple_flux) {
+    unsafeAtomicAdd(row, wfl);
+    return;
+  }
+  const int nsc = T.n_scores;
+  for (int ni = 0; ni < T.n_nuclides; ++ni) {
+    const int nuc = T.nuclides[ni];
+    Macro M = {0.0, 0.0, 0.0, 0.0};
+    if (T.needs_xs) {
+      if (nuc < 0) {
+        macro_all(X, mat, E, M);
+      } else if (mat >= 0) {
+        for (int32_t q = X.mat_off[mat]; q < X.mat_off[mat + 1]; ++q)
+          if (X.mat_nuc[q] == nuc) {
+            Micro mu;
+            micro_xs(X, nuc, E, mu);
+            double dn = X.mat_dens[q];
+            M.t = dn * mu.t;
+            M.a = dn * mu.a;
+            M.s = dn * mu.s;
+            M.f = dn * mu.f;
+          }
+      }
+    }
+    for (int si = 0; si < nsc; ++si) {
+      double m;
+      switch (T.scores[si]) {
+        case 0: m = 1.0; break;
+        case 1: m = M.t; break;
+        case 2: m = M.a; break;
+        case 3: m = M.s; break;
+        default: m = M.f; break;
+      }
+      unsafeAtomicAdd(row + ni * nsc + si, wfl * m);
+    }
+  }
+}
+
+} // namespace mcw

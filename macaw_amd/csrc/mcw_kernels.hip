@@ -1,0 +1,678 @@
+// mcw_kernels.hip -- gfx950 transport kernels.
+//
+// Two schedules over the same per-history arithmetic:
+//
+//  PLAIN (event-based, host-driven): k_source fills a structure-of-arrays
+//  bank; each cycle runs k_segment over the live queue (one segment per
+//  particle: xs if the material or energy changed, exit side, distance draw,
+//  track-length tally, cross / reflect / leak) which appends colliding
+//  particles to a collision queue and the others to the next live queue, then
+//  k_collide (sample nuclide, absorb or scatter, collision tally) appends the
+//  survivors.  Queues are compacted with wavefront ballots: one atomic per
+//  wavefront per append.
+//
+//  BATCHED (persistent, fused): k_persistent keeps each particle in
+//  registers; every loop trip performs one segment (and its collision) for
+//  every lane, and a lane whose history ended takes the next history index
+//  from a wavefront-aggregated work counter and samples its source inline.
+//
+// Both reproduce, per history, the FP64 CPU reference to the bit.
+#include <hip/hip_runtime.h>
+
+#include "mcw_device.h"
+#include "mcw_types.h"
+
+namespace mcw {
+
+enum { EV_CROSS = 0, EV_REFLECT = 1, EV_LEAK = 2, EV_SCATTER = 3, EV_ABSORB = 4, EV_CUTOFF = 5,
+       EV_LOST = 6, EV_COLLIDE = 7 };
+enum { FATE_ALIVE = 0, FATE_LEAKED = 1, FATE_ABSORBED = 2, FATE_CUTOFF = 3, FATE_LOST = 4 };
+
+struct PState {
+  double x, y, z, u, v, w, E, sigt;
+  uint64_t seed;
+  int32_t elem, inface, mat;
+  int64_t hid;
+  uint64_t h;
+  int32_t nseg, ncoll;
+};
+
+struct LocalCount {
+  unsigned long long seg, coll, absn, leak, cut, lost, refl;
+  double track;
+  __device__ void zero() { seg = coll = absn = leak = cut = lost = refl = 0; track = 0.0; }
+};
+
+// ------------------------------------------------------------------ wave helpers
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += (unsigned long long)__shfl_xor((long long)v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ void flush_counts(Counters *c, LocalCount &l)
+{
+  unsigned long long s0 = wave_sum(l.seg), s1 = wave_sum(l.coll), s2 = wave_sum(l.absn),
+                     s3 = wave_sum(l.leak), s4 = wave_sum(l.cut), s5 = wave_sum(l.lost),
+                     s6 = wave_sum(l.refl);
+  double tr = wave_sum_d(l.track);
+  if (__lane_id() == 0) {
+    if (s0) atomicAdd(&c->segments, s0);
+    if (s1) atomicAdd(&c->collisions, s1);
+    if (s2) atomicAdd(&c->absorptions, s2);
+    if (s3) atomicAdd(&c->leaks, s3);
+    if (s4) atomicAdd(&c->cutoffs, s4);
+    if (s5) atomicAdd(&c->lost, s5);
+    if (s6) atomicAdd(&c->reflections, s6);
+    if (tr != 0.0) unsafeAtomicAdd(&c->track, tr);
+  }
+}
+
+// Wavefront-aggregated append: every lane of the wavefront must call it.
+__device__ __forceinline__ int64_t wave_append(unsigned long long *count, bool want)
+{
+  const unsigned long long mask = __ballot(want);
+  if (mask == 0) return -1;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
+  base = (unsigned long long)__shfl((long long)base, leader, 64);
+  const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+  return want ? (int64_t)(base + __popcll(below)) : -1;
+}
+
+// ------------------------------------------------------------------ source
+
+__device__ void sample_point_in_tet(const double *p0, const double *p1, const double *p2, const double *p3,
+                                    uint64_t &seed, double *out)
+{
+  // Rocchini & Cignoni, Journal of Graphics Tools 5(4), 2000.
+  double s = prn(seed), t = prn(seed), u = prn(seed);
+  if (s + t > 1.0) {
+    s = 1.0 - s;
+    t = 1.0 - t;
+  }
+  if (t + u > 1.0) {
+    double tmp = u;
+    u = 1.0 - s - t;
+    t = 1.0 - tmp;
+  } else if (s + t + u > 1.0) {
+    double tmp = u;
+    u = s + t + u - 1.0;
+    s = 1.0 - t - tmp;
+  }
+  double a = 1.0 - s - t - u;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) out[k] = ((a * p0[k] + s * p1[k]) + t * p2[k]) + u * p3[k];
+}
+
+__constant__ int8_t c_kuhn[6][4] = {{0, 1, 2, 6}, {0, 2, 3, 6}, {0, 3, 7, 6},
+                                    {0, 7, 4, 6}, {0, 4, 5, 6}, {0, 5, 1, 6}};
+
+__device__ double tet_volume6_d(const double *a, const double *b, const double *c, const double *d)
+{
+  double ax = b[0] - a[0], ay = b[1] - a[1], az = b[2] - a[2];
+  double bx = c[0] - a[0], by = c[1] - a[1], bz = c[2] - a[2];
+  double cx = d[0] - a[0], cy = d[1] - a[1], cz = d[2] - a[2];
+  double det = (ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx)) + az * (bx * cy - by * cx);
+  return fabs(det);
+}
+
+// OpenMC fixed-source initialize_history: seeds for particle id
+// (batch-1)*n + i + 1, then space, angle, energy sampled on stream 1.
+__device__ void source_init(const Params &P, int64_t hid, PState &st)
+{
+  const int64_t id = (P.batch - 1) * P.n_particles + hid + 1;
+  st.seed = init_seed(id, 0, P.master_seed);
+  uint64_t ss = init_seed(id, 1, P.master_seed);
+  const SourceView &S = P.src;
+  const MeshView &M = P.mesh;
+  double p[3];
+  int64_t elem;
+  if (S.space == 0) {
+    p[0] = S.point[0];
+    p[1] = S.point[1];
+    p[2] = S.point[2];
+    elem = S.point_elem;
+  } else {
+    const int64_t ne = M.n_elem;
+    double target = prn(ss) * M.cdf[ne - 1];
+    int64_t lo = 0, hi = ne; // first e with cdf[e] > target
+    while (lo < hi) {
+      int64_t mid = lo + ((hi - lo) >> 1);
+      if (M.cdf[mid] > target) hi = mid;
+      else lo = mid + 1;
+    }
+    elem = lo < ne ? lo : ne - 1;
+    const int32_t *c = M.conn + M.conn_off[elem];
+    const double *X = M.xyz;
+    if (M.type[elem] == 4) {
+      sample_point_in_tet(X + 3 * (int64_t)c[0], X + 3 * (int64_t)c[1], X + 3 * (int64_t)c[2],
+                          X + 3 * (int64_t)c[3], ss, p);
+    } else {
+      double vols[6], tot = 0.0;
+      for (int k = 0; k < 6; ++k) {
+        tot = tot + tet_volume6_d(X + 3 * (int64_t)c[c_kuhn[k][0]], X + 3 * (int64_t)c[c_kuhn[k][1]],
+                                  X + 3 * (int64_t)c[c_kuhn[k][2]], X + 3 * (int64_t)c[c_kuhn[k][3]]);
+        vols[k] = tot;
+      }
+      double tt = prn(ss) * tot;
+      int k = 0;
+      while (k < 5 && !(vols[k] > tt)) ++k;
+      sample_point_in_tet(X + 3 * (int64_t)c[c_kuhn[k][0]], X + 3 * (int64_t)c[c_kuhn[k][1]],
+                          X + 3 * (int64_t)c[c_kuhn[k][2]], X + 3 * (int64_t)c[c_kuhn[k][3]], ss, p);
+    }
+  }
+  st.x = p[0];
+  st.y = p[1];
+  st.z = p[2];
+  {
+    double xi_phi = prn(ss);
+    double mu = 2.0 * prn(ss) - 1.0;
+    double s, c;
+    sincos_2pi_d(xi_phi, &s, &c);
+    double r = sqrt(1.0 - mu * mu);
+    st.u = mu;
+    st.v = r * c;
+    st.w = r * s;
+  }
+  if (S.energy == 0) {
+    st.E = S.E0;
+  } else {
+    const double a = S.watt_a, b = S.watt_b;
+    for (int it = 0;; ++it) {
+      double r1 = prn(ss), r2 = prn(ss), r3 = prn(ss);
+      double c = cos_halfpi_d(r3);
+      double wmax = -a * (log_d(r1) + log_d(r2) * c * c);
+      double E = wmax + 0.25 * a * a * b + (-1.0 + 2.0 * prn(ss)) * sqrt(a * a * b * wmax);
+      if (E < P.xs.e_max || it >= 1000) {
+        st.E = E < P.xs.e_max ? E : P.xs.e_min;
+        break;
+      }
+    }
+  }
+  if (P.roundtrip) st.seed = seed_roundtrip(st.seed);
+  st.elem = (int32_t)elem;
+  st.inface = -1;
+  st.mat = -2;
+  st.sigt = 0.0;
+  st.hid = hid;
+  st.h = 0xcbf29ce484222325ull;
+  st.nseg = 0;
+  st.ncoll = 0;
+}
+
+// ------------------------------------------------------------------ one segment
+
+__device__ __forceinline__ int64_t orig_elem(const MeshView &M, int32_t e)
+{
+  return M.orig_id ? M.orig_id[e] : (int64_t)e;
+}
+
+// Geometry and distance draw of one segment.  Returns the event that ends it;
+// EV_COLLIDE leaves the particle at the collision point for collide().
+template <int NS>
+__device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &lc)
+{
+  const MeshView &M = P.mesh;
+  if (st.nseg >= P.max_seg) {
+    if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
+    ++lc.lost;
+    return EV_LOST;
+  }
+  const int32_t mat = M.elem_mat[st.elem];
+  if (mat != st.mat) {
+    st.sigt = macro_total(P.xs, mat, st.E);
+    st.mat = mat;
+  }
+  double L;
+  int fx;
+  int64_t slot0;
+  if (NS > 0) {
+    slot0 = (int64_t)st.elem * NS;
+    fx = exit_side<NS>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+  } else {
+    slot0 = M.face_off[st.elem];
+    fx = exit_side_dyn(M.plane + slot0, M.n_sides[st.elem], st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+  }
+  if (fx < 0) {
+    if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
+    ++lc.lost;
+    return EV_LOST;
+  }
+  const double xi = prn(st.seed);
+  const double d = st.sigt > 0.0 ? -log_d(xi) / st.sigt : __builtin_inf();
+  const double track = (d < L) ? d : L;
+  ++st.nseg;
+  ++lc.seg;
+  lc.track += track;
+  for (int k = 0; k < P.n_tl; ++k) tally_score(P.tally[k], P.xs, st.elem, st.E, track, mat);
+  if (d > L) {
+    st.x = st.x + L * st.u;
+    st.y = st.y + L * st.v;
+    st.z = st.z + L * st.w;
+    const int32_t nb = M.nbr[slot0 + fx];
+    int ev;
+    if (nb >= 0) {
+      ev = EV_CROSS;
+      if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), ev);
+      st.elem = nb >> 3;
+      st.inface = nb & 7;
+    } else {
+      const int32_t bid = -1 - nb;
+      const int kind = bid < M.n_bc ? M.bc_kind[bid] : 0;
+      if (kind == 1) {
+        ev = EV_REFLECT;
+        if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), ev);
+        const double4 q = M.plane[slot0 + fx];
+        double dot = (st.u * q.x + st.v * q.y) + st.w * q.z;
+        st.u = st.u - 2.0 * dot * q.x;
+        st.v = st.v - 2.0 * dot * q.y;
+        st.w = st.w - 2.0 * dot * q.z;
+        st.inface = fx;
+        ++lc.refl;
+      } else {
+        ev = EV_LEAK;
+        if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), ev);
+        ++lc.leak;
+      }
+    }
+    if (P.roundtrip) st.seed = seed_roundtrip(st.seed);
+    return ev;
+  }
+  // collision: back-step from the exit point (CollisionKernel.C:172-175)
+  const double back = L - d;
+  st.x = (st.x + L * st.u) - back * st.u;
+  st.y = (st.y + L * st.v) - back * st.v;
+  st.z = (st.z + L * st.w) - back * st.w;
+  return EV_COLLIDE;
+}
+
+// Collision at the current point: collision-estimator tallies, sample the
+// nuclide (OpenMC sample_nuclide), absorb (analog) or scatter elastically
+// (isotropic in CM, target at rest) with an energy cutoff at the grid minimum.
+__device__ __forceinline__ int collide(const Params &P, PState &st, LocalCount &lc)
+{
+  const XSView &X = P.xs;
+  const int32_t mat = st.mat;
+  ++st.ncoll;
+  ++lc.coll;
+  for (int k = P.n_tl; k < P.n_tl + P.n_coll; ++k) tally_score(P.tally[k], X, st.elem, st.E, 1.0 / st.sigt, mat);
+  const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
+  const double cutoff = prn(st.seed) * st.sigt;
+  double prob = 0.0;
+  int32_t pick = e - 1;
+  Micro mu;
+  for (int32_t q = b; q < e; ++q) {
+    micro_xs(X, X.mat_nuc[q], st.E, mu);
+    prob = prob + X.mat_dens[q] * mu.t;
+    if (prob >= cutoff) {
+      pick = q;
+      break;
+    }
+  }
+  const int32_t nuc = X.mat_nuc[pick];
+  micro_xs(X, nuc, st.E, mu);
+  int ev;
+  if (mu.a > prn(st.seed) * mu.t) {
+    ev = EV_ABSORB;
+    ++lc.absn;
+  } else {
+    const double A = X.awr[nuc];
+    const double mu_cm = 2.0 * prn(st.seed) - 1.0;
+    const double q2 = (A * A + 2.0 * A * mu_cm) + 1.0;
+    const double Ep = st.E * q2 / ((A + 1.0) * (A + 1.0));
+    if (Ep < X.e_min) {
+      ev = EV_CUTOFF;
+      st.E = Ep;
+      ++lc.cut;
+    } else {
+      const double mu_lab = (1.0 + A * mu_cm) / sqrt(q2);
+      const double xi_phi = prn(st.seed);
+      rotate_angle(st.u, st.v, st.w, mu_lab, xi_phi);
+      st.E = Ep;
+      st.mat = -2; // energy changed: recompute Sigma_t
+      ev = EV_SCATTER;
+    }
+  }
+  if (P.record) st.h = digest_step(st.h, orig_elem(P.mesh, st.elem), ev);
+  st.inface = -1;
+  if (P.roundtrip) st.seed = seed_roundtrip(st.seed);
+  return ev;
+}
+
+__device__ __forceinline__ int fate_of(int ev)
+{
+  switch (ev) {
+    case EV_LEAK: return FATE_LEAKED;
+    case EV_ABSORB: return FATE_ABSORBED;
+    case EV_CUTOFF: return FATE_CUTOFF;
+    case EV_LOST: return FATE_LOST;
+    default: return FATE_ALIVE;
+  }
+}
+
+__device__ void write_record(const Params &P, const PState &st, int fate)
+{
+  const int64_t i = st.hid;
+  const RecordView &R = P.rec;
+  R.digest[i] = st.h;
+  R.seed[i] = st.seed;
+  R.E[i] = st.E;
+  R.x[i] = st.x;
+  R.y[i] = st.y;
+  R.z[i] = st.z;
+  R.nseg[i] = st.nseg;
+  R.ncoll[i] = st.ncoll;
+  R.elem[i] = (int32_t)orig_elem(P.mesh, st.elem);
+  R.fate[i] = fate;
+}
+
+// ------------------------------------------------------------------ bank I/O (plain)
+
+__device__ __forceinline__ void load(const BankView &B, int32_t s, PState &st, bool record)
+{
+  st.x = B.x[s]; st.y = B.y[s]; st.z = B.z[s];
+  st.u = B.u[s]; st.v = B.v[s]; st.w = B.w[s];
+  st.E = B.E[s]; st.sigt = B.sigt[s];
+  st.seed = B.seed[s];
+  st.elem = B.elem[s]; st.inface = B.inface[s]; st.mat = B.mat[s];
+  st.hid = B.hid[s];
+  if (record) {
+    st.h = B.digest[s];
+    st.nseg = B.nseg[s];
+    st.ncoll = B.ncoll[s];
+  } else {
+    st.nseg = B.nseg[s];
+    st.ncoll = 0;
+  }
+}
+
+__device__ __forceinline__ void store(const BankView &B, int32_t s, const PState &st, bool record)
+{
+  B.x[s] = st.x; B.y[s] = st.y; B.z[s] = st.z;
+  B.u[s] = st.u; B.v[s] = st.v; B.w[s] = st.w;
+  B.E[s] = st.E; B.sigt[s] = st.sigt;
+  B.seed[s] = st.seed;
+  B.elem[s] = st.elem; B.inface[s] = st.inface; B.mat[s] = st.mat;
+  B.nseg[s] = st.nseg;
+  if (record) {
+    B.digest[s] = st.h;
+    B.ncoll[s] = st.ncoll;
+  }
+}
+
+} // namespace mcw
+
+using namespace mcw;
+
+// ------------------------------------------------------------------ PLAIN kernels
+
+// Bank slot s <- history hist_begin + s; live queue = all slots.
+extern "C" __global__ void __launch_bounds__(256) k_source(Params P, BankView B, int32_t *live, int64_t n)
+{
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) P.cnt->q_count[0] = (unsigned long long)n;
+  if (s >= n) return;
+  PState st;
+  source_init(P, P.hist_begin + s, st);
+  store(B, (int32_t)s, st, P.record != 0);
+  B.hid[s] = st.hid;
+  live[s] = (int32_t)s;
+}
+
+// One segment for every particle of the live queue.
+template <int NS>
+__global__ void __launch_bounds__(256) k_segment(Params P, BankView B, const int32_t *live_in, int qin,
+                                                 int32_t *live_out, int qout, int32_t *coll_q, int qcoll)
+{
+  Counters *C = P.cnt;
+  const int64_t n = (int64_t)C->q_count[qin];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  LocalCount lc;
+  lc.zero();
+  const bool record = P.record != 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    int ev = -1;
+    int32_t s = 0;
+    PState st;
+    if (valid) {
+      s = live_in[i];
+      load(B, s, st, record);
+      ev = segment<NS>(P, st, lc);
+      if (ev != EV_LOST && ev != EV_LEAK) store(B, s, st, record);
+      else {
+        B.nseg[s] = st.nseg;
+        if (record) write_record(P, st, fate_of(ev));
+      }
+    }
+    const bool to_coll = valid && ev == EV_COLLIDE;
+    const bool to_live = valid && (ev == EV_CROSS || ev == EV_REFLECT);
+    int64_t pc = wave_append(&C->q_count[qcoll], to_coll);
+    if (to_coll) coll_q[pc] = s;
+    int64_t pl = wave_append(&C->q_count[qout], to_live);
+    if (to_live) live_out[pl] = s;
+  }
+  flush_counts(C, lc);
+}
+
+extern "C" __global__ void __launch_bounds__(256) k_collide(Params P, BankView B, const int32_t *coll_q, int qcoll,
+                                                           int32_t *live_out, int qout)
+{
+  Counters *C = P.cnt;
+  const int64_t n = (int64_t)C->q_count[qcoll];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  LocalCount lc;
+  lc.zero();
+  const bool record = P.record != 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const bool valid = i < n;
+    int ev = -1;
+    int32_t s = 0;
+    if (valid) {
+      s = coll_q[i];
+      PState st;
+      load(B, s, st, record);
+      ev = collide(P, st, lc);
+      if (ev == EV_SCATTER) store(B, s, st, record);
+      else if (record) write_record(P, st, fate_of(ev));
+    }
+    const bool to_live = valid && ev == EV_SCATTER;
+    int64_t pl = wave_append(&C->q_count[qout], to_live);
+    if (to_live) live_out[pl] = s;
+  }
+  flush_counts(C, lc);
+}
+
+extern "C" __global__ void k_reset_work(Counters *C)
+{
+  if (threadIdx.x == 0) C->next_hist = 0ull;
+}
+
+extern "C" __global__ void k_reset_queue(Counters *C, int q0, int q1)
+{
+  if (threadIdx.x == 0) {
+    if (q0 >= 0) C->q_count[q0] = 0;
+    if (q1 >= 0) C->q_count[q1] = 0;
+  }
+}
+
+// ------------------------------------------------------------------ BATCHED kernel
+
+template <int NS>
+__global__ void __launch_bounds__(256) k_persistent(Params P)
+{
+  Counters *C = P.cnt;
+  LocalCount lc;
+  lc.zero();
+  PState st;
+  bool active = false;
+  bool exhausted = false;
+  const bool record = P.record != 0;
+  for (;;) {
+    // refill idle lanes from the work counter (one atomic per wavefront)
+    const bool need = !active && !exhausted;
+    int64_t slot = wave_append(&C->next_hist, need);
+    if (need) {
+      const int64_t hid = P.hist_begin + slot;
+      if (hid < P.hist_end) {
+        source_init(P, hid, st);
+        active = true;
+      } else {
+        exhausted = true;
+      }
+    }
+    if (__ballot(active) == 0) break;
+    if (active) {
+      int ev = segment<NS>(P, st, lc);
+      if (ev == EV_COLLIDE) ev = collide(P, st, lc);
+      if (ev == EV_LEAK || ev == EV_LOST || ev == EV_ABSORB || ev == EV_CUTOFF) {
+        if (record) write_record(P, st, fate_of(ev));
+        active = false;
+      }
+    }
+  }
+  flush_counts(C, lc);
+}
+
+// ------------------------------------------------------------------ utilities
+
+extern "C" __global__ void k_fill(double *p, int64_t n, double v)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+extern "C" __global__ void k_zero_counters(Counters *C)
+{
+  unsigned long long *p = (unsigned long long *)C;
+  const int n = sizeof(Counters) / 8;
+  if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// VALUE -> SUM += VALUE/norm, SUM_SQ += (VALUE/norm)^2, VALUE = 0
+extern "C" __global__ void k_accumulate(double *value, double *sum, double *sum_sq, int64_t n, double norm)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double v = value[i] / norm;
+    sum[i] += v;
+    sum_sq[i] += v * v;
+  }
+}
+
+// Elementary functions and RNG on the device, for bit-equality tests.
+extern "C" __global__ void k_math_probe(const double *x, int64_t n, double *log_out, double *sin_out,
+                                        double *cos_out, double *sqrt_out, const uint64_t *seeds,
+                                        double *prn_out, uint64_t *seed_out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  log_out[i] = log_d(x[i]);
+  double s, c;
+  sincos_2pi_d(x[i] - floor(x[i]), &s, &c);
+  sin_out[i] = s;
+  cos_out[i] = c;
+  sqrt_out[i] = sqrt(x[i]);
+  uint64_t sd = seeds[i];
+  prn_out[i] = prn(sd);
+  seed_out[i] = future_seed((uint64_t)i * PRN_STRIDE + 7, seeds[i]);
+}
+
+// explicit instantiations, launched from the host runtime
+template __global__ void k_segment<4>(Params, BankView, const int32_t *, int, int32_t *, int, int32_t *, int);
+template __global__ void k_segment<6>(Params, BankView, const int32_t *, int, int32_t *, int, int32_t *, int);
+template __global__ void k_segment<0>(Params, BankView, const int32_t *, int, int32_t *, int, int32_t *, int);
+template __global__ void k_persistent<4>(Params);
+template __global__ void k_persistent<6>(Params);
+template __global__ void k_persistent<0>(Params);
+
+namespace mcw {
+// Host-side launch helpers (defined here so the templates live in one TU).
+hipError_t launch_segment(int ns, dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B,
+                          const int32_t *live_in, int qin, int32_t *live_out, int qout, int32_t *coll_q, int qcoll)
+{
+  if (ns == 4) hipLaunchKernelGGL(k_segment<4>, g, b, 0, s, P, B, live_in, qin, live_out, qout, coll_q, qcoll);
+  else if (ns == 6) hipLaunchKernelGGL(k_segment<6>, g, b, 0, s, P, B, live_in, qin, live_out, qout, coll_q, qcoll);
+  else hipLaunchKernelGGL(k_segment<0>, g, b, 0, s, P, B, live_in, qin, live_out, qout, coll_q, qcoll);
+  return hipGetLastError();
+}
+
+hipError_t launch_persistent(int ns, dim3 g, dim3 b, hipStream_t s, const Params &P)
+{
+  if (ns == 4) hipLaunchKernelGGL(k_persistent<4>, g, b, 0, s, P);
+  else if (ns == 6) hipLaunchKernelGGL(k_persistent<6>, g, b, 0, s, P);
+  else hipLaunchKernelGGL(k_persistent<0>, g, b, 0, s, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_source(dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B, int32_t *live, int64_t n)
+{
+  hipLaunchKernelGGL(k_source, g, b, 0, s, P, B, live, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_collide(dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B, const int32_t *coll_q,
+                          int qcoll, int32_t *live_out, int qout)
+{
+  hipLaunchKernelGGL(k_collide, g, b, 0, s, P, B, coll_q, qcoll, live_out, qout);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1)
+{
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, s, C, q0, q1);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_work(hipStream_t s, Counters *C)
+{
+  hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(64), 0, s, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t s, double *p, int64_t n, double v)
+{
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_counters(hipStream_t s, Counters *C)
+{
+  hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_accumulate(hipStream_t s, double *value, double *sum, double *sum_sq, int64_t n, double norm)
+{
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_accumulate, dim3((unsigned)blocks), dim3(256), 0, s, value, sum, sum_sq, n, norm);
+  return hipGetLastError();
+}
+
+hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *lo, double *so, double *co,
+                             double *sq, const uint64_t *seeds, double *po, uint64_t *seo)
+{
+  int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_math_probe, dim3((unsigned)blocks), dim3(256), 0, s, x, n, lo, so, co, sq, seeds, po, seo);
+  return hipGetLastError();
+}
+} // namespace mcw

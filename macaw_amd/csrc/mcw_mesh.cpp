@@ -1,0 +1,246 @@
+// mcw_mesh.cpp -- host-side derivation of the device mesh layout.
+//
+// From libMesh-ordered connectivity (the mesh OpenMCStudy maps to OpenMC
+// cells, src/userobjects/OpenMCStudy.C:152-187) this builds what the
+// element walk needs on the device:
+//   - one outward unit plane per element side.  A side shared by two
+//     elements gets one canonical plane (through its three lowest-numbered
+//     nodes) used with opposite signs on the two sides, so a point that
+//     leaves one element through the plane is exactly on the neighbour's;
+//   - face neighbours by matching sorted side-node tuples;
+//   - boundary ids of external sides;
+//   - the element-volume CDF of volumetric sources (sequential sum).
+// Planar sides are required (warn_non_planar in the reference's inputs,
+// examples/beavrs_core/run_3d_core_cycle1.i:102-104): a HEX8 side whose
+// fourth node lies off the plane beyond a relative 1e-9 is rejected.
+#include "mcw_mesh.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <thread>
+
+namespace mcw {
+
+static const int TET_SIDE[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}};
+static const int HEX_SIDE[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                                   {2, 3, 7, 6}, {3, 0, 4, 7}, {4, 5, 6, 7}};
+const int HEX_KUHN_TETS[6][4] = {{0, 1, 2, 6}, {0, 2, 3, 6}, {0, 3, 7, 6},
+                                 {0, 7, 4, 6}, {0, 4, 5, 6}, {0, 5, 1, 6}};
+
+static inline int sides_of(uint8_t t) { return t == 4 ? 4 : 6; }
+
+static void side_nodes(const HostMesh &m, int64_t e, int s, int32_t *out, int &nn)
+{
+  const int32_t *c = m.conn.data() + m.conn_off[e];
+  if (m.type[e] == 4) {
+    nn = 3;
+    for (int k = 0; k < 3; ++k) out[k] = c[TET_SIDE[s][k]];
+  } else {
+    nn = 4;
+    for (int k = 0; k < 4; ++k) out[k] = c[HEX_SIDE[s][k]];
+  }
+}
+
+double tet_volume6(const double *a, const double *b, const double *c, const double *d)
+{
+  double ax = b[0] - a[0], ay = b[1] - a[1], az = b[2] - a[2];
+  double bx = c[0] - a[0], by = c[1] - a[1], bz = c[2] - a[2];
+  double cx = d[0] - a[0], cy = d[1] - a[1], cz = d[2] - a[2];
+  double det = (ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx)) + az * (bx * cy - by * cx);
+  return std::fabs(det);
+}
+
+static double volume6(const HostMesh &m, int64_t e)
+{
+  const int32_t *c = m.conn.data() + m.conn_off[e];
+  const double *X = m.xyz.data();
+  if (m.type[e] == 4)
+    return tet_volume6(X + 3 * (int64_t)c[0], X + 3 * (int64_t)c[1], X + 3 * (int64_t)c[2], X + 3 * (int64_t)c[3]);
+  double v = 0.0;
+  for (int k = 0; k < 6; ++k)
+    v = v + tet_volume6(X + 3 * (int64_t)c[HEX_KUHN_TETS[k][0]], X + 3 * (int64_t)c[HEX_KUHN_TETS[k][1]],
+                        X + 3 * (int64_t)c[HEX_KUHN_TETS[k][2]], X + 3 * (int64_t)c[HEX_KUHN_TETS[k][3]]);
+  return v;
+}
+
+std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
+{
+  const int64_t ne = m.n_elem;
+  m.face_off.assign(ne + 1, 0);
+  m.uniform_sides = 0;
+  bool all_tet = true, all_hex = true;
+  for (int64_t e = 0; e < ne; ++e) {
+    uint8_t t = m.type[e];
+    if (t != 4 && t != 8) return "unsupported element type " + std::to_string((int)t) + " at element " + std::to_string(e);
+    if (m.conn_off[e + 1] - m.conn_off[e] != t) return "connectivity size does not match element type at element " + std::to_string(e);
+    all_tet &= t == 4;
+    all_hex &= t == 8;
+    m.face_off[e + 1] = m.face_off[e] + sides_of(t);
+  }
+  if (all_tet) m.uniform_sides = 4;
+  if (all_hex) m.uniform_sides = 6;
+  const int64_t nf = m.face_off[ne];
+  if (nf >= ((int64_t)1 << 31)) return "mesh too large: more than 2^31 element sides";
+  if (ne >= ((int64_t)1 << 28)) return "mesh too large: more than 2^28 elements";
+  for (int64_t i = 0; i < m.conn_off[ne]; ++i)
+    if (m.conn[i] < 0 || m.conn[i] >= m.n_nodes) return "connectivity refers to a node out of range";
+
+  // -- neighbours: sort side keys
+  struct Key {
+    int32_t n[4];
+    int32_t slot;
+  };
+  std::vector<Key> keys(nf);
+  std::vector<int32_t> slot_elem(nf);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int s = 0; s < sides_of(m.type[e]); ++s) {
+      int32_t fn[4];
+      int nn;
+      side_nodes(m, e, s, fn, nn);
+      std::sort(fn, fn + nn);
+      Key &k = keys[m.face_off[e] + s];
+      for (int q = 0; q < 4; ++q) k.n[q] = q < nn ? fn[q] : -1;
+      k.slot = (int32_t)(m.face_off[e] + s);
+      slot_elem[m.face_off[e] + s] = (int32_t)e;
+    }
+  std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
+    if (a.n[0] != b.n[0]) return a.n[0] < b.n[0];
+    if (a.n[1] != b.n[1]) return a.n[1] < b.n[1];
+    if (a.n[2] != b.n[2]) return a.n[2] < b.n[2];
+    if (a.n[3] != b.n[3]) return a.n[3] < b.n[3];
+    return a.slot < b.slot;
+  });
+  m.nbr.assign(nf, 0);
+  m.boundary_ids.clear();
+  for (size_t i = 0; i < keys.size();) {
+    size_t j = i + 1;
+    while (j < keys.size() && keys[j].n[0] == keys[i].n[0] && keys[j].n[1] == keys[i].n[1] &&
+           keys[j].n[2] == keys[i].n[2] && keys[j].n[3] == keys[i].n[3])
+      ++j;
+    if (j - i > 2) return "non-manifold side shared by more than two elements";
+    if (j - i == 2) {
+      int32_t a = keys[i].slot, b = keys[i + 1].slot;
+      int32_t ea = slot_elem[a], eb = slot_elem[b];
+      m.nbr[a] = eb * 8 + (int32_t)(b - m.face_off[eb]);
+      m.nbr[b] = ea * 8 + (int32_t)(a - m.face_off[ea]);
+    } else {
+      int32_t a = keys[i].slot;
+      int32_t bid = face_boundary ? face_boundary[a] : 0;
+      if (bid < 0) return "external side without a boundary id (element " + std::to_string(slot_elem[a]) + ")";
+      m.nbr[a] = -1 - bid;
+      m.boundary_ids.insert(bid);
+    }
+    i = j;
+  }
+  keys.clear();
+  keys.shrink_to_fit();
+
+  // -- outward planes
+  m.plane.assign(4 * nf, 0.0);
+  const double *X = m.xyz.data();
+  std::string err;
+  auto planes = [&](int64_t lo, int64_t hi, std::string *perr) {
+    for (int64_t e = lo; e < hi; ++e) {
+      const int32_t *c = m.conn.data() + m.conn_off[e];
+      const int nn = m.type[e];
+      double cx = 0.0, cy = 0.0, cz = 0.0;
+      for (int k = 0; k < nn; ++k) {
+        cx = cx + X[3 * (int64_t)c[k]];
+        cy = cy + X[3 * (int64_t)c[k] + 1];
+        cz = cz + X[3 * (int64_t)c[k] + 2];
+      }
+      cx = cx / nn;
+      cy = cy / nn;
+      cz = cz / nn;
+      for (int s = 0; s < sides_of(m.type[e]); ++s) {
+        int32_t fn[4];
+        int fnn;
+        side_nodes(m, e, s, fn, fnn);
+        std::sort(fn, fn + fnn);
+        const double *p0 = X + 3 * (int64_t)fn[0];
+        const double *p1 = X + 3 * (int64_t)fn[1];
+        const double *p2 = X + 3 * (int64_t)fn[2];
+        double ax = p1[0] - p0[0], ay = p1[1] - p0[1], az = p1[2] - p0[2];
+        double bx = p2[0] - p0[0], by = p2[1] - p0[1], bz = p2[2] - p0[2];
+        double nx = ay * bz - az * by;
+        double ny = az * bx - ax * bz;
+        double nz = ax * by - ay * bx;
+        double len = std::sqrt((nx * nx + ny * ny) + nz * nz);
+        if (!(len > 0.0)) {
+          *perr = "degenerate side " + std::to_string(s) + " of element " + std::to_string(e);
+          return;
+        }
+        nx = nx / len;
+        ny = ny / len;
+        nz = nz / len;
+        double cc = (nx * p0[0] + ny * p0[1]) + nz * p0[2];
+        if (fnn == 4) {
+          const double *p3 = X + 3 * (int64_t)fn[3];
+          double off = ((nx * p3[0] + ny * p3[1]) + nz * p3[2]) - cc;
+          double scale = std::sqrt((ax * ax + ay * ay) + az * az) + std::sqrt((bx * bx + by * by) + bz * bz);
+          if (std::fabs(off) > 1e-9 * scale) {
+            *perr = "non-planar side " + std::to_string(s) + " of element " + std::to_string(e);
+            return;
+          }
+        }
+        double side = ((nx * cx + ny * cy) + nz * cz) - cc;
+        double *p = m.plane.data() + 4 * (m.face_off[e] + s);
+        if (side > 0.0) {
+          p[0] = -nx;
+          p[1] = -ny;
+          p[2] = -nz;
+          p[3] = -cc;
+        } else {
+          p[0] = nx;
+          p[1] = ny;
+          p[2] = nz;
+          p[3] = cc;
+        }
+      }
+    }
+  };
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (ne < 100000) nt = 1;
+  std::vector<std::string> errs(nt);
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < nt; ++t)
+    pool.emplace_back(planes, ne * t / nt, ne * (t + 1) / nt, &errs[t]);
+  for (auto &th : pool) th.join();
+  for (auto &e : errs)
+    if (!e.empty()) return e;
+
+  // -- volume CDF, sequential in element order
+  m.cdf.assign(ne, 0.0);
+  std::vector<double> vol(ne);
+  {
+    std::vector<std::thread> p2;
+    for (unsigned t = 0; t < nt; ++t)
+      p2.emplace_back([&, t] {
+        for (int64_t e = ne * t / nt; e < ne * (t + 1) / nt; ++e) vol[e] = volume6(m, e);
+      });
+    for (auto &th : p2) th.join();
+  }
+  double acc = 0.0;
+  for (int64_t e = 0; e < ne; ++e) {
+    acc = acc + vol[e];
+    m.cdf[e] = acc;
+  }
+  return "";
+}
+
+int64_t locate_point(const HostMesh &m, const double *p, double tol)
+{
+  for (int64_t e = 0; e < m.n_elem; ++e) {
+    bool in = true;
+    for (int64_t s = m.face_off[e]; s < m.face_off[e + 1] && in; ++s) {
+      const double *q = m.plane.data() + 4 * s;
+      double v = ((q[0] * p[0] + q[1] * p[1]) + q[2] * p[2]) - q[3];
+      if (v > tol) in = false;
+    }
+    if (in) return e;
+  }
+  return -1;
+}
+
+} // namespace mcw

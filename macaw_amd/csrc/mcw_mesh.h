@@ -1,0 +1,38 @@
+// mcw_mesh.h -- host mesh container and derivation (planes, neighbours, CDF).
+#pragma once
+
+#include <stdint.h>
+
+#include <set>
+#include <string>
+#include <vector>
+
+namespace mcw {
+
+struct HostMesh {
+  int64_t n_nodes = 0, n_elem = 0;
+  std::vector<double> xyz;
+  std::vector<uint8_t> type;
+  std::vector<int64_t> conn_off;
+  std::vector<int32_t> conn;
+  std::vector<int32_t> subdomain;
+  // derived
+  int uniform_sides = 0;            // 4 all-TET4, 6 all-HEX8, 0 mixed
+  std::vector<int64_t> face_off;    // n_elem + 1
+  std::vector<double> plane;        // 4 per side slot
+  std::vector<int32_t> nbr;         // neighbour*8 + side, or -1 - boundary id
+  std::set<int32_t> boundary_ids;   // ids of external sides
+  std::vector<double> cdf;          // element volume CDF (6x volumes)
+};
+
+extern const int HEX_KUHN_TETS[6][4];
+
+// Returns an error message, or "" on success.
+std::string derive_mesh(HostMesh &m, const int32_t *face_boundary);
+
+// Lowest-index element whose side planes all contain p within tol, or -1.
+int64_t locate_point(const HostMesh &m, const double *p, double tol);
+
+double tet_volume6(const double *a, const double *b, const double *c, const double *d);
+
+} // namespace mcw

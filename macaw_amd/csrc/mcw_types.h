@@ -1,0 +1,125 @@
+// mcw_types.h -- device views shared by the host runtime and the kernels.
+//
+// HBM layout (all arrays device-resident, built once per problem):
+//   mesh   plane[slot]   double4 (nx, ny, nz, c), outward unit normal; a
+//                        TET4 element's 4 planes are one 128-byte record
+//          nbr[slot]     int32: neighbour*8 + neighbour side, or -1 - boundary
+//          elem_mat[e]   int32 material index (-1 void)
+//          face_off[e]   int32 first slot (mixed meshes only)
+//   xs     energy[]      per-nuclide grids, packed (grid_off[n] .. +n_points[n])
+//          values[]      double4 (total, absorption, scatter, fission) per point
+//   bank   SoA particle state (x,y,z,u,v,w,E,sigt f64; seed u64; elem, inface,
+//          mat i32) sized for one chunk of histories
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mcw {
+
+constexpr int MAX_TALLIES = 8;
+
+struct MeshView {
+  const double4 *plane;
+  const int32_t *nbr;
+  const int32_t *elem_mat;
+  const int32_t *face_off;   // nullptr for single-type meshes
+  const uint8_t *n_sides;    // per element, mixed meshes only
+  int32_t stride;            // sides per element for single-type meshes (4 or 6), 0 = mixed
+  int32_t n_bc;
+  const int8_t *bc_kind;     // per boundary id
+  int64_t n_elem;
+  // for volumetric sources
+  const double *cdf;
+  const double *xyz;
+  const int32_t *conn;
+  const int64_t *conn_off;
+  const uint8_t *type;
+  const int64_t *orig_id;    // internal element -> caller's element index (nullptr = identity)
+};
+
+struct XSView {
+  const int64_t *grid_off;
+  const int32_t *n_points;
+  const double *energy;
+  const double4 *values;
+  const double *awr;
+  const int32_t *mat_off;
+  const int32_t *mat_nuc;
+  const double *mat_dens;
+  int32_t n_nuc;
+  int32_t n_mat;
+  double e_min, e_max;
+};
+
+struct SourceView {
+  int32_t space, angle, energy, pad;
+  double point[3];
+  double box_lo[3], box_hi[3];
+  double axis[3];
+  double beta, E0, watt_a, watt_b;
+  int64_t point_elem;
+};
+
+struct TallyView {
+  int32_t estimator;
+  int32_t n_scores;
+  int32_t n_nuclides;
+  int32_t n_score_bins;
+  int32_t simple_flux;       // one flux score, one nuclide bin
+  int32_t needs_xs;
+  int32_t n_edges;           // energy filter edges (0 = none)
+  int32_t pad;
+  int64_t energy_stride;
+  const int64_t *elem_base;  // spatial filter index per element (-1 = no bin), nullptr = element id
+  const double *edges;
+  const int32_t *scores;
+  const int32_t *nuclides;
+  double *value;
+};
+
+struct BankView {
+  double *x, *y, *z, *u, *v, *w, *E, *sigt;
+  uint64_t *seed;
+  int32_t *elem, *inface, *mat;
+  int64_t *hid;      // history index within the batch (0-based)
+  uint64_t *digest;  // record mode only
+  int32_t *nseg, *ncoll;
+};
+
+struct RecordView {
+  uint64_t *digest, *seed;
+  double *E, *x, *y, *z;
+  int32_t *nseg, *ncoll, *elem, *fate;
+};
+
+// Global counters, accumulated with one atomic per wavefront.
+struct Counters {
+  unsigned long long segments, collisions, absorptions, leaks, cutoffs, lost, reflections, pad;
+  double track;
+  unsigned long long next_hist;   // persistent kernel work queue
+  unsigned long long q_count[4];  // event queue lengths
+};
+
+struct Params {
+  MeshView mesh;
+  XSView xs;
+  SourceView src;
+  TallyView tally[MAX_TALLIES];
+  int32_t n_tallies;
+  int32_t n_tl;            // tallies with the track-length estimator
+  int32_t n_coll;          // tallies with the collision estimator
+  int32_t roundtrip;
+  int32_t record;
+  int32_t pad;
+  int64_t batch;
+  int64_t n_particles;     // particles in the batch (seed id stride)
+  int64_t hist_begin;      // first history index handled by this launch
+  int64_t hist_end;        // one past the last
+  int64_t max_seg;
+  uint64_t master_seed;
+  Counters *cnt;
+  RecordView rec;
+};
+
+} // namespace mcw

@@ -1,0 +1,167 @@
+"""ctypes binding of libmacaw_amd.so (the C ABI in include/macaw_amd.h).
+
+The library is the product: this module only loads it and declares the
+signatures.  There is no fallback -- a missing or unloadable library is an
+error.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .problems import Problem, load_into
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmacaw_amd.so")
+_lib = None
+
+# every symbol include/macaw_amd.h declares
+EXPORTS = (
+    "mcw_create", "mcw_destroy", "mcw_last_error", "mcw_default_options", "mcw_mesh_set", "mcw_bc_set",
+    "mcw_xs_set", "mcw_materials_set", "mcw_block_material_map", "mcw_source_set", "mcw_tally_add",
+    "mcw_tally_shape", "mcw_run_batch", "mcw_tally_get", "mcw_tally_reset", "mcw_history_get",
+    "mcw_server_create", "mcw_server_destroy", "mcw_server_last_error", "mcw_submit", "mcw_poll",
+    "mcw_result", "mcw_debug_math_probe",
+)
+
+
+class McwError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise McwError(f"{LIB_PATH} is not built: run `python -m macaw_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, u64, dp = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.POINTER(C.c_double)
+    i32p, i64p, u8p, u64p = C.POINTER(i32), C.POINTER(i64), C.POINTER(C.c_uint8), C.POINTER(u64)
+    sig = {
+        "mcw_create": ([C.POINTER(vp), C.POINTER(abi.Options)], C.c_int),
+        "mcw_destroy": ([vp], C.c_int),
+        "mcw_last_error": ([vp], C.c_char_p),
+        "mcw_default_options": ([C.POINTER(abi.Options)], None),
+        "mcw_mesh_set": ([vp, i64, dp, i64, u8p, i64p, i32p, i32p, i32p], C.c_int),
+        "mcw_bc_set": ([vp, i32, i32], C.c_int),
+        "mcw_xs_set": ([vp, i32, C.POINTER(abi.Nuclide)], C.c_int),
+        "mcw_materials_set": ([vp, i32, C.POINTER(abi.Material)], C.c_int),
+        "mcw_block_material_map": ([vp, i32, i32p, i32p], C.c_int),
+        "mcw_source_set": ([vp, C.POINTER(abi.Source)], C.c_int),
+        "mcw_tally_add": ([vp, i32, i32, i32, C.POINTER(abi.Filter), i32, i32p, i32, i32p], C.c_int),
+        "mcw_tally_shape": ([vp, i32, i64p, i64p], C.c_int),
+        "mcw_run_batch": ([vp, i64, i64, C.POINTER(abi.BatchStats)], C.c_int),
+        "mcw_tally_get": ([vp, i32, i32, dp, C.c_size_t], C.c_int),
+        "mcw_tally_reset": ([vp], C.c_int),
+        "mcw_history_get": ([vp, vp, C.c_size_t], C.c_int),
+        "mcw_server_create": ([C.POINTER(vp), C.POINTER(vp), i32, i32, i64], C.c_int),
+        "mcw_server_destroy": ([vp], C.c_int),
+        "mcw_server_last_error": ([vp], C.c_char_p),
+        "mcw_submit": ([vp, C.POINTER(abi.Request)], C.c_int),
+        "mcw_poll": ([vp, i32p], C.c_int),
+        "mcw_result": ([vp, C.POINTER(abi.RequestStatus), dp, C.c_size_t], C.c_int),
+        "mcw_debug_math_probe": ([vp, dp, i64, dp, dp, dp, dp, u64p, dp, u64p], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def default_options(**kw) -> abi.Options:
+    o = abi.Options()
+    lib().mcw_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One device context (mcw_ctx): a problem resident in HBM and its tallies."""
+
+    def __init__(self, problem: Problem = None, **options):
+        L = lib()
+        self.L = L
+        self.opts = default_options(**options)
+        h = C.c_void_p()
+        rc = L.mcw_create(C.byref(h), C.byref(self.opts))
+        self.h = h
+        if rc != 0:
+            msg = L.mcw_last_error(h).decode() if h.value else "mcw_create failed"
+            L.mcw_destroy(h)
+            self.h = None
+            raise McwError(msg)
+        self.problem = None
+        if problem is not None:
+            self.load(problem)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise McwError(self.L.mcw_last_error(self.h).decode())
+        return rc
+
+    def load(self, p: Problem):
+        L, h = self.L, self.h
+        calls = {
+            "mesh_set": lambda *a: self._chk(L.mcw_mesh_set(h, *a)),
+            "bc_set": lambda *a: self._chk(L.mcw_bc_set(h, *a)),
+            "xs_set": lambda *a: self._chk(L.mcw_xs_set(h, *a)),
+            "materials_set": lambda *a: self._chk(L.mcw_materials_set(h, *a)),
+            "block_material_map": lambda *a: self._chk(L.mcw_block_material_map(h, *a)),
+            "source_set": lambda *a: self._chk(L.mcw_source_set(h, *a)),
+            "tally_add": lambda *a: self._chk(L.mcw_tally_add(h, *a)),
+        }
+        self._marshal = load_into(calls, p)
+        self.problem = p
+        return self
+
+    def run_batch(self, batch, n_particles):
+        st = abi.BatchStats()
+        self._chk(self.L.mcw_run_batch(self.h, batch, n_particles, C.byref(st)))
+        return st.as_dict()
+
+    def tally_shape(self, tally_id):
+        nf, ns = C.c_int64(), C.c_int64()
+        self._chk(self.L.mcw_tally_shape(self.h, tally_id, C.byref(nf), C.byref(ns)))
+        return nf.value, ns.value
+
+    def tally(self, tally_id, kind=abi.RESULT_SUM):
+        nf, ns = self.tally_shape(tally_id)
+        out = np.zeros(nf * ns)
+        self._chk(self.L.mcw_tally_get(self.h, tally_id, kind, out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
+        return out
+
+    def tally_reset(self):
+        self._chk(self.L.mcw_tally_reset(self.h))
+
+    def histories(self, n):
+        out = np.zeros(n, abi.history_dtype())
+        self._chk(self.L.mcw_history_get(self.h, out.ctypes.data, n))
+        return out
+
+    def math_probe(self, x, seeds):
+        x = np.ascontiguousarray(x, np.float64)
+        seeds = np.ascontiguousarray(seeds, np.uint64)
+        n = x.size
+        outs = [np.zeros(n) for _ in range(5)]
+        so = np.zeros(n, np.uint64)
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        self._chk(self.L.mcw_debug_math_probe(self.h, dp(x), n, dp(outs[0]), dp(outs[1]), dp(outs[2]), dp(outs[3]),
+                                              seeds.ctypes.data_as(C.POINTER(C.c_uint64)), dp(outs[4]),
+                                              so.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return {"log": outs[0], "sin": outs[1], "cos": outs[2], "sqrt": outs[3], "prn": outs[4], "future": so}
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.L.mcw_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

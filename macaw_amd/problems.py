@@ -1,0 +1,353 @@
+"""Synthetic fixed-source problems: meshes, nuclides, materials, sources.
+
+Every generator is seeded and deterministic, so the GPU library and the CPU
+oracle are fed identical inputs.  A `Problem` is plain numpy data laid out
+as the C ABI takes it (mcw_mesh_set, mcw_xs_set, ...).
+
+Shapes follow the reference's inputs:
+  * `hex_box` is MOOSE's GeneratedMeshGenerator (HEX8, libMesh node order,
+    boundary ids 0 back, 1 bottom, 2 right, 3 top, 4 left, 5 front;
+    test/tests/study/openmc_study.i:1-23);
+  * `tet_cube` is a Delaunay triangulation of a jittered lattice;
+  * nuclides are pointwise tables of 1/v absorption plus single-level
+    Breit-Wigner resonances with constant potential scattering.
+"""
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+import os
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import abi
+
+
+@dataclasses.dataclass
+class Mesh:
+    xyz: np.ndarray            # (n_nodes, 3) f64
+    elem_type: np.ndarray      # (n_elem,) u8
+    conn_offsets: np.ndarray   # (n_elem+1,) i64
+    conn: np.ndarray           # i32
+    subdomain: np.ndarray      # (n_elem,) i32
+    face_boundary: Optional[np.ndarray] = None  # per side slot i32
+
+    @property
+    def n_nodes(self):
+        return self.xyz.shape[0]
+
+    @property
+    def n_elem(self):
+        return self.elem_type.shape[0]
+
+
+@dataclasses.dataclass
+class NuclideData:
+    awr: float
+    energy: np.ndarray
+    total: np.ndarray
+    absorption: np.ndarray
+    scatter: np.ndarray
+    fission: Optional[np.ndarray] = None
+
+
+@dataclasses.dataclass
+class MaterialData:
+    id: int
+    nuclides: np.ndarray   # i32
+    densities: np.ndarray  # f64 [1/(b cm)]
+
+
+@dataclasses.dataclass
+class TallySpec:
+    id: int
+    estimator: int
+    filters: List[dict]      # {"type": FILTER_*, "ids": [...]} or {"type": ENERGY, "edges": [...]}
+    scores: List[int]
+    nuclides: List[int] = dataclasses.field(default_factory=lambda: [-1])
+
+
+@dataclasses.dataclass
+class Problem:
+    name: str
+    mesh: Mesh
+    nuclides: List[NuclideData]
+    materials: List[MaterialData]
+    block_material: Dict[int, int]
+    bcs: Dict[int, int]
+    source: abi.Source
+    tallies: List[TallySpec]
+    n_particles: int
+    n_batches: int
+
+
+# ----------------------------------------------------------------- nuclides
+
+def synth_nuclide(n_points: int, awr: float, seed: int, sigma_a0: float = 1.0,
+                  sigma_p: float = 4.7, n_res: int = 50, e_min: float = 1e-5,
+                  e_max: float = 2e7) -> NuclideData:
+    """A pointwise nuclide: sigma_a = sigma_a0 sqrt(0.0253/E) + SLBW capture,
+    sigma_s = sigma_p + SLBW elastic, on a log grid of n_points in [e_min, e_max] eV.
+
+    Resonances: E_r log-uniform in [1, 1e5] eV, Gamma log-uniform in
+    [1e-2, 10] eV, peak log-uniform in [10, 1e4] b, split Gamma_n/Gamma_gamma
+    by a uniform fraction in [0.2, 0.8]."""
+    rng = np.random.default_rng(seed)
+    E = np.geomspace(e_min, e_max, n_points)
+    E[0], E[-1] = e_min, e_max
+    Er = np.exp(rng.uniform(np.log(1.0), np.log(1e5), n_res))
+    G = np.exp(rng.uniform(np.log(1e-2), np.log(10.0), n_res))
+    peak = np.exp(rng.uniform(np.log(10.0), np.log(1e4), n_res))
+    fg = rng.uniform(0.2, 0.8, n_res)
+    sa = sigma_a0 * np.sqrt(0.0253 / E)
+    ss = np.full_like(E, sigma_p)
+    for r in range(n_res):
+        x = 2.0 * (E - Er[r]) / G[r]
+        shape = peak[r] / (1.0 + x * x)
+        sa += shape * fg[r] * np.sqrt(Er[r] / E)
+        ss += shape * (1.0 - fg[r])
+    st = sa + ss
+    return NuclideData(awr=float(awr), energy=E, total=st, absorption=sa, scatter=ss)
+
+
+# ----------------------------------------------------------------- meshes
+
+HEX_SIDE_BOUNDARY = np.array([0, 1, 2, 3, 4, 5], np.int32)  # side s lies on boundary s
+
+
+def hex_box(n: int, lo: float, hi: float) -> Mesh:
+    """n^3 HEX8 box, GeneratedMeshGenerator layout (x fastest)."""
+    m = n + 1
+    g = np.linspace(lo, hi, m)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")          # x fastest
+    xyz = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.float64)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()             # element i + n (j + n k)
+    nid = lambda a, b, c: a + m * (b + m * c)
+    conn = np.stack([nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                     nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1),
+                     nid(i, j + 1, k + 1)], 1).astype(np.int32)
+    ne = n ** 3
+    fb = np.tile(HEX_SIDE_BOUNDARY, ne)
+    return Mesh(xyz=xyz, elem_type=np.full(ne, abi.HEX8, np.uint8),
+                conn_offsets=np.arange(ne + 1, dtype=np.int64) * 8, conn=conn.ravel(),
+                subdomain=np.zeros(ne, np.int32), face_boundary=fb)
+
+
+def _cache_dir():
+    d = os.environ.get("MACAW_AMD_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "macaw_amd"))
+    os.makedirs(d, exist_ok=True)
+    return d
+
+
+def tet_cube(n_lattice: int, seed: int = 101, jitter: float = 0.25, cache: bool = True) -> Mesh:
+    """Delaunay TET4 mesh of the unit cube from a jittered n^3 point lattice.
+
+    Interior points move by up to +-jitter lattice spacings; points on a face,
+    edge or corner stay on it.  Tets are oriented positively.  All external
+    sides are boundary 0."""
+    key = hashlib.sha1(f"tet_cube-{n_lattice}-{seed}-{jitter}".encode()).hexdigest()[:16]
+    path = os.path.join(_cache_dir(), f"tet_cube_{n_lattice}_{key}.npz")
+    if cache and os.path.exists(path):
+        z = np.load(path)
+        xyz, tets = z["xyz"], z["tets"]
+    else:
+        from scipy.spatial import Delaunay
+        rng = np.random.default_rng(seed)
+        g = np.linspace(0.0, 1.0, n_lattice)
+        X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+        P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+        h = 1.0 / (n_lattice - 1)
+        J = rng.uniform(-jitter * h, jitter * h, P.shape)
+        J[(P == 0.0) | (P == 1.0)] = 0.0
+        xyz = P + J
+        tets = Delaunay(xyz).simplices.astype(np.int32)
+        a = xyz[tets[:, 1]] - xyz[tets[:, 0]]
+        b = xyz[tets[:, 2]] - xyz[tets[:, 0]]
+        c = xyz[tets[:, 3]] - xyz[tets[:, 0]]
+        vol = np.einsum("ij,ij->i", a, np.cross(b, c))
+        neg = vol < 0
+        tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
+        if cache:
+            np.savez(path, xyz=xyz, tets=tets)
+    ne = tets.shape[0]
+    return Mesh(xyz=np.ascontiguousarray(xyz, np.float64), elem_type=np.full(ne, abi.TET4, np.uint8),
+                conn_offsets=np.arange(ne + 1, dtype=np.int64) * 4,
+                conn=np.ascontiguousarray(tets.ravel(), np.int32),
+                subdomain=np.zeros(ne, np.int32), face_boundary=None)
+
+
+def voronoi_blocks(mesh: Mesh, n_regions: int, seed: int) -> np.ndarray:
+    """Subdomain per element: index of the nearest of n_regions seeded centres
+    to the element centroid."""
+    rng = np.random.default_rng(seed)
+    lo, hi = mesh.xyz.min(0), mesh.xyz.max(0)
+    centres = rng.uniform(lo, hi, (n_regions, 3))
+    nn = np.diff(mesh.conn_offsets)
+    cen = np.zeros((mesh.n_elem, 3))
+    # all elements of one type here
+    k = int(nn[0])
+    cen = mesh.xyz[mesh.conn.reshape(-1, k)].mean(1)
+    d = ((cen[:, None, :] - centres[None, :, :]) ** 2).sum(-1)
+    return d.argmin(1).astype(np.int32)
+
+
+# ----------------------------------------------------------------- sources
+
+def point_source(p, E0=2.0e6) -> abi.Source:
+    s = abi.Source()
+    s.space, s.angle, s.energy = abi.SPACE_POINT, abi.ANGLE_ISOTROPIC, abi.ENERGY_MONO
+    s.point[:] = list(p)
+    s.E0 = E0
+    return s
+
+
+def volume_watt_source(a=0.988e6, b=2.249e-6) -> abi.Source:
+    s = abi.Source()
+    s.space, s.angle, s.energy = abi.SPACE_MESH_VOLUME, abi.ANGLE_ISOTROPIC, abi.ENERGY_WATT
+    s.watt_a, s.watt_b = a, b
+    return s
+
+
+def box_source(lo, hi, E0=2.0e6) -> abi.Source:
+    s = abi.Source()
+    s.space, s.angle, s.energy = abi.SPACE_BOX, abi.ANGLE_ISOTROPIC, abi.ENERGY_MONO
+    s.box_lo[:] = list(lo)
+    s.box_hi[:] = list(hi)
+    s.E0 = E0
+    return s
+
+
+# ----------------------------------------------------------------- configs
+
+def flux_tallies(both=True):
+    t = [TallySpec(id=1, estimator=abi.EST_TRACKLENGTH,
+                   filters=[{"type": abi.FILTER_CELL}], scores=[abi.SCORE_FLUX])]
+    if both:
+        t.append(TallySpec(id=2, estimator=abi.EST_COLLISION,
+                           filters=[{"type": abi.FILTER_CELL}], scores=[abi.SCORE_FLUX]))
+    return t
+
+
+def c1_hex_reflective(n_particles=10_000, n_batches=10, n=8, n_points=20_000) -> Problem:
+    """8x8x8 HEX8 reflective 10 cm cube, one A=12 nuclide at 0.08 /b-cm,
+    isotropic 2 MeV point source at the centre, track-length + collision flux."""
+    mesh = hex_box(n, -5.0, 5.0)
+    nuc = synth_nuclide(n_points, 12.0, seed=202, sigma_a0=1.0, sigma_p=4.7)
+    mat = MaterialData(id=1, nuclides=np.array([0], np.int32), densities=np.array([0.08]))
+    return Problem(name="c1_hex8_reflective", mesh=mesh, nuclides=[nuc], materials=[mat],
+                   block_material={0: 1}, bcs={b: abi.BC_REFLECT for b in range(6)},
+                   source=point_source((0.0, 0.0, 0.0), 2.0e6), tallies=flux_tallies(),
+                   n_particles=n_particles, n_batches=n_batches)
+
+
+C2_NUCLIDES = [  # (A, sigma_p [b], sigma_a0 [b])
+    (1.0, 20.0, 0.33),
+    (12.0, 4.7, 0.0035),
+    (16.0, 3.8, 0.0002),
+    (56.0, 11.0, 2.56),
+]
+
+
+def c2_nuclides(n_points=50_000, n_nuclides=4):
+    out = []
+    for i in range(n_nuclides):
+        A, sp, sa = C2_NUCLIDES[i % len(C2_NUCLIDES)]
+        out.append(synth_nuclide(n_points, A, seed=1000 + i, sigma_a0=sa, sigma_p=sp))
+    return out
+
+
+def c2_materials(n_materials=4, n_nuclides=4, seed=303):
+    rng = np.random.default_rng(seed + 1)
+    mats = []
+    for m in range(n_materials):
+        nuc = np.arange(n_nuclides, dtype=np.int32)
+        if n_nuclides > 4:
+            nuc = np.sort(rng.choice(n_nuclides, size=4, replace=False)).astype(np.int32)
+        mats.append(MaterialData(id=m + 1, nuclides=nuc,
+                                 densities=rng.uniform(0.01, 0.1, nuc.size)))
+    return mats
+
+
+def c2_tet_vacuum(n_lattice=54, n_particles=10_000_000, n_batches=1, n_points=50_000,
+                  n_materials=4, n_nuclides=4, tallies=None) -> Problem:
+    """Unit cube, vacuum, Delaunay TET4 (54^3 lattice -> 1.0M tets), Voronoi
+    materials from 4 nuclides, uniform volumetric Watt source."""
+    mesh = tet_cube(n_lattice, seed=101)
+    mesh.subdomain = voronoi_blocks(mesh, n_materials, seed=303)
+    nucs = c2_nuclides(n_points, n_nuclides)
+    mats = c2_materials(n_materials, n_nuclides)
+    return Problem(name=f"c2_tet4_vacuum_{mesh.n_elem}", mesh=mesh, nuclides=nucs, materials=mats,
+                   block_material={b: b + 1 for b in range(n_materials)}, bcs={0: abi.BC_VACUUM},
+                   source=volume_watt_source(),
+                   tallies=tallies if tallies is not None else flux_tallies(both=False),
+                   n_particles=n_particles, n_batches=n_batches)
+
+
+# ----------------------------------------------------------------- marshalling
+
+class Marshal:
+    """Holds ctypes views of a Problem's arrays alive while a library copies them."""
+
+    def __init__(self, p: Problem):
+        self.p = p
+        self.keep = []
+        self.nuclides = (abi.Nuclide * len(p.nuclides))()
+        for i, n in enumerate(p.nuclides):
+            arrs = [np.ascontiguousarray(a, np.float64) for a in (n.energy, n.total, n.absorption, n.scatter)]
+            fis = None if n.fission is None else np.ascontiguousarray(n.fission, np.float64)
+            self.keep += arrs + [fis]
+            self.nuclides[i] = abi.Nuclide(n.energy.size, n.awr, *[abi.ptr(a, abi.C.c_double) for a in arrs],
+                                           abi.ptr(fis, abi.C.c_double))
+        self.materials = (abi.Material * len(p.materials))()
+        for i, m in enumerate(p.materials):
+            nu = np.ascontiguousarray(m.nuclides, np.int32)
+            de = np.ascontiguousarray(m.densities, np.float64)
+            self.keep += [nu, de]
+            self.materials[i] = abi.Material(m.id, nu.size, abi.ptr(nu, abi.C.c_int32),
+                                             abi.ptr(de, abi.C.c_double))
+        self.blocks = np.array(list(p.block_material.keys()), np.int32)
+        self.block_mats = np.array(list(p.block_material.values()), np.int32)
+
+    def filters(self, t: TallySpec):
+        arr = (abi.Filter * max(1, len(t.filters)))()
+        for k, f in enumerate(t.filters):
+            ids = f.get("ids")
+            edges = f.get("edges")
+            ida = None if ids is None else np.ascontiguousarray(ids, np.int64)
+            eda = None if edges is None else np.ascontiguousarray(edges, np.float64)
+            self.keep += [ida, eda]
+            n = len(ids) if ids is not None else (len(edges) if edges is not None else 0)
+            arr[k] = abi.Filter(f["type"], n, abi.ptr(ida, abi.C.c_int64), abi.ptr(eda, abi.C.c_double))
+        scores = np.ascontiguousarray(t.scores, np.int32)
+        nucs = np.ascontiguousarray(t.nuclides, np.int32)
+        self.keep += [scores, nucs]
+        return arr, scores, nucs
+
+
+def load_into(lib_prefix_calls, p: Problem):
+    """Feed a Problem through a set of setter callables with the mcw_* signatures.
+
+    `lib_prefix_calls` maps names ('mesh_set', 'bc_set', 'xs_set', 'materials_set',
+    'block_material_map', 'source_set', 'tally_add') to callables returning 0 on success
+    and raising/returning non-zero on error."""
+    m = p.mesh
+    M = Marshal(p)
+    c = lib_prefix_calls
+    fb = None if m.face_boundary is None else np.ascontiguousarray(m.face_boundary, np.int32)
+    c["mesh_set"](m.n_nodes, abi.ptr(np.ascontiguousarray(m.xyz, np.float64), abi.C.c_double), m.n_elem,
+                  abi.ptr(m.elem_type, abi.C.c_uint8), abi.ptr(m.conn_offsets, abi.C.c_int64),
+                  abi.ptr(m.conn, abi.C.c_int32), abi.ptr(m.subdomain, abi.C.c_int32),
+                  abi.ptr(fb, abi.C.c_int32))
+    for b, k in p.bcs.items():
+        c["bc_set"](int(b), int(k))
+    c["xs_set"](len(p.nuclides), M.nuclides)
+    c["materials_set"](len(p.materials), M.materials)
+    c["block_material_map"](len(M.blocks), abi.ptr(M.blocks, abi.C.c_int32), abi.ptr(M.block_mats, abi.C.c_int32))
+    c["source_set"](abi.C.byref(p.source))
+    for t in p.tallies:
+        f, s, n = M.filters(t)
+        c["tally_add"](t.id, t.estimator, len(t.filters), f, len(t.scores), abi.ptr(s, abi.C.c_int32),
+                       len(t.nuclides), abi.ptr(n, abi.C.c_int32))
+    return M

@@ -1,0 +1,180 @@
+"""GPU parity: the HIP library against the CPU oracle, through the C ABI.
+
+Per history the crossed-element sequence (a hash chain over element and event
+of every segment), the segment and collision counts, the fate and the final
+state (energy, position, tracking seed) must be bit-identical; tallies agree to
+1e-10 relative (FP64 atomics change only the summation order)."""
+import numpy as np
+import pytest
+
+from macaw_amd import abi, problems as P
+from macaw_amd.lib import Context
+
+pytestmark = pytest.mark.gpu
+
+TALLY_RTOL = 1e-10
+
+
+def small_tet(n_lattice=9, n_materials=4, n_particles=3000):
+    mesh = P.tet_cube(n_lattice, seed=101, cache=False)
+    mesh.subdomain = P.voronoi_blocks(mesh, n_materials, seed=303)
+    return P.Problem(name="tet_small", mesh=mesh, nuclides=P.c2_nuclides(2000, 4),
+                     materials=P.c2_materials(n_materials, 4),
+                     block_material={b: b + 1 for b in range(n_materials)}, bcs={0: abi.BC_VACUUM},
+                     source=P.volume_watt_source(), tallies=P.flux_tallies(both=True),
+                     n_particles=n_particles, n_batches=2)
+
+
+def c1_small(n_particles=1000):
+    return P.c1_hex_reflective(n_particles=n_particles, n_batches=2, n_points=20000)
+
+
+def assert_records_equal(g, o):
+    assert g.shape == o.shape
+    for f in ("id", "digest", "seed", "n_segments", "n_collisions", "elem", "fate"):
+        bad = np.nonzero(g[f] != o[f])[0]
+        assert bad.size == 0, f"{f}: {bad.size} histories differ, first {bad[:5]}: {g[f][bad[:3]]} vs {o[f][bad[:3]]}"
+    for f in ("E", "x", "y", "z"):
+        gb, ob = g[f].view(np.uint64), o[f].view(np.uint64)
+        bad = np.nonzero(gb != ob)[0]
+        assert bad.size == 0, f"{f}: {bad.size} histories differ in bits, first {bad[:5]}"
+
+
+def assert_tally_close(a, b, rtol=TALLY_RTOL):
+    scale = np.maximum(np.abs(b), 1e-300)
+    rel = np.abs(a - b) / scale
+    assert rel.max() <= rtol, f"max rel diff {rel.max():.3e} at {rel.argmax()}"
+
+
+@pytest.fixture(scope="module")
+def ctx_probe():
+    c = Context()
+    yield c
+    c.close()
+
+
+def test_device_math_bit_identical(ctx_probe, oracle_lib):
+    po = oracle_lib
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(0, 1, 40000), 2.0 ** -rng.uniform(0, 64, 20000),
+                        rng.uniform(0, 1e7, 20000), [1.0, 0.5, 0.25, 1e-300, 2.0**-64]])
+    seeds = rng.integers(0, 2**63, x.size, dtype=np.int64).astype(np.uint64) * np.uint64(2) + np.uint64(1)
+    d = ctx_probe.math_probe(x, seeds)
+    lg = np.array([po.log(float(v)) for v in x])
+    assert np.array_equal(d["log"].view(np.uint64), lg.view(np.uint64))
+    sq = np.sqrt(x)
+    assert np.array_equal(d["sqrt"].view(np.uint64), sq.view(np.uint64))
+    fr = x - np.floor(x)
+    sc = np.array([po.sincos_2pi(float(v)) for v in fr])
+    assert np.array_equal(d["sin"].view(np.uint64), sc[:, 0].view(np.uint64))
+    assert np.array_equal(d["cos"].view(np.uint64), sc[:, 1].view(np.uint64))
+    pr = np.array([po.prn(int(s))[0] for s in seeds])
+    assert np.array_equal(d["prn"].view(np.uint64), pr.view(np.uint64))
+    fs = np.array([po.future_seed(i * 152917 + 7, int(s)) for i, s in enumerate(seeds)], np.uint64)
+    assert np.array_equal(d["future"], fs)
+
+
+def run_both(problem, mode, batches=(1,), roundtrip=False, **opt):
+    from oracle.pyoracle import Oracle
+    g = Context(problem, mode=mode, record_histories=1, seed_roundtrip=int(roundtrip), **opt)
+    o = Oracle(problem, seed_roundtrip=roundtrip)
+    out = []
+    for b in batches:
+        sg = g.run_batch(b, problem.n_particles)
+        so = o.run_batch(b, problem.n_particles, threads=4, record=True)
+        out.append((sg, so, g.histories(problem.n_particles), o.histories(problem.n_particles)))
+    return g, o, out
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_PLAIN, abi.MODE_BATCHED])
+def test_c1_hex_reflective_parity(mode):
+    p = c1_small(1000)
+    g, o, res = run_both(p, mode, batches=(1, 2))
+    for sg, so, hg, ho in res:
+        assert_records_equal(hg, ho)
+        for k in ("segments", "collisions", "absorptions", "leaks", "cutoffs", "lost", "reflections"):
+            assert sg[k] == so[k], k
+        assert sg["lost"] == 0 and sg["leaks"] == 0
+        assert abs(sg["track_length"] - so["track_length"]) <= 1e-10 * so["track_length"]
+    for tid in (1, 2):
+        for kind in (abi.RESULT_VALUE, abi.RESULT_SUM, abi.RESULT_SUM_SQ):
+            assert_tally_close(g.tally(tid, kind), o.tally(tid, kind))
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_PLAIN, abi.MODE_BATCHED])
+def test_tet_vacuum_watt_parity(mode):
+    p = small_tet()
+    g, o, res = run_both(p, mode, batches=(1,))
+    sg, so, hg, ho = res[0]
+    assert_records_equal(hg, ho)
+    assert sg["segments"] == so["segments"] and sg["leaks"] == so["leaks"]
+    assert sg["leaks"] > 0 and sg["lost"] == 0
+    for tid in (1, 2):
+        assert_tally_close(g.tally(tid), o.tally(tid))
+
+
+def test_seed_roundtrip_mode_parity():
+    """MaCaw's Real-stored seeds (round to double each segment) on both sides."""
+    p = c1_small(500)
+    g, o, res = run_both(p, abi.MODE_BATCHED, roundtrip=True)
+    sg, so, hg, ho = res[0]
+    assert_records_equal(hg, ho)
+    assert np.all(hg["seed"] < 2**64)
+    # seeds stay representable as doubles
+    assert np.all(hg["seed"].astype(np.float64).astype(np.uint64) == hg["seed"])
+
+
+def test_plain_and_batched_agree_bitwise():
+    p = small_tet(n_particles=4000)
+    a = Context(p, mode=abi.MODE_PLAIN, record_histories=1)
+    b = Context(p, mode=abi.MODE_BATCHED, record_histories=1)
+    sa = a.run_batch(3, p.n_particles)
+    sb = b.run_batch(3, p.n_particles)
+    assert_records_equal(a.histories(p.n_particles), b.histories(p.n_particles))
+    assert sa["segments"] == sb["segments"]
+    assert_tally_close(a.tally(1), b.tally(1))
+
+
+def test_chunked_bank_matches_single_chunk():
+    p = small_tet(n_particles=3000)
+    a = Context(p, mode=abi.MODE_PLAIN, record_histories=1, bank_capacity=700)
+    b = Context(p, mode=abi.MODE_BATCHED, record_histories=1, bank_capacity=1100)
+    c = Context(p, mode=abi.MODE_BATCHED, record_histories=1)
+    for x in (a, b, c):
+        x.run_batch(1, p.n_particles)
+    ha, hb, hc = (x.histories(p.n_particles) for x in (a, b, c))
+    assert_records_equal(ha, hc)
+    assert_records_equal(hb, hc)
+
+
+def test_tally_reset_and_accumulation():
+    p = c1_small(300)
+    g = Context(p)
+    g.run_batch(1, 300)
+    v1 = g.tally(1, abi.RESULT_VALUE).copy()
+    g.run_batch(2, 300)
+    v2 = g.tally(1, abi.RESULT_VALUE)
+    s = g.tally(1, abi.RESULT_SUM)
+    ss = g.tally(1, abi.RESULT_SUM_SQ)
+    assert_tally_close(s, v1 / 300 + v2 / 300, 1e-13)
+    assert_tally_close(ss, (v1 / 300) ** 2 + (v2 / 300) ** 2, 1e-12)
+    g.tally_reset()
+    assert not g.tally(1).any()
+
+
+def test_errors_are_reported():
+    from macaw_amd.lib import McwError
+    p = c1_small(10)
+    g = Context(p)
+    with pytest.raises(McwError, match="batch numbers start at 1"):
+        g.run_batch(0, 10)
+    p2 = c1_small(10)
+    p2.block_material = {0: 7}
+    g2 = Context(p2)
+    with pytest.raises(McwError, match="Could not find material 7"):
+        g2.run_batch(1, 10)
+    p3 = c1_small(10)
+    p3.bcs = {b: abi.BC_REFLECT for b in range(5)}
+    g3 = Context(p3)
+    with pytest.raises(McwError, match="boundary 5"):
+        g3.run_batch(1, 10)
