@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: histories/s and segments/s of the particle-tracking hot path.
+
+Workload (one "step" = one batch): unit-cube TET4 mesh from a Delaunay
+triangulation of a jittered 54^3 lattice (1,008,162 tets), vacuum
+boundaries, 4 Voronoi materials of 4 synthetic nuclides (50k-point grids),
+uniform volumetric Watt source, 1e7 histories per batch, FP64, a per-element
+track-length flux tally.  Inputs are synthetic and resident in HBM before
+the timed region.
+
+Reported beside the batched (persistent, fused) path: the plain event-based
+GPU path on the same batch, the dominant kernel's roofline (algorithmic bytes
+per launch over its HIP-event duration) and a CPU baseline (the FP64 oracle,
+std::thread on the host cores, on a bounded sample of the same histories).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--histories N]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "histories/sec + segments/sec at 1M tets vs plain GPU; p50/p99 serving latency"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(stats, n_nuc_per_mat, b_seg=276.0):
+    """92 B per history (bank record), B_seg per segment (TET4 FP64: particle
+    read 76 + write 36, element record 148, tally 16), 64*n_nuc + 64 per
+    collision (two bracketing points of 4 tables per nuclide + state)."""
+    return 92.0 * stats["histories"] + b_seg * stats["segments"] + (64.0 * n_nuc_per_mat + 64.0) * stats["collisions"]
+
+
+def read_traffic(workload):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", f"k_persistent_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split("=")[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(problem, n_hist_full, seconds=12.0):
+    from oracle.pyoracle import Oracle
+    cores = os.cpu_count() or 1
+    o = Oracle(problem)
+    pilot = 2000
+    t = time.perf_counter()
+    o.run_batch(1, n_hist_full if False else pilot, threads=cores)
+    dt = time.perf_counter() - t
+    n = int(min(max(pilot * seconds / max(dt, 1e-3), 10000), 2_000_000))
+    log(f"cpu baseline: pilot {pilot} histories in {dt:.2f}s -> sample {n} histories on {cores} threads")
+    o2 = Oracle(problem)
+    t = time.perf_counter()
+    st = o2.run_batch(1, n, threads=cores)
+    dt = time.perf_counter() - t
+    return {"value": n / dt, "unit": "histories/s", "cores": cores, "kind": "port",
+            "sample": f"batch 1, histories 1..{n} of the same workload (seed ids (b-1)*N+i with N={n}), "
+                      f"{st['segments']} segments",
+            "segments_per_s": st["segments"] / dt, "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--histories", type=int, default=10_000_000)
+    ap.add_argument("--lattice", type=int, default=54)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(histories shard by id range across GPUs with one tally all-reduce per batch; "
+              f"that path is not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from macaw_amd import abi, problems as P
+    from macaw_amd.lib import Context
+
+    t0 = time.perf_counter()
+    log(f"generating workload: Delaunay lattice {args.lattice}^3")
+    prob = P.c2_tet_vacuum(n_lattice=args.lattice, n_particles=args.histories)
+    n_elem = prob.mesh.n_elem
+    workload = f"c2_tet4_{n_elem // 1000}k_vacuum"
+    log(f"mesh {n_elem} tets, {prob.mesh.n_nodes} nodes ({time.perf_counter() - t0:.1f}s)")
+
+    ctx = Context(prob, mode=abi.MODE_BATCHED, block_size=args.block)
+    N = args.histories
+    for w in range(args.warmup):
+        st = ctx.run_batch(100 + w, N)
+        log(f"warmup {w}: {st['time_ms']:.1f} ms, {st['segments']} segments")
+    steps = []
+    t = time.perf_counter()
+    for k in range(args.steps):
+        steps.append(ctx.run_batch(k + 1, N))
+    wall = time.perf_counter() - t
+    for k, st in enumerate(steps):
+        log(f"step {k}: {st['time_ms']:.1f} ms device, {st['segments']} segments, {st['collisions']} collisions, "
+            f"leaks {st['leaks']}, lost {st['lost']}")
+    segs = sum(s["segments"] for s in steps)
+    colls = sum(s["collisions"] for s in steps)
+    kern_ms = sum(s["kernel_ms"] for s in steps)
+    launches = sum(s["kernel_launches"] for s in steps)
+    hist_s = N * args.steps / wall
+    seg_s = segs / wall
+
+    # dominant kernel: k_persistent, one launch per batch
+    agg = {"histories": N * args.steps, "segments": segs, "collisions": colls}
+    b_alg = algorithmic_bytes(agg, 4)
+    per_launch_bytes = b_alg / launches
+    per_launch_s = kern_ms / launches / 1e3
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    traffic = read_traffic(workload)
+
+    plain = None
+    if not args.no_plain:
+        pctx = Context(prob, mode=abi.MODE_PLAIN)
+        pctx.run_batch(200, N)  # warm
+        t = time.perf_counter()
+        pst = pctx.run_batch(1, N)
+        pw = time.perf_counter() - t
+        plain = {"histories_per_s": N / pw, "segments_per_s": pst["segments"] / pw, "ms_per_step": pw * 1e3}
+        if pst["segments"] != steps[0]["segments"]:
+            log("WARNING: plain and batched segment counts differ")
+        log(f"plain: {pw * 1e3:.1f} ms per batch")
+        pctx.close()
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(prob, N)
+        log(f"cpu: {cpu['value']:.3e} histories/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": hist_s,
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": workload, "n_elem": n_elem, "element": "TET4", "histories_per_step": N,
+                   "materials": 4, "nuclides": 4, "grid_points": 50000, "source": "volume Watt",
+                   "boundary": "vacuum", "tally": "per-element track-length flux", "mode": "batched",
+                   "precision": "fp64", "parallelism": "1 GPU"},
+        "segments_per_s": seg_s,
+        "segments_per_history": segs / (N * args.steps),
+        "collisions_per_history": colls / (N * args.steps),
+        "plain_gpu": plain,
+        "speedup_vs_plain": (hist_s / plain["histories_per_s"]) if plain else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_persistent<4>", "alg_bytes_per_launch": per_launch_bytes,
+                     "launch_ms": per_launch_s * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
