@@ -92,7 +92,9 @@ typedef struct {
   int64_t max_segments;     /* per-history guard; beyond it the history is lost; 0 = default */
   uint64_t master_seed;     /* OpenMC settings::seed, default 1 */
   int32_t sort_cadence;     /* BATCHED: refills between element sorts of the bank, 0 = never */
-  int32_t reserved[7];
+  int32_t lanes_min;        /* BATCHED: a wavefront keeps stepping segments while at least this
+                               many of its 64 lanes are in flight (1..64, 0 = default) */
+  int32_t reserved[6];
 } mcw_options;
 
 /* One synthetic continuous-energy nuclide: pointwise lin-lin tables on an
@@ -237,6 +239,12 @@ int mcw_history_get(mcw_ctx *ctx, mcw_history *out, size_t n);
 int mcw_debug_math_probe(mcw_ctx *ctx, const double *x, int64_t n, double *log_out, double *sin_out,
                          double *cos_out, double *sqrt_out, const uint64_t *seeds, double *prn_out,
                          uint64_t *seed_out);
+
+/* Diagnostics (no reference counterpart): the device's union-grid lookup of
+ * nuclide `nuc` at n host energies: union index and (total, absorption,
+ * scatter, fission) per energy, for comparison with a per-nuclide search. */
+int mcw_debug_xs_probe(mcw_ctx *ctx, int32_t nuc, const double *E, int64_t n, int32_t *k_out,
+                       double *xs_out);
 
 /* ---- serving (continuous batching of fixed-source requests) -------------- */
 

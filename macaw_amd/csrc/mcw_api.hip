@@ -22,12 +22,14 @@
 #include "mcw_types.h"
 
 namespace mcw {
-hipError_t launch_segment(int ns, dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B,
-                          const int32_t *live_in, int qin, int32_t *live_out, int qout, int32_t *coll_q, int qcoll);
-hipError_t launch_persistent(int ns, dim3 g, dim3 b, hipStream_t s, const Params &P);
-hipError_t launch_source(dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B, int32_t *live, int64_t n);
-hipError_t launch_collide(dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B, const int32_t *coll_q,
-                          int qcoll, int32_t *live_out, int qout);
+hipError_t launch_source_bank(hipStream_t s, const Params &P, const SourceBank &S, int64_t n);
+hipError_t launch_bank_init(hipStream_t s, const Params &P, const SourceBank &S, const BankView &B, int64_t n, int q);
+hipError_t launch_segment(int ns, unsigned blocks, hipStream_t s, const Params &P, const BankView &in, int qin,
+                          const BankView &out, int qout, const BankView &coll, int qcoll);
+hipError_t launch_collide(unsigned blocks, hipStream_t s, const Params &P, const BankView &coll, int qcoll,
+                          const BankView &out, int qout);
+hipError_t launch_persistent(int ns, unsigned blocks, int block, hipStream_t s, const Params &P, const SourceBank &S);
+const void *persistent_kernel(int ns);
 hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1);
 hipError_t launch_reset_work(hipStream_t s, Counters *C);
 hipError_t launch_fill(hipStream_t s, double *p, int64_t n, double v);
@@ -35,6 +37,8 @@ hipError_t launch_zero_counters(hipStream_t s, Counters *C);
 hipError_t launch_accumulate(hipStream_t s, double *value, double *sum, double *sum_sq, int64_t n, double norm);
 hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *lo, double *so, double *co,
                              double *sq, const uint64_t *seeds, double *po, uint64_t *seo);
+hipError_t launch_xs_probe(hipStream_t s, const XSView &X, int32_t nuc, const double *E, int64_t n, int32_t *k,
+                           double *xs);
 } // namespace mcw
 
 using namespace mcw;
@@ -133,12 +137,13 @@ struct mcw_ctx {
 
   // device problem
   DevBuf plane, nbr, elem_mat, face_off, n_sides, bc_kind, cdf, xyz, conn, conn_off, type;
-  DevBuf grid_off, npts, energy, values, awr, mat_off, mat_nuc, mat_dens;
+  DevBuf grid_off, npts, rec, fission, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
   Counters *h_counters = nullptr; // pinned
-  // bank (plain) and records
-  DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_sigt, b_seed, b_elem, b_inface, b_mat, b_hid, b_digest, b_nseg,
-      b_ncoll, q_live0, q_live1, q_coll;
+  // source bank, live/collision banks of the event-based path, records
+  DevBuf sb[9];
+  int64_t sb_cap = 0;
+  DevBuf bank[3][17];
   int64_t bank_cap = 0;
   DevBuf r_digest, r_seed, r_E, r_x, r_y, r_z, r_nseg, r_ncoll, r_elem, r_fate;
   int64_t rec_n = 0;
@@ -236,29 +241,61 @@ void prepare(mcw_ctx *c)
     c->n_sides.upload(ns, s);
   }
 
-  // nuclear data
+  // nuclear data: per-point records (E, total, absorption, scatter), fission apart
   std::vector<int64_t> goff;
   std::vector<int32_t> np;
-  std::vector<double> en, aw;
-  std::vector<double4> vals;
+  std::vector<double> aw, fis;
+  std::vector<double4> recs;
+  bool any_fission = false;
+  for (auto &n : c->nucs) any_fission |= !n.f.empty();
   c->e_min = INFINITY;
   c->e_max = -INFINITY;
+  std::vector<double> uni;
   for (auto &n : c->nucs) {
-    goff.push_back((int64_t)en.size());
+    goff.push_back((int64_t)recs.size());
     np.push_back((int32_t)n.E.size());
     aw.push_back(n.awr);
     for (size_t j = 0; j < n.E.size(); ++j) {
-      en.push_back(n.E[j]);
-      vals.push_back(make_double4(n.t[j], n.a[j], n.s[j], n.f.empty() ? 0.0 : n.f[j]));
+      recs.push_back(make_double4(n.E[j], n.t[j], n.a[j], n.s[j]));
+      if (any_fission) fis.push_back(n.f.empty() ? 0.0 : n.f[j]);
     }
+    uni.insert(uni.end(), n.E.begin(), n.E.end());
     c->e_min = std::min(c->e_min, n.E.front());
     c->e_max = std::max(c->e_max, n.E.back());
   }
   c->grid_off.upload(goff, s);
   c->npts.upload(np, s);
-  c->energy.upload(en, s);
-  c->values.upload(vals, s);
+  c->rec.upload(recs, s);
+  if (any_fission) c->fission.upload(fis, s);
   c->awr.upload(aw, s);
+  // unionized grid, per-nuclide index map and log-bucket table
+  std::sort(uni.begin(), uni.end());
+  uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+  const int64_t nu = (int64_t)uni.size();
+  const int32_t nn = (int32_t)c->nucs.size();
+  require(nu < ((int64_t)1 << 31) / std::max(1, nn), "unionized grid too large");
+  std::vector<int32_t> uidx((size_t)nu * nn);
+  for (int32_t q = 0; q < nn; ++q) {
+    const std::vector<double> &E = c->nucs[q].E;
+    int64_t j = -1;
+    for (int64_t k = 0; k < nu; ++k) {
+      while (j + 1 < (int64_t)E.size() && E[j + 1] <= uni[k]) ++j;
+      uidx[(size_t)k * nn + q] = (int32_t)j;
+    }
+  }
+  const int32_t nh = MAX_HASH;
+  const double logU0 = std::log(uni.front());
+  const double dl = (std::log(uni.back()) - logU0) / nh;
+  std::vector<int32_t> hash(nh + 1);
+  for (int32_t b = 0; b < nh; ++b) {
+    const double Eb = std::exp(logU0 + b * dl);
+    int64_t k = (int64_t)(std::upper_bound(uni.begin(), uni.end(), Eb) - uni.begin()) - 1;
+    hash[b] = (int32_t)std::max<int64_t>(0, k);
+  }
+  hash[nh] = (int32_t)(nu - 1);
+  c->uni.upload(uni, s);
+  c->uidx.upload(uidx, s);
+  c->uhash.upload(hash, s);
   std::vector<int32_t> moff{0}, mnuc;
   std::vector<double> mden;
   for (auto &mt : c->mats) {
@@ -308,9 +345,16 @@ void prepare(mcw_ctx *c)
   P.mesh.orig_id = nullptr;
   P.xs.grid_off = c->grid_off.as<int64_t>();
   P.xs.n_points = c->npts.as<int32_t>();
-  P.xs.energy = c->energy.as<double>();
-  P.xs.values = c->values.as<double4>();
+  P.xs.rec = c->rec.as<double4>();
+  P.xs.fission = any_fission ? c->fission.as<double>() : nullptr;
   P.xs.awr = c->awr.as<double>();
+  P.xs.U = c->uni.as<double>();
+  P.xs.uidx = c->uidx.as<int32_t>();
+  P.xs.hash = c->uhash.as<int32_t>();
+  P.xs.n_union = (int32_t)nu;
+  P.xs.n_hash = nh;
+  P.xs.logU0 = logU0;
+  P.xs.inv_dl = 1.0 / dl;
   P.xs.mat_off = c->mat_off.as<int32_t>();
   P.xs.mat_nuc = c->mat_nuc.as<int32_t>();
   P.xs.mat_dens = c->mat_dens.as<double>();
@@ -364,6 +408,7 @@ void prepare(mcw_ctx *c)
   }
   P.master_seed = c->opt.master_seed ? c->opt.master_seed : 1;
   P.roundtrip = c->opt.seed_roundtrip;
+  P.lanes_min = c->opt.lanes_min > 0 ? std::min(c->opt.lanes_min, 64) : 40;
   P.max_seg = c->opt.max_segments > 0 ? c->opt.max_segments : ((int64_t)1 << 20);
   P.cnt = c->counters.as<Counters>();
   HIPCHK(hipStreamSynchronize(s));
@@ -457,16 +502,28 @@ void build_tally(mcw_ctx *c, HostTally &t)
   HIPCHK(launch_fill(c->stream, t.sum_sq.as<double>(), (int64_t)n, 0.0));
 }
 
+// Three structure-of-arrays banks of the event-based path: live (in),
+// live (out) and collision.  17 arrays each, see BankView.
 void ensure_bank(mcw_ctx *c, int64_t n)
 {
   if (c->bank_cap >= n) return;
-  size_t d = (size_t)n * 8, i = (size_t)n * 4;
-  c->b_x.alloc(d); c->b_y.alloc(d); c->b_z.alloc(d);
-  c->b_u.alloc(d); c->b_v.alloc(d); c->b_w.alloc(d);
-  c->b_E.alloc(d); c->b_sigt.alloc(d); c->b_seed.alloc(d); c->b_hid.alloc(d); c->b_digest.alloc(d);
-  c->b_elem.alloc(i); c->b_inface.alloc(i); c->b_mat.alloc(i); c->b_nseg.alloc(i); c->b_ncoll.alloc(i);
-  c->q_live0.alloc(i); c->q_live1.alloc(i); c->q_coll.alloc(i);
+  const size_t d = (size_t)n * 8, i = (size_t)n * 4;
+  for (int b = 0; b < 3; ++b)
+    for (int k = 0; k < 17; ++k) c->bank[b][k].alloc(k < 11 ? d : i);
   c->bank_cap = n;
+}
+
+BankView bank_view(mcw_ctx *c, int b)
+{
+  DevBuf *a = c->bank[b];
+  BankView B;
+  B.x = a[0].as<double>(); B.y = a[1].as<double>(); B.z = a[2].as<double>();
+  B.u = a[3].as<double>(); B.v = a[4].as<double>(); B.w = a[5].as<double>();
+  B.E = a[6].as<double>(); B.sigt = a[7].as<double>();
+  B.seed = a[8].as<uint64_t>(); B.hid = a[9].as<int64_t>(); B.digest = a[10].as<uint64_t>();
+  B.elem = a[11].as<int32_t>(); B.inface = a[12].as<int32_t>(); B.mat = a[13].as<int32_t>();
+  B.k = a[14].as<int32_t>(); B.nseg = a[15].as<int32_t>(); B.ncoll = a[16].as<int32_t>();
+  return B;
 }
 
 void ensure_records(mcw_ctx *c, int64_t n)
@@ -478,40 +535,40 @@ void ensure_records(mcw_ctx *c, int64_t n)
   c->rec_n = n;
 }
 
-BankView bank_view(mcw_ctx *c)
+SourceBank source_bank(mcw_ctx *c, int64_t n)
 {
-  BankView B;
-  B.x = c->b_x.as<double>(); B.y = c->b_y.as<double>(); B.z = c->b_z.as<double>();
-  B.u = c->b_u.as<double>(); B.v = c->b_v.as<double>(); B.w = c->b_w.as<double>();
-  B.E = c->b_E.as<double>(); B.sigt = c->b_sigt.as<double>();
-  B.seed = c->b_seed.as<uint64_t>();
-  B.elem = c->b_elem.as<int32_t>(); B.inface = c->b_inface.as<int32_t>(); B.mat = c->b_mat.as<int32_t>();
-  B.hid = c->b_hid.as<int64_t>();
-  B.digest = c->b_digest.as<uint64_t>();
-  B.nseg = c->b_nseg.as<int32_t>(); B.ncoll = c->b_ncoll.as<int32_t>();
-  return B;
+  if (c->sb_cap < n) {
+    for (int k = 0; k < 9; ++k) c->sb[k].alloc((size_t)n * (k < 8 ? 8 : 4));
+    c->sb_cap = n;
+  }
+  SourceBank S;
+  S.x = c->sb[0].as<double>(); S.y = c->sb[1].as<double>(); S.z = c->sb[2].as<double>();
+  S.u = c->sb[3].as<double>(); S.v = c->sb[4].as<double>(); S.w = c->sb[5].as<double>();
+  S.E = c->sb[6].as<double>(); S.seed = c->sb[7].as<uint64_t>(); S.elem = c->sb[8].as<int32_t>();
+  return S;
 }
 
+// Resident workgroups of the persistent kernel over the whole device.
 int persistent_blocks(mcw_ctx *c, int ns, int block)
 {
-  (void)ns;
-  // A wavefront's register file is the limit; 8 workgroups of 256 per CU keep
-  // 32 wavefronts resident when the kernel fits, fewer otherwise (the
-  // hardware queues the rest).
-  return c->n_cu * 8 * 256 / block;
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns), block, 0));
+  if (per_cu < 1) per_cu = 1;
+  return c->n_cu * per_cu;
 }
 
-void run_plain(mcw_ctx *c, Params &P, int64_t n)
+// Event-based transport of histories hist_begin .. hist_end (sites already
+// in the source bank): segment and collision kernels alternate until no
+// particle is alive.  The live count is read back every 8 cycles to size
+// the grids; kernels read the exact count from device memory.
+void run_plain(mcw_ctx *c, Params &P, const SourceBank &S, int64_t n)
 {
   hipStream_t s = c->stream;
   const int ns = c->mesh.uniform_sides;
   const int block = 256;
   ensure_bank(c, n);
-  BankView B = bank_view(c);
-  int32_t *q[2] = {c->q_live0.as<int32_t>(), c->q_live1.as<int32_t>()};
-  int32_t *qc = c->q_coll.as<int32_t>();
-  HIPCHK(launch_reset_queue(s, P.cnt, 1, 2));
-  HIPCHK(launch_source(dim3((unsigned)((n + block - 1) / block)), dim3(block), s, P, B, q[0], n));
+  BankView B[3] = {bank_view(c, 0), bank_view(c, 1), bank_view(c, 2)};
+  HIPCHK(launch_bank_init(s, P, S, B[0], n, 0));
   const int64_t max_blocks = (int64_t)c->n_cu * 8;
   int64_t bound = n;
   int cur = 0;
@@ -520,8 +577,8 @@ void run_plain(mcw_ctx *c, Params &P, int64_t n)
     int64_t blocks = std::min<int64_t>((bound + block - 1) / block, max_blocks);
     if (blocks < 1) blocks = 1;
     HIPCHK(launch_reset_queue(s, P.cnt, nxt, 2));
-    HIPCHK(launch_segment(ns, dim3((unsigned)blocks), dim3(block), s, P, B, q[cur], cur, q[nxt], nxt, qc, 2));
-    HIPCHK(launch_collide(dim3((unsigned)blocks), dim3(block), s, P, B, qc, 2, q[nxt], nxt));
+    HIPCHK(launch_segment(ns, (unsigned)blocks, s, P, B[cur], cur, B[nxt], nxt, B[2], 2));
+    HIPCHK(launch_collide((unsigned)blocks, s, P, B[2], 2, B[nxt], nxt));
     cur = nxt;
     if ((it & 7) == 7) {
       HIPCHK(hipMemcpyAsync(&c->h_counters->q_count[cur], &P.cnt->q_count[cur], 8, hipMemcpyDeviceToHost, s));
@@ -635,6 +692,7 @@ int mcw_xs_set(mcw_ctx *c, int32_t n, const mcw_nuclide *nuc)
       HostNuclide h;
       h.awr = q.awr;
       h.E.assign(q.energy, q.energy + q.n_points);
+      require(h.E[0] > 0.0, "nuclide " + std::to_string(i) + ": energies must be positive");
       for (int j = 1; j < q.n_points; ++j)
         require(h.E[j] > h.E[j - 1], "nuclide " + std::to_string(i) + ": energy grid must increase");
       h.t.assign(q.total, q.total + q.n_points);
@@ -774,7 +832,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
     }
     for (auto &t : c->tallies) HIPCHK(launch_fill(s, t->value.as<double>(), t->n_filter_bins * t->n_score_bins, 0.0));
     HIPCHK(launch_zero_counters(s, P.cnt));
-    const int64_t cap_default = c->opt.mode == MCW_MODE_PLAIN ? ((int64_t)1 << 24) : ((int64_t)1 << 30);
+    const int64_t cap_default = (int64_t)1 << 24;
     const int64_t cap = c->opt.bank_capacity > 0 ? c->opt.bank_capacity : cap_default;
     const int ns = c->mesh.uniform_sides;
     float kernel_ms = 0.0f;
@@ -784,17 +842,19 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       const int64_t h1 = std::min(n_particles, h0 + cap);
       P.hist_begin = h0;
       P.hist_end = h1;
+      const SourceBank S = source_bank(c, h1 - h0);
+      HIPCHK(launch_source_bank(s, P, S, h1 - h0));
       if (c->opt.mode == MCW_MODE_PLAIN) {
-        run_plain(c, P, h1 - h0);
+        run_plain(c, P, S, h1 - h0);
       } else {
         const int block = c->opt.block_size > 0 ? c->opt.block_size : 256;
-        require(block % 64 == 0 && block <= 1024, "block_size must be a multiple of 64, at most 1024");
+        require(block % 64 == 0 && block <= 256, "block_size must be 64, 128, 192 or 256");
         int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, block), (h1 - h0 + block - 1) / block);
         if (blocks < 1) blocks = 1;
         // the work counter restarts for every chunk
         HIPCHK(launch_reset_work(s, P.cnt));
         HIPCHK(hipEventRecord(c->ev[2], s));
-        HIPCHK(launch_persistent(ns, dim3((unsigned)blocks), dim3(block), s, P));
+        HIPCHK(launch_persistent(ns, (unsigned)blocks, block, s, P, S));
         HIPCHK(hipEventRecord(c->ev[3], s));
         HIPCHK(hipEventSynchronize(c->ev[3]));
         float ms = 0.0f;
@@ -909,6 +969,24 @@ int mcw_debug_math_probe(mcw_ctx *c, const double *x, int64_t n, double *log_out
     HIPCHK(hipMemcpyAsync(sqrt_out, dq.p, b, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(prn_out, dp.p, b, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(seed_out, dso.p, b, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  });
+}
+
+int mcw_debug_xs_probe(mcw_ctx *c, int32_t nuc, const double *E, int64_t n, int32_t *k_out, double *xs_out)
+{
+  return guard(c, [&] {
+    prepare(c);
+    require(nuc >= 0 && nuc < (int32_t)c->nucs.size(), "no such nuclide");
+    hipStream_t s = c->stream;
+    DevBuf dE, dk, dx;
+    dE.alloc((size_t)n * 8);
+    dk.alloc((size_t)n * 4);
+    dx.alloc((size_t)n * 32);
+    HIPCHK(hipMemcpyAsync(dE.p, E, (size_t)n * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_xs_probe(s, c->base.xs, nuc, dE.as<double>(), n, dk.as<int32_t>(), dx.as<double>()));
+    HIPCHK(hipMemcpyAsync(k_out, dk.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(xs_out, dx.p, (size_t)n * 32, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
   });
 }

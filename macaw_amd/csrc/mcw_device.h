@@ -200,54 +200,76 @@ __device__ __forceinline__ uint64_t digest_step(uint64_t h, int64_t elem, int ev
 }
 
 // ------------------------------------------------------------------ cross sections
+//
+// Pointwise nuclides on a unionized grid U (all nuclides' points merged).
+// union_index finds k = largest index with U[k] <= E: a log-spaced bucket
+// table (staged in LDS by the kernels) narrows the range to a few entries,
+// then a binary search in U finishes.  For nuclide n, uidx[k][n] is then the
+// bracketing point of its own grid (no point of any grid lies strictly
+// between U[k] and U[k+1]), so the interpolation equals OpenMC's per-nuclide
+// lookup: largest j with E_j <= E, clamped to [0, n-2], lin-lin.
 
 struct Micro {
   double t, a, s, f;
 };
 
-// Largest j with E_j <= E, clamped to [0, n-2], then lin-lin interpolation
-// (OpenMC Nuclide::calculate_xs on a pointwise grid).  lo/hi give an initial
-// bracket E_lo <= E < E_hi (0, n-1 when unknown).
-__device__ __forceinline__ void micro_xs(const XSView &X, int nuc, double E, Micro &m)
+// -1 when E < U[0].
+__device__ __forceinline__ int32_t union_index(const XSView &X, const int32_t *hash, double E)
+{
+  const int32_t n = X.n_union;
+  const double *__restrict__ U = X.U;
+  if (E < U[0]) return -1;
+  if (E >= U[n - 1]) return n - 1;
+  double pos = (log_d(E) - X.logU0) * X.inv_dl;
+  int32_t b = pos > 0.0 ? (int32_t)pos : 0;
+  if (b > X.n_hash - 1) b = X.n_hash - 1;
+  const int32_t b0 = b > 0 ? b - 1 : 0;
+  const int32_t b1 = b + 2 < X.n_hash ? b + 2 : X.n_hash;
+  int32_t lo = hash[b0];
+  int32_t hi = hash[b1] + 1;
+  if (hi > n - 1) hi = n - 1;
+  // U[lo] <= E < U[hi]
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (U[mid] <= E) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void micro_at(const XSView &X, int32_t nuc, int32_t k, double E, Micro &m)
 {
   const int64_t base = X.grid_off[nuc];
   const int32_t np = X.n_points[nuc];
-  const double *__restrict__ Eg = X.energy + base;
-  int64_t j;
+  int32_t j = k < 0 ? -1 : X.uidx[(int64_t)k * X.n_nuc + nuc];
   double f;
-  if (!(E > Eg[0])) {
+  if (j < 0) {
     j = 0;
     f = 0.0;
-  } else if (E >= Eg[np - 1]) {
+  } else if (j >= np - 1) {
     j = np - 2;
     f = 1.0;
   } else {
-    int64_t lo = 0, hi = np - 1;
-    while (hi - lo > 1) {
-      int64_t mid = lo + (hi - lo) / 2;
-      if (Eg[mid] <= E) lo = mid;
-      else hi = mid;
-    }
-    j = lo;
-    f = (E - Eg[j]) / (Eg[j + 1] - Eg[j]);
+    f = -1.0; // set below from the bracketing energies
   }
-  const double4 v0 = X.values[base + j];
-  const double4 v1 = X.values[base + j + 1];
-  double g = 1.0 - f;
-  m.t = g * v0.x + f * v1.x;
-  m.a = g * v0.y + f * v1.y;
-  m.s = g * v0.z + f * v1.z;
-  m.f = g * v0.w + f * v1.w;
+  const double4 r0 = X.rec[base + j];
+  const double4 r1 = X.rec[base + j + 1];
+  if (f < 0.0) f = (E - r0.x) / (r1.x - r0.x);
+  const double g = 1.0 - f;
+  m.t = g * r0.y + f * r1.y;
+  m.a = g * r0.z + f * r1.z;
+  m.s = g * r0.w + f * r1.w;
+  m.f = X.fission ? g * X.fission[base + j] + f * X.fission[base + j + 1] : 0.0;
 }
 
-__device__ __forceinline__ double macro_total(const XSView &X, int mat, double E)
+__device__ __forceinline__ double macro_total(const XSView &X, int mat, int32_t k, double E)
 {
   if (mat < 0) return 0.0;
   double t = 0.0;
   const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
   for (int32_t q = b; q < e; ++q) {
     Micro u;
-    micro_xs(X, X.mat_nuc[q], E, u);
+    micro_at(X, X.mat_nuc[q], k, E, u);
     t = t + X.mat_dens[q] * u.t;
   }
   return t;
@@ -257,14 +279,14 @@ struct Macro {
   double t, a, s, f;
 };
 
-__device__ __forceinline__ void macro_all(const XSView &X, int mat, double E, Macro &M)
+__device__ __forceinline__ void macro_all(const XSView &X, int mat, int32_t k, double E, Macro &M)
 {
   M.t = M.a = M.s = M.f = 0.0;
   if (mat < 0) return;
   const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
   for (int32_t q = b; q < e; ++q) {
     Micro u;
-    micro_xs(X, X.mat_nuc[q], E, u);
+    micro_at(X, X.mat_nuc[q], k, E, u);
     double dn = X.mat_dens[q];
     M.t = M.t + dn * u.t;
     M.a = M.a + dn * u.a;
@@ -344,7 +366,7 @@ __device__ __forceinline__ void rotate_angle(double &u, double &v, double &w, do
 // ------------------------------------------------------------------ tallies
 
 __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X, int64_t elem, double E,
-                                            double wfl, int mat)
+                                            int32_t k, double wfl, int mat)
 {
   int64_t fi;
   if (T.elem_base) {
@@ -376,12 +398,12 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
     Macro M = {0.0, 0.0, 0.0, 0.0};
     if (T.needs_xs) {
       if (nuc < 0) {
-        macro_all(X, mat, E, M);
+        macro_all(X, mat, k, E, M);
       } else if (mat >= 0) {
         for (int32_t q = X.mat_off[mat]; q < X.mat_off[mat + 1]; ++q)
           if (X.mat_nuc[q] == nuc) {
             Micro mu;
-            micro_xs(X, nuc, E, mu);
+            micro_at(X, nuc, k, E, mu);
             double dn = X.mat_dens[q];
             M.t = dn * mu.t;
             M.a = dn * mu.a;

@@ -1,20 +1,26 @@
 // mcw_kernels.hip -- gfx950 transport kernels.
 //
-// Two schedules over the same per-history arithmetic:
+// Source sites for a chunk of histories are sampled first (k_source_bank:
+// one thread per history, coalesced structure-of-arrays writes).  Two
+// schedules then transport them with the same per-history arithmetic:
 //
-//  PLAIN (event-based, host-driven): k_source fills a structure-of-arrays
-//  bank; each cycle runs k_segment over the live queue (one segment per
-//  particle: xs if the material or energy changed, exit side, distance draw,
-//  track-length tally, cross / reflect / leak) which appends colliding
-//  particles to a collision queue and the others to the next live queue, then
-//  k_collide (sample nuclide, absorb or scatter, collision tally) appends the
-//  survivors.  Queues are compacted with wavefront ballots: one atomic per
-//  wavefront per append.
+//  PLAIN (event-based, host-driven).  The live particles sit compacted in a
+//  structure-of-arrays bank.  Each cycle k_segment advances every live
+//  particle by one segment (exit side, distance draw, track-length tally,
+//  cross / reflect / leak) and appends it to the next live bank or to the
+//  collision bank; k_collide then collides the latter (sample nuclide,
+//  absorb or scatter, collision tally, new union-grid index) and appends the
+//  survivors to the next live bank.  Appends are wavefront-aggregated
+//  (ballot + prefix: one atomic per wavefront), so reads and writes stay
+//  coalesced.
 //
-//  BATCHED (persistent, fused): k_persistent keeps each particle in
-//  registers; every loop trip performs one segment (and its collision) for
-//  every lane, and a lane whose history ended takes the next history index
-//  from a wavefront-aggregated work counter and samples its source inline.
+//  BATCHED (persistent, fused).  k_persistent keeps particles in registers.
+//  A wavefront steps segments for its lanes while at least `lanes_min` of
+//  them are in flight; lanes that reach a collision or die wait, then the
+//  wavefront runs the collisions together and refills idle lanes together
+//  from the source bank.  Expensive events (collisions, refills) are thus
+//  paid once per batch of lanes instead of once per segment.  The unionized
+//  energy-grid bucket table is staged in LDS once per workgroup.
 //
 // Both reproduce, per history, the FP64 CPU reference to the bit.
 #include <hip/hip_runtime.h>
@@ -31,9 +37,9 @@ enum { FATE_ALIVE = 0, FATE_LEAKED = 1, FATE_ABSORBED = 2, FATE_CUTOFF = 3, FATE
 struct PState {
   double x, y, z, u, v, w, E, sigt;
   uint64_t seed;
-  int32_t elem, inface, mat;
   int64_t hid;
   uint64_t h;
+  int32_t elem, inface, mat, k;
   int32_t nseg, ncoll;
 };
 
@@ -78,6 +84,7 @@ __device__ void flush_counts(Counters *c, LocalCount &l)
 }
 
 // Wavefront-aggregated append: every lane of the wavefront must call it.
+// Returns this lane's slot (consecutive in lane order), or -1.
 __device__ __forceinline__ int64_t wave_append(unsigned long long *count, bool want)
 {
   const unsigned long long mask = __ballot(want);
@@ -90,6 +97,8 @@ __device__ __forceinline__ int64_t wave_append(unsigned long long *count, bool w
   const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
   return want ? (int64_t)(base + __popcll(below)) : -1;
 }
+
+__device__ __forceinline__ int wave_count(bool p) { return __popcll(__ballot(p)); }
 
 // ------------------------------------------------------------------ source
 
@@ -128,16 +137,17 @@ __device__ double tet_volume6_d(const double *a, const double *b, const double *
   return fabs(det);
 }
 
-// OpenMC fixed-source initialize_history: seeds for particle id
-// (batch-1)*n + i + 1, then space, angle, energy sampled on stream 1.
-__device__ void source_init(const Params &P, int64_t hid, PState &st)
+// OpenMC fixed-source initialize_history (OpenMCStudy.C:317-363 drives it):
+// seeds for particle id (batch-1)*n + i + 1, then space, angle, energy
+// sampled on stream 1 (sample_external_source).
+__device__ void sample_site(const Params &P, int64_t hid, double *p, double *dir, double &E, uint64_t &track_seed,
+                            int32_t &elem_out)
 {
   const int64_t id = (P.batch - 1) * P.n_particles + hid + 1;
-  st.seed = init_seed(id, 0, P.master_seed);
+  track_seed = init_seed(id, 0, P.master_seed);
   uint64_t ss = init_seed(id, 1, P.master_seed);
   const SourceView &S = P.src;
   const MeshView &M = P.mesh;
-  double p[3];
   int64_t elem;
   if (S.space == 0) {
     p[0] = S.point[0];
@@ -173,38 +183,39 @@ __device__ void source_init(const Params &P, int64_t hid, PState &st)
                           X + 3 * (int64_t)c[c_kuhn[k][2]], X + 3 * (int64_t)c[c_kuhn[k][3]], ss, p);
     }
   }
-  st.x = p[0];
-  st.y = p[1];
-  st.z = p[2];
   {
     double xi_phi = prn(ss);
     double mu = 2.0 * prn(ss) - 1.0;
     double s, c;
     sincos_2pi_d(xi_phi, &s, &c);
     double r = sqrt(1.0 - mu * mu);
-    st.u = mu;
-    st.v = r * c;
-    st.w = r * s;
+    dir[0] = mu;
+    dir[1] = r * c;
+    dir[2] = r * s;
   }
   if (S.energy == 0) {
-    st.E = S.E0;
+    E = S.E0;
   } else {
     const double a = S.watt_a, b = S.watt_b;
     for (int it = 0;; ++it) {
       double r1 = prn(ss), r2 = prn(ss), r3 = prn(ss);
       double c = cos_halfpi_d(r3);
       double wmax = -a * (log_d(r1) + log_d(r2) * c * c);
-      double E = wmax + 0.25 * a * a * b + (-1.0 + 2.0 * prn(ss)) * sqrt(a * a * b * wmax);
-      if (E < P.xs.e_max || it >= 1000) {
-        st.E = E < P.xs.e_max ? E : P.xs.e_min;
+      double Ew = wmax + 0.25 * a * a * b + (-1.0 + 2.0 * prn(ss)) * sqrt(a * a * b * wmax);
+      if (Ew < P.xs.e_max || it >= 1000) {
+        E = Ew < P.xs.e_max ? Ew : P.xs.e_min;
         break;
       }
     }
   }
-  if (P.roundtrip) st.seed = seed_roundtrip(st.seed);
-  st.elem = (int32_t)elem;
+  if (P.roundtrip) track_seed = seed_roundtrip(track_seed);
+  elem_out = (int32_t)elem;
+}
+
+__device__ __forceinline__ void start_history(PState &st, int64_t hid)
+{
   st.inface = -1;
-  st.mat = -2;
+  st.mat = -2; // forces Sigma_t at the first segment
   st.sigt = 0.0;
   st.hid = hid;
   st.h = 0xcbf29ce484222325ull;
@@ -232,7 +243,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
   }
   const int32_t mat = M.elem_mat[st.elem];
   if (mat != st.mat) {
-    st.sigt = macro_total(P.xs, mat, st.E);
+    st.sigt = macro_total(P.xs, mat, st.k, st.E);
     st.mat = mat;
   }
   double L;
@@ -256,7 +267,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
   ++st.nseg;
   ++lc.seg;
   lc.track += track;
-  for (int k = 0; k < P.n_tl; ++k) tally_score(P.tally[k], P.xs, st.elem, st.E, track, mat);
+  for (int t = 0; t < P.n_tl; ++t) tally_score(P.tally[t], P.xs, st.elem, st.E, st.k, track, mat);
   if (d > L) {
     st.x = st.x + L * st.u;
     st.y = st.y + L * st.v;
@@ -301,20 +312,22 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
 // Collision at the current point: collision-estimator tallies, sample the
 // nuclide (OpenMC sample_nuclide), absorb (analog) or scatter elastically
 // (isotropic in CM, target at rest) with an energy cutoff at the grid minimum.
-__device__ __forceinline__ int collide(const Params &P, PState &st, LocalCount &lc)
+// A scatter looks up the new energy's union-grid index through `hash`.
+__device__ __forceinline__ int collide(const Params &P, PState &st, LocalCount &lc, const int32_t *hash)
 {
   const XSView &X = P.xs;
   const int32_t mat = st.mat;
   ++st.ncoll;
   ++lc.coll;
-  for (int k = P.n_tl; k < P.n_tl + P.n_coll; ++k) tally_score(P.tally[k], X, st.elem, st.E, 1.0 / st.sigt, mat);
+  for (int t = P.n_tl; t < P.n_tl + P.n_coll; ++t)
+    tally_score(P.tally[t], X, st.elem, st.E, st.k, 1.0 / st.sigt, mat);
   const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
   const double cutoff = prn(st.seed) * st.sigt;
   double prob = 0.0;
   int32_t pick = e - 1;
   Micro mu;
   for (int32_t q = b; q < e; ++q) {
-    micro_xs(X, X.mat_nuc[q], st.E, mu);
+    micro_at(X, X.mat_nuc[q], st.k, st.E, mu);
     prob = prob + X.mat_dens[q] * mu.t;
     if (prob >= cutoff) {
       pick = q;
@@ -322,7 +335,6 @@ __device__ __forceinline__ int collide(const Params &P, PState &st, LocalCount &
     }
   }
   const int32_t nuc = X.mat_nuc[pick];
-  micro_xs(X, nuc, st.E, mu);
   int ev;
   if (mu.a > prn(st.seed) * mu.t) {
     ev = EV_ABSORB;
@@ -341,6 +353,7 @@ __device__ __forceinline__ int collide(const Params &P, PState &st, LocalCount &
       const double xi_phi = prn(st.seed);
       rotate_angle(st.u, st.v, st.w, mu_lab, xi_phi);
       st.E = Ep;
+      st.k = union_index(X, hash, Ep);
       st.mat = -2; // energy changed: recompute Sigma_t
       ev = EV_SCATTER;
     }
@@ -362,6 +375,11 @@ __device__ __forceinline__ int fate_of(int ev)
   }
 }
 
+__device__ __forceinline__ bool is_death(int ev)
+{
+  return ev == EV_LEAK || ev == EV_LOST || ev == EV_ABSORB || ev == EV_CUTOFF;
+}
+
 __device__ void write_record(const Params &P, const PState &st, int fate)
 {
   const int64_t i = st.hid;
@@ -378,33 +396,34 @@ __device__ void write_record(const Params &P, const PState &st, int fate)
   R.fate[i] = fate;
 }
 
-// ------------------------------------------------------------------ bank I/O (plain)
+// ------------------------------------------------------------------ bank I/O
 
-__device__ __forceinline__ void load(const BankView &B, int32_t s, PState &st, bool record)
+__device__ __forceinline__ void load(const BankView &B, int64_t s, PState &st, bool record)
 {
   st.x = B.x[s]; st.y = B.y[s]; st.z = B.z[s];
   st.u = B.u[s]; st.v = B.v[s]; st.w = B.w[s];
   st.E = B.E[s]; st.sigt = B.sigt[s];
   st.seed = B.seed[s];
-  st.elem = B.elem[s]; st.inface = B.inface[s]; st.mat = B.mat[s];
   st.hid = B.hid[s];
+  st.elem = B.elem[s]; st.inface = B.inface[s]; st.mat = B.mat[s]; st.k = B.k[s];
+  st.nseg = B.nseg[s];
   if (record) {
     st.h = B.digest[s];
-    st.nseg = B.nseg[s];
     st.ncoll = B.ncoll[s];
   } else {
-    st.nseg = B.nseg[s];
+    st.h = 0;
     st.ncoll = 0;
   }
 }
 
-__device__ __forceinline__ void store(const BankView &B, int32_t s, const PState &st, bool record)
+__device__ __forceinline__ void store(const BankView &B, int64_t s, const PState &st, bool record)
 {
   B.x[s] = st.x; B.y[s] = st.y; B.z[s] = st.z;
   B.u[s] = st.u; B.v[s] = st.v; B.w[s] = st.w;
   B.E[s] = st.E; B.sigt[s] = st.sigt;
   B.seed[s] = st.seed;
-  B.elem[s] = st.elem; B.inface[s] = st.inface; B.mat[s] = st.mat;
+  B.hid[s] = st.hid;
+  B.elem[s] = st.elem; B.inface[s] = st.inface; B.mat[s] = st.mat; B.k[s] = st.k;
   B.nseg[s] = st.nseg;
   if (record) {
     B.digest[s] = st.h;
@@ -412,29 +431,63 @@ __device__ __forceinline__ void store(const BankView &B, int32_t s, const PState
   }
 }
 
+__device__ __forceinline__ void load_site(const SourceBank &S, int64_t i, PState &st)
+{
+  st.x = S.x[i]; st.y = S.y[i]; st.z = S.z[i];
+  st.u = S.u[i]; st.v = S.v[i]; st.w = S.w[i];
+  st.E = S.E[i];
+  st.seed = S.seed[i];
+  st.elem = S.elem[i];
+}
+
+__device__ __forceinline__ void stage_hash(const XSView &X, int32_t *s_hash)
+{
+  for (int i = threadIdx.x; i <= X.n_hash; i += blockDim.x) s_hash[i] = X.hash[i];
+  __syncthreads();
+}
+
 } // namespace mcw
 
 using namespace mcw;
 
-// ------------------------------------------------------------------ PLAIN kernels
+// ------------------------------------------------------------------ source bank
 
-// Bank slot s <- history hist_begin + s; live queue = all slots.
-extern "C" __global__ void __launch_bounds__(256) k_source(Params P, BankView B, int32_t *live, int64_t n)
+// Sites of histories hist_begin .. hist_begin + n - 1 into slots 0 .. n-1.
+extern "C" __global__ void __launch_bounds__(256) k_source_bank(Params P, SourceBank S, int64_t n)
 {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) P.cnt->q_count[0] = (unsigned long long)n;
-  if (s >= n) return;
-  PState st;
-  source_init(P, P.hist_begin + s, st);
-  store(B, (int32_t)s, st, P.record != 0);
-  B.hid[s] = st.hid;
-  live[s] = (int32_t)s;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[3], d[3], E;
+  uint64_t seed;
+  int32_t elem;
+  sample_site(P, P.hist_begin + i, p, d, E, seed, elem);
+  S.x[i] = p[0]; S.y[i] = p[1]; S.z[i] = p[2];
+  S.u[i] = d[0]; S.v[i] = d[1]; S.w[i] = d[2];
+  S.E[i] = E;
+  S.seed[i] = seed;
+  S.elem[i] = elem;
 }
 
-// One segment for every particle of the live queue.
+// ------------------------------------------------------------------ PLAIN kernels
+
+// Live bank <- source bank (slot i = history hist_begin + i).
+extern "C" __global__ void __launch_bounds__(256) k_bank_init(Params P, SourceBank S, BankView B, int64_t n,
+                                                              int q)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) P.cnt->q_count[q] = (unsigned long long)n;
+  if (i >= n) return;
+  PState st;
+  load_site(S, i, st);
+  start_history(st, P.hist_begin + i);
+  st.k = union_index(P.xs, P.xs.hash, st.E);
+  store(B, i, st, P.record != 0);
+}
+
+// One segment for every particle of the live bank `in` (q_count[qin] of them).
 template <int NS>
-__global__ void __launch_bounds__(256) k_segment(Params P, BankView B, const int32_t *live_in, int qin,
-                                                 int32_t *live_out, int qout, int32_t *coll_q, int qcoll)
+__global__ void __launch_bounds__(256) k_segment(Params P, BankView in, int qin, BankView out, int qout,
+                                                 BankView coll, int qcoll)
 {
   Counters *C = P.cnt;
   const int64_t n = (int64_t)C->q_count[qin];
@@ -446,30 +499,23 @@ __global__ void __launch_bounds__(256) k_segment(Params P, BankView B, const int
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n;
     int ev = -1;
-    int32_t s = 0;
     PState st;
     if (valid) {
-      s = live_in[i];
-      load(B, s, st, record);
+      load(in, i, st, record);
       ev = segment<NS>(P, st, lc);
-      if (ev != EV_LOST && ev != EV_LEAK) store(B, s, st, record);
-      else {
-        B.nseg[s] = st.nseg;
-        if (record) write_record(P, st, fate_of(ev));
-      }
+      if (is_death(ev) && record) write_record(P, st, fate_of(ev));
     }
     const bool to_coll = valid && ev == EV_COLLIDE;
     const bool to_live = valid && (ev == EV_CROSS || ev == EV_REFLECT);
-    int64_t pc = wave_append(&C->q_count[qcoll], to_coll);
-    if (to_coll) coll_q[pc] = s;
-    int64_t pl = wave_append(&C->q_count[qout], to_live);
-    if (to_live) live_out[pl] = s;
+    const int64_t pc = wave_append(&C->q_count[qcoll], to_coll);
+    const int64_t pl = wave_append(&C->q_count[qout], to_live);
+    if (to_coll) store(coll, pc, st, record);
+    if (to_live) store(out, pl, st, record);
   }
   flush_counts(C, lc);
 }
 
-extern "C" __global__ void __launch_bounds__(256) k_collide(Params P, BankView B, const int32_t *coll_q, int qcoll,
-                                                           int32_t *live_out, int qout)
+extern "C" __global__ void __launch_bounds__(256) k_collide(Params P, BankView coll, int qcoll, BankView out, int qout)
 {
   Counters *C = P.cnt;
   const int64_t n = (int64_t)C->q_count[qcoll];
@@ -481,18 +527,15 @@ extern "C" __global__ void __launch_bounds__(256) k_collide(Params P, BankView B
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n;
     int ev = -1;
-    int32_t s = 0;
+    PState st;
     if (valid) {
-      s = coll_q[i];
-      PState st;
-      load(B, s, st, record);
-      ev = collide(P, st, lc);
-      if (ev == EV_SCATTER) store(B, s, st, record);
-      else if (record) write_record(P, st, fate_of(ev));
+      load(coll, i, st, record);
+      ev = collide(P, st, lc, P.xs.hash);
+      if (ev != EV_SCATTER && record) write_record(P, st, fate_of(ev));
     }
     const bool to_live = valid && ev == EV_SCATTER;
-    int64_t pl = wave_append(&C->q_count[qout], to_live);
-    if (to_live) live_out[pl] = s;
+    const int64_t pl = wave_append(&C->q_count[qout], to_live);
+    if (to_live) store(out, pl, st, record);
   }
   flush_counts(C, lc);
 }
@@ -512,36 +555,58 @@ extern "C" __global__ void k_reset_queue(Counters *C, int q0, int q1)
 
 // ------------------------------------------------------------------ BATCHED kernel
 
+enum { PH_IDLE = 0, PH_TRANSPORT = 1, PH_COLLIDE = 2 };
+
 template <int NS>
-__global__ void __launch_bounds__(256) k_persistent(Params P)
+__global__ void __launch_bounds__(256) k_persistent(Params P, SourceBank S)
 {
+  __shared__ int32_t s_hash[MAX_HASH + 1];
+  stage_hash(P.xs, s_hash);
   Counters *C = P.cnt;
   LocalCount lc;
   lc.zero();
   PState st;
-  bool active = false;
+  st.elem = 0;
+  int phase = PH_IDLE;
   bool exhausted = false;
   const bool record = P.record != 0;
+  const int64_t n_hist = P.hist_end - P.hist_begin;
+  const int lanes_min = P.lanes_min < 1 ? 1 : (P.lanes_min > 64 ? 64 : P.lanes_min);
   for (;;) {
-    // refill idle lanes from the work counter (one atomic per wavefront)
-    const bool need = !active && !exhausted;
-    int64_t slot = wave_append(&C->next_hist, need);
+    // refill idle lanes together (one atomic per wavefront)
+    const bool need = phase == PH_IDLE && !exhausted;
+    const int64_t slot = wave_append(&C->next_hist, need);
     if (need) {
-      const int64_t hid = P.hist_begin + slot;
-      if (hid < P.hist_end) {
-        source_init(P, hid, st);
-        active = true;
+      if (slot < n_hist) {
+        load_site(S, slot, st);
+        start_history(st, P.hist_begin + slot);
+        st.k = union_index(P.xs, s_hash, st.E);
+        phase = PH_TRANSPORT;
       } else {
         exhausted = true;
       }
     }
-    if (__ballot(active) == 0) break;
-    if (active) {
-      int ev = segment<NS>(P, st, lc);
-      if (ev == EV_COLLIDE) ev = collide(P, st, lc);
-      if (ev == EV_LEAK || ev == EV_LOST || ev == EV_ABSORB || ev == EV_CUTOFF) {
+    if (__ballot(phase != PH_IDLE) == 0) break;
+    // segments while enough lanes are in flight
+    do {
+      if (phase == PH_TRANSPORT) {
+        const int ev = segment<NS>(P, st, lc);
+        if (ev == EV_COLLIDE) {
+          phase = PH_COLLIDE;
+        } else if (is_death(ev)) {
+          if (record) write_record(P, st, fate_of(ev));
+          phase = PH_IDLE;
+        }
+      }
+    } while (wave_count(phase == PH_TRANSPORT) >= lanes_min);
+    // collisions together
+    if (phase == PH_COLLIDE) {
+      const int ev = collide(P, st, lc, s_hash);
+      if (ev == EV_SCATTER) {
+        phase = PH_TRANSPORT;
+      } else {
         if (record) write_record(P, st, fate_of(ev));
-        active = false;
+        phase = PH_IDLE;
       }
     }
   }
@@ -563,7 +628,7 @@ extern "C" __global__ void k_zero_counters(Counters *C)
   if (threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
-// VALUE -> SUM += VALUE/norm, SUM_SQ += (VALUE/norm)^2, VALUE = 0
+// VALUE -> SUM += VALUE/norm, SUM_SQ += (VALUE/norm)^2
 extern "C" __global__ void k_accumulate(double *value, double *sum, double *sum_sq, int64_t n, double norm)
 {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -592,44 +657,80 @@ extern "C" __global__ void k_math_probe(const double *x, int64_t n, double *log_
   seed_out[i] = future_seed((uint64_t)i * PRN_STRIDE + 7, seeds[i]);
 }
 
-// explicit instantiations, launched from the host runtime
-template __global__ void k_segment<4>(Params, BankView, const int32_t *, int, int32_t *, int, int32_t *, int);
-template __global__ void k_segment<6>(Params, BankView, const int32_t *, int, int32_t *, int, int32_t *, int);
-template __global__ void k_segment<0>(Params, BankView, const int32_t *, int, int32_t *, int, int32_t *, int);
-template __global__ void k_persistent<4>(Params);
-template __global__ void k_persistent<6>(Params);
-template __global__ void k_persistent<0>(Params);
+// Union-grid lookup on the device, for tests: k and (total, absorption,
+// scatter, fission) of nuclide `nuc` at each energy.
+extern "C" __global__ void k_xs_probe(XSView X, int32_t nuc, const double *E, int64_t n, int32_t *k_out,
+                                      double *xs_out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int32_t k = union_index(X, X.hash, E[i]);
+  Micro m;
+  micro_at(X, nuc, k, E[i], m);
+  k_out[i] = k;
+  xs_out[4 * i] = m.t;
+  xs_out[4 * i + 1] = m.a;
+  xs_out[4 * i + 2] = m.s;
+  xs_out[4 * i + 3] = m.f;
+}
+
+template __global__ void k_segment<4>(Params, BankView, int, BankView, int, BankView, int);
+template __global__ void k_segment<6>(Params, BankView, int, BankView, int, BankView, int);
+template __global__ void k_segment<0>(Params, BankView, int, BankView, int, BankView, int);
+template __global__ void k_persistent<4>(Params, SourceBank);
+template __global__ void k_persistent<6>(Params, SourceBank);
+template __global__ void k_persistent<0>(Params, SourceBank);
 
 namespace mcw {
-// Host-side launch helpers (defined here so the templates live in one TU).
-hipError_t launch_segment(int ns, dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B,
-                          const int32_t *live_in, int qin, int32_t *live_out, int qout, int32_t *coll_q, int qcoll)
+// Host-side launch helpers (the templates live in this translation unit).
+static inline unsigned grid_for(int64_t n, int block, int64_t cap)
 {
-  if (ns == 4) hipLaunchKernelGGL(k_segment<4>, g, b, 0, s, P, B, live_in, qin, live_out, qout, coll_q, qcoll);
-  else if (ns == 6) hipLaunchKernelGGL(k_segment<6>, g, b, 0, s, P, B, live_in, qin, live_out, qout, coll_q, qcoll);
-  else hipLaunchKernelGGL(k_segment<0>, g, b, 0, s, P, B, live_in, qin, live_out, qout, coll_q, qcoll);
+  int64_t b = (n + block - 1) / block;
+  if (cap > 0 && b > cap) b = cap;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+hipError_t launch_source_bank(hipStream_t s, const Params &P, const SourceBank &S, int64_t n)
+{
+  hipLaunchKernelGGL(k_source_bank, dim3(grid_for(n, 256, 0)), dim3(256), 0, s, P, S, n);
   return hipGetLastError();
 }
 
-hipError_t launch_persistent(int ns, dim3 g, dim3 b, hipStream_t s, const Params &P)
+hipError_t launch_bank_init(hipStream_t s, const Params &P, const SourceBank &S, const BankView &B, int64_t n, int q)
 {
-  if (ns == 4) hipLaunchKernelGGL(k_persistent<4>, g, b, 0, s, P);
-  else if (ns == 6) hipLaunchKernelGGL(k_persistent<6>, g, b, 0, s, P);
-  else hipLaunchKernelGGL(k_persistent<0>, g, b, 0, s, P);
+  hipLaunchKernelGGL(k_bank_init, dim3(grid_for(n, 256, 0)), dim3(256), 0, s, P, S, B, n, q);
   return hipGetLastError();
 }
 
-hipError_t launch_source(dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B, int32_t *live, int64_t n)
+hipError_t launch_segment(int ns, unsigned blocks, hipStream_t s, const Params &P, const BankView &in, int qin,
+                          const BankView &out, int qout, const BankView &coll, int qcoll)
 {
-  hipLaunchKernelGGL(k_source, g, b, 0, s, P, B, live, n);
+  if (ns == 4) hipLaunchKernelGGL(k_segment<4>, dim3(blocks), dim3(256), 0, s, P, in, qin, out, qout, coll, qcoll);
+  else if (ns == 6) hipLaunchKernelGGL(k_segment<6>, dim3(blocks), dim3(256), 0, s, P, in, qin, out, qout, coll, qcoll);
+  else hipLaunchKernelGGL(k_segment<0>, dim3(blocks), dim3(256), 0, s, P, in, qin, out, qout, coll, qcoll);
   return hipGetLastError();
 }
 
-hipError_t launch_collide(dim3 g, dim3 b, hipStream_t s, const Params &P, const BankView &B, const int32_t *coll_q,
-                          int qcoll, int32_t *live_out, int qout)
+hipError_t launch_collide(unsigned blocks, hipStream_t s, const Params &P, const BankView &coll, int qcoll,
+                          const BankView &out, int qout)
 {
-  hipLaunchKernelGGL(k_collide, g, b, 0, s, P, B, coll_q, qcoll, live_out, qout);
+  hipLaunchKernelGGL(k_collide, dim3(blocks), dim3(256), 0, s, P, coll, qcoll, out, qout);
   return hipGetLastError();
+}
+
+hipError_t launch_persistent(int ns, unsigned blocks, int block, hipStream_t s, const Params &P, const SourceBank &S)
+{
+  if (ns == 4) hipLaunchKernelGGL(k_persistent<4>, dim3(blocks), dim3(block), 0, s, P, S);
+  else if (ns == 6) hipLaunchKernelGGL(k_persistent<6>, dim3(blocks), dim3(block), 0, s, P, S);
+  else hipLaunchKernelGGL(k_persistent<0>, dim3(blocks), dim3(block), 0, s, P, S);
+  return hipGetLastError();
+}
+
+const void *persistent_kernel(int ns)
+{
+  if (ns == 4) return (const void *)k_persistent<4>;
+  if (ns == 6) return (const void *)k_persistent<6>;
+  return (const void *)k_persistent<0>;
 }
 
 hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1)
@@ -647,9 +748,7 @@ hipError_t launch_reset_work(hipStream_t s, Counters *C)
 hipError_t launch_fill(hipStream_t s, double *p, int64_t n, double v)
 {
   if (n <= 0) return hipSuccess;
-  int64_t blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
 }
 
@@ -662,17 +761,21 @@ hipError_t launch_zero_counters(hipStream_t s, Counters *C)
 hipError_t launch_accumulate(hipStream_t s, double *value, double *sum, double *sum_sq, int64_t n, double norm)
 {
   if (n <= 0) return hipSuccess;
-  int64_t blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_accumulate, dim3((unsigned)blocks), dim3(256), 0, s, value, sum, sum_sq, n, norm);
+  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, value, sum, sum_sq, n, norm);
   return hipGetLastError();
 }
 
 hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *lo, double *so, double *co,
                              double *sq, const uint64_t *seeds, double *po, uint64_t *seo)
 {
-  int64_t blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(k_math_probe, dim3((unsigned)blocks), dim3(256), 0, s, x, n, lo, so, co, sq, seeds, po, seo);
+  hipLaunchKernelGGL(k_math_probe, dim3(grid_for(n, 256, 0)), dim3(256), 0, s, x, n, lo, so, co, sq, seeds, po, seo);
+  return hipGetLastError();
+}
+
+hipError_t launch_xs_probe(hipStream_t s, const XSView &X, int32_t nuc, const double *E, int64_t n, int32_t *k,
+                           double *xs)
+{
+  hipLaunchKernelGGL(k_xs_probe, dim3(grid_for(n, 256, 0)), dim3(256), 0, s, X, nuc, E, n, k, xs);
   return hipGetLastError();
 }
 } // namespace mcw

@@ -39,18 +39,27 @@ struct MeshView {
 };
 
 struct XSView {
-  const int64_t *grid_off;
+  const int64_t *grid_off;   // per nuclide: first point in rec
   const int32_t *n_points;
-  const double *energy;
-  const double4 *values;
+  const double4 *rec;        // per point: (E, total, absorption, scatter)
+  const double *fission;     // per point, nullptr when no nuclide has fission data
   const double *awr;
+  // unionized grid: every nuclide's points merged, sorted, unique
+  const double *U;
+  const int32_t *uidx;       // n_union x n_nuc: largest j with E_n[j] <= U[k], -1 if none
+  const int32_t *hash;       // n_hash + 1: largest k with U[k] <= lower edge of log bucket b
+  int32_t n_union;
+  int32_t n_nuc;
+  int32_t n_hash;
+  int32_t n_mat;
+  double logU0, inv_dl;      // bucket b = floor((log E - logU0) * inv_dl)
   const int32_t *mat_off;
   const int32_t *mat_nuc;
   const double *mat_dens;
-  int32_t n_nuc;
-  int32_t n_mat;
   double e_min, e_max;
 };
+
+constexpr int MAX_HASH = 8192;
 
 struct SourceView {
   int32_t space, angle, energy, pad;
@@ -78,13 +87,21 @@ struct TallyView {
   double *value;
 };
 
+// Structure-of-arrays particle bank of the event-based path.
 struct BankView {
   double *x, *y, *z, *u, *v, *w, *E, *sigt;
   uint64_t *seed;
-  int32_t *elem, *inface, *mat;
   int64_t *hid;      // history index within the batch (0-based)
+  int32_t *elem, *inface, *mat, *k, *nseg;
   uint64_t *digest;  // record mode only
-  int32_t *nseg, *ncoll;
+  int32_t *ncoll;    // record mode only
+};
+
+// Source sites sampled ahead of transport (one per history of a chunk).
+struct SourceBank {
+  double *x, *y, *z, *u, *v, *w, *E;
+  uint64_t *seed;    // tracking-stream seed
+  int32_t *elem;
 };
 
 struct RecordView {
@@ -111,7 +128,7 @@ struct Params {
   int32_t n_coll;          // tallies with the collision estimator
   int32_t roundtrip;
   int32_t record;
-  int32_t pad;
+  int32_t lanes_min;       // persistent kernel: keep stepping segments while this many lanes are in flight
   int64_t batch;
   int64_t n_particles;     // particles in the batch (seed id stride)
   int64_t hist_begin;      // first history index handled by this launch

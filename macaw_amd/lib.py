@@ -22,7 +22,7 @@ EXPORTS = (
     "mcw_xs_set", "mcw_materials_set", "mcw_block_material_map", "mcw_source_set", "mcw_tally_add",
     "mcw_tally_shape", "mcw_run_batch", "mcw_tally_get", "mcw_tally_reset", "mcw_history_get",
     "mcw_server_create", "mcw_server_destroy", "mcw_server_last_error", "mcw_submit", "mcw_poll",
-    "mcw_result", "mcw_debug_math_probe",
+    "mcw_result", "mcw_debug_math_probe", "mcw_debug_xs_probe",
 )
 
 
@@ -63,6 +63,7 @@ def lib():
         "mcw_poll": ([vp, i32p], C.c_int),
         "mcw_result": ([vp, C.POINTER(abi.RequestStatus), dp, C.c_size_t], C.c_int),
         "mcw_debug_math_probe": ([vp, dp, i64, dp, dp, dp, dp, u64p, dp, u64p], C.c_int),
+        "mcw_debug_xs_probe": ([vp, i32, dp, i64, i32p, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -154,6 +155,15 @@ class Context:
                                               seeds.ctypes.data_as(C.POINTER(C.c_uint64)), dp(outs[4]),
                                               so.ctypes.data_as(C.POINTER(C.c_uint64))))
         return {"log": outs[0], "sin": outs[1], "cos": outs[2], "sqrt": outs[3], "prn": outs[4], "future": so}
+
+    def xs_probe(self, nuc, E):
+        E = np.ascontiguousarray(E, np.float64)
+        k = np.zeros(E.size, np.int32)
+        xs = np.zeros((E.size, 4))
+        self._chk(self.L.mcw_debug_xs_probe(self.h, nuc, E.ctypes.data_as(C.POINTER(C.c_double)), E.size,
+                                            k.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            xs.ctypes.data_as(C.POINTER(C.c_double))))
+        return k, xs
 
     def close(self):
         if self.h is not None and self.h.value:

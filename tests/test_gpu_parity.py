@@ -178,3 +178,26 @@ def test_errors_are_reported():
     g3 = Context(p3)
     with pytest.raises(McwError, match="boundary 5"):
         g3.run_batch(1, 10)
+
+
+def test_union_grid_lookup_matches_per_nuclide_search(oracle_lib):
+    """Unionized-grid lookup (LDS bucket table + union index map) gives the
+    same bits as the oracle's per-nuclide binary search, including energies
+    at grid points, below/above a nuclide's range and between grids."""
+    from oracle.pyoracle import Oracle
+    nucs = [P.synth_nuclide(3000, 12.0, seed=1),
+            P.synth_nuclide(700, 1.0, seed=2, e_min=1e-3, e_max=1e6),
+            P.synth_nuclide(50, 56.0, seed=3, e_min=10.0, e_max=2e7)]
+    p = c1_small(10)
+    p.nuclides = nucs
+    p.materials = [P.MaterialData(id=1, nuclides=np.array([0, 1, 2], np.int32), densities=np.array([0.02, 0.03, 0.01]))]
+    g = Context(p)
+    o = Oracle(p)
+    rng = np.random.default_rng(3)
+    for q, n in enumerate(nucs):
+        E = np.concatenate([np.exp(rng.uniform(np.log(1e-6), np.log(3e7), 20000)), n.energy,
+                            0.5 * (n.energy[1:] + n.energy[:-1]), [n.energy[0], n.energy[-1], 1e-7, 1e8],
+                            nucs[(q + 1) % 3].energy])
+        k, xs = g.xs_probe(q, E)
+        want = np.array([o.micro_xs(q, float(e)) for e in E])
+        assert np.array_equal(xs.view(np.uint64), want.view(np.uint64)), f"nuclide {q}"
