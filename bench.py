@@ -58,9 +58,9 @@ def cpu_baseline(problem, n_hist_full, seconds=12.0):
     from oracle.pyoracle import Oracle
     cores = os.cpu_count() or 1
     o = Oracle(problem)
-    pilot = 2000
+    pilot = 40000
     t = time.perf_counter()
-    o.run_batch(1, n_hist_full if False else pilot, threads=cores)
+    o.run_batch(1, pilot, threads=cores)
     dt = time.perf_counter() - t
     n = int(min(max(pilot * seconds / max(dt, 1e-3), 10000), 2_000_000))
     log(f"cpu baseline: pilot {pilot} histories in {dt:.2f}s -> sample {n} histories on {cores} threads")
@@ -84,6 +84,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--lanes-min", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
+                    help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
+                         "bytes): the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -95,6 +99,14 @@ def main():
     from macaw_amd import abi, problems as P
     from macaw_amd.lib import Context
 
+    if args.calibrate_copy:
+        import numpy as np
+        n = args.calibrate_copy
+        c = Context()
+        c.math_probe(np.linspace(0.1, 0.9, n), np.arange(n, dtype=np.uint64))
+        print(json.dumps({"calibration": "k_math_probe", "n": n, "read_bytes": 16 * n, "write_bytes": 48 * n}))
+        return
+
     t0 = time.perf_counter()
     log(f"generating workload: Delaunay lattice {args.lattice}^3")
     prob = P.c2_tet_vacuum(n_lattice=args.lattice, n_particles=args.histories)
@@ -102,7 +114,7 @@ def main():
     workload = f"c2_tet4_{n_elem // 1000}k_vacuum"
     log(f"mesh {n_elem} tets, {prob.mesh.n_nodes} nodes ({time.perf_counter() - t0:.1f}s)")
 
-    ctx = Context(prob, mode=abi.MODE_BATCHED, block_size=args.block)
+    ctx = Context(prob, mode=abi.MODE_BATCHED, block_size=args.block, lanes_min=args.lanes_min)
     N = args.histories
     for w in range(args.warmup):
         st = ctx.run_batch(100 + w, N)

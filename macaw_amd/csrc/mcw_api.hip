@@ -136,7 +136,7 @@ struct mcw_ctx {
   bool dirty = true;
 
   // device problem
-  DevBuf plane, nbr, elem_mat, face_off, n_sides, bc_kind, cdf, xyz, conn, conn_off, type;
+  DevBuf plane, erec, face_off, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf grid_off, npts, rec, fission, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
   Counters *h_counters = nullptr; // pinned
@@ -217,29 +217,41 @@ void prepare(mcw_ctx *c)
     if (b != c->block_mat.end()) emat[e] = id_to_idx[b->second];
   }
 
-  // mesh
+  // mesh in internal (Morton) order: planes per side slot, and per element
+  // one 32-byte record of neighbour codes, material and side count
+  const int64_t ne = m.n_elem;
   std::vector<double4> planes(m.plane.size() / 4);
-  for (size_t i = 0; i < planes.size(); ++i)
-    planes[i] = make_double4(m.plane[4 * i], m.plane[4 * i + 1], m.plane[4 * i + 2], m.plane[4 * i + 3]);
+  std::vector<int4> erec(2 * ne);
+  std::vector<int32_t> fo;
+  if (m.uniform_sides == 0) fo.resize(ne);
+  int64_t slot = 0;
+  for (int64_t i = 0; i < ne; ++i) {
+    const int64_t o = m.orig_of_int[i];
+    const int64_t f0 = m.face_off[o];
+    const int ns = (int)(m.face_off[o + 1] - f0);
+    if (m.uniform_sides == 0) fo[i] = (int32_t)slot;
+    int32_t code[6] = {-1, -1, -1, -1, -1, -1};
+    for (int k = 0; k < ns; ++k) {
+      const double *q = &m.plane[4 * (f0 + k)];
+      planes[slot + k] = make_double4(q[0], q[1], q[2], q[3]);
+      const int32_t nb = m.nbr[f0 + k];
+      code[k] = nb >= 0 ? m.int_of_orig[nb >> 3] * 8 + (nb & 7) : nb;
+    }
+    slot += ns;
+    erec[2 * i] = make_int4(code[0], code[1], code[2], code[3]);
+    erec[2 * i + 1] = make_int4(code[4], code[5], emat[o], ns);
+  }
   c->plane.upload(planes, s);
-  c->nbr.upload(m.nbr, s);
-  c->elem_mat.upload(emat, s);
+  c->erec.upload(erec, s);
+  if (m.uniform_sides == 0) c->face_off.upload(fo, s);
   c->bc_kind.upload(kinds, s);
   c->cdf.upload(m.cdf, s);
   c->xyz.upload(m.xyz, s);
   c->conn.upload(m.conn, s);
   c->conn_off.upload(m.conn_off, s);
   c->type.upload(m.type, s);
-  if (m.uniform_sides == 0) {
-    std::vector<int32_t> fo(m.n_elem);
-    std::vector<uint8_t> ns(m.n_elem);
-    for (int64_t e = 0; e < m.n_elem; ++e) {
-      fo[e] = (int32_t)m.face_off[e];
-      ns[e] = (uint8_t)(m.face_off[e + 1] - m.face_off[e]);
-    }
-    c->face_off.upload(fo, s);
-    c->n_sides.upload(ns, s);
-  }
+  c->int_of_orig.upload(m.int_of_orig, s);
+  c->orig_of_int.upload(m.orig_of_int, s);
 
   // nuclear data: per-point records (E, total, absorption, scatter), fission apart
   std::vector<int64_t> goff;
@@ -329,10 +341,8 @@ void prepare(mcw_ctx *c)
   Params &P = c->base;
   P = Params{};
   P.mesh.plane = c->plane.as<double4>();
-  P.mesh.nbr = c->nbr.as<int32_t>();
-  P.mesh.elem_mat = c->elem_mat.as<int32_t>();
+  P.mesh.erec = c->erec.as<int4>();
   P.mesh.face_off = m.uniform_sides ? nullptr : c->face_off.as<int32_t>();
-  P.mesh.n_sides = m.uniform_sides ? nullptr : c->n_sides.as<uint8_t>();
   P.mesh.stride = m.uniform_sides;
   P.mesh.n_bc = c->n_bc;
   P.mesh.bc_kind = c->bc_kind.as<int8_t>();
@@ -342,7 +352,8 @@ void prepare(mcw_ctx *c)
   P.mesh.conn = c->conn.as<int32_t>();
   P.mesh.conn_off = c->conn_off.as<int64_t>();
   P.mesh.type = c->type.as<uint8_t>();
-  P.mesh.orig_id = nullptr;
+  P.mesh.int_of_orig = c->int_of_orig.as<int32_t>();
+  P.mesh.orig_of_int = c->orig_of_int.as<int32_t>();
   P.xs.grid_off = c->grid_off.as<int64_t>();
   P.xs.n_points = c->npts.as<int32_t>();
   P.xs.rec = c->rec.as<double4>();
@@ -375,7 +386,7 @@ void prepare(mcw_ctx *c)
   P.src.E0 = c->src.E0;
   P.src.watt_a = c->src.watt_a;
   P.src.watt_b = c->src.watt_b;
-  P.src.point_elem = c->src_elem;
+  P.src.point_elem = c->src_elem >= 0 ? m.int_of_orig[c->src_elem] : -1;
 
   // tallies: track-length ones first, then collision ones
   std::vector<HostTally *> order;
@@ -490,7 +501,11 @@ void build_tally(mcw_ctx *c, HostTally &t)
     // no spatial filter: every element scores into spatial bin 0
     t.identity = false;
   }
-  t.elem_base.upload(base, c->stream);
+  // device order: base of internal element i is that of caller element orig_of_int[i];
+  // an identity cell tally is stored by internal id and permuted on read-out
+  std::vector<int64_t> base_int(m.n_elem);
+  for (int64_t i = 0; i < m.n_elem; ++i) base_int[i] = base[m.orig_of_int[i]];
+  t.elem_base.upload(base_int, c->stream);
   t.d_scores.upload(t.scores, c->stream);
   t.d_nuclides.upload(t.nuclides, c->stream);
   const size_t n = (size_t)(t.n_filter_bins * t.n_score_bins);
@@ -900,10 +915,20 @@ int mcw_tally_get(mcw_ctx *c, int32_t id, int32_t kind, double *out, size_t n)
       if (t->id == id) {
         const size_t need = (size_t)(t->n_filter_bins * t->n_score_bins);
         require(n >= need, "output buffer holds " + std::to_string(n) + " values, tally has " + std::to_string(need));
-        const DevBuf &b = kind == MCW_RESULT_VALUE ? t->value : kind == MCW_RESULT_SUM ? t->sum : t->sum_sq;
         require(kind >= MCW_RESULT_VALUE && kind <= MCW_RESULT_SUM_SQ, "unknown result kind");
-        HIPCHK(hipMemcpyAsync(out, b.p, need * 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+        const DevBuf &b = kind == MCW_RESULT_VALUE ? t->value : kind == MCW_RESULT_SUM ? t->sum : t->sum_sq;
+        if (!t->identity) {
+          HIPCHK(hipMemcpyAsync(out, b.p, need * 8, hipMemcpyDeviceToHost, c->stream));
+          HIPCHK(hipStreamSynchronize(c->stream));
+        } else {
+          std::vector<double> tmp(need);
+          HIPCHK(hipMemcpyAsync(tmp.data(), b.p, need * 8, hipMemcpyDeviceToHost, c->stream));
+          HIPCHK(hipStreamSynchronize(c->stream));
+          const int64_t w = t->n_score_bins;
+          const std::vector<int32_t> &io = c->mesh.int_of_orig;
+          for (int64_t e = 0; e < (int64_t)io.size(); ++e)
+            std::copy(&tmp[(size_t)io[e] * w], &tmp[(size_t)io[e] * w] + w, out + (size_t)e * w);
+        }
         return;
       }
     throw McwError{"no tally with id " + std::to_string(id)};

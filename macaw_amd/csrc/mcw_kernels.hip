@@ -209,7 +209,7 @@ __device__ void sample_site(const Params &P, int64_t hid, double *p, double *dir
     }
   }
   if (P.roundtrip) track_seed = seed_roundtrip(track_seed);
-  elem_out = (int32_t)elem;
+  elem_out = S.space == 0 ? (int32_t)elem : M.int_of_orig[elem];
 }
 
 __device__ __forceinline__ void start_history(PState &st, int64_t hid)
@@ -225,9 +225,17 @@ __device__ __forceinline__ void start_history(PState &st, int64_t hid)
 
 // ------------------------------------------------------------------ one segment
 
-__device__ __forceinline__ int64_t orig_elem(const MeshView &M, int32_t e)
+__device__ __forceinline__ int64_t orig_elem(const MeshView &M, int32_t e) { return M.orig_of_int[e]; }
+
+__device__ __forceinline__ int32_t pick_nbr(const int4 &r0, const int4 &r1, int fx)
 {
-  return M.orig_id ? M.orig_id[e] : (int64_t)e;
+  int32_t nb = r0.x;
+  nb = fx == 1 ? r0.y : nb;
+  nb = fx == 2 ? r0.z : nb;
+  nb = fx == 3 ? r0.w : nb;
+  nb = fx == 4 ? r1.x : nb;
+  nb = fx == 5 ? r1.y : nb;
+  return nb;
 }
 
 // Geometry and distance draw of one segment.  Returns the event that ends it;
@@ -241,7 +249,9 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
     ++lc.lost;
     return EV_LOST;
   }
-  const int32_t mat = M.elem_mat[st.elem];
+  const int4 r0 = M.erec[2 * (int64_t)st.elem];
+  const int4 r1 = M.erec[2 * (int64_t)st.elem + 1];
+  const int32_t mat = r1.z;
   if (mat != st.mat) {
     st.sigt = macro_total(P.xs, mat, st.k, st.E);
     st.mat = mat;
@@ -254,7 +264,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
     fx = exit_side<NS>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
   } else {
     slot0 = M.face_off[st.elem];
-    fx = exit_side_dyn(M.plane + slot0, M.n_sides[st.elem], st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+    fx = exit_side_dyn(M.plane + slot0, r1.w, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
   }
   if (fx < 0) {
     if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
@@ -272,7 +282,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
     st.x = st.x + L * st.u;
     st.y = st.y + L * st.v;
     st.z = st.z + L * st.w;
-    const int32_t nb = M.nbr[slot0 + fx];
+    const int32_t nb = pick_nbr(r0, r1, fx);
     int ev;
     if (nb >= 0) {
       ev = EV_CROSS;

@@ -226,6 +226,45 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
     acc = acc + vol[e];
     m.cdf[e] = acc;
   }
+
+  // -- Morton order of element centroids (21 bits per axis), ties by index
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = 0; i < m.n_nodes; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], X[3 * i + k]);
+      hi[k] = std::max(hi[k], X[3 * i + k]);
+    }
+  auto spread = [](uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+  };
+  std::vector<std::pair<uint64_t, int32_t>> key(ne);
+  for (int64_t e = 0; e < ne; ++e) {
+    const int32_t *c = m.conn.data() + m.conn_off[e];
+    const int nn = m.type[e];
+    uint64_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      double cc = 0.0;
+      for (int q = 0; q < nn; ++q) cc += X[3 * (int64_t)c[q] + k];
+      cc /= nn;
+      double t = hi[k] > lo[k] ? (cc - lo[k]) / (hi[k] - lo[k]) : 0.0;
+      uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097152.0));
+      code |= spread(q) << k;
+    }
+    key[e] = {code, (int32_t)e};
+  }
+  std::sort(key.begin(), key.end());
+  m.orig_of_int.resize(ne);
+  m.int_of_orig.resize(ne);
+  for (int64_t i = 0; i < ne; ++i) {
+    m.orig_of_int[i] = key[i].second;
+    m.int_of_orig[key[i].second] = (int32_t)i;
+  }
   return "";
 }
 

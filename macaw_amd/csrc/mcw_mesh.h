@@ -23,6 +23,9 @@ struct HostMesh {
   std::vector<int32_t> nbr;         // neighbour*8 + side, or -1 - boundary id
   std::set<int32_t> boundary_ids;   // ids of external sides
   std::vector<double> cdf;          // element volume CDF (6x volumes)
+  // device order: elements sorted along a Morton curve of their centroids
+  std::vector<int32_t> orig_of_int; // internal position -> caller element
+  std::vector<int32_t> int_of_orig; // caller element -> internal position
 };
 
 extern const int HEX_KUHN_TETS[6][4];

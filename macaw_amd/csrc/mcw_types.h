@@ -19,23 +19,26 @@ namespace mcw {
 
 constexpr int MAX_TALLIES = 8;
 
+// Elements live on the device in Morton order of their centroids
+// ("internal" ids); the caller's ids appear only in source sampling (the
+// volume CDF is in caller order), records and tally read-out.
 struct MeshView {
-  const double4 *plane;
-  const int32_t *nbr;
-  const int32_t *elem_mat;
-  const int32_t *face_off;   // nullptr for single-type meshes
-  const uint8_t *n_sides;    // per element, mixed meshes only
+  const double4 *plane;      // per side slot (internal order): outward unit normal, offset
+  const int4 *erec;          // per element, 2 x int4: neighbour code of sides 0..5
+                             // (internal id * 8 + side, or -1 - boundary id), material, side count
+  const int32_t *face_off;   // first side slot per element, mixed meshes only
   int32_t stride;            // sides per element for single-type meshes (4 or 6), 0 = mixed
   int32_t n_bc;
   const int8_t *bc_kind;     // per boundary id
   int64_t n_elem;
-  // for volumetric sources
+  // caller order, for volumetric sources
   const double *cdf;
   const double *xyz;
   const int32_t *conn;
   const int64_t *conn_off;
   const uint8_t *type;
-  const int64_t *orig_id;    // internal element -> caller's element index (nullptr = identity)
+  const int32_t *int_of_orig; // caller element -> internal id
+  const int32_t *orig_of_int; // internal id -> caller element
 };
 
 struct XSView {
