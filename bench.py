@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--lanes-min", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
                          "bytes): the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling")
@@ -116,6 +117,14 @@ def main():
 
     ctx = Context(prob, mode=abi.MODE_BATCHED, block_size=args.block, lanes_min=args.lanes_min)
     N = args.histories
+    tuning = {"block_size": ctx.opts.block_size, "lanes_min": ctx.opts.lanes_min,
+              "waves_per_eu": ctx.opts.waves_per_eu}
+    if not args.no_autotune:
+        from macaw_amd.autotune import autotune
+        t = time.perf_counter()
+        tuning, rows = autotune(ctx, N, log=log)
+        ctx.set_options(**tuning)
+        log(f"autotune: best {tuning} ({time.perf_counter() - t:.1f}s)")
     for w in range(args.warmup):
         st = ctx.run_batch(100 + w, N)
         log(f"warmup {w}: {st['time_ms']:.1f} ms, {st['segments']} segments")
@@ -177,7 +186,7 @@ def main():
         "config": {"workload": workload, "n_elem": n_elem, "element": "TET4", "histories_per_step": N,
                    "materials": 4, "nuclides": 4, "grid_points": 50000, "source": "volume Watt",
                    "boundary": "vacuum", "tally": "per-element track-length flux", "mode": "batched",
-                   "precision": "fp64", "parallelism": "1 GPU"},
+                   "precision": "fp64", "parallelism": "1 GPU", "tuning": tuning},
         "segments_per_s": seg_s,
         "segments_per_history": segs / (N * args.steps),
         "collisions_per_history": colls / (N * args.steps),

@@ -94,7 +94,9 @@ typedef struct {
   int32_t sort_cadence;     /* BATCHED: refills between element sorts of the bank, 0 = never */
   int32_t lanes_min;        /* BATCHED: a wavefront keeps stepping segments while at least this
                                many of its 64 lanes are in flight (1..64, 0 = default) */
-  int32_t reserved[6];
+  int32_t waves_per_eu;     /* BATCHED: 5 = kernel variant built for 5 wavefronts per SIMD
+                               (fewer registers, some spills), 0 = compiler's choice */
+  int32_t reserved[5];
 } mcw_options;
 
 /* One synthetic continuous-energy nuclide: pointwise lin-lin tables on an
@@ -175,6 +177,11 @@ typedef struct {
 int mcw_create(mcw_ctx **ctx, const mcw_options *opts);
 /* Replaces openmc_finalize (OpenMCStudy.C:213-215). */
 int mcw_destroy(mcw_ctx *ctx);
+/* Change the schedule and tuning fields of a live context (mode, precision,
+ * block_size, bank_capacity, lanes_min, waves_per_eu, sort_cadence,
+ * record_histories, max_segments) without re-uploading the problem; device,
+ * master_seed and seed_roundtrip cannot change.  Used by the autotuner. */
+int mcw_options_set(mcw_ctx *ctx, const mcw_options *opts);
 /* Replaces openmc_err_msg. */
 const char *mcw_last_error(const mcw_ctx *ctx);
 void mcw_default_options(mcw_options *opts);

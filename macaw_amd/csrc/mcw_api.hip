@@ -28,8 +28,9 @@ hipError_t launch_segment(int ns, unsigned blocks, hipStream_t s, const Params &
                           const BankView &out, int qout, const BankView &coll, int qcoll);
 hipError_t launch_collide(unsigned blocks, hipStream_t s, const Params &P, const BankView &coll, int qcoll,
                           const BankView &out, int qout);
-hipError_t launch_persistent(int ns, unsigned blocks, int block, hipStream_t s, const Params &P, const SourceBank &S);
-const void *persistent_kernel(int ns);
+hipError_t launch_persistent(int ns, int wpe, unsigned blocks, int block, hipStream_t s, const Params &P,
+                             const SourceBank &S);
+const void *persistent_kernel(int ns, int wpe, bool rec);
 hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1);
 hipError_t launch_reset_work(hipStream_t s, Counters *C);
 hipError_t launch_fill(hipStream_t s, double *p, int64_t n, double v);
@@ -564,10 +565,10 @@ SourceBank source_bank(mcw_ctx *c, int64_t n)
 }
 
 // Resident workgroups of the persistent kernel over the whole device.
-int persistent_blocks(mcw_ctx *c, int ns, int block)
+int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, int block)
 {
   int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns), block, 0));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns, wpe, rec), block, 0));
   if (per_cu < 1) per_cu = 1;
   return c->n_cu * per_cu;
 }
@@ -656,6 +657,25 @@ int mcw_destroy(mcw_ctx *c)
 }
 
 const char *mcw_last_error(const mcw_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int mcw_options_set(mcw_ctx *c, const mcw_options *o)
+{
+  return guard(c, [&] {
+    require(o != nullptr, "null options");
+    require(o->device == c->opt.device, "the device of a context cannot change");
+    require((o->master_seed ? o->master_seed : 1) == c->opt.master_seed, "master_seed of a context cannot change");
+    require(o->seed_roundtrip == c->opt.seed_roundtrip, "seed_roundtrip of a context cannot change");
+    require(o->mode == MCW_MODE_PLAIN || o->mode == MCW_MODE_BATCHED, "unknown mode");
+    require(o->precision == MCW_PREC_FP64 || o->precision == MCW_PREC_MIXED, "unknown precision");
+    require(o->lanes_min >= 0 && o->lanes_min <= 64, "lanes_min must be in 0..64");
+    require(o->block_size == 0 || (o->block_size % 64 == 0 && o->block_size <= 256),
+            "block_size must be 64, 128, 192 or 256");
+    const bool reprep = o->max_segments != c->opt.max_segments || o->lanes_min != c->opt.lanes_min;
+    c->opt = *o;
+    if (c->opt.master_seed == 0) c->opt.master_seed = 1;
+    if (reprep) c->dirty = true;
+  });
+}
 
 int mcw_mesh_set(mcw_ctx *c, int64_t n_nodes, const double *xyz, int64_t n_elem, const uint8_t *elem_type,
                  const int64_t *conn_offsets, const int32_t *conn, const int32_t *subdomain,
@@ -863,13 +883,14 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
         run_plain(c, P, S, h1 - h0);
       } else {
         const int block = c->opt.block_size > 0 ? c->opt.block_size : 256;
+        const int wpe = c->opt.waves_per_eu == 5 ? 5 : 0;
         require(block % 64 == 0 && block <= 256, "block_size must be 64, 128, 192 or 256");
-        int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, block), (h1 - h0 + block - 1) / block);
+        int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, block), (h1 - h0 + block - 1) / block);
         if (blocks < 1) blocks = 1;
         // the work counter restarts for every chunk
         HIPCHK(launch_reset_work(s, P.cnt));
         HIPCHK(hipEventRecord(c->ev[2], s));
-        HIPCHK(launch_persistent(ns, (unsigned)blocks, block, s, P, S));
+        HIPCHK(launch_persistent(ns, wpe, (unsigned)blocks, block, s, P, S));
         HIPCHK(hipEventRecord(c->ev[3], s));
         HIPCHK(hipEventSynchronize(c->ev[3]));
         float ms = 0.0f;

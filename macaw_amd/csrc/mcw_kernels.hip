@@ -43,10 +43,11 @@ struct PState {
   int32_t nseg, ncoll;
 };
 
-struct LocalCount {
+// Event counts of one wavefront.  Every field is wavefront-uniform (sums of
+// ballots), so the compiler keeps them in scalar registers.
+struct WaveCount {
   unsigned long long seg, coll, absn, leak, cut, lost, refl;
-  double track;
-  __device__ void zero() { seg = coll = absn = leak = cut = lost = refl = 0; track = 0.0; }
+  __device__ void zero() { seg = coll = absn = leak = cut = lost = refl = 0; }
 };
 
 // ------------------------------------------------------------------ wave helpers
@@ -65,20 +66,36 @@ __device__ __forceinline__ double wave_sum_d(double v)
   return v;
 }
 
-__device__ void flush_counts(Counters *c, LocalCount &l)
+__device__ __forceinline__ unsigned long long wave_pop(bool p) { return (unsigned long long)__popcll(__ballot(p)); }
+
+// Count the outcome of one segment step (every lane of the wavefront calls it).
+__device__ __forceinline__ void count_segment(WaveCount &w, bool stepped, int ev)
 {
-  unsigned long long s0 = wave_sum(l.seg), s1 = wave_sum(l.coll), s2 = wave_sum(l.absn),
-                     s3 = wave_sum(l.leak), s4 = wave_sum(l.cut), s5 = wave_sum(l.lost),
-                     s6 = wave_sum(l.refl);
-  double tr = wave_sum_d(l.track);
+  w.seg += wave_pop(stepped);
+  w.leak += wave_pop(stepped && ev == 2);
+  w.refl += wave_pop(stepped && ev == 1);
+  w.lost += wave_pop(stepped && ev == 6);
+}
+
+// Count the outcome of one collision step (every lane calls it).
+__device__ __forceinline__ void count_collision(WaveCount &w, bool collided, int ev)
+{
+  w.coll += wave_pop(collided);
+  w.absn += wave_pop(collided && ev == 4);
+  w.cut += wave_pop(collided && ev == 5);
+}
+
+__device__ void flush_counts(Counters *c, const WaveCount &l, double track)
+{
+  const double tr = wave_sum_d(track);
   if (__lane_id() == 0) {
-    if (s0) atomicAdd(&c->segments, s0);
-    if (s1) atomicAdd(&c->collisions, s1);
-    if (s2) atomicAdd(&c->absorptions, s2);
-    if (s3) atomicAdd(&c->leaks, s3);
-    if (s4) atomicAdd(&c->cutoffs, s4);
-    if (s5) atomicAdd(&c->lost, s5);
-    if (s6) atomicAdd(&c->reflections, s6);
+    if (l.seg) atomicAdd(&c->segments, l.seg);
+    if (l.coll) atomicAdd(&c->collisions, l.coll);
+    if (l.absn) atomicAdd(&c->absorptions, l.absn);
+    if (l.leak) atomicAdd(&c->leaks, l.leak);
+    if (l.cut) atomicAdd(&c->cutoffs, l.cut);
+    if (l.lost) atomicAdd(&c->lost, l.lost);
+    if (l.refl) atomicAdd(&c->reflections, l.refl);
     if (tr != 0.0) unsafeAtomicAdd(&c->track, tr);
   }
 }
@@ -240,13 +257,12 @@ __device__ __forceinline__ int32_t pick_nbr(const int4 &r0, const int4 &r1, int 
 
 // Geometry and distance draw of one segment.  Returns the event that ends it;
 // EV_COLLIDE leaves the particle at the collision point for collide().
-template <int NS>
-__device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &lc)
+template <int NS, bool REC>
+__device__ __forceinline__ int segment(const Params &P, PState &st, double &track_sum)
 {
   const MeshView &M = P.mesh;
   if (st.nseg >= P.max_seg) {
-    if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
-    ++lc.lost;
+    if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
     return EV_LOST;
   }
   const int4 r0 = M.erec[2 * (int64_t)st.elem];
@@ -267,16 +283,14 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
     fx = exit_side_dyn(M.plane + slot0, r1.w, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
   }
   if (fx < 0) {
-    if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
-    ++lc.lost;
+    if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
     return EV_LOST;
   }
   const double xi = prn(st.seed);
   const double d = st.sigt > 0.0 ? -log_d(xi) / st.sigt : __builtin_inf();
   const double track = (d < L) ? d : L;
   ++st.nseg;
-  ++lc.seg;
-  lc.track += track;
+  track_sum += track;
   for (int t = 0; t < P.n_tl; ++t) tally_score(P.tally[t], P.xs, st.elem, st.E, st.k, track, mat);
   if (d > L) {
     st.x = st.x + L * st.u;
@@ -286,7 +300,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
     int ev;
     if (nb >= 0) {
       ev = EV_CROSS;
-      if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), ev);
+      if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), ev);
       st.elem = nb >> 3;
       st.inface = nb & 7;
     } else {
@@ -294,18 +308,16 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
       const int kind = bid < M.n_bc ? M.bc_kind[bid] : 0;
       if (kind == 1) {
         ev = EV_REFLECT;
-        if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), ev);
+        if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), ev);
         const double4 q = M.plane[slot0 + fx];
         double dot = (st.u * q.x + st.v * q.y) + st.w * q.z;
         st.u = st.u - 2.0 * dot * q.x;
         st.v = st.v - 2.0 * dot * q.y;
         st.w = st.w - 2.0 * dot * q.z;
         st.inface = fx;
-        ++lc.refl;
       } else {
         ev = EV_LEAK;
-        if (P.record) st.h = digest_step(st.h, orig_elem(M, st.elem), ev);
-        ++lc.leak;
+        if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), ev);
       }
     }
     if (P.roundtrip) st.seed = seed_roundtrip(st.seed);
@@ -323,12 +335,12 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, LocalCount &
 // nuclide (OpenMC sample_nuclide), absorb (analog) or scatter elastically
 // (isotropic in CM, target at rest) with an energy cutoff at the grid minimum.
 // A scatter looks up the new energy's union-grid index through `hash`.
-__device__ __forceinline__ int collide(const Params &P, PState &st, LocalCount &lc, const int32_t *hash)
+template <bool REC>
+__device__ __forceinline__ int collide(const Params &P, PState &st, const int32_t *hash)
 {
   const XSView &X = P.xs;
   const int32_t mat = st.mat;
-  ++st.ncoll;
-  ++lc.coll;
+  if (REC) ++st.ncoll;
   for (int t = P.n_tl; t < P.n_tl + P.n_coll; ++t)
     tally_score(P.tally[t], X, st.elem, st.E, st.k, 1.0 / st.sigt, mat);
   const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
@@ -348,7 +360,6 @@ __device__ __forceinline__ int collide(const Params &P, PState &st, LocalCount &
   int ev;
   if (mu.a > prn(st.seed) * mu.t) {
     ev = EV_ABSORB;
-    ++lc.absn;
   } else {
     const double A = X.awr[nuc];
     const double mu_cm = 2.0 * prn(st.seed) - 1.0;
@@ -357,7 +368,6 @@ __device__ __forceinline__ int collide(const Params &P, PState &st, LocalCount &
     if (Ep < X.e_min) {
       ev = EV_CUTOFF;
       st.E = Ep;
-      ++lc.cut;
     } else {
       const double mu_lab = (1.0 + A * mu_cm) / sqrt(q2);
       const double xi_phi = prn(st.seed);
@@ -368,7 +378,7 @@ __device__ __forceinline__ int collide(const Params &P, PState &st, LocalCount &
       ev = EV_SCATTER;
     }
   }
-  if (P.record) st.h = digest_step(st.h, orig_elem(P.mesh, st.elem), ev);
+  if (REC) st.h = digest_step(st.h, orig_elem(P.mesh, st.elem), ev);
   st.inface = -1;
   if (P.roundtrip) st.seed = seed_roundtrip(st.seed);
   return ev;
@@ -408,7 +418,8 @@ __device__ void write_record(const Params &P, const PState &st, int fate)
 
 // ------------------------------------------------------------------ bank I/O
 
-__device__ __forceinline__ void load(const BankView &B, int64_t s, PState &st, bool record)
+template <bool record>
+__device__ __forceinline__ void load(const BankView &B, int64_t s, PState &st)
 {
   st.x = B.x[s]; st.y = B.y[s]; st.z = B.z[s];
   st.u = B.u[s]; st.v = B.v[s]; st.w = B.w[s];
@@ -426,7 +437,8 @@ __device__ __forceinline__ void load(const BankView &B, int64_t s, PState &st, b
   }
 }
 
-__device__ __forceinline__ void store(const BankView &B, int64_t s, const PState &st, bool record)
+template <bool record>
+__device__ __forceinline__ void store(const BankView &B, int64_t s, const PState &st)
 {
   B.x[s] = st.x; B.y[s] = st.y; B.z[s] = st.z;
   B.u[s] = st.u; B.v[s] = st.v; B.w[s] = st.w;
@@ -481,8 +493,8 @@ extern "C" __global__ void __launch_bounds__(256) k_source_bank(Params P, Source
 // ------------------------------------------------------------------ PLAIN kernels
 
 // Live bank <- source bank (slot i = history hist_begin + i).
-extern "C" __global__ void __launch_bounds__(256) k_bank_init(Params P, SourceBank S, BankView B, int64_t n,
-                                                              int q)
+template <bool REC>
+__global__ void __launch_bounds__(256) k_bank_init(Params P, SourceBank S, BankView B, int64_t n, int q)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) P.cnt->q_count[q] = (unsigned long long)n;
@@ -491,63 +503,65 @@ extern "C" __global__ void __launch_bounds__(256) k_bank_init(Params P, SourceBa
   load_site(S, i, st);
   start_history(st, P.hist_begin + i);
   st.k = union_index(P.xs, P.xs.hash, st.E);
-  store(B, i, st, P.record != 0);
+  store<REC>(B, i, st);
 }
 
 // One segment for every particle of the live bank `in` (q_count[qin] of them).
-template <int NS>
+template <int NS, bool REC>
 __global__ void __launch_bounds__(256) k_segment(Params P, BankView in, int qin, BankView out, int qout,
                                                  BankView coll, int qcoll)
 {
   Counters *C = P.cnt;
   const int64_t n = (int64_t)C->q_count[qin];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  LocalCount lc;
-  lc.zero();
-  const bool record = P.record != 0;
+  WaveCount wc;
+  wc.zero();
+  double track = 0.0;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n;
     int ev = -1;
     PState st;
     if (valid) {
-      load(in, i, st, record);
-      ev = segment<NS>(P, st, lc);
-      if (is_death(ev) && record) write_record(P, st, fate_of(ev));
+      load<REC>(in, i, st);
+      ev = segment<NS, REC>(P, st, track);
+      if (REC && is_death(ev)) write_record(P, st, fate_of(ev));
     }
+    count_segment(wc, valid, ev);
     const bool to_coll = valid && ev == EV_COLLIDE;
     const bool to_live = valid && (ev == EV_CROSS || ev == EV_REFLECT);
     const int64_t pc = wave_append(&C->q_count[qcoll], to_coll);
     const int64_t pl = wave_append(&C->q_count[qout], to_live);
-    if (to_coll) store(coll, pc, st, record);
-    if (to_live) store(out, pl, st, record);
+    if (to_coll) store<REC>(coll, pc, st);
+    if (to_live) store<REC>(out, pl, st);
   }
-  flush_counts(C, lc);
+  flush_counts(C, wc, track);
 }
 
-extern "C" __global__ void __launch_bounds__(256) k_collide(Params P, BankView coll, int qcoll, BankView out, int qout)
+template <bool REC>
+__global__ void __launch_bounds__(256) k_collide(Params P, BankView coll, int qcoll, BankView out, int qout)
 {
   Counters *C = P.cnt;
   const int64_t n = (int64_t)C->q_count[qcoll];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  LocalCount lc;
-  lc.zero();
-  const bool record = P.record != 0;
+  WaveCount wc;
+  wc.zero();
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n;
     int ev = -1;
     PState st;
     if (valid) {
-      load(coll, i, st, record);
-      ev = collide(P, st, lc, P.xs.hash);
-      if (ev != EV_SCATTER && record) write_record(P, st, fate_of(ev));
+      load<REC>(coll, i, st);
+      ev = collide<REC>(P, st, P.xs.hash);
+      if (REC && ev != EV_SCATTER) write_record(P, st, fate_of(ev));
     }
+    count_collision(wc, valid, ev);
     const bool to_live = valid && ev == EV_SCATTER;
     const int64_t pl = wave_append(&C->q_count[qout], to_live);
-    if (to_live) store(out, pl, st, record);
+    if (to_live) store<REC>(out, pl, st);
   }
-  flush_counts(C, lc);
+  flush_counts(C, wc, 0.0);
 }
 
 extern "C" __global__ void k_reset_work(Counters *C)
@@ -567,19 +581,22 @@ extern "C" __global__ void k_reset_queue(Counters *C, int q0, int q1)
 
 enum { PH_IDLE = 0, PH_TRANSPORT = 1, PH_COLLIDE = 2 };
 
-template <int NS>
-__global__ void __launch_bounds__(256) k_persistent(Params P, SourceBank S)
+// WPE: minimum wavefronts per SIMD requested from the register allocator
+// (0 = compiler's choice); a tuning knob trading spills for occupancy.
+template <int NS, bool REC, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
+k_persistent(Params P, SourceBank S)
 {
   __shared__ int32_t s_hash[MAX_HASH + 1];
   stage_hash(P.xs, s_hash);
   Counters *C = P.cnt;
-  LocalCount lc;
-  lc.zero();
+  WaveCount wc;
+  wc.zero();
+  double track = 0.0;
   PState st;
   st.elem = 0;
   int phase = PH_IDLE;
   bool exhausted = false;
-  const bool record = P.record != 0;
   const int64_t n_hist = P.hist_end - P.hist_begin;
   const int lanes_min = P.lanes_min < 1 ? 1 : (P.lanes_min > 64 ? 64 : P.lanes_min);
   for (;;) {
@@ -599,28 +616,34 @@ __global__ void __launch_bounds__(256) k_persistent(Params P, SourceBank S)
     if (__ballot(phase != PH_IDLE) == 0) break;
     // segments while enough lanes are in flight
     do {
-      if (phase == PH_TRANSPORT) {
-        const int ev = segment<NS>(P, st, lc);
+      const bool go = phase == PH_TRANSPORT;
+      int ev = -1;
+      if (go) {
+        ev = segment<NS, REC>(P, st, track);
         if (ev == EV_COLLIDE) {
           phase = PH_COLLIDE;
         } else if (is_death(ev)) {
-          if (record) write_record(P, st, fate_of(ev));
+          if (REC) write_record(P, st, fate_of(ev));
           phase = PH_IDLE;
         }
       }
+      count_segment(wc, go, ev);
     } while (wave_count(phase == PH_TRANSPORT) >= lanes_min);
     // collisions together
-    if (phase == PH_COLLIDE) {
-      const int ev = collide(P, st, lc, s_hash);
+    const bool hit = phase == PH_COLLIDE;
+    int ev = -1;
+    if (hit) {
+      ev = collide<REC>(P, st, s_hash);
       if (ev == EV_SCATTER) {
         phase = PH_TRANSPORT;
       } else {
-        if (record) write_record(P, st, fate_of(ev));
+        if (REC) write_record(P, st, fate_of(ev));
         phase = PH_IDLE;
       }
     }
+    count_collision(wc, hit, ev);
   }
-  flush_counts(C, lc);
+  flush_counts(C, wc, track);
 }
 
 // ------------------------------------------------------------------ utilities
@@ -684,13 +707,6 @@ extern "C" __global__ void k_xs_probe(XSView X, int32_t nuc, const double *E, in
   xs_out[4 * i + 3] = m.f;
 }
 
-template __global__ void k_segment<4>(Params, BankView, int, BankView, int, BankView, int);
-template __global__ void k_segment<6>(Params, BankView, int, BankView, int, BankView, int);
-template __global__ void k_segment<0>(Params, BankView, int, BankView, int, BankView, int);
-template __global__ void k_persistent<4>(Params, SourceBank);
-template __global__ void k_persistent<6>(Params, SourceBank);
-template __global__ void k_persistent<0>(Params, SourceBank);
-
 namespace mcw {
 // Host-side launch helpers (the templates live in this translation unit).
 static inline unsigned grid_for(int64_t n, int block, int64_t cap)
@@ -706,41 +722,69 @@ hipError_t launch_source_bank(hipStream_t s, const Params &P, const SourceBank &
   return hipGetLastError();
 }
 
+// Kernels are instantiated per side count (4 TET4, 6 HEX8, 0 mixed) and
+// record mode, so that neither costs registers where it is not used.
+#define MCW_BY_NS_REC(ns, rec, F, ...)                                                            \
+  do {                                                                                           \
+    if (rec) {                                                                                   \
+      if (ns == 4) F(4, true, __VA_ARGS__);                                                      \
+      else if (ns == 6) F(6, true, __VA_ARGS__);                                                 \
+      else F(0, true, __VA_ARGS__);                                                              \
+    } else {                                                                                     \
+      if (ns == 4) F(4, false, __VA_ARGS__);                                                     \
+      else if (ns == 6) F(6, false, __VA_ARGS__);                                                \
+      else F(0, false, __VA_ARGS__);                                                             \
+    }                                                                                            \
+  } while (0)
+
 hipError_t launch_bank_init(hipStream_t s, const Params &P, const SourceBank &S, const BankView &B, int64_t n, int q)
 {
-  hipLaunchKernelGGL(k_bank_init, dim3(grid_for(n, 256, 0)), dim3(256), 0, s, P, S, B, n, q);
+  if (P.record) hipLaunchKernelGGL(k_bank_init<true>, dim3(grid_for(n, 256, 0)), dim3(256), 0, s, P, S, B, n, q);
+  else hipLaunchKernelGGL(k_bank_init<false>, dim3(grid_for(n, 256, 0)), dim3(256), 0, s, P, S, B, n, q);
   return hipGetLastError();
 }
 
+#define MCW_SEG(NS, REC, ...) hipLaunchKernelGGL((k_segment<NS, REC>), __VA_ARGS__)
 hipError_t launch_segment(int ns, unsigned blocks, hipStream_t s, const Params &P, const BankView &in, int qin,
                           const BankView &out, int qout, const BankView &coll, int qcoll)
 {
-  if (ns == 4) hipLaunchKernelGGL(k_segment<4>, dim3(blocks), dim3(256), 0, s, P, in, qin, out, qout, coll, qcoll);
-  else if (ns == 6) hipLaunchKernelGGL(k_segment<6>, dim3(blocks), dim3(256), 0, s, P, in, qin, out, qout, coll, qcoll);
-  else hipLaunchKernelGGL(k_segment<0>, dim3(blocks), dim3(256), 0, s, P, in, qin, out, qout, coll, qcoll);
+  MCW_BY_NS_REC(ns, P.record, MCW_SEG, dim3(blocks), dim3(256), 0, s, P, in, qin, out, qout, coll, qcoll);
   return hipGetLastError();
 }
 
 hipError_t launch_collide(unsigned blocks, hipStream_t s, const Params &P, const BankView &coll, int qcoll,
                           const BankView &out, int qout)
 {
-  hipLaunchKernelGGL(k_collide, dim3(blocks), dim3(256), 0, s, P, coll, qcoll, out, qout);
+  if (P.record) hipLaunchKernelGGL(k_collide<true>, dim3(blocks), dim3(256), 0, s, P, coll, qcoll, out, qout);
+  else hipLaunchKernelGGL(k_collide<false>, dim3(blocks), dim3(256), 0, s, P, coll, qcoll, out, qout);
   return hipGetLastError();
 }
 
-hipError_t launch_persistent(int ns, unsigned blocks, int block, hipStream_t s, const Params &P, const SourceBank &S)
+#define MCW_PERS0(NS, REC, ...) hipLaunchKernelGGL((k_persistent<NS, REC, 0>), __VA_ARGS__)
+#define MCW_PERS5(NS, REC, ...) hipLaunchKernelGGL((k_persistent<NS, REC, 5>), __VA_ARGS__)
+hipError_t launch_persistent(int ns, int wpe, unsigned blocks, int block, hipStream_t s, const Params &P,
+                             const SourceBank &S)
 {
-  if (ns == 4) hipLaunchKernelGGL(k_persistent<4>, dim3(blocks), dim3(block), 0, s, P, S);
-  else if (ns == 6) hipLaunchKernelGGL(k_persistent<6>, dim3(blocks), dim3(block), 0, s, P, S);
-  else hipLaunchKernelGGL(k_persistent<0>, dim3(blocks), dim3(block), 0, s, P, S);
+  if (wpe == 5) MCW_BY_NS_REC(ns, P.record, MCW_PERS5, dim3(blocks), dim3(block), 0, s, P, S);
+  else MCW_BY_NS_REC(ns, P.record, MCW_PERS0, dim3(blocks), dim3(block), 0, s, P, S);
   return hipGetLastError();
 }
 
-const void *persistent_kernel(int ns)
+template <int WPE> const void *persistent_kernel_w(int ns, bool rec)
 {
-  if (ns == 4) return (const void *)k_persistent<4>;
-  if (ns == 6) return (const void *)k_persistent<6>;
-  return (const void *)k_persistent<0>;
+  if (rec) {
+    if (ns == 4) return (const void *)k_persistent<4, true, WPE>;
+    if (ns == 6) return (const void *)k_persistent<6, true, WPE>;
+    return (const void *)k_persistent<0, true, WPE>;
+  }
+  if (ns == 4) return (const void *)k_persistent<4, false, WPE>;
+  if (ns == 6) return (const void *)k_persistent<6, false, WPE>;
+  return (const void *)k_persistent<0, false, WPE>;
+}
+
+const void *persistent_kernel(int ns, int wpe, bool rec)
+{
+  return wpe == 5 ? persistent_kernel_w<5>(ns, rec) : persistent_kernel_w<0>(ns, rec);
 }
 
 hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1)

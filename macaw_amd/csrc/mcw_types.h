@@ -62,7 +62,7 @@ struct XSView {
   double e_min, e_max;
 };
 
-constexpr int MAX_HASH = 8192;
+constexpr int MAX_HASH = 8190; // LDS table of MAX_HASH + 1 ints: five 256-thread workgroups fit in 160 KB
 
 struct SourceView {
   int32_t space, angle, energy, pad;

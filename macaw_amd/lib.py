@@ -18,7 +18,7 @@ _lib = None
 
 # every symbol include/macaw_amd.h declares
 EXPORTS = (
-    "mcw_create", "mcw_destroy", "mcw_last_error", "mcw_default_options", "mcw_mesh_set", "mcw_bc_set",
+    "mcw_create", "mcw_destroy", "mcw_options_set", "mcw_last_error", "mcw_default_options", "mcw_mesh_set", "mcw_bc_set",
     "mcw_xs_set", "mcw_materials_set", "mcw_block_material_map", "mcw_source_set", "mcw_tally_add",
     "mcw_tally_shape", "mcw_run_batch", "mcw_tally_get", "mcw_tally_reset", "mcw_history_get",
     "mcw_server_create", "mcw_server_destroy", "mcw_server_last_error", "mcw_submit", "mcw_poll",
@@ -42,6 +42,7 @@ def lib():
     sig = {
         "mcw_create": ([C.POINTER(vp), C.POINTER(abi.Options)], C.c_int),
         "mcw_destroy": ([vp], C.c_int),
+        "mcw_options_set": ([vp, C.POINTER(abi.Options)], C.c_int),
         "mcw_last_error": ([vp], C.c_char_p),
         "mcw_default_options": ([C.POINTER(abi.Options)], None),
         "mcw_mesh_set": ([vp, i64, dp, i64, u8p, i64p, i32p, i32p, i32p], C.c_int),
@@ -119,6 +120,12 @@ class Context:
         self._marshal = load_into(calls, p)
         self.problem = p
         return self
+
+    def set_options(self, **kw):
+        """Change schedule/tuning options in place (mcw_options_set)."""
+        for k, v in kw.items():
+            setattr(self.opts, k, v)
+        self._chk(self.L.mcw_options_set(self.h, C.byref(self.opts)))
 
     def run_batch(self, batch, n_particles):
         st = abi.BatchStats()
