@@ -390,6 +390,15 @@ void rotate_angle(double *u, double *v, double *w, double mu, double xi)
   }
 }
 
+// Elastic scattering off a target at rest, isotropic in the CM frame:
+// E' = E (A^2 + 2 A mu + 1)/(A+1)^2, mu_lab = (1 + A mu)/sqrt(A^2 + 2 A mu + 1).
+void elastic_kinematics(double E, double A, double mu_cm, double *Ep, double *mu_lab)
+{
+  double q2 = (A * A + 2.0 * A * mu_cm) + 1.0;
+  *Ep = E * q2 / ((A + 1.0) * (A + 1.0));
+  *mu_lab = (1.0 + A * mu_cm) / std::sqrt(q2);
+}
+
 // ---------------------------------------------------------------- source
 
 struct Site {
@@ -702,17 +711,15 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
         fate = MCW_FATE_ABSORBED;
         ++cnt.absorptions;
       } else {
-        const double A = nuc.awr;
         double mu_cm = 2.0 * orc_prn(&seed) - 1.0;
-        double q2 = (A * A + 2.0 * A * mu_cm) + 1.0;
-        double Ep = E * q2 / ((A + 1.0) * (A + 1.0));
+        double Ep, mu_lab;
+        elastic_kinematics(E, nuc.awr, mu_cm, &Ep, &mu_lab);
         if (Ep < o.energy_min) {
           ev = EV_CUTOFF;
           fate = MCW_FATE_CUTOFF;
           E = Ep;
           ++cnt.cutoffs;
         } else {
-          double mu_lab = (1.0 + A * mu_cm) / std::sqrt(q2);
           double xi_phi = orc_prn(&seed);
           rotate_angle(&u, &v, &w, mu_lab, xi_phi);
           E = Ep;
@@ -1096,5 +1103,9 @@ uint64_t orc_future_seed_x(uint64_t n, uint64_t s) { return orc_future_seed(n, s
 uint64_t orc_init_seed_x(int64_t id, int32_t stream, uint64_t master) { return orc_init_seed(id, stream, master); }
 uint64_t orc_seed_roundtrip_x(uint64_t s) { return orc_seed_roundtrip(s); }
 void orc_rotate_angle_x(double *uvw, double mu, double xi) { rotate_angle(&uvw[0], &uvw[1], &uvw[2], mu, xi); }
+void orc_elastic_x(double E, double A, double mu_cm, double *Ep, double *mu_lab)
+{
+  elastic_kinematics(E, A, mu_cm, Ep, mu_lab);
+}
 
 } // extern "C"

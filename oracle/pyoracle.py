@@ -75,6 +75,7 @@ def lib():
         L.orc_seed_roundtrip_x.restype = u64
         L.orc_seed_roundtrip_x.argtypes = [u64]
         L.orc_rotate_angle_x.argtypes = [dp, C.c_double, C.c_double]
+        L.orc_elastic_x.argtypes = [C.c_double, C.c_double, C.c_double, dp, dp]
         _lib = L
     return _lib
 
@@ -234,3 +235,9 @@ def rotate_angle(u, mu, xi):
     a = np.ascontiguousarray(u, np.float64).copy()
     lib().orc_rotate_angle_x(a.ctypes.data_as(C.POINTER(C.c_double)), mu, xi)
     return a
+
+
+def elastic(E, A, mu_cm):
+    ep, ml = C.c_double(), C.c_double()
+    lib().orc_elastic_x(E, A, mu_cm, C.byref(ep), C.byref(ml))
+    return ep.value, ml.value

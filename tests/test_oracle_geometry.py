@@ -1,0 +1,190 @@
+"""The oracle's mesh derivation and element walk against brute force and
+exact (rational) arithmetic: neighbours by face matching, planes from the
+elements' own vertices, Amanatides-Woo traversal of hex grids, exact ray/tet
+traversal of a Delaunay mesh, volume CDF, and segment lengths summing to the
+straight-line distance to the boundary."""
+from fractions import Fraction as F
+
+import numpy as np
+import pytest
+
+from macaw_amd import abi, problems as P
+
+TET_SIDES = [(0, 2, 1), (0, 1, 3), (1, 2, 3), (2, 0, 3)]
+HEX_SIDES = [(0, 3, 2, 1), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7), (4, 5, 6, 7)]
+
+
+def void_problem(mesh, src_point, bcs):
+    """Nearly void medium (no collisions) so that histories are straight lines."""
+    nuc = P.NuclideData(awr=12.0, energy=np.array([1e-5, 2e7]), total=np.array([1e-30, 1e-30]),
+                        absorption=np.array([1e-31, 1e-31]), scatter=np.array([9e-31, 9e-31]))
+    mat = P.MaterialData(id=1, nuclides=np.array([0], np.int32), densities=np.array([1e-10]))
+    blocks = {int(b): 1 for b in np.unique(mesh.subdomain)}
+    return P.Problem(name="void", mesh=mesh, nuclides=[nuc], materials=[mat], block_material=blocks,
+                     bcs=bcs, source=P.point_source(src_point, 1e6), tallies=P.flux_tallies(both=False),
+                     n_particles=100, n_batches=1)
+
+
+def elem_nodes(mesh, e):
+    return mesh.conn[mesh.conn_offsets[e]:mesh.conn_offsets[e + 1]]
+
+
+def test_neighbours_and_planes_brute_force(oracle_lib):
+    mesh = P.tet_cube(6, seed=5, cache=False)
+    o = oracle_lib.Oracle(void_problem(mesh, (0.41, 0.43, 0.47), {0: abi.BC_VACUUM}))
+    nb, side = o.neighbors()
+    planes = o.planes()
+    faces = {}
+    for e in range(mesh.n_elem):
+        c = elem_nodes(mesh, e)
+        for s, sn in enumerate(TET_SIDES):
+            faces.setdefault(frozenset(int(c[k]) for k in sn), []).append((e, s))
+    for key, lst in faces.items():
+        assert len(lst) in (1, 2)
+        if len(lst) == 2:
+            (ea, sa), (eb, sb) = lst
+            assert nb[4 * ea + sa] == eb and side[4 * ea + sa] == sb
+            assert nb[4 * eb + sb] == ea and side[4 * eb + sb] == sa
+            # one canonical plane, opposite signs: bit-exact negation
+            assert np.array_equal(planes[4 * ea + sa], -planes[4 * eb + sb])
+        else:
+            (ea, sa), = lst
+            assert nb[4 * ea + sa] == -1  # boundary id 0
+    X = mesh.xyz
+    assert np.allclose(np.linalg.norm(planes[:, :3], axis=1), 1.0, atol=1e-15)
+    for e in range(mesh.n_elem):
+        c = elem_nodes(mesh, e)
+        pts = X[c]
+        for s, sn in enumerate(TET_SIDES):
+            q = planes[4 * e + s]
+            d = pts @ q[:3] - q[3]
+            assert np.all(np.abs(d[list(sn)]) < 1e-14)        # the side's nodes lie on it
+            assert d[[k for k in range(4) if k not in sn][0]] < -1e-6  # the opposite node inside
+    cdf = o.cdf()
+    assert abs(cdf[-1] / 6.0 - 1.0) < 1e-12
+    assert np.all(np.diff(cdf) > 0)
+
+
+def dda_cells(p, u, n, h):
+    """Exact Amanatides-Woo walk of a ray through an n^3 grid of cells of size h
+    on [0, n h]^3: list of (cell index, length)."""
+    p = [F(x) for x in p]
+    u = [F(x) for x in u]
+    h = F(h)
+    cell = [int(p[k] // h) for k in range(3)]
+    out = []
+    t = F(0)
+    while all(0 <= cell[k] < n for k in range(3)):
+        tn = []
+        for k in range(3):
+            if u[k] > 0:
+                tn.append(((cell[k] + 1) * h - p[k]) / u[k])
+            elif u[k] < 0:
+                tn.append((cell[k] * h - p[k]) / u[k])
+            else:
+                tn.append(None)
+        k = min((k for k in range(3) if tn[k] is not None), key=lambda k: tn[k])
+        out.append((cell[0] + n * (cell[1] + n * cell[2]), tn[k] - t))
+        t = tn[k]
+        cell[k] += 1 if u[k] > 0 else -1
+    return out, t
+
+
+def test_hex_walk_matches_amanatides_woo(oracle_lib):
+    n, L = 4, 1.0
+    mesh = P.hex_box(n, 0.0, L)
+    src = (0.3, 0.55, 0.8)
+    prob = void_problem(mesh, src, {b: abi.BC_VACUUM for b in range(6)})
+    o = oracle_lib.Oracle(prob)
+    sites = o.sources(1, prob.n_particles, 60)
+    for i in range(60):
+        el, ev, le = o.trace(1, prob.n_particles, i)
+        want, t_exit = dda_cells(sites[i, :3], sites[i, 3:6], n, L / n)
+        assert list(el) == [c for c, _ in want], i
+        assert list(ev[:-1]) == [0] * (len(ev) - 1) and ev[-1] == 2  # crossings, then leak
+        assert np.allclose(le, [float(x) for _, x in want], rtol=0, atol=1e-13)
+        assert abs(le.sum() - float(t_exit)) < 1e-13
+
+
+def exact_tet_walk(mesh, faces, e, p, u):
+    """Exact rational walk through a tet mesh from point p in element e."""
+    X = mesh.xyz
+    p = [F(x) for x in p]
+    u = [F(x) for x in u]
+    seq = []
+    incoming = None
+    while True:
+        c = [int(v) for v in elem_nodes(mesh, e)]
+        V = [[F(float(X[v][k])) for k in range(3)] for v in c]
+        best, bf = None, None
+        for s, sn in enumerate(TET_SIDES):
+            if s == incoming:
+                continue
+            a, b, cc = (V[k] for k in sn)
+            opp = V[[k for k in range(4) if k not in sn][0]]
+            ab = [b[k] - a[k] for k in range(3)]
+            ac = [cc[k] - a[k] for k in range(3)]
+            nrm = [ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]]
+            if sum(nrm[k] * (opp[k] - a[k]) for k in range(3)) > 0:
+                nrm = [-x for x in nrm]
+            den = sum(nrm[k] * u[k] for k in range(3))
+            if den <= 0:
+                continue
+            t = sum(nrm[k] * (a[k] - p[k]) for k in range(3)) / den
+            t = max(t, F(0))
+            if best is None or t < best:
+                best, bf = t, s
+        seq.append(e)
+        p = [p[k] + best * u[k] for k in range(3)]
+        key = frozenset(c[k] for k in TET_SIDES[bf])
+        nxt = [x for x in faces[key] if x[0] != e]
+        if not nxt:
+            return seq
+        e, incoming = nxt[0]
+
+
+def test_tet_walk_matches_exact_arithmetic(oracle_lib):
+    mesh = P.tet_cube(5, seed=7, cache=False)
+    prob = void_problem(mesh, (0.41, 0.43, 0.47), {0: abi.BC_VACUUM})
+    o = oracle_lib.Oracle(prob)
+    faces = {}
+    for e in range(mesh.n_elem):
+        c = elem_nodes(mesh, e)
+        for s, sn in enumerate(TET_SIDES):
+            faces.setdefault(frozenset(int(c[k]) for k in sn), []).append((e, s))
+    sites = o.sources(1, prob.n_particles, 25)
+    for i in range(25):
+        el, ev, le = o.trace(1, prob.n_particles, i)
+        want = exact_tet_walk(mesh, faces, int(sites[i, 7]), sites[i, :3], sites[i, 3:6])
+        assert list(el) == want, i
+        # total length = distance from the source to the cube's surface along u
+        p, u = sites[i, :3], sites[i, 3:6]
+        tb = min(((1.0 - p[k]) / u[k]) if u[k] > 0 else (-p[k] / u[k]) for k in range(3) if u[k] != 0)
+        assert abs(le.sum() - tb) < 1e-12
+
+
+def test_point_location_lowest_index_rule(oracle_lib):
+    mesh = P.hex_box(2, 0.0, 1.0)
+    prob = void_problem(mesh, (0.5, 0.5, 0.5), {b: abi.BC_VACUUM for b in range(6)})
+    o = oracle_lib.Oracle(prob)
+    import ctypes as C
+    pt = (C.c_double * 3)(0.5, 0.5, 0.5)
+    # the centre is a vertex of all 8 elements: the lowest index claims it
+    assert o.L.orc_locate(o.h, pt, 1e-12) == 0
+    pt = (C.c_double * 3)(0.75, 0.75, 0.25)
+    assert o.L.orc_locate(o.h, pt, 1e-12) == 3
+    pt = (C.c_double * 3)(1.5, 0.5, 0.5)
+    assert o.L.orc_locate(o.h, pt, 1e-12) == -1
+
+
+def test_reflection_keeps_unit_direction(oracle_lib):
+    """Reflective box, no collisions: each history runs until the segment cap;
+    the walk never loses the particle and every segment stays in the box."""
+    mesh = P.hex_box(3, -1.0, 1.0)
+    prob = void_problem(mesh, (0.1, -0.2, 0.3), {b: abi.BC_REFLECT for b in range(6)})
+    o = oracle_lib.Oracle(prob, max_segments=400)
+    st = o.run_batch(1, 50, threads=1, record=True)
+    h = o.histories(50)
+    assert st["lost"] == 50 and st["leaks"] == 0 and st["reflections"] > 0
+    assert np.all(h["n_segments"] == 400)
+    assert np.all(np.abs([h["x"], h["y"], h["z"]]) <= 1.0 + 1e-12)
