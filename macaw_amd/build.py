@@ -54,6 +54,21 @@ def build_library(verbose=False):
     return LIB
 
 
+HOST = os.path.join(HERE, "host")
+DRIVER = os.path.join(LIBDIR, "macaw-amd-opt")
+
+
+def build_driver():
+    """The MOOSE-shaped host layer and its input driver (host C++ only, linked
+    against the library)."""
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+    tmp = DRIVER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wno-unused-parameter", "-I", os.path.join(ROOT, "include")] + srcs
+         + ["-o", tmp, "-L", LIBDIR, "-lmacaw_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build_oracle():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 
@@ -61,6 +76,7 @@ def build_oracle():
 def main():
     lib = build_library(verbose="-v" in sys.argv)
     print("built", lib)
+    print("built", build_driver())
     if "--no-oracle" not in sys.argv:
         build_oracle()
         print("built oracle")

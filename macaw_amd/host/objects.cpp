@@ -1,0 +1,869 @@
+// objects.cpp -- MOOSE-shaped host objects (see objects.h).
+#include "objects.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <set>
+#include <sstream>
+
+namespace mcwhost {
+
+// ------------------------------------------------------------------ parameters
+
+std::string InputParameters::getString(const std::string &k) const { return node_->get(k); }
+std::string InputParameters::getString(const std::string &k, const std::string &d) const
+{
+  return node_->has(k) ? node_->get(k) : d;
+}
+std::vector<std::string> InputParameters::getWords(const std::string &k) const
+{
+  return node_->has(k) ? split_words(node_->get(k)) : std::vector<std::string>{};
+}
+
+static long long to_int(const InputParameters &p, const std::string &k, const std::string &v)
+{
+  try {
+    size_t pos = 0;
+    long long x = std::stoll(v, &pos);
+    if (pos != v.size()) throw std::invalid_argument(v);
+    return x;
+  } catch (const std::exception &) {
+    paramError(p, k, "expected an integer, got '" + v + "'");
+  }
+}
+
+static double to_real(const InputParameters &p, const std::string &k, const std::string &v)
+{
+  try {
+    size_t pos = 0;
+    double x = std::stod(v, &pos);
+    if (pos != v.size()) throw std::invalid_argument(v);
+    return x;
+  } catch (const std::exception &) {
+    paramError(p, k, "expected a number, got '" + v + "'");
+  }
+}
+
+long long InputParameters::getInt(const std::string &k) const { return to_int(*this, k, node_->get(k)); }
+long long InputParameters::getInt(const std::string &k, long long d) const { return node_->has(k) ? getInt(k) : d; }
+double InputParameters::getReal(const std::string &k) const { return to_real(*this, k, node_->get(k)); }
+double InputParameters::getReal(const std::string &k, double d) const { return node_->has(k) ? getReal(k) : d; }
+bool InputParameters::getBool(const std::string &k, bool d) const
+{
+  if (!node_->has(k)) return d;
+  std::string v = node_->get(k);
+  std::transform(v.begin(), v.end(), v.begin(), ::tolower);
+  if (v == "true" || v == "1" || v == "yes") return true;
+  if (v == "false" || v == "0" || v == "no") return false;
+  paramError(*this, k, "expected a boolean, got '" + v + "'");
+}
+std::vector<long long> InputParameters::getInts(const std::string &k) const
+{
+  std::vector<long long> out;
+  for (auto &w : getWords(k)) out.push_back(to_int(*this, k, w));
+  return out;
+}
+std::vector<double> InputParameters::getReals(const std::string &k) const
+{
+  std::vector<double> out;
+  for (auto &w : getWords(k)) out.push_back(to_real(*this, k, w));
+  return out;
+}
+
+void paramError(const InputParameters &p, const std::string &param, const std::string &msg)
+{
+  throw MooseError(p.path() + "/" + param + ": " + msg);
+}
+
+void mooseError(const std::string &msg) { throw MooseError(msg); }
+
+static void check(mcw_ctx *ctx, int rc, const std::string &what)
+{
+  if (rc != 0) mooseError(what + ": " + (ctx ? mcw_last_error(ctx) : "no context"));
+}
+
+// ------------------------------------------------------------------ mesh
+
+// GeneratedMeshGenerator boundary ids (MOOSE): back, bottom, right, top, left, front.
+static const char *GEN_BOUNDARY[6] = {"back", "bottom", "right", "top", "left", "front"};
+// Kuhn split of a hex (libMesh node order) into 6 tets around diagonal 0-6.
+static const int KUHN[6][4] = {{0, 1, 2, 6}, {0, 2, 3, 6}, {0, 3, 7, 6}, {0, 7, 4, 6}, {0, 4, 5, 6}, {0, 5, 1, 6}};
+static const int TET_SIDE[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}};
+
+static void generated_mesh(const InputParameters &p, MeshData &m)
+{
+  if (p.getInt("dim", 3) != 3) paramError(p, "dim", "only dim = 3 meshes are traced here");
+  const long long nx = p.getInt("nx", 1), ny = p.getInt("ny", 1), nz = p.getInt("nz", 1);
+  const double x0 = p.getReal("xmin", 0), x1 = p.getReal("xmax", 1), y0 = p.getReal("ymin", 0),
+               y1 = p.getReal("ymax", 1), z0 = p.getReal("zmin", 0), z1 = p.getReal("zmax", 1);
+  const std::string et = p.getString("elem_type", "HEX8");
+  if (et != "HEX8" && et != "TET4") paramError(p, "elem_type", "supported element types: HEX8, TET4");
+  m = MeshData();
+  for (long long k = 0; k <= nz; ++k)
+    for (long long j = 0; j <= ny; ++j)
+      for (long long i = 0; i <= nx; ++i) {
+        m.xyz.push_back(x0 + (x1 - x0) * i / nx);
+        m.xyz.push_back(y0 + (y1 - y0) * j / ny);
+        m.xyz.push_back(z0 + (z1 - z0) * k / nz);
+      }
+  auto nid = [&](long long i, long long j, long long k) { return (int32_t)(i + (nx + 1) * (j + (ny + 1) * k)); };
+  for (int b = 0; b < 6; ++b) m.boundary_names[GEN_BOUNDARY[b]] = b;
+  for (long long k = 0; k < nz; ++k)
+    for (long long j = 0; j < ny; ++j)
+      for (long long i = 0; i < nx; ++i) {
+        int32_t h[8] = {nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                        nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)};
+        if (et == "HEX8") {
+          m.type.push_back(MCW_HEX8);
+          m.conn.insert(m.conn.end(), h, h + 8);
+          m.conn_off.push_back((int64_t)m.conn.size());
+          m.subdomain.push_back(0);
+          for (int s = 0; s < 6; ++s) m.face_boundary.push_back(s);
+        } else {
+          for (int t = 0; t < 6; ++t) {
+            int32_t tet[4];
+            for (int q = 0; q < 4; ++q) tet[q] = h[KUHN[t][q]];
+            m.type.push_back(MCW_TET4);
+            m.conn.insert(m.conn.end(), tet, tet + 4);
+            m.conn_off.push_back((int64_t)m.conn.size());
+            m.subdomain.push_back(0);
+            // boundary id of an external side: the cube face it lies on
+            for (int s = 0; s < 4; ++s) {
+              double c[3] = {0, 0, 0};
+              for (int q = 0; q < 3; ++q)
+                for (int d = 0; d < 3; ++d) c[d] += m.xyz[3 * (int64_t)tet[TET_SIDE[s][q]] + d] / 3.0;
+              int b = 0;
+              const double tol = 1e-12 * (std::fabs(x1 - x0) + std::fabs(y1 - y0) + std::fabs(z1 - z0));
+              if (std::fabs(c[2] - z0) < tol) b = 0;
+              else if (std::fabs(c[1] - y0) < tol) b = 1;
+              else if (std::fabs(c[0] - x1) < tol) b = 2;
+              else if (std::fabs(c[1] - y1) < tol) b = 3;
+              else if (std::fabs(c[0] - x0) < tol) b = 4;
+              else if (std::fabs(c[2] - z1) < tol) b = 5;
+              m.face_boundary.push_back(b);
+            }
+          }
+        }
+      }
+}
+
+static void subdomain_bbox(const InputParameters &p, MeshData &m)
+{
+  auto bl = p.getReals("bottom_left"), tr = p.getReals("top_right");
+  if (bl.size() != 3 || tr.size() != 3) paramError(p, "bottom_left", "bottom_left and top_right need 3 coordinates");
+  const int32_t id = (int32_t)p.getInt("block_id");
+  const bool inside = p.getString("location", "INSIDE") == "INSIDE";
+  for (int64_t e = 0; e < m.n_elem(); ++e) {
+    double c[3] = {0, 0, 0};
+    const int nn = (int)(m.conn_off[e + 1] - m.conn_off[e]);
+    for (int64_t q = m.conn_off[e]; q < m.conn_off[e + 1]; ++q)
+      for (int d = 0; d < 3; ++d) c[d] += m.xyz[3 * (int64_t)m.conn[q] + d] / nn;
+    bool in = true;
+    for (int d = 0; d < 3; ++d) in &= c[d] >= std::min(bl[d], tr[d]) && c[d] <= std::max(bl[d], tr[d]);
+    if (in == inside) m.subdomain[e] = id;
+  }
+}
+
+MeshData build_mesh(const HitNode &mb)
+{
+  MeshData m;
+  bool any = false;
+  for (auto &c : mb.children) {
+    InputParameters p(c.get());
+    const std::string type = p.getString("type");
+    if (type == "GeneratedMeshGenerator") {
+      generated_mesh(p, m);
+      any = true;
+    } else if (type == "SubdomainBoundingBoxGenerator") {
+      if (!any) paramError(p, "input", "needs a mesh generated before it");
+      subdomain_bbox(p, m);
+    } else {
+      paramError(p, "type", "unsupported mesh generator '" + type + "'");
+    }
+  }
+  if (!any) mooseError("[Mesh] does not generate a mesh");
+  return m;
+}
+
+// ------------------------------------------------------------------ synthetic nuclides
+
+// Pointwise nuclide: 1/v absorption plus single-level Breit-Wigner resonances
+// and constant potential scattering on a log grid; resonance parameters are
+// drawn from the OpenMC generator seeded with init_seed(seed, 0).
+static NuclideData synthetic_nuclide(const InputParameters &p)
+{
+  NuclideData n;
+  n.name = p.path().substr(p.path().rfind('/') + 1);
+  n.awr = p.getReal("awr");
+  const long long np = p.getInt("n_points", 20000);
+  const double e0 = p.getReal("e_min", 1e-5), e1 = p.getReal("e_max", 2e7);
+  const double sa0 = p.getReal("sigma_a0", 1.0), sp = p.getReal("sigma_p", 4.7);
+  const long long nres = p.getInt("resonances", 50);
+  if (np < 2) paramError(p, "n_points", "a grid needs at least 2 points");
+  uint64_t s = (uint64_t)p.getInt("seed", 1) * 152917ull;
+  auto prn = [&]() {
+    s = 6364136223846793005ull * s + 1442695040888963407ull;
+    uint64_t w = ((s >> ((s >> 59u) + 5u)) ^ s) * 12605985483714917081ull;
+    return (double)((w >> 43u) ^ w) * 0x1p-64;
+  };
+  std::vector<double> Er(nres), G(nres), pk(nres), fg(nres);
+  for (long long r = 0; r < nres; ++r) {
+    Er[r] = std::exp(std::log(1.0) + prn() * (std::log(1e5) - std::log(1.0)));
+    G[r] = std::exp(std::log(1e-2) + prn() * (std::log(10.0) - std::log(1e-2)));
+    pk[r] = std::exp(std::log(10.0) + prn() * (std::log(1e4) - std::log(10.0)));
+    fg[r] = 0.2 + 0.6 * prn();
+  }
+  n.E.resize(np);
+  n.t.resize(np);
+  n.a.resize(np);
+  n.s.resize(np);
+  for (long long j = 0; j < np; ++j) {
+    double E = j == 0 ? e0 : (j == np - 1 ? e1 : e0 * std::pow(e1 / e0, (double)j / (np - 1)));
+    double a = sa0 * std::sqrt(0.0253 / E), sc = sp;
+    for (long long r = 0; r < nres; ++r) {
+      double x = 2.0 * (E - Er[r]) / G[r];
+      double shape = pk[r] / (1.0 + x * x);
+      a += shape * fg[r] * std::sqrt(Er[r] / E);
+      sc += shape * (1.0 - fg[r]);
+    }
+    n.E[j] = E;
+    n.a[j] = a;
+    n.s[j] = sc;
+    n.t[j] = a + sc;
+  }
+  return n;
+}
+
+// ------------------------------------------------------------------ CollisionKernel
+
+CollisionKernel::CollisionKernel(const InputParameters &p, Problem &fe) : _params(p), _fe(fe)
+{
+  _verbose = p.getBool("verbose", false);
+  const std::string T = p.getString("temperature");
+  auto it = fe.aux_vars.find(T);
+  if (it == fe.aux_vars.end()) paramError(p, "temperature", "no variable named '" + T + "'");
+  // Check that the temperature variable is a constant monomial (CollisionKernel.C:52-57)
+  if (it->second != 0)
+    paramError(p, "temperature", "Only CONST MONOMIAL temperatures (current order: " + std::to_string(it->second) +
+                                     ") are supported.");
+  auto blocks = p.getInts("blocks");
+  auto materials = p.getInts("materials");
+  if (blocks.size() != materials.size())
+    paramError(p, "blocks", "The blocks parameter must be the same size as the materials parameter.");
+  for (size_t i = 0; i < blocks.size(); ++i) _block_to_openmc_materials.emplace((int)blocks[i], (int)materials[i]);
+}
+
+void CollisionKernel::initialSetup()
+{
+  // Check that all materials do exist, otherwise it will crash at XS calculation
+  for (auto &m : _block_to_openmc_materials) {
+    bool found = false;
+    for (auto &mat : _fe.materials) found |= mat.id == m.second;
+    if (!found) mooseError("Could not find material " + std::to_string(m.second) + " in OpenMC materials.");
+  }
+  std::set<int> present(_fe.mesh.subdomain.begin(), _fe.mesh.subdomain.end());
+  for (auto &b : _block_to_openmc_materials)
+    if (!present.count(b.first))
+      std::cerr << "*** Warning *** Could not find specified block " << b.first << " for the collision kernel.\n";
+}
+
+// ------------------------------------------------------------------ RayBCs
+
+RayBC::RayBC(const InputParameters &p, Problem &fe, int kind) : _kind(kind)
+{
+  for (auto &name : p.getWords("boundary")) {
+    auto it = fe.mesh.boundary_names.find(name);
+    if (it != fe.mesh.boundary_names.end()) {
+      _ids.push_back(it->second);
+      continue;
+    }
+    try {
+      _ids.push_back((int32_t)std::stoi(name));
+    } catch (const std::exception &) {
+      paramError(p, "boundary", "the boundary '" + name + "' was not found in the mesh");
+    }
+  }
+}
+
+void RayBC::apply(mcw_ctx *ctx) const
+{
+  for (int32_t b : _ids) check(ctx, mcw_bc_set(ctx, b, _kind), "RayBC");
+}
+
+// ------------------------------------------------------------------ OpenMCTally
+
+// OpenMCTally.h:71-88: universe < cell < energy < others
+static int filter_rank(const std::string &f)
+{
+  if (f == "universe") return 0;
+  if (f == "cell") return 1;
+  if (f == "energy") return 2;
+  return 3;
+}
+
+OpenMCTally::OpenMCTally(const InputParameters &p, Problem &fe) : _params(p), _fe(fe)
+{
+  _id = (int)p.getInt("id");
+  const std::string particle = p.getString("particle_type", "neutron");
+  const std::string est = p.getString("estimator", "COLLISION");
+  _scores = p.getWords("scores");
+  _filters = p.getWords("filters");
+  if (!p.isParamValid("scores")) paramError(p, "scores", "missing required parameter");
+  if (!p.isParamValid("filters")) paramError(p, "filters", "missing required parameter");
+  _nuclides = p.isParamValid("nuclides") ? p.getWords("nuclides") : std::vector<std::string>{"total"};
+  _energy_bins = p.getReals("energy_bins");
+  _cell_bins = p.getInts("cell_bins");
+  _block_bins = p.getInts("block_bins");
+  // Parameter checks (OpenMCTally.C:87-101); the track-length estimator is
+  // supported here as well as the collision estimator
+  if (particle != "neutron") paramError(p, "particle_type", "Only neutrons are currently supported.");
+  if (est == "COLLISION") _estimator = MCW_EST_COLLISION;
+  else if (est == "TRACKLENGTH") _estimator = MCW_EST_TRACKLENGTH;
+  else paramError(p, "estimator", "Only collision and tracklength estimators are currently supported");
+  const std::string exec = p.getString("execute_on", "initial");
+  std::string lower = exec;
+  std::transform(lower.begin(), lower.end(), lower.begin(), ::tolower);
+  if (lower != "initial")
+    paramError(p, "execute_on",
+               "execute_on must be INITIAL to ensure tallies are created once at the beginning of the simulation");
+  if (std::find(_filters.begin(), _filters.end(), "energy") != _filters.end() && !p.isParamValid("energy_bins"))
+    mooseError("The energy bins parameter should be supplied if an energy filter is requested.");
+}
+
+void OpenMCTally::initialize()
+{
+  // Sort the filters into a specific order
+  std::stable_sort(_filters.begin(), _filters.end(),
+                   [](const std::string &a, const std::string &b) { return filter_rank(a) < filter_rank(b); });
+  std::vector<mcw_filter> f(_filters.size());
+  std::vector<std::vector<int64_t>> ids(_filters.size());
+  _filter_bins.clear();
+  _universe_ids.clear();
+  for (size_t i = 0; i < _filters.size(); ++i) {
+    const std::string &name = _filters[i];
+    memset(&f[i], 0, sizeof f[i]);
+    if (name == "energy") {
+      f[i].type = MCW_FILTER_ENERGY;
+      f[i].n_bins = (int32_t)_energy_bins.size();
+      f[i].edges = _energy_bins.data();
+      _filter_bins.push_back((int64_t)_energy_bins.size() - 1);
+    } else if (name == "particle") {
+      f[i].type = MCW_FILTER_PARTICLE;
+      _filter_bins.push_back(1);
+    } else if (name == "universe") {
+      f[i].type = MCW_FILTER_UNIVERSE;
+      if (_params.isParamValid("block_bins")) {
+        for (auto b : _block_bins) ids[i].push_back(b);
+      } else {
+        std::set<int64_t> s(_fe.mesh.subdomain.begin(), _fe.mesh.subdomain.end());
+        ids[i].assign(s.begin(), s.end());
+      }
+      _universe_ids = ids[i];
+      f[i].n_bins = (int32_t)ids[i].size();
+      f[i].ids = ids[i].data();
+      _filter_bins.push_back((int64_t)ids[i].size());
+    } else if (name == "cell") {
+      f[i].type = MCW_FILTER_CELL;
+      if (_params.isParamValid("cell_bins"))
+        for (auto c : _cell_bins) ids[i].push_back(c);
+      else
+        for (int64_t e = 0; e < _fe.mesh.n_elem(); ++e) ids[i].push_back(e);
+      f[i].n_bins = (int32_t)ids[i].size();
+      f[i].ids = ids[i].data();
+      _filter_bins.push_back((int64_t)ids[i].size());
+      _cell_pos.clear();
+      for (int64_t b = 0; b < (int64_t)ids[i].size(); ++b) _cell_pos.emplace(ids[i][b], b);
+    } else {
+      mooseError("Unrecognized filter");
+    }
+  }
+  _strides.assign(_filters.size(), 0);
+  int64_t stride = 1;
+  for (int k = (int)_filters.size() - 1; k >= 0; --k) {
+    _strides[k] = stride;
+    stride *= _filter_bins[k];
+  }
+  std::vector<int32_t> sc;
+  for (auto &s : _scores) {
+    if (s == "flux") sc.push_back(MCW_SCORE_FLUX);
+    else if (s == "total") sc.push_back(MCW_SCORE_TOTAL);
+    else if (s == "absorption") sc.push_back(MCW_SCORE_ABSORPTION);
+    else if (s == "scatter" || s == "elastic") sc.push_back(MCW_SCORE_SCATTER);
+    else if (s == "fission") sc.push_back(MCW_SCORE_FISSION);
+    else paramError(_params, "scores", "unknown score '" + s + "'");
+  }
+  // nuclide bins: "all" = every nuclide + total, "total" alone, else the list + total
+  _nuclide_idx.clear();
+  auto index_of = [&](const std::string &n) {
+    for (size_t q = 0; q < _fe.nuclides.size(); ++q)
+      if (_fe.nuclides[q].name == n) return (int32_t)q;
+    paramError(_params, "nuclides", "unknown nuclide '" + n + "'");
+  };
+  if (_nuclides[0] == "all") {
+    for (size_t q = 0; q < _fe.nuclides.size(); ++q) _nuclide_idx.push_back((int32_t)q);
+    _nuclide_idx.push_back(-1);
+  } else if (_nuclides[0] != "total") {
+    for (auto &n : _nuclides) _nuclide_idx.push_back(index_of(n));
+    _nuclide_idx.push_back(-1);
+  } else {
+    _nuclide_idx.push_back(-1);
+  }
+  check(_fe.study->ctx(),
+        mcw_tally_add(_fe.study->ctx(), _id, _estimator, (int32_t)f.size(), f.data(), (int32_t)sc.size(), sc.data(),
+                      (int32_t)_nuclide_idx.size(), _nuclide_idx.data()),
+        "OpenMCTally " + _params.path());
+}
+
+int64_t OpenMCTally::cellBin(int64_t elem) const
+{
+  auto it = _cell_pos.find(elem);
+  return it == _cell_pos.end() ? -1 : it->second;
+}
+
+// ------------------------------------------------------------------ OpenMCTallyAux
+
+OpenMCTallyAux::OpenMCTallyAux(const InputParameters &p, Problem &fe) : _params(p), _fe(fe)
+{
+  if (!p.isParamValid("tally_id")) mooseError("Use tally ids, for now");
+  _tally_id = (int)p.getInt("tally_id");
+  const std::string g = p.getString("granularity");
+  if (g == "global") _granularity = 0;
+  else if (g == "universe") _granularity = 1;
+  else if (g == "cell") _granularity = 2;
+  else paramError(p, "granularity", "granularity must be one of global, universe, cell");
+  _score = p.getString("score");
+  p.getString("estimator");
+  p.getString("particle_type");
+  _nuclide = p.getString("nuclide", "all");
+  _energy_bin = (int)p.getInt("energy_bin", -1);
+  _variable = p.getString("variable");
+}
+
+double OpenMCTallyAux::computeValue(int64_t elem, const std::vector<double> &sum) const
+{
+  const OpenMCTally *t = _fe.tally(_tally_id);
+  if (!t) mooseError("no tally with id " + std::to_string(_tally_id));
+  // nuclide bin: the last bin is the total when individual nuclides are binned
+  const std::vector<int32_t> &nb = t->nuclideIndices();
+  int nuc_bin = 0;
+  if (nb.size() > 1) {
+    if (_nuclide == "all") {
+      nuc_bin = (int)nb.size() - 1;
+    } else {
+      nuc_bin = -1;
+      for (size_t q = 0; q + 1 < nb.size(); ++q)
+        if (_fe.nuclides[nb[q]].name == _nuclide) nuc_bin = (int)q;
+      if (nuc_bin < 0) paramError(_params, "nuclide", "nuclide '" + _nuclide + "' is not binned in the tally");
+    }
+  }
+  auto it = std::find(t->scores().begin(), t->scores().end(), _score);
+  if (it == t->scores().end()) paramError(_params, "score", "The specified score does not exist in given tally");
+  const int score_bin = (int)(it - t->scores().begin());
+  // OpenMC results_ layout: nuclide-major score bins
+  const int64_t score_index = (int64_t)nuc_bin * (int64_t)t->scores().size() + score_bin;
+  const int64_t nsb = t->nScoreBins();
+
+  int64_t univ_end = 1, univ_stride = 1, energy_start = 0, energy_end = 1, energy_stride = 1, cell_stride = 1;
+  int64_t cell_bin = 0, univ_bin = -1;
+  bool has_cell = false, has_univ = false;
+  for (size_t i = 0; i < t->filters().size(); ++i) {
+    const std::string &f = t->filters()[i];
+    if (f == "universe") {
+      univ_end = t->filterBins()[i];
+      univ_stride = t->strides()[i];
+      has_univ = true;
+      const auto &ids = t->universeIds();
+      auto u = std::find(ids.begin(), ids.end(), (int64_t)_fe.mesh.subdomain[elem]);
+      univ_bin = u == ids.end() ? -1 : (int64_t)(u - ids.begin());
+    } else if (f == "energy") {
+      if (_energy_bin < 0) {
+        energy_end = t->filterBins()[i];
+      } else {
+        energy_start = _energy_bin;
+        energy_end = energy_start + 1;
+      }
+      energy_stride = t->strides()[i];
+    } else if (f == "cell") {
+      // If the local cell is not in the tally cell filter, tally value is 0
+      has_cell = true;
+      cell_stride = t->strides()[i];
+      cell_bin = t->cellBin(elem);
+      if (cell_bin < 0) return 0.0;
+    }
+  }
+  double val = 0.0;
+  switch (_granularity) {
+    case 0:
+      mooseError("Global Tally retrieving currently not supported");
+    case 1:
+      if (!has_univ) mooseError("Tally does not have a universe filter");
+      if (has_cell)
+        mooseError("Universe granularity does not currently support tallies"
+                   "with both universe and cell filters");
+      if (univ_bin < 0) return 0.0;
+      for (int64_t j = energy_start; j < energy_end; ++j)
+        val += sum[(univ_bin * univ_stride + j * energy_stride) * nsb + score_index];
+      break;
+    default:
+      if (!has_cell) mooseError("Specified tally does not contain a cell filter");
+      for (int64_t i = 0; i < univ_end; ++i)
+        for (int64_t j = energy_start; j < energy_end; ++j)
+          val += sum[(i * univ_stride + cell_bin * cell_stride + j * energy_stride) * nsb + score_index];
+      break;
+  }
+  return val;
+}
+
+// ------------------------------------------------------------------ OpenMCStudy
+
+OpenMCStudy::OpenMCStudy(const InputParameters &p, Problem &fe) : _params(p), _fe(fe)
+{
+  _verbose = p.getBool("verbose", true);
+  const std::string exec = p.getString("execute_on", "TIMESTEP_END");
+  if (exec != "TIMESTEP_END" && exec != "timestep_end")
+    paramError(p, "execute_on", "the study runs one batch per time step: execute_on = TIMESTEP_END");
+}
+
+void OpenMCStudy::initialSetup()
+{
+  if (_verbose) std::cout << "Initializing OpenMC simulation" << std::endl;
+  Problem &fe = _fe;
+  int rc = mcw_create(&_ctx, &fe.options);
+  check(_ctx, rc, "mcw_create");
+  const MeshData &m = fe.mesh;
+  check(_ctx,
+        mcw_mesh_set(_ctx, m.n_nodes(), m.xyz.data(), m.n_elem(), m.type.data(), m.conn_off.data(), m.conn.data(),
+                     m.subdomain.data(), m.face_boundary.data()),
+        "mcw_mesh_set");
+  std::vector<mcw_nuclide> nd;
+  for (auto &n : fe.nuclides)
+    nd.push_back(mcw_nuclide{(int32_t)n.E.size(), n.awr, n.E.data(), n.t.data(), n.a.data(), n.s.data(), nullptr});
+  check(_ctx, mcw_xs_set(_ctx, (int32_t)nd.size(), nd.data()), "mcw_xs_set");
+  std::vector<mcw_material> md;
+  for (auto &mt : fe.materials)
+    md.push_back(mcw_material{mt.id, (int32_t)mt.nuclides.size(), mt.nuclides.data(), mt.densities.data()});
+  check(_ctx, mcw_materials_set(_ctx, (int32_t)md.size(), md.data()), "mcw_materials_set");
+  check(_ctx, mcw_source_set(_ctx, &fe.source), "mcw_source_set");
+}
+
+OpenMCStudy::~OpenMCStudy()
+{
+  if (!_ctx) return;
+  if (_verbose) std::cout << "Finalizing OpenMC" << std::endl;
+  mcw_destroy(_ctx);
+}
+
+void OpenMCStudy::generateRays()
+{
+  ++_t_step;
+  if (_verbose) std::cout << "Generating new rays" << std::endl;
+  if (_fe.particles < 1) mooseError("No Rays were moved to _rays in defineRays()");
+}
+
+void OpenMCStudy::execute()
+{
+  if (_verbose) std::cout << "Executing study" << std::endl;
+  check(_ctx, mcw_run_batch(_ctx, _t_step, _fe.particles, &_stats), "mcw_run_batch");
+}
+
+void OpenMCStudy::postExecuteStudy()
+{
+  if (_verbose)
+    std::cout << "Batch " << _t_step << ": " << _stats.histories << " histories, " << _stats.segments << " segments, "
+              << _stats.collisions << " collisions, leakage " << (double)_stats.leaks / _stats.histories << ", "
+              << _stats.time_ms << " ms" << std::endl;
+}
+
+// ------------------------------------------------------------------ Problem
+
+static bool active(const HitNode &parent, const HitNode &child)
+{
+  if (!parent.has("active")) return !parent.has("inactive") || [&] {
+    for (auto &w : split_words(parent.get("inactive")))
+      if (w == child.name) return false;
+    return true;
+  }();
+  for (auto &w : split_words(parent.get("active")))
+    if (w == child.name) return true;
+  return false;
+}
+
+Problem::Problem(std::unique_ptr<HitNode> root, const std::string &input) : _root(std::move(root)), _input(input)
+{
+  HitNode &R = *_root;
+  HitNode *mb = R.child("Mesh");
+  if (!mb) mooseError("the input has no [Mesh] block");
+  mesh = build_mesh(*mb);
+  for (const char *vb : {"Variables", "AuxVariables"})
+    if (HitNode *v = R.child(vb))
+      for (auto &c : v->children) {
+        InputParameters p(c.get());
+        std::string order = p.getString("order", "FIRST");
+        aux_vars[c->name] = order == "CONSTANT" ? 0 : (order == "FIRST" ? 1 : 2);
+        aux_family[c->name] = p.getString("family", "LAGRANGE");
+      }
+  HitNode *mc = R.child("MonteCarlo");
+  if (!mc) mooseError("the input has no [MonteCarlo] block (settings, source, nuclides, materials)");
+  read_monte_carlo(*mc);
+
+  if (HitNode *rk = R.child("RayKernels"))
+    for (auto &c : rk->children) {
+      if (!active(*rk, *c)) continue;
+      InputParameters p(c.get());
+      if (p.getString("type") != "CollisionKernel") paramError(p, "type", "unknown RayKernel type");
+      raykernels.push_back(std::make_unique<CollisionKernel>(p, *this));
+    }
+  if (HitNode *rb = R.child("RayBCs"))
+    for (auto &c : rb->children) {
+      if (!active(*rb, *c)) continue;
+      InputParameters p(c.get());
+      const std::string t = p.getString("type");
+      if (t == "ReflectRayBC") raybcs.push_back(std::make_unique<RayBC>(p, *this, MCW_BC_REFLECT));
+      else if (t == "KillRayBC") raybcs.push_back(std::make_unique<RayBC>(p, *this, MCW_BC_VACUUM));
+      else paramError(p, "type", "unknown RayBC type");
+    }
+  HitNode *uo = R.child("UserObjects");
+  if (!uo) mooseError("the input has no [UserObjects] block");
+  for (auto &c : uo->children) {
+    if (!active(*uo, *c)) continue;
+    InputParameters p(c.get());
+    const std::string t = p.getString("type");
+    if (t == "OpenMCStudy") {
+      if (study) paramError(p, "type", "only one OpenMCStudy per input");
+      study = std::make_unique<OpenMCStudy>(p, *this);
+    }
+  }
+  if (!study) mooseError("the input has no OpenMCStudy user object");
+  for (auto &c : uo->children) {
+    if (!active(*uo, *c)) continue;
+    InputParameters p(c.get());
+    const std::string t = p.getString("type");
+    if (t == "OpenMCTally") tallies.push_back(std::make_unique<OpenMCTally>(p, *this));
+    else if (t != "OpenMCStudy") paramError(p, "type", "unknown UserObject type '" + t + "'");
+  }
+  if (HitNode *ak = R.child("AuxKernels"))
+    for (auto &c : ak->children) {
+      if (!active(*ak, *c)) continue;
+      InputParameters p(c.get());
+      const std::string t = p.getString("type");
+      if (t != "OpenMCTallyAux") continue; // other aux kernels are not on this path
+      auxkernels.push_back(std::make_unique<OpenMCTallyAux>(p, *this));
+      if (!aux_vars.count(auxkernels.back()->variable()))
+        paramError(p, "variable", "no auxiliary variable '" + auxkernels.back()->variable() + "'");
+    }
+  if (HitNode *ex = R.child("Executioner")) {
+    InputParameters p(ex);
+    if (p.getString("type", "Transient") != "Transient")
+      mooseError("Unable to set the number of batches for a non Transient Executioner");
+  }
+  file_base = input.substr(0, input.rfind('.')) + "_out";
+  if (HitNode *o = R.child("Outputs")) {
+    InputParameters p(o);
+    csv = p.getBool("csv", true);
+    if (p.isParamValid("file_base")) file_base = p.getString("file_base");
+  }
+}
+
+Problem::~Problem() = default;
+
+const OpenMCTally *Problem::tally(int id) const
+{
+  for (auto &t : tallies)
+    if (t->id() == id) return t.get();
+  return nullptr;
+}
+
+void Problem::read_monte_carlo(const HitNode &mcb)
+{
+  InputParameters p(&mcb);
+  mcw_default_options(&options);
+  particles = p.getInt("particles");
+  n_batches = (int)p.getInt("batches");
+  options.master_seed = (uint64_t)p.getInt("seed", 1);
+  options.seed_roundtrip = p.getBool("seed_roundtrip", false) ? 1 : 0;
+  const std::string mode = p.getString("mode", "batched");
+  if (mode == "batched") options.mode = MCW_MODE_BATCHED;
+  else if (mode == "plain") options.mode = MCW_MODE_PLAIN;
+  else paramError(p, "mode", "mode must be batched or plain");
+  options.lanes_min = (int32_t)p.getInt("lanes_min", 0);
+  HitNode *nb = mcb.child("Nuclides");
+  HitNode *mb = mcb.child("Materials");
+  HitNode *sb = mcb.child("source");
+  if (!nb || !mb || !sb) mooseError("[MonteCarlo] needs [Nuclides], [Materials] and [source] sub-blocks");
+  for (auto &c : nb->children) {
+    InputParameters q(c.get());
+    if (q.getString("type", "SyntheticNuclide") != "SyntheticNuclide") paramError(q, "type", "unknown nuclide type");
+    nuclides.push_back(synthetic_nuclide(q));
+  }
+  for (auto &c : mb->children) {
+    InputParameters q(c.get());
+    MaterialData m;
+    m.id = (int32_t)q.getInt("id");
+    for (auto &n : q.getWords("nuclides")) {
+      int32_t idx = -1;
+      for (size_t k = 0; k < nuclides.size(); ++k)
+        if (nuclides[k].name == n) idx = (int32_t)k;
+      if (idx < 0) paramError(q, "nuclides", "unknown nuclide '" + n + "'");
+      m.nuclides.push_back(idx);
+    }
+    m.densities = q.getReals("densities");
+    if (m.densities.size() != m.nuclides.size()) paramError(q, "densities", "one density per nuclide");
+    materials.push_back(m);
+  }
+  InputParameters s(sb);
+  memset(&source, 0, sizeof source);
+  const std::string space = s.getString("space", "point");
+  if (space == "point") {
+    source.space = MCW_SPACE_POINT;
+    auto pt = s.getReals("point");
+    if (pt.size() != 3) paramError(s, "point", "a point needs 3 coordinates");
+    for (int k = 0; k < 3; ++k) source.point[k] = pt[k];
+  } else if (space == "volume") {
+    source.space = MCW_SPACE_MESH_VOLUME;
+  } else {
+    paramError(s, "space", "space must be point or volume");
+  }
+  source.angle = MCW_ANGLE_ISOTROPIC;
+  if (s.isParamValid("watt_a")) {
+    source.energy = MCW_ENERGY_WATT;
+    source.watt_a = s.getReal("watt_a");
+    source.watt_b = s.getReal("watt_b");
+  } else {
+    source.energy = MCW_ENERGY_MONO;
+    source.E0 = s.getReal("energy", 2e6);
+  }
+}
+
+std::map<int, int> Problem::blockMap() const
+{
+  std::map<int, int> bm;
+  for (auto &k : raykernels)
+    for (auto &e : k->blockToMaterial()) bm[e.first] = e.second;
+  return bm;
+}
+
+void Problem::run()
+{
+  // initial setup (EXEC_INITIAL): study, kernels, BCs, tallies
+  for (auto &k : raykernels) k->initialSetup();
+  study->initialSetup();
+  std::map<int, int> bm = blockMap();
+  std::vector<int32_t> blocks, mats;
+  for (auto &e : bm) {
+    blocks.push_back(e.first);
+    mats.push_back(e.second);
+  }
+  mcw_ctx *ctx = study->ctx();
+  if (blocks.empty()) mooseError("no CollisionKernel maps a block to a material");
+  check(ctx, mcw_block_material_map(ctx, (int32_t)blocks.size(), blocks.data(), mats.data()), "block map");
+  for (auto &b : raybcs) b->apply(ctx);
+  for (auto &t : tallies) {
+    std::cout << "Creating new tally" << std::endl;
+    t->initialize();
+  }
+  std::vector<std::vector<double>> rows;
+  std::map<int, std::vector<double>> sums;
+  for (int step = 1; step <= n_batches; ++step) {
+    study->generateRays();
+    study->execute();
+    study->postExecuteStudy();
+    const mcw_batch_stats &st = study->lastStats();
+    std::vector<double> row = {(double)step, (double)st.segments, (double)st.collisions,
+                               (double)st.leaks / (double)st.histories, st.histories / (st.time_ms * 1e-3)};
+    for (auto &a : auxkernels) {
+      int64_t nf = 0, ns = 0;
+      check(ctx, mcw_tally_shape(ctx, a->tallyId(), &nf, &ns), "mcw_tally_shape");
+      std::vector<double> &sum = sums[a->tallyId()];
+      sum.assign((size_t)(nf * ns), 0.0);
+      check(ctx, mcw_tally_get(ctx, a->tallyId(), MCW_RESULT_SUM, sum.data(), sum.size()), "mcw_tally_get");
+      double tot = 0.0;
+      for (int64_t e = 0; e < mesh.n_elem(); ++e) tot += a->computeValue(e, sum);
+      row.push_back(tot);
+    }
+    rows.push_back(row);
+  }
+  if (csv) {
+    write_outputs(rows);
+    for (auto &a : auxkernels) {
+      std::ofstream f(file_base + "_" + a->variable() + ".csv");
+      f << "elem," << a->variable() << "\n";
+      f.precision(17);
+      for (int64_t e = 0; e < mesh.n_elem(); ++e) f << e << "," << a->computeValue(e, sums[a->tallyId()]) << "\n";
+    }
+  }
+}
+
+void Problem::write_outputs(const std::vector<std::vector<double>> &rows)
+{
+  std::ofstream f(file_base + ".csv");
+  f << "time,segments,collisions,leakage,histories_per_s";
+  for (auto &a : auxkernels) f << "," << a->variable() << "_total";
+  f << "\n";
+  f.precision(17);
+  for (auto &r : rows) {
+    for (size_t k = 0; k < r.size(); ++k) f << (k ? "," : "") << r[k];
+    f << "\n";
+  }
+}
+
+// Binary dump of the problem as the library received it, for replay on a
+// CPU reference: little-endian, int64 counts then raw arrays.
+void Problem::dump(const std::string &path) const
+{
+  std::ofstream f(path, std::ios::binary);
+  auto w64 = [&](int64_t v) { f.write((const char *)&v, 8); };
+  auto wd = [&](const void *p, size_t n) { f.write((const char *)p, n); };
+  f.write("MCWD0001", 8);
+  w64(mesh.n_nodes());
+  wd(mesh.xyz.data(), mesh.xyz.size() * 8);
+  w64(mesh.n_elem());
+  wd(mesh.type.data(), mesh.type.size());
+  wd(mesh.conn_off.data(), mesh.conn_off.size() * 8);
+  w64((int64_t)mesh.conn.size());
+  wd(mesh.conn.data(), mesh.conn.size() * 4);
+  wd(mesh.subdomain.data(), mesh.subdomain.size() * 4);
+  w64((int64_t)mesh.face_boundary.size());
+  wd(mesh.face_boundary.data(), mesh.face_boundary.size() * 4);
+  std::map<int, int> bc;
+  for (auto &b : raybcs)
+    for (int32_t id : b->ids()) bc[id] = b->kind();
+  w64((int64_t)bc.size());
+  for (auto &e : bc) {
+    w64(e.first);
+    w64(e.second);
+  }
+  std::map<int, int> bm = blockMap();
+  w64((int64_t)bm.size());
+  for (auto &e : bm) {
+    w64(e.first);
+    w64(e.second);
+  }
+  w64((int64_t)nuclides.size());
+  for (auto &n : nuclides) {
+    wd(&n.awr, 8);
+    w64((int64_t)n.E.size());
+    wd(n.E.data(), n.E.size() * 8);
+    wd(n.t.data(), n.t.size() * 8);
+    wd(n.a.data(), n.a.size() * 8);
+    wd(n.s.data(), n.s.size() * 8);
+  }
+  w64((int64_t)materials.size());
+  for (auto &m : materials) {
+    w64(m.id);
+    w64((int64_t)m.nuclides.size());
+    wd(m.nuclides.data(), m.nuclides.size() * 4);
+    wd(m.densities.data(), m.densities.size() * 8);
+  }
+  wd(&source, sizeof source);
+  w64(particles);
+  w64(n_batches);
+  w64(options.seed_roundtrip);
+  w64((int64_t)options.master_seed);
+}
+
+} // namespace mcwhost

@@ -351,3 +351,64 @@ def load_into(lib_prefix_calls, p: Problem):
         c["tally_add"](t.id, t.estimator, len(t.filters), f, len(t.scores), abi.ptr(s, abi.C.c_int32),
                        len(t.nuclides), abi.ptr(n, abi.C.c_int32))
     return M
+
+
+# ----------------------------------------------------------------- driver dumps
+
+def load_dump(path) -> Problem:
+    """Problem written by `macaw-amd-opt --dump-problem` (the exact bytes the
+    driver handed to the library), for replay on a CPU reference."""
+    raw = open(path, "rb").read()
+    if raw[:8] != b"MCWD0001":
+        raise ValueError(f"{path}: not a problem dump")
+    pos = [8]
+
+    def i64():
+        v = int(np.frombuffer(raw, "<i8", 1, pos[0])[0])
+        pos[0] += 8
+        return v
+
+    def arr(dtype, n):
+        a = np.frombuffer(raw, dtype, n, pos[0]).copy()
+        pos[0] += a.nbytes
+        return a
+
+    nn = i64()
+    xyz = arr("<f8", 3 * nn).reshape(nn, 3)
+    ne = i64()
+    etype = arr("u1", ne)
+    off = arr("<i8", ne + 1)
+    nc = i64()
+    conn = arr("<i4", nc)
+    sub = arr("<i4", ne)
+    nfb = i64()
+    fb = arr("<i4", nfb)
+    bcs = {}
+    for _ in range(i64()):
+        b = i64()
+        bcs[b] = i64()
+    bm = {}
+    for _ in range(i64()):
+        b = i64()
+        bm[b] = i64()
+    nucs = []
+    for _ in range(i64()):
+        awr = float(arr("<f8", 1)[0])
+        npt = i64()
+        E, t, a, s = (arr("<f8", npt) for _ in range(4))
+        nucs.append(NuclideData(awr=awr, energy=E, total=t, absorption=a, scatter=s))
+    mats = []
+    for _ in range(i64()):
+        mid = i64()
+        k = i64()
+        mats.append(MaterialData(id=mid, nuclides=arr("<i4", k), densities=arr("<f8", k)))
+    src = abi.Source.from_buffer_copy(raw[pos[0]:pos[0] + abi.C.sizeof(abi.Source)])
+    pos[0] += abi.C.sizeof(abi.Source)
+    particles, batches, roundtrip, seed = i64(), i64(), i64(), i64()
+    mesh = Mesh(xyz=np.ascontiguousarray(xyz), elem_type=etype, conn_offsets=off, conn=conn, subdomain=sub,
+                face_boundary=fb if nfb else None)
+    p = Problem(name=os.path.basename(path), mesh=mesh, nuclides=nucs, materials=mats, block_material=bm,
+                bcs=bcs, source=src, tallies=[], n_particles=particles, n_batches=batches)
+    p.seed_roundtrip = bool(roundtrip)
+    p.master_seed = seed
+    return p

@@ -1,0 +1,96 @@
+"""The MOOSE-shaped host layer (macaw_amd/host) and its driver on the CPU:
+inputs laid out as the reference's test inputs are read and validated, the
+reference's parameter errors are reproduced, and the problem the driver hands
+to the library can be replayed on the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from macaw_amd import abi, build, problems as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INPUTS = os.path.join(ROOT, "tests", "inputs")
+DRIVER = build.DRIVER
+
+
+@pytest.fixture(scope="module", autouse=True)
+def driver_built():
+    if not os.path.exists(DRIVER):
+        build.build_driver()
+
+
+def run(args, cwd=None):
+    return subprocess.run([DRIVER] + args, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                          timeout=120)
+
+
+@pytest.mark.parametrize("name", ["c1_hex8_reflective.i", "openmc_tally.i"])
+def test_inputs_validate(name):
+    r = run(["-i", os.path.join(INPUTS, name), "--check-input"])
+    assert r.returncode == 0, r.stderr
+    assert "input ok" in r.stdout
+
+
+ERRORS = [
+    ("c1_hex8_reflective.i", "UserObjects/tracklength/particle_type=photon", "Only neutrons are currently supported."),
+    ("c1_hex8_reflective.i", "UserObjects/collision/estimator=ANALOG",
+     "Only collision and tracklength estimators are currently supported"),
+    ("c1_hex8_reflective.i", "UserObjects/collision/execute_on=timestep_end",
+     "execute_on must be INITIAL to ensure tallies are created once at the beginning of the simulation"),
+    ("c1_hex8_reflective.i", "UserObjects/collision/filters=energy",
+     "The energy bins parameter should be supplied if an energy filter is requested."),
+    ("c1_hex8_reflective.i", "RayKernels/collision/blocks=0 1",
+     "The blocks parameter must be the same size as the materials parameter."),
+    ("c1_hex8_reflective.i", "Variables/temperature/order=FIRST",
+     "Only CONST MONOMIAL temperatures (current order: 1) are supported."),
+    ("c1_hex8_reflective.i", "RayBCs/reflect/boundary=back nowhere", "the boundary 'nowhere' was not found"),
+    ("c1_hex8_reflective.i", "MonteCarlo/Materials/graphite/nuclides=U235", "unknown nuclide 'U235'"),
+]
+
+
+@pytest.mark.parametrize("name,override,message", ERRORS)
+def test_reference_parameter_errors(name, override, message):
+    r = run(["-i", os.path.join(INPUTS, name), override, "--check-input"])
+    assert r.returncode == 1
+    assert message in r.stderr, r.stderr
+
+
+def test_missing_material_is_reported_at_initial_setup(tmp_path):
+    # CollisionKernel::initialSetup (CollisionKernel.C:84-90): a material id
+    # that does not exist; found before the device is touched.
+    r = run(["-i", os.path.join(INPUTS, "c1_hex8_reflective.i"), "RayKernels/collision/materials=7",
+             "Outputs/file_base=" + str(tmp_path / "x")])
+    assert r.returncode == 1
+    assert "Could not find material 7 in OpenMC materials." in r.stderr
+
+
+def test_hit_syntax(tmp_path):
+    text = open(os.path.join(INPUTS, "c1_hex8_reflective.i")).read()
+    text = text.replace("boundary = 'back front top right left bottom'",
+                        "boundary = 'back front\n                top right\n left bottom'  # a comment")
+    text = text.replace("[Executioner]\n  type = Transient\n[]", "[Executioner]\n  type = Transient\n[../]")
+    f = tmp_path / "x.i"
+    f.write_text(text)
+    r = run(["-i", str(f), "--check-input"])
+    assert r.returncode == 0, r.stderr
+    bad = tmp_path / "bad.i"
+    bad.write_text("[Mesh]\n  [gmg]\n    type = GeneratedMeshGenerator\n  []\n")
+    r = run(["-i", str(bad), "--check-input"])
+    assert r.returncode == 1 and "is not closed" in r.stderr
+
+
+def test_dump_replays_on_the_oracle(tmp_path, oracle_lib):
+    d = tmp_path / "c1.dump"
+    r = run(["-i", os.path.join(INPUTS, "c1_hex8_reflective.i"), "MonteCarlo/particles=300", "--dump-problem", str(d)])
+    assert r.returncode == 0, r.stderr
+    p = P.load_dump(str(d))
+    assert p.mesh.n_elem == 512 and p.n_particles == 300 and p.block_material == {0: 1}
+    assert p.bcs == {b: abi.BC_REFLECT for b in range(6)}
+    assert p.nuclides[0].energy.size == 20000 and p.nuclides[0].awr == 12.0
+    p.tallies = P.flux_tallies(both=True)
+    o = oracle_lib.Oracle(p)
+    st = o.run_batch(1, 300, threads=2)
+    assert st["lost"] == 0 and st["leaks"] == 0 and st["absorptions"] + st["cutoffs"] == 300
+    assert o.tally(1).sum() > 0
