@@ -67,8 +67,11 @@ enum { MCW_ENERGY_MONO = 0, MCW_ENERGY_WATT = 1 };
  * bank chunk.  Both give bit-identical histories. */
 enum { MCW_MODE_PLAIN = 0, MCW_MODE_BATCHED = 1 };
 
-/* Exit-face distance arithmetic.  FP64 everywhere, or FP32 candidate
- * distances with an FP64 recheck of near ties and an FP64 final distance. */
+/* Exit-side arithmetic.  FP64 everywhere, or (BATCHED, TET4 meshes) FP32
+ * candidate distances from one 128-byte record per element with an FP64
+ * re-solve of near ties: the exit side then always equals FP64's choice,
+ * segment lengths carry FP32 rounding; positions, cross sections and
+ * tallies stay FP64.  Other meshes and the PLAIN mode run FP64. */
 enum { MCW_PREC_FP64 = 0, MCW_PREC_MIXED = 1 };
 
 /* Per-history fates. */
@@ -156,6 +159,7 @@ typedef struct {
   double time_ms;       /* device time of the batch, transport only */
   double kernel_ms;     /* device time of the dominant kernel(s) */
   int64_t kernel_launches;
+  int64_t fp64_rechecks; /* precision MIXED: segments whose exit side was re-solved in FP64 */
 } mcw_batch_stats;
 
 /* Per-history record, for parity checks against a history-based reference. */

@@ -28,9 +28,9 @@ hipError_t launch_segment(int ns, unsigned blocks, hipStream_t s, const Params &
                           const BankView &out, int qout, const BankView &coll, int qcoll);
 hipError_t launch_collide(unsigned blocks, hipStream_t s, const Params &P, const BankView &coll, int qcoll,
                           const BankView &out, int qout);
-hipError_t launch_persistent(int ns, int wpe, unsigned blocks, int block, hipStream_t s, const Params &P,
+hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int block, hipStream_t s, const Params &P,
                              const SourceBank &S);
-const void *persistent_kernel(int ns, int wpe, bool rec);
+const void *persistent_kernel(int ns, int wpe, bool rec, bool p32);
 hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1);
 hipError_t launch_reset_work(hipStream_t s, Counters *C);
 hipError_t launch_fill(hipStream_t s, double *p, int64_t n, double v);
@@ -137,7 +137,7 @@ struct mcw_ctx {
   bool dirty = true;
 
   // device problem
-  DevBuf plane, erec, face_off, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
+  DevBuf plane, erec, mtet, face_off, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf grid_off, npts, rec, fission, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
   Counters *h_counters = nullptr; // pinned
@@ -244,6 +244,28 @@ void prepare(mcw_ctx *c)
   }
   c->plane.upload(planes, s);
   c->erec.upload(erec, s);
+  // mixed-precision records of TET4 meshes: FP32 planes relative to vertex 0
+  if (m.uniform_sides == 4) {
+    std::vector<MixedTet> mt(ne);
+    for (int64_t i = 0; i < ne; ++i) {
+      const int64_t o = m.orig_of_int[i];
+      const double *v0 = &m.xyz[3 * (int64_t)m.conn[m.conn_off[o]]];
+      MixedTet &R = mt[i];
+      memset(&R, 0, sizeof R);
+      for (int k = 0; k < 3; ++k) R.v0[k] = v0[k];
+      for (int sd = 0; sd < 4; ++sd) {
+        const double4 q = planes[4 * i + sd];
+        const double rel = q.w - ((q.x * v0[0] + q.y * v0[1]) + q.z * v0[2]);
+        R.pl[sd] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)rel);
+      }
+      R.code[0] = erec[2 * i].x;
+      R.code[1] = erec[2 * i].y;
+      R.code[2] = erec[2 * i].z;
+      R.code[3] = erec[2 * i].w;
+      R.mat = erec[2 * i + 1].z;
+    }
+    c->mtet.upload(mt, s);
+  }
   if (m.uniform_sides == 0) c->face_off.upload(fo, s);
   c->bc_kind.upload(kinds, s);
   c->cdf.upload(m.cdf, s);
@@ -355,6 +377,7 @@ void prepare(mcw_ctx *c)
   P.mesh.type = c->type.as<uint8_t>();
   P.mesh.int_of_orig = c->int_of_orig.as<int32_t>();
   P.mesh.orig_of_int = c->orig_of_int.as<int32_t>();
+  P.mesh.mtet = m.uniform_sides == 4 ? c->mtet.as<MixedTet>() : nullptr;
   P.xs.grid_off = c->grid_off.as<int64_t>();
   P.xs.n_points = c->npts.as<int32_t>();
   P.xs.rec = c->rec.as<double4>();
@@ -565,10 +588,10 @@ SourceBank source_bank(mcw_ctx *c, int64_t n)
 }
 
 // Resident workgroups of the persistent kernel over the whole device.
-int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, int block)
+int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, bool p32, int block)
 {
   int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns, wpe, rec), block, 0));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns, wpe, rec, p32), block, 0));
   if (per_cu < 1) per_cu = 1;
   return c->n_cu * per_cu;
 }
@@ -884,13 +907,14 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       } else {
         const int block = c->opt.block_size > 0 ? c->opt.block_size : 256;
         const int wpe = c->opt.waves_per_eu == 5 ? 5 : 0;
+        const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns == 4;
         require(block % 64 == 0 && block <= 256, "block_size must be 64, 128, 192 or 256");
-        int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, block), (h1 - h0 + block - 1) / block);
+        int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, p32, block), (h1 - h0 + block - 1) / block);
         if (blocks < 1) blocks = 1;
         // the work counter restarts for every chunk
         HIPCHK(launch_reset_work(s, P.cnt));
         HIPCHK(hipEventRecord(c->ev[2], s));
-        HIPCHK(launch_persistent(ns, wpe, (unsigned)blocks, block, s, P, S));
+        HIPCHK(launch_persistent(ns, wpe, p32, (unsigned)blocks, block, s, P, S));
         HIPCHK(hipEventRecord(c->ev[3], s));
         HIPCHK(hipEventSynchronize(c->ev[3]));
         float ms = 0.0f;
@@ -925,6 +949,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       out->time_ms = total_ms;
       out->kernel_ms = c->opt.mode == MCW_MODE_PLAIN ? total_ms : kernel_ms;
       out->kernel_launches = launches;
+      out->fp64_rechecks = (int64_t)k.rechecks;
     }
   });
 }

@@ -343,6 +343,55 @@ __device__ __forceinline__ int exit_side_dyn(const double4 *__restrict__ pl, int
   return fx;
 }
 
+// FP32 exit side of a TET4 from a mixed-precision record.  Each candidate
+// distance t carries an error bound e (evaluation and rounding of FP32
+// operands: |dt| <= (16u(|c'| + |dx|+|dy|+|dz|) + 16u|t|)/den + 2u|t|,
+// u = 2^-24).  When the best candidate's interval overlaps another's, or a
+// side is within 1e-5 of grazing, `recheck` is set and the caller must redo
+// the selection in FP64; otherwise the chosen side is the one FP64 would
+// choose, and L is the FP32 distance.
+__device__ __forceinline__ int exit_side_p32(const MixedTet &R, double x, double y, double z, double u, double v,
+                                             double w, int inface, double &L, bool &recheck)
+{
+  const float uu = 0x1p-24f;
+  const float dx = (float)(x - R.v0[0]), dy = (float)(y - R.v0[1]), dz = (float)(z - R.v0[2]);
+  const float ux = (float)u, uy = (float)v, uz = (float)w;
+  const float S = fabsf(dx) + fabsf(dy) + fabsf(dz);
+  float t[4], e[4];
+  bool ok[4];
+  bool graze = false;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const float4 q = R.pl[s];
+    const float den = (q.x * ux + q.y * uy) + q.z * uz;
+    const float num = q.w - ((q.x * dx + q.y * dy) + q.z * dz);
+    float ts = num / den;
+    ts = ts > 0.0f ? ts : 0.0f;
+    ok[s] = (s != inface) && (den > 0.0f);
+    graze |= (s != inface) && (fabsf(den) <= 1e-5f);
+    t[s] = ts;
+    e[s] = (16.0f * uu * (fabsf(q.w) + S) + 16.0f * uu * ts) / den + 2.0f * uu * ts;
+  }
+  int fx = -1;
+  float best = __builtin_inff();
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    if (ok[s] && t[s] < best) {
+      best = t[s];
+      fx = s;
+    }
+  bool overlap = false;
+  if (fx >= 0) {
+    const float hi = best + e[fx];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      if (ok[s] && s != fx && t[s] - e[s] <= hi) overlap = true;
+  }
+  recheck = graze || overlap || fx < 0;
+  L = (double)best;
+  return fx;
+}
+
 // OpenMC rotate_angle with phi = 2 pi xi.
 __device__ __forceinline__ void rotate_angle(double &u, double &v, double &w, double mu, double xi)
 {

@@ -246,41 +246,58 @@ __device__ __forceinline__ int64_t orig_elem(const MeshView &M, int32_t e) { ret
 
 __device__ __forceinline__ int32_t pick_nbr(const int4 &r0, const int4 &r1, int fx)
 {
-  int32_t nb = r0.x;
-  nb = fx == 1 ? r0.y : nb;
-  nb = fx == 2 ? r0.z : nb;
-  nb = fx == 3 ? r0.w : nb;
-  nb = fx == 4 ? r1.x : nb;
-  nb = fx == 5 ? r1.y : nb;
-  return nb;
+  // masked OR over the six codes: keeps the record in registers (a select
+  // chain is turned into an indexed stack array by the compiler)
+  const int32_t m0 = -(int32_t)(fx == 0), m1 = -(int32_t)(fx == 1), m2 = -(int32_t)(fx == 2);
+  const int32_t m3 = -(int32_t)(fx == 3), m4 = -(int32_t)(fx == 4), m5 = -(int32_t)(fx == 5);
+  return (r0.x & m0) | (r0.y & m1) | (r0.z & m2) | (r0.w & m3) | (r1.x & m4) | (r1.y & m5);
 }
 
 // Geometry and distance draw of one segment.  Returns the event that ends it;
 // EV_COLLIDE leaves the particle at the collision point for collide().
-template <int NS, bool REC>
-__device__ __forceinline__ int segment(const Params &P, PState &st, double &track_sum)
+// P32 (TET4 only): exit side from the FP32 mixed record, FP64 recheck of
+// near ties (counted in `rechecks`).
+template <int NS, bool REC, bool P32>
+__device__ __forceinline__ int segment(const Params &P, PState &st, double &track_sum, unsigned &rechecks)
 {
   const MeshView &M = P.mesh;
   if (st.nseg >= P.max_seg) {
     if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
     return EV_LOST;
   }
-  const int4 r0 = M.erec[2 * (int64_t)st.elem];
-  const int4 r1 = M.erec[2 * (int64_t)st.elem + 1];
-  const int32_t mat = r1.z;
-  if (mat != st.mat) {
-    st.sigt = macro_total(P.xs, mat, st.k, st.E);
-    st.mat = mat;
-  }
+  int4 r0, r1;
+  int32_t mat;
   double L;
   int fx;
   int64_t slot0;
-  if (NS > 0) {
-    slot0 = (int64_t)st.elem * NS;
-    fx = exit_side<NS>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+  if (P32) {
+    const MixedTet &R = M.mtet[st.elem];
+    const int4 code = *(const int4 *)R.code;
+    r0 = code;
+    r1 = make_int4(-1, -1, R.mat, 4);
+    mat = R.mat;
+    slot0 = (int64_t)st.elem * 4;
+    bool recheck;
+    fx = exit_side_p32(R, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L, recheck);
+    if (recheck) {
+      fx = exit_side<4>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+      ++rechecks;
+    }
   } else {
-    slot0 = M.face_off[st.elem];
-    fx = exit_side_dyn(M.plane + slot0, r1.w, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+    r0 = M.erec[2 * (int64_t)st.elem];
+    r1 = M.erec[2 * (int64_t)st.elem + 1];
+    mat = r1.z;
+    if (NS > 0) {
+      slot0 = (int64_t)st.elem * NS;
+      fx = exit_side<NS>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+    } else {
+      slot0 = M.face_off[st.elem];
+      fx = exit_side_dyn(M.plane + slot0, r1.w, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+    }
+  }
+  if (mat != st.mat) {
+    st.sigt = macro_total(P.xs, mat, st.k, st.E);
+    st.mat = mat;
   }
   if (fx < 0) {
     if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
@@ -517,6 +534,7 @@ __global__ void __launch_bounds__(256) k_segment(Params P, BankView in, int qin,
   WaveCount wc;
   wc.zero();
   double track = 0.0;
+  unsigned rechecks = 0;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n;
@@ -524,7 +542,7 @@ __global__ void __launch_bounds__(256) k_segment(Params P, BankView in, int qin,
     PState st;
     if (valid) {
       load<REC>(in, i, st);
-      ev = segment<NS, REC>(P, st, track);
+      ev = segment<NS, REC, false>(P, st, track, rechecks);
       if (REC && is_death(ev)) write_record(P, st, fate_of(ev));
     }
     count_segment(wc, valid, ev);
@@ -583,7 +601,7 @@ enum { PH_IDLE = 0, PH_TRANSPORT = 1, PH_COLLIDE = 2 };
 
 // WPE: minimum wavefronts per SIMD requested from the register allocator
 // (0 = compiler's choice); a tuning knob trading spills for occupancy.
-template <int NS, bool REC, int WPE>
+template <int NS, bool REC, int WPE, bool P32>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 k_persistent(Params P, SourceBank S)
 {
@@ -593,6 +611,7 @@ k_persistent(Params P, SourceBank S)
   WaveCount wc;
   wc.zero();
   double track = 0.0;
+  unsigned rechecks = 0;
   PState st;
   st.elem = 0;
   int phase = PH_IDLE;
@@ -619,7 +638,7 @@ k_persistent(Params P, SourceBank S)
       const bool go = phase == PH_TRANSPORT;
       int ev = -1;
       if (go) {
-        ev = segment<NS, REC>(P, st, track);
+        ev = segment<NS, REC, P32>(P, st, track, rechecks);
         if (ev == EV_COLLIDE) {
           phase = PH_COLLIDE;
         } else if (is_death(ev)) {
@@ -644,6 +663,10 @@ k_persistent(Params P, SourceBank S)
     count_collision(wc, hit, ev);
   }
   flush_counts(C, wc, track);
+  if (P32) {
+    const unsigned long long r = wave_sum(rechecks);
+    if (__lane_id() == 0 && r) atomicAdd(&C->rechecks, r);
+  }
 }
 
 // ------------------------------------------------------------------ utilities
@@ -760,31 +783,31 @@ hipError_t launch_collide(unsigned blocks, hipStream_t s, const Params &P, const
   return hipGetLastError();
 }
 
-#define MCW_PERS0(NS, REC, ...) hipLaunchKernelGGL((k_persistent<NS, REC, 0>), __VA_ARGS__)
-#define MCW_PERS5(NS, REC, ...) hipLaunchKernelGGL((k_persistent<NS, REC, 5>), __VA_ARGS__)
-hipError_t launch_persistent(int ns, int wpe, unsigned blocks, int block, hipStream_t s, const Params &P,
+template <int WPE> const void *persistent_kernel_w(int ns, bool rec, bool p32)
+{
+  if (p32 && ns == 4) return rec ? (const void *)k_persistent<4, true, WPE, true> : (const void *)k_persistent<4, false, WPE, true>;
+  if (rec) {
+    if (ns == 4) return (const void *)k_persistent<4, true, WPE, false>;
+    if (ns == 6) return (const void *)k_persistent<6, true, WPE, false>;
+    return (const void *)k_persistent<0, true, WPE, false>;
+  }
+  if (ns == 4) return (const void *)k_persistent<4, false, WPE, false>;
+  if (ns == 6) return (const void *)k_persistent<6, false, WPE, false>;
+  return (const void *)k_persistent<0, false, WPE, false>;
+}
+
+// The kernel variant for side count, record mode, occupancy hint and precision.
+const void *persistent_kernel(int ns, int wpe, bool rec, bool p32)
+{
+  return wpe == 5 ? persistent_kernel_w<5>(ns, rec, p32) : persistent_kernel_w<0>(ns, rec, p32);
+}
+
+hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int block, hipStream_t s, const Params &P,
                              const SourceBank &S)
 {
-  if (wpe == 5) MCW_BY_NS_REC(ns, P.record, MCW_PERS5, dim3(blocks), dim3(block), 0, s, P, S);
-  else MCW_BY_NS_REC(ns, P.record, MCW_PERS0, dim3(blocks), dim3(block), 0, s, P, S);
-  return hipGetLastError();
-}
-
-template <int WPE> const void *persistent_kernel_w(int ns, bool rec)
-{
-  if (rec) {
-    if (ns == 4) return (const void *)k_persistent<4, true, WPE>;
-    if (ns == 6) return (const void *)k_persistent<6, true, WPE>;
-    return (const void *)k_persistent<0, true, WPE>;
-  }
-  if (ns == 4) return (const void *)k_persistent<4, false, WPE>;
-  if (ns == 6) return (const void *)k_persistent<6, false, WPE>;
-  return (const void *)k_persistent<0, false, WPE>;
-}
-
-const void *persistent_kernel(int ns, int wpe, bool rec)
-{
-  return wpe == 5 ? persistent_kernel_w<5>(ns, rec) : persistent_kernel_w<0>(ns, rec);
+  const void *k = persistent_kernel(ns, wpe, P.record != 0, p32);
+  void *args[] = {(void *)&P, (void *)&S};
+  return hipLaunchKernel(k, dim3(blocks), dim3(block), args, 0, s);
 }
 
 hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1)

@@ -19,6 +19,18 @@ namespace mcw {
 
 constexpr int MAX_TALLIES = 8;
 
+// Mixed-precision TET4 record: one 128-byte line per element.  Planes are
+// FP32, with offsets relative to the element's first vertex v0 (kept in
+// FP64), so FP32 distances carry errors of order 2^-24 of the element size
+// rather than of the domain size.
+struct __align__(128) MixedTet {
+  float4 pl[4];      // nx, ny, nz, n.(p_side - v0)
+  double v0[3];
+  int32_t code[4];   // neighbour codes, as in erec
+  int32_t mat;
+  int32_t pad[5];
+};
+
 // Elements live on the device in Morton order of their centroids
 // ("internal" ids); the caller's ids appear only in source sampling (the
 // volume CDF is in caller order), records and tally read-out.
@@ -39,6 +51,7 @@ struct MeshView {
   const uint8_t *type;
   const int32_t *int_of_orig; // caller element -> internal id
   const int32_t *orig_of_int; // internal id -> caller element
+  const MixedTet *mtet;       // mixed-precision records (TET4 meshes, precision MIXED), else nullptr
 };
 
 struct XSView {
@@ -115,7 +128,7 @@ struct RecordView {
 
 // Global counters, accumulated with one atomic per wavefront.
 struct Counters {
-  unsigned long long segments, collisions, absorptions, leaks, cutoffs, lost, reflections, pad;
+  unsigned long long segments, collisions, absorptions, leaks, cutoffs, lost, reflections, rechecks;
   double track;
   unsigned long long next_hist;   // persistent kernel work queue
   unsigned long long q_count[4];  // event queue lengths

@@ -237,3 +237,33 @@ def test_general_tallies_match_oracle():
             assert a.shape == b.shape, t.id
             assert b.any(), t.id
             assert_tally_close(a, b)
+
+
+def test_mixed_precision_against_fp64():
+    """FP32 exit-side distances with FP64 re-solve of near ties: the exit side
+    always equals FP64's choice, so crossed-element sequences agree for all but
+    a handful of histories (segment lengths carry FP32 rounding, which can flip
+    a collision test d > L that FP64 decides by less than ~1e-7 cm); tallies
+    agree far inside their statistical error."""
+    mesh = P.tet_cube(14, seed=101, cache=False)
+    mesh.subdomain = P.voronoi_blocks(mesh, 4, seed=303)
+    p = P.Problem(name="tet_mixed", mesh=mesh, nuclides=P.c2_nuclides(5000, 4), materials=P.c2_materials(4, 4),
+                  block_material={b: b + 1 for b in range(4)}, bcs={0: abi.BC_VACUUM},
+                  source=P.volume_watt_source(), tallies=P.flux_tallies(both=False), n_particles=20000, n_batches=1)
+    a = Context(p, mode=abi.MODE_BATCHED, record_histories=1)
+    b = Context(p, mode=abi.MODE_BATCHED, record_histories=1, precision=abi.PREC_MIXED)
+    sa = a.run_batch(1, p.n_particles)
+    sb = b.run_batch(1, p.n_particles)
+    ha, hb = a.histories(p.n_particles), b.histories(p.n_particles)
+    same = (ha["digest"] == hb["digest"]) & (ha["n_segments"] == hb["n_segments"])
+    frac_diff = 1.0 - same.mean()
+    assert frac_diff < 1e-3, frac_diff
+    dx = np.sqrt((ha["x"] - hb["x"]) ** 2 + (ha["y"] - hb["y"]) ** 2 + (ha["z"] - hb["z"]) ** 2)[same]
+    # final positions: FP32 segment lengths leave ~1e-9 cm per history (1e-5 worst case)
+    assert np.median(dx) < 1e-8 and dx.max() < 1e-5
+    assert np.array_equal(ha["E"][same], hb["E"][same])
+    assert sb["fp64_rechecks"] > 0 and sa["fp64_rechecks"] == 0
+    assert sb["fp64_rechecks"] < 0.02 * sb["segments"]
+    ta, tb = a.tally(1), b.tally(1)
+    rel = abs(tb.sum() - ta.sum()) / ta.sum()
+    assert rel < 1e-5, rel
