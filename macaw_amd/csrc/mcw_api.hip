@@ -138,6 +138,7 @@ struct mcw_ctx {
 
   // device problem
   DevBuf plane, erec, mtet, face_off, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
+  DevBuf cell_start, cell_elem;
   DevBuf grid_off, npts, rec, fission, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
   Counters *h_counters = nullptr; // pinned
@@ -351,9 +352,16 @@ void prepare(mcw_ctx *c)
   if (c->src.space == MCW_SPACE_POINT) {
     c->src_elem = locate_point(m, c->src.point, 1e-12);
     require(c->src_elem >= 0, "point source outside the mesh");
+  } else if (c->src.space == MCW_SPACE_BOX) {
+    for (int k = 0; k < 3; ++k)
+      require(c->src.box_hi[k] >= c->src.box_lo[k], "box source: box_hi must not be below box_lo");
   } else {
-    require(c->src.space == MCW_SPACE_MESH_VOLUME, "source space type not supported by this path");
+    require(c->src.space == MCW_SPACE_MESH_VOLUME, "unknown source space type");
   }
+  // point locator for box sources (and for the request server)
+  build_locator(m);
+  c->cell_start.upload(m.cell_start, s);
+  c->cell_elem.upload(m.cell_elem, s);
   require(c->src.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
   require(c->src.energy == MCW_ENERGY_MONO || c->src.energy == MCW_ENERGY_WATT, "source energy type not supported");
 
@@ -378,6 +386,13 @@ void prepare(mcw_ctx *c)
   P.mesh.int_of_orig = c->int_of_orig.as<int32_t>();
   P.mesh.orig_of_int = c->orig_of_int.as<int32_t>();
   P.mesh.mtet = m.uniform_sides == 4 ? c->mtet.as<MixedTet>() : nullptr;
+  for (int k = 0; k < 3; ++k) {
+    P.mesh.grid_lo[k] = m.grid_lo[k];
+    P.mesh.grid_inv[k] = m.grid_inv[k];
+    P.mesh.grid_n[k] = m.grid_n[k];
+  }
+  P.mesh.cell_start = c->cell_start.as<int32_t>();
+  P.mesh.cell_elem = c->cell_elem.as<int32_t>();
   P.xs.grid_off = c->grid_off.as<int64_t>();
   P.xs.n_points = c->npts.as<int32_t>();
   P.xs.rec = c->rec.as<double4>();

@@ -71,7 +71,7 @@ __device__ __forceinline__ unsigned long long wave_pop(bool p) { return (unsigne
 // Count the outcome of one segment step (every lane of the wavefront calls it).
 __device__ __forceinline__ void count_segment(WaveCount &w, bool stepped, int ev)
 {
-  w.seg += wave_pop(stepped);
+  w.seg += wave_pop(stepped && ev != 6); // a history lost before its segment adds none
   w.leak += wave_pop(stepped && ev == 2);
   w.refl += wave_pop(stepped && ev == 1);
   w.lost += wave_pop(stepped && ev == 6);
@@ -154,6 +154,47 @@ __device__ double tet_volume6_d(const double *a, const double *b, const double *
   return fabs(det);
 }
 
+// Lowest caller-index element whose side planes contain p within tol, or -1
+// (the rule of the host-side locate and of the CPU reference).
+__device__ int32_t locate_point_d(const MeshView &M, const double *p, double tol)
+{
+  int32_t cell[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double f = (p[k] - M.grid_lo[k]) * M.grid_inv[k];
+    int32_t c = f > 0.0 ? (int32_t)f : 0;
+    cell[k] = c < M.grid_n[k] ? c : M.grid_n[k] - 1;
+  }
+  const int64_t ci = (int64_t)cell[0] + (int64_t)M.grid_n[0] * ((int64_t)cell[1] + (int64_t)M.grid_n[1] * cell[2]);
+  int32_t best = -1, best_orig = 0x7fffffff;
+  for (int32_t q = M.cell_start[ci]; q < M.cell_start[ci + 1]; ++q) {
+    const int32_t e = M.cell_elem[q];
+    int64_t s0;
+    int ns;
+    if (M.stride > 0) {
+      s0 = (int64_t)e * M.stride;
+      ns = M.stride;
+    } else {
+      s0 = M.face_off[e];
+      ns = M.erec[2 * (int64_t)e + 1].w;
+    }
+    bool in = true;
+    for (int s = 0; s < ns && in; ++s) {
+      const double4 pl = M.plane[s0 + s];
+      const double v = ((pl.x * p[0] + pl.y * p[1]) + pl.z * p[2]) - pl.w;
+      if (v > tol) in = false;
+    }
+    if (in) {
+      const int32_t o = M.orig_of_int[e];
+      if (o < best_orig) {
+        best_orig = o;
+        best = e;
+      }
+    }
+  }
+  return best;
+}
+
 // OpenMC fixed-source initialize_history (OpenMCStudy.C:317-363 drives it):
 // seeds for particle id (batch-1)*n + i + 1, then space, angle, energy
 // sampled on stream 1 (sample_external_source).
@@ -171,6 +212,15 @@ __device__ void sample_site(const Params &P, int64_t hid, double *p, double *dir
     p[1] = S.point[1];
     p[2] = S.point[2];
     elem = S.point_elem;
+  } else if (S.space == 1) {
+    // uniform in the box, resampled while outside the mesh
+    int32_t e = -1;
+    for (int it = 0; it < 1000 && e < 0; ++it) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) p[k] = S.box_lo[k] + prn(ss) * (S.box_hi[k] - S.box_lo[k]);
+      e = locate_point_d(M, p, 1e-12);
+    }
+    elem = e; // internal id, -1 when the box misses the mesh
   } else {
     const int64_t ne = M.n_elem;
     double target = prn(ss) * M.cdf[ne - 1];
@@ -226,7 +276,7 @@ __device__ void sample_site(const Params &P, int64_t hid, double *p, double *dir
     }
   }
   if (P.roundtrip) track_seed = seed_roundtrip(track_seed);
-  elem_out = S.space == 0 ? (int32_t)elem : M.int_of_orig[elem];
+  elem_out = S.space == 2 ? M.int_of_orig[elem] : (int32_t)elem;
 }
 
 __device__ __forceinline__ void start_history(PState &st, int64_t hid)
@@ -242,7 +292,7 @@ __device__ __forceinline__ void start_history(PState &st, int64_t hid)
 
 // ------------------------------------------------------------------ one segment
 
-__device__ __forceinline__ int64_t orig_elem(const MeshView &M, int32_t e) { return M.orig_of_int[e]; }
+__device__ __forceinline__ int64_t orig_elem(const MeshView &M, int32_t e) { return e < 0 ? -1 : M.orig_of_int[e]; }
 
 __device__ __forceinline__ int32_t pick_nbr(const int4 &r0, const int4 &r1, int fx)
 {
@@ -261,6 +311,7 @@ template <int NS, bool REC, bool P32>
 __device__ __forceinline__ int segment(const Params &P, PState &st, double &track_sum, unsigned &rechecks)
 {
   const MeshView &M = P.mesh;
+  if (st.elem < 0) return EV_LOST; // source site outside the mesh
   if (st.nseg >= P.max_seg) {
     if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
     return EV_LOST;

@@ -268,6 +268,66 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
   return "";
 }
 
+void build_locator(HostMesh &m)
+{
+  if (m.have_grid) return;
+  const int64_t ne = m.n_elem;
+  const double *X = m.xyz.data();
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = 0; i < m.n_nodes; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], X[3 * i + k]);
+      hi[k] = std::max(hi[k], X[3 * i + k]);
+    }
+  double ext[3], vol = 1.0;
+  for (int k = 0; k < 3; ++k) {
+    ext[k] = std::max(hi[k] - lo[k], 1e-300);
+    vol *= ext[k];
+  }
+  const double h = std::cbrt(vol / (double)std::max<int64_t>(1, ne));
+  for (int k = 0; k < 3; ++k) {
+    m.grid_n[k] = (int32_t)std::min(1024.0, std::max(1.0, std::ceil(ext[k] / h)));
+    m.grid_lo[k] = lo[k];
+    m.grid_inv[k] = m.grid_n[k] / ext[k];
+  }
+  const int64_t nc = (int64_t)m.grid_n[0] * m.grid_n[1] * m.grid_n[2];
+  // element bounding boxes (internal order), widened by a tolerance
+  auto cells_of = [&](int64_t i, int32_t *c0, int32_t *c1) {
+    const int64_t o = m.orig_of_int[i];
+    double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t q = m.conn_off[o]; q < m.conn_off[o + 1]; ++q)
+      for (int k = 0; k < 3; ++k) {
+        blo[k] = std::min(blo[k], X[3 * (int64_t)m.conn[q] + k]);
+        bhi[k] = std::max(bhi[k], X[3 * (int64_t)m.conn[q] + k]);
+      }
+    for (int k = 0; k < 3; ++k) {
+      const double tol = 1e-9 * ext[k];
+      c0[k] = (int32_t)std::max(0.0, std::floor((blo[k] - tol - lo[k]) * m.grid_inv[k]));
+      c1[k] = (int32_t)std::min((double)m.grid_n[k] - 1, std::floor((bhi[k] + tol - lo[k]) * m.grid_inv[k]));
+    }
+  };
+  m.cell_start.assign(nc + 1, 0);
+  for (int64_t i = 0; i < ne; ++i) {
+    int32_t c0[3], c1[3];
+    cells_of(i, c0, c1);
+    for (int32_t z = c0[2]; z <= c1[2]; ++z)
+      for (int32_t y = c0[1]; y <= c1[1]; ++y)
+        for (int32_t x = c0[0]; x <= c1[0]; ++x) ++m.cell_start[x + (int64_t)m.grid_n[0] * (y + (int64_t)m.grid_n[1] * z) + 1];
+  }
+  for (int64_t c = 0; c < nc; ++c) m.cell_start[c + 1] += m.cell_start[c];
+  m.cell_elem.resize(m.cell_start[nc]);
+  std::vector<int32_t> fill(m.cell_start.begin(), m.cell_start.end() - 1);
+  for (int64_t i = 0; i < ne; ++i) {
+    int32_t c0[3], c1[3];
+    cells_of(i, c0, c1);
+    for (int32_t z = c0[2]; z <= c1[2]; ++z)
+      for (int32_t y = c0[1]; y <= c1[1]; ++y)
+        for (int32_t x = c0[0]; x <= c1[0]; ++x)
+          m.cell_elem[fill[x + (int64_t)m.grid_n[0] * (y + (int64_t)m.grid_n[1] * z)]++] = (int32_t)i;
+  }
+  m.have_grid = true;
+}
+
 int64_t locate_point(const HostMesh &m, const double *p, double tol)
 {
   for (int64_t e = 0; e < m.n_elem; ++e) {

@@ -26,7 +26,15 @@ struct HostMesh {
   // device order: elements sorted along a Morton curve of their centroids
   std::vector<int32_t> orig_of_int; // internal position -> caller element
   std::vector<int32_t> int_of_orig; // caller element -> internal position
+  // point locator (built on demand): uniform grid, CSR of internal ids
+  bool have_grid = false;
+  double grid_lo[3] = {0, 0, 0}, grid_inv[3] = {0, 0, 0};
+  int32_t grid_n[3] = {1, 1, 1};
+  std::vector<int32_t> cell_start, cell_elem;
 };
+
+// Build the uniform-grid point locator (about one element per cell).
+void build_locator(HostMesh &m);
 
 extern const int HEX_KUHN_TETS[6][4];
 

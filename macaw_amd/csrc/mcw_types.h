@@ -52,6 +52,13 @@ struct MeshView {
   const int32_t *int_of_orig; // caller element -> internal id
   const int32_t *orig_of_int; // internal id -> caller element
   const MixedTet *mtet;       // mixed-precision records (TET4 meshes, precision MIXED), else nullptr
+  // point location: uniform grid over the mesh bounding box, CSR lists of
+  // the (internal) elements whose bounding box meets each cell
+  double grid_lo[3], grid_inv[3];
+  int32_t grid_n[3];
+  int32_t pad_grid;
+  const int32_t *cell_start;
+  const int32_t *cell_elem;
 };
 
 struct XSView {

@@ -461,8 +461,18 @@ bool sample_source(const Oracle &o, uint64_t *seed, Site *st)
                           p);
     }
     st->elem = e;
+  } else if (S.space == MCW_SPACE_BOX) {
+    // uniform in the box, resampled while outside the mesh (OpenMC rejects
+    // source sites that are not in a cell); the element is the lowest-index
+    // one containing the point
+    int64_t e = -1;
+    for (int it = 0; it < 1000 && e < 0; ++it) {
+      for (int k = 0; k < 3; ++k) p[k] = S.box_lo[k] + orc_prn(seed) * (S.box_hi[k] - S.box_lo[k]);
+      e = locate(o, p, 1e-12);
+    }
+    st->elem = e;
   } else {
-    return false; // BOX sources are located on the device; not in the oracle yet
+    return false;
   }
   st->x = p[0];
   st->y = p[1];
