@@ -10,7 +10,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <deque>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -40,6 +42,9 @@ hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *
                              double *sq, const uint64_t *seeds, double *po, uint64_t *seo);
 hipError_t launch_xs_probe(hipStream_t s, const XSView &X, int32_t nuc, const double *E, int64_t n, int32_t *k,
                            double *xs);
+hipError_t launch_source_req(hipStream_t s, const Params &P, const SourceBank &S, int32_t n_slices, int64_t n);
+const void *serve_kernel(int ns, bool p32);
+hipError_t launch_serve(int ns, bool p32, unsigned blocks, hipStream_t s, const Params &P, const SourceBank &S);
 } // namespace mcw
 
 using namespace mcw;
@@ -186,14 +191,19 @@ void require(bool ok, const std::string &m)
 
 // Upload everything the kernels read; validate the problem as
 // CollisionKernel::initialSetup does (CollisionKernel.C:78-96).
-void prepare(mcw_ctx *c)
+void prepare(mcw_ctx *c, bool need_source = true)
 {
-  if (!c->dirty) return;
+  if (!c->dirty && (c->have_src || !need_source)) return;
   require(c->have_mesh, "no mesh: call mcw_mesh_set");
   require(!c->nucs.empty(), "no nuclear data: call mcw_xs_set");
   require(!c->mats.empty(), "no materials: call mcw_materials_set");
   require(c->have_block_map, "no block to material map: call mcw_block_material_map");
-  require(c->have_src, "no source: call mcw_source_set");
+  require(c->have_src || !need_source, "no source: call mcw_source_set");
+  if (!c->have_src) {
+    memset(&c->src, 0, sizeof c->src);
+    c->src.space = MCW_SPACE_MESH_VOLUME;
+    c->src.E0 = 1e6;
+  }
   hipStream_t s = c->stream;
   HostMesh &m = c->mesh;
 
@@ -1074,6 +1084,311 @@ int mcw_debug_xs_probe(mcw_ctx *c, int32_t nuc, const double *E, int64_t n, int3
     HIPCHK(hipMemcpyAsync(k_out, dk.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(xs_out, dx.p, (size_t)n * 32, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+  });
+}
+
+} // extern "C"
+
+// ======================================================================== request server
+//
+// Continuous batching of fixed-source requests over resident meshes.  A
+// request is admitted when an in-flight slot is free and its memory (flux
+// tally + its share of a round's source bank) fits the budget.  Each mesh
+// runs rounds: one fused launch (k_source_req + the persistent kernel in its
+// request-server form) covering a slice of every active request on that mesh,
+// at most `round_budget` histories in all; requests that arrive meanwhile join
+// at the next round, and rounds of different meshes overlap on their own
+// streams.  A request's histories use seeds (batch 1, its particle count,
+// master seed request_id + 1), so its result is the same alone or in a mix up
+// to the summation order of the FP64 tally atomics.
+
+namespace {
+
+struct ServeReq {
+  mcw_request r;
+  double t_submit = 0, t_admit = -1, t_done = -1;
+  int64_t scheduled = 0, done = 0, bytes = 0;
+  DevBuf tally, segs;
+  std::vector<double> flux;
+  unsigned long long segments = 0;
+};
+
+struct ServeMesh {
+  mcw_ctx *ctx = nullptr;
+  hipEvent_t done_ev = nullptr;
+  bool busy = false;
+  std::vector<std::pair<ServeReq *, int64_t>> round;
+  DevBuf slices;
+  DevBuf sb[10];
+  int64_t sb_cap = 0;
+  int blocks = 0;
+};
+
+} // namespace
+
+struct mcw_server {
+  std::string err;
+  std::vector<std::unique_ptr<ServeMesh>> meshes;
+  std::deque<ServeReq *> pending, done;
+  std::vector<ServeReq *> active;
+  int32_t max_inflight = 8;
+  int64_t bank_bytes = 0, used = 0;
+  int64_t round_budget = (int64_t)1 << 21;
+  std::chrono::steady_clock::time_point t0;
+  double now() const
+  {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+int sfail(mcw_server *s, const std::string &m)
+{
+  if (s) s->err = m;
+  return 1;
+}
+
+template <class F> int sguard(mcw_server *s, F &&f)
+{
+  if (!s) return 1;
+  try {
+    f();
+    return 0;
+  } catch (const McwError &e) {
+    return sfail(s, e.msg);
+  } catch (const std::exception &e) {
+    return sfail(s, e.what());
+  }
+}
+
+int64_t request_bytes(const mcw_server *s, const ServeReq *q)
+{
+  const mcw_ctx *c = s->meshes[q->r.mesh_slot]->ctx;
+  return c->mesh.n_elem * 8 + std::min(q->r.n_particles, s->round_budget) * 76;
+}
+
+void finish(mcw_server *s, ServeMesh &m, ServeReq *q)
+{
+  mcw_ctx *c = m.ctx;
+  const int64_t ne = c->mesh.n_elem;
+  std::vector<double> tmp(ne);
+  HIPCHK(hipMemcpyAsync(tmp.data(), q->tally.p, ne * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&q->segments, q->segs.p, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  q->flux.resize(ne);
+  const double norm = (double)q->r.n_particles;
+  for (int64_t e = 0; e < ne; ++e) q->flux[e] = tmp[c->mesh.int_of_orig[e]] / norm;
+  q->tally.release();
+  q->segs.release();
+  q->t_done = s->now();
+  s->used -= q->bytes;
+  s->active.erase(std::find(s->active.begin(), s->active.end(), q));
+  s->done.push_back(q);
+}
+
+void launch_round(mcw_server *s, int mi)
+{
+  ServeMesh &m = *s->meshes[mi];
+  mcw_ctx *c = m.ctx;
+  std::vector<ServeReq *> mine;
+  for (ServeReq *q : s->active)
+    if (q->r.mesh_slot == mi && q->scheduled < q->r.n_particles) mine.push_back(q);
+  if (mine.empty()) return;
+  const int64_t quota = std::max<int64_t>(s->round_budget / (int64_t)mine.size(), 65536);
+  std::vector<ReqSlice> sl;
+  m.round.clear();
+  int64_t total = 0;
+  for (ServeReq *q : mine) {
+    if (total >= s->round_budget) break;
+    const int64_t cnt = std::min(q->r.n_particles - q->scheduled, quota);
+    ReqSlice r;
+    memset(&r, 0, sizeof r);
+    r.first = total;
+    r.count = cnt;
+    r.hist0 = q->scheduled;
+    r.n_particles = q->r.n_particles;
+    r.master_seed = (uint64_t)q->r.request_id + 1;
+    r.tally = q->tally.as<double>();
+    r.segments = q->segs.as<unsigned long long>();
+    const mcw_source &src = q->r.source;
+    r.src.space = src.space;
+    r.src.angle = src.angle;
+    r.src.energy = src.energy;
+    for (int k = 0; k < 3; ++k) {
+      r.src.point[k] = src.point[k];
+      r.src.box_lo[k] = src.box_lo[k];
+      r.src.box_hi[k] = src.box_hi[k];
+      r.src.axis[k] = src.axis[k];
+    }
+    r.src.beta = src.beta;
+    r.src.E0 = src.E0;
+    r.src.watt_a = src.watt_a;
+    r.src.watt_b = src.watt_b;
+    r.src.point_elem = -1;
+    if (src.space == MCW_SPACE_POINT) {
+      const int64_t e = locate_point(c->mesh, src.point, 1e-12);
+      r.src.point_elem = e >= 0 ? c->mesh.int_of_orig[e] : -1;
+    }
+    sl.push_back(r);
+    m.round.push_back({q, cnt});
+    q->scheduled += cnt;
+    total += cnt;
+  }
+  hipStream_t st = c->stream;
+  if (m.sb_cap < total) {
+    for (int k = 0; k < 10; ++k) m.sb[k].alloc((size_t)s->round_budget * (k < 8 ? 8 : 4) + 64);
+    m.sb_cap = s->round_budget;
+  }
+  SourceBank S;
+  S.x = m.sb[0].as<double>(); S.y = m.sb[1].as<double>(); S.z = m.sb[2].as<double>();
+  S.u = m.sb[3].as<double>(); S.v = m.sb[4].as<double>(); S.w = m.sb[5].as<double>();
+  S.E = m.sb[6].as<double>(); S.seed = m.sb[7].as<uint64_t>(); S.elem = m.sb[8].as<int32_t>();
+  S.req = m.sb[9].as<int32_t>();
+  m.slices.upload(sl, st);
+  Params P = c->base;
+  P.n_tallies = P.n_tl = P.n_coll = 0;
+  P.record = 0;
+  P.batch = 1;
+  P.hist_begin = 0;
+  P.hist_end = total;
+  P.slices = m.slices.as<ReqSlice>();
+  const int ns = c->mesh.uniform_sides;
+  const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns == 4;
+  HIPCHK(launch_reset_work(st, P.cnt));
+  HIPCHK(launch_source_req(st, P, S, (int32_t)sl.size(), total));
+  const unsigned blocks = (unsigned)std::min<int64_t>(m.blocks, (total + 255) / 256);
+  HIPCHK(launch_serve(ns, p32, blocks < 1 ? 1 : blocks, st, P, S));
+  HIPCHK(hipEventRecord(m.done_ev, st));
+  m.busy = true;
+}
+
+} // namespace
+
+extern "C" {
+
+int mcw_server_create(mcw_server **out, mcw_ctx *const *meshes, int32_t n_meshes, int32_t max_inflight,
+                      int64_t bank_bytes)
+{
+  if (!out) return 1;
+  mcw_server *s = new mcw_server();
+  *out = s;
+  return sguard(s, [&] {
+    require(n_meshes > 0 && meshes, "mcw_server_create: no meshes");
+    require(max_inflight > 0, "max_inflight must be positive");
+    s->max_inflight = max_inflight;
+    s->bank_bytes = bank_bytes > 0 ? bank_bytes : ((int64_t)16 << 30);
+    s->t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < n_meshes; ++i) s->meshes.push_back(std::make_unique<ServeMesh>());
+    for (int i = 0; i < n_meshes; ++i) {
+      mcw_ctx *c = meshes[i];
+      require(c != nullptr, "null mesh context");
+      HIPCHK(hipSetDevice(c->opt.device));
+      if (guard(c, [&] { prepare(c, false); }) != 0) throw McwError{"mesh " + std::to_string(i) + ": " + c->err};
+      ServeMesh &m = *s->meshes[i];
+      m.ctx = c;
+      HIPCHK(hipEventCreateWithFlags(&m.done_ev, hipEventDisableTiming));
+      int per_cu = 0;
+      const int ns = c->mesh.uniform_sides;
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, serve_kernel(ns, c->opt.precision == MCW_PREC_MIXED && ns == 4), 256, 0));
+      m.blocks = c->n_cu * std::max(1, per_cu);
+    }
+  });
+}
+
+int mcw_server_destroy(mcw_server *s)
+{
+  if (!s) return 0;
+  for (auto &mp : s->meshes) {
+    ServeMesh &m = *mp;
+    if (m.ctx && m.ctx->stream) (void)hipStreamSynchronize(m.ctx->stream);
+    if (m.done_ev) (void)hipEventDestroy(m.done_ev);
+  }
+  for (auto *q : s->pending) delete q;
+  for (auto *q : s->active) delete q;
+  for (auto *q : s->done) delete q;
+  delete s;
+  return 0;
+}
+
+const char *mcw_server_last_error(const mcw_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int mcw_submit(mcw_server *s, const mcw_request *req)
+{
+  return sguard(s, [&] {
+    require(req != nullptr, "null request");
+    require(req->mesh_slot >= 0 && req->mesh_slot < (int32_t)s->meshes.size(), "request names an unknown mesh slot");
+    require(req->n_particles >= 1, "a request needs at least one particle");
+    const int sp = req->source.space;
+    require(sp == MCW_SPACE_POINT || sp == MCW_SPACE_BOX || sp == MCW_SPACE_MESH_VOLUME, "unknown source space type");
+    require(req->source.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
+    auto *q = new ServeReq();
+    q->r = *req;
+    q->t_submit = s->now();
+    s->pending.push_back(q);
+  });
+}
+
+int mcw_poll(mcw_server *s, int32_t *n_done)
+{
+  return sguard(s, [&] {
+    // finished rounds
+    for (auto &mp : s->meshes) {
+      ServeMesh &m = *mp;
+      if (!m.busy) continue;
+      HIPCHK(hipSetDevice(m.ctx->opt.device));
+      hipError_t e = hipEventQuery(m.done_ev);
+      if (e == hipErrorNotReady) continue;
+      HIPCHK(e);
+      m.busy = false;
+      for (auto &rc : m.round) {
+        rc.first->done += rc.second;
+        if (rc.first->done == rc.first->r.n_particles) finish(s, m, rc.first);
+      }
+      m.round.clear();
+    }
+    // admissions
+    while (!s->pending.empty() && (int32_t)s->active.size() < s->max_inflight) {
+      ServeReq *q = s->pending.front();
+      const int64_t need = request_bytes(s, q);
+      if (s->used + need > s->bank_bytes && !s->active.empty()) break;
+      mcw_ctx *c = s->meshes[q->r.mesh_slot]->ctx;
+      q->tally.alloc(c->mesh.n_elem * 8);
+      q->segs.alloc(8);
+      HIPCHK(launch_fill(c->stream, q->tally.as<double>(), c->mesh.n_elem, 0.0));
+      HIPCHK(launch_fill(c->stream, q->segs.as<double>(), 1, 0.0));
+      q->bytes = need;
+      s->used += need;
+      q->t_admit = s->now();
+      s->pending.pop_front();
+      s->active.push_back(q);
+    }
+    // next rounds
+    for (int i = 0; i < (int)s->meshes.size(); ++i)
+      if (!s->meshes[i]->busy) launch_round(s, i);
+    if (n_done) *n_done = (int32_t)s->done.size();
+  });
+}
+
+int mcw_result(mcw_server *s, mcw_request_status *status, double *flux, size_t n)
+{
+  return sguard(s, [&] {
+    require(!s->done.empty(), "no finished request");
+    ServeReq *q = s->done.front();
+    require(flux == nullptr || n >= q->flux.size(), "flux buffer too small for the request's mesh");
+    if (status) {
+      status->request_id = q->r.request_id;
+      status->histories = q->r.n_particles;
+      status->segments = (int64_t)q->segments;
+      status->collisions = 0;
+      status->t_submit_ms = q->t_submit;
+      status->t_admit_ms = q->t_admit;
+      status->t_done_ms = q->t_done;
+    }
+    if (flux) std::copy(q->flux.begin(), q->flux.end(), flux);
+    s->done.pop_front();
+    delete q;
   });
 }
 

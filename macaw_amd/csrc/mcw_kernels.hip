@@ -41,6 +41,8 @@ struct PState {
   uint64_t h;
   int32_t elem, inface, mat, k;
   int32_t nseg, ncoll;
+  double *tv;   // request server: the request's flux tally
+  int32_t req;  // request server: slice of the round
 };
 
 // Event counts of one wavefront.  Every field is wavefront-uniform (sums of
@@ -198,13 +200,12 @@ __device__ int32_t locate_point_d(const MeshView &M, const double *p, double tol
 // OpenMC fixed-source initialize_history (OpenMCStudy.C:317-363 drives it):
 // seeds for particle id (batch-1)*n + i + 1, then space, angle, energy
 // sampled on stream 1 (sample_external_source).
-__device__ void sample_site(const Params &P, int64_t hid, double *p, double *dir, double &E, uint64_t &track_seed,
-                            int32_t &elem_out)
+__device__ void sample_site(const Params &P, const SourceView &S, uint64_t master, int64_t batch, int64_t n_particles,
+                            int64_t hid, double *p, double *dir, double &E, uint64_t &track_seed, int32_t &elem_out)
 {
-  const int64_t id = (P.batch - 1) * P.n_particles + hid + 1;
-  track_seed = init_seed(id, 0, P.master_seed);
-  uint64_t ss = init_seed(id, 1, P.master_seed);
-  const SourceView &S = P.src;
+  const int64_t id = (batch - 1) * n_particles + hid + 1;
+  track_seed = init_seed(id, 0, master);
+  uint64_t ss = init_seed(id, 1, master);
   const MeshView &M = P.mesh;
   int64_t elem;
   if (S.space == 0) {
@@ -307,7 +308,7 @@ __device__ __forceinline__ int32_t pick_nbr(const int4 &r0, const int4 &r1, int 
 // EV_COLLIDE leaves the particle at the collision point for collide().
 // P32 (TET4 only): exit side from the FP32 mixed record, FP64 recheck of
 // near ties (counted in `rechecks`).
-template <int NS, bool REC, bool P32>
+template <int NS, bool REC, bool P32, bool SERVE = false>
 __device__ __forceinline__ int segment(const Params &P, PState &st, double &track_sum, unsigned &rechecks)
 {
   const MeshView &M = P.mesh;
@@ -359,7 +360,9 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   const double track = (d < L) ? d : L;
   ++st.nseg;
   track_sum += track;
-  for (int t = 0; t < P.n_tl; ++t) tally_score(P.tally[t], P.xs, st.elem, st.E, st.k, track, mat);
+  if (SERVE) unsafeAtomicAdd(st.tv + st.elem, track);
+  else
+    for (int t = 0; t < P.n_tl; ++t) tally_score(P.tally[t], P.xs, st.elem, st.E, st.k, track, mat);
   if (d > L) {
     st.x = st.x + L * st.u;
     st.y = st.y + L * st.v;
@@ -550,12 +553,38 @@ extern "C" __global__ void __launch_bounds__(256) k_source_bank(Params P, Source
   double p[3], d[3], E;
   uint64_t seed;
   int32_t elem;
-  sample_site(P, P.hist_begin + i, p, d, E, seed, elem);
+  sample_site(P, P.src, P.master_seed, P.batch, P.n_particles, P.hist_begin + i, p, d, E, seed, elem);
   S.x[i] = p[0]; S.y[i] = p[1]; S.z[i] = p[2];
   S.u[i] = d[0]; S.v[i] = d[1]; S.w[i] = d[2];
   S.E[i] = E;
   S.seed[i] = seed;
   S.elem[i] = elem;
+}
+
+// Sites of a request-server round: slot i belongs to the slice whose range
+// holds it.
+extern "C" __global__ void __launch_bounds__(256) k_source_req(Params P, SourceBank S, int32_t n_slices, int64_t n)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const ReqSlice *R = P.slices;
+  int32_t lo = 0, hi = n_slices; // last slice with first <= i
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (R[mid].first <= i) lo = mid;
+    else hi = mid;
+  }
+  const ReqSlice &r = R[lo];
+  double p[3], d[3], E;
+  uint64_t seed;
+  int32_t elem;
+  sample_site(P, r.src, r.master_seed, 1, r.n_particles, r.hist0 + (i - r.first), p, d, E, seed, elem);
+  S.x[i] = p[0]; S.y[i] = p[1]; S.z[i] = p[2];
+  S.u[i] = d[0]; S.v[i] = d[1]; S.w[i] = d[2];
+  S.E[i] = E;
+  S.seed[i] = seed;
+  S.elem[i] = elem;
+  S.req[i] = lo;
 }
 
 // ------------------------------------------------------------------ PLAIN kernels
@@ -652,7 +681,7 @@ enum { PH_IDLE = 0, PH_TRANSPORT = 1, PH_COLLIDE = 2 };
 
 // WPE: minimum wavefronts per SIMD requested from the register allocator
 // (0 = compiler's choice); a tuning knob trading spills for occupancy.
-template <int NS, bool REC, int WPE, bool P32>
+template <int NS, bool REC, int WPE, bool P32, bool SERVE = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 k_persistent(Params P, SourceBank S)
 {
@@ -677,6 +706,10 @@ k_persistent(Params P, SourceBank S)
       if (slot < n_hist) {
         load_site(S, slot, st);
         start_history(st, P.hist_begin + slot);
+        if (SERVE) {
+          st.req = S.req[slot];
+          st.tv = P.slices[st.req].tally;
+        }
         st.k = union_index(P.xs, s_hash, st.E);
         phase = PH_TRANSPORT;
       } else {
@@ -689,11 +722,12 @@ k_persistent(Params P, SourceBank S)
       const bool go = phase == PH_TRANSPORT;
       int ev = -1;
       if (go) {
-        ev = segment<NS, REC, P32>(P, st, track, rechecks);
+        ev = segment<NS, REC, P32, SERVE>(P, st, track, rechecks);
         if (ev == EV_COLLIDE) {
           phase = PH_COLLIDE;
         } else if (is_death(ev)) {
           if (REC) write_record(P, st, fate_of(ev));
+          if (SERVE) atomicAdd(P.slices[st.req].segments, (unsigned long long)st.nseg);
           phase = PH_IDLE;
         }
       }
@@ -708,6 +742,7 @@ k_persistent(Params P, SourceBank S)
         phase = PH_TRANSPORT;
       } else {
         if (REC) write_record(P, st, fate_of(ev));
+        if (SERVE) atomicAdd(P.slices[st.req].segments, (unsigned long long)st.nseg);
         phase = PH_IDLE;
       }
     }
@@ -859,6 +894,26 @@ hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int blo
   const void *k = persistent_kernel(ns, wpe, P.record != 0, p32);
   void *args[] = {(void *)&P, (void *)&S};
   return hipLaunchKernel(k, dim3(blocks), dim3(block), args, 0, s);
+}
+
+hipError_t launch_source_req(hipStream_t s, const Params &P, const SourceBank &S, int32_t n_slices, int64_t n)
+{
+  hipLaunchKernelGGL(k_source_req, dim3(grid_for(n, 256, 0)), dim3(256), 0, s, P, S, n_slices, n);
+  return hipGetLastError();
+}
+
+// Request-server transport kernel (no records, compiler-chosen occupancy).
+const void *serve_kernel(int ns, bool p32)
+{
+  if (ns == 4) return p32 ? (const void *)k_persistent<4, false, 0, true, true> : (const void *)k_persistent<4, false, 0, false, true>;
+  if (ns == 6) return (const void *)k_persistent<6, false, 0, false, true>;
+  return (const void *)k_persistent<0, false, 0, false, true>;
+}
+
+hipError_t launch_serve(int ns, bool p32, unsigned blocks, hipStream_t s, const Params &P, const SourceBank &S)
+{
+  void *args[] = {(void *)&P, (void *)&S};
+  return hipLaunchKernel(serve_kernel(ns, p32), dim3(blocks), dim3(256), args, 0, s);
 }
 
 hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1)

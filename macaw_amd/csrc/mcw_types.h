@@ -125,6 +125,23 @@ struct SourceBank {
   double *x, *y, *z, *u, *v, *w, *E;
   uint64_t *seed;    // tracking-stream seed
   int32_t *elem;
+  int32_t *req;      // request-server rounds: slice index of each site, else nullptr
+};
+
+// One request's slice of a request-server round: histories hist0 ..
+// hist0 + count - 1 of the request occupy source-bank slots first .. first +
+// count - 1.  Seeds follow OpenMC's formula with batch 1, the request's
+// particle count and a master seed derived from the request id, so a result
+// does not depend on what else runs beside it.
+struct ReqSlice {
+  int64_t first;
+  int64_t count;
+  int64_t hist0;
+  int64_t n_particles;
+  uint64_t master_seed;
+  double *tally;                   // per-element track-length flux (internal order)
+  unsigned long long *segments;    // per-request segment counter
+  SourceView src;
 };
 
 struct RecordView {
@@ -160,6 +177,7 @@ struct Params {
   uint64_t master_seed;
   Counters *cnt;
   RecordView rec;
+  const ReqSlice *slices;  // request-server rounds, else nullptr
 };
 
 } // namespace mcw

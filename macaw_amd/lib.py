@@ -182,3 +182,73 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Request server over resident mesh contexts (mcw_server_*): continuous
+    batching of fixed-source requests, each with a per-element track-length
+    flux tally (per source particle)."""
+
+    def __init__(self, contexts, max_inflight=8, bank_bytes=0):
+        L = lib()
+        self.L = L
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h.value for c in self.contexts])
+        h = C.c_void_p()
+        rc = L.mcw_server_create(C.byref(h), arr, len(self.contexts), max_inflight, bank_bytes)
+        self.h = h
+        if rc != 0:
+            msg = L.mcw_server_last_error(h).decode() if h.value else "mcw_server_create failed"
+            raise McwError(msg)
+        self.n_elem = [c.problem.mesh.n_elem for c in self.contexts]
+        self.mesh_of = {}
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise McwError(self.L.mcw_server_last_error(self.h).decode())
+
+    def submit(self, request_id, mesh_slot, n_particles, source):
+        r = abi.Request()
+        r.request_id = request_id
+        r.mesh_slot = mesh_slot
+        r.n_particles = n_particles
+        r.source = source
+        self.mesh_of[request_id] = mesh_slot
+        self._chk(self.L.mcw_submit(self.h, C.byref(r)))
+
+    def poll(self):
+        n = C.c_int32()
+        self._chk(self.L.mcw_poll(self.h, C.byref(n)))
+        return n.value
+
+    def result(self):
+        st = abi.RequestStatus()
+        buf = np.zeros(max(self.n_elem))
+        self._chk(self.L.mcw_result(self.h, C.byref(st), buf.ctypes.data_as(C.POINTER(C.c_double)), buf.size))
+        d = {k: getattr(st, k) for k, _ in st._fields_}
+        return d, buf[:self.n_elem[self.mesh_of[st.request_id]]].copy()
+
+    def run_all(self, sleep=0.0):
+        """Poll until every submitted request has finished; returns {id: (status, flux)}."""
+        import time
+        out = {}
+        want = len(self.mesh_of)
+        while len(out) < want:
+            n = self.poll()
+            for _ in range(n):
+                st, fl = self.result()
+                out[st["request_id"]] = (st, fl)
+            if sleep:
+                time.sleep(sleep)
+        return out
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.L.mcw_server_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

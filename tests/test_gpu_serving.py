@@ -1,0 +1,90 @@
+"""Request server (continuous batching): every request's per-element flux is
+the same alone and in a mix (1e-10 relative: only the FP64 atomic summation
+order differs), and equals the oracle's track-length flux for that request
+(batch 1, the request's particle count, master seed request_id + 1)."""
+import numpy as np
+import pytest
+
+from macaw_amd import abi, problems as P
+from macaw_amd.lib import Context, Server
+
+pytestmark = pytest.mark.gpu
+
+
+def meshes():
+    t = P.tet_cube(9, seed=101, cache=False)
+    t.subdomain = P.voronoi_blocks(t, 4, seed=303)
+    tet = P.Problem(name="tet", mesh=t, nuclides=P.c2_nuclides(2000, 4), materials=P.c2_materials(4, 4),
+                    block_material={b: b + 1 for b in range(4)}, bcs={0: abi.BC_VACUUM},
+                    source=P.volume_watt_source(), tallies=[], n_particles=1, n_batches=1)
+    hexp = P.c1_hex_reflective(n_particles=1, n_points=20000)
+    hexp.tallies = []
+    return [tet, hexp]
+
+
+def requests():
+    out = []
+    srcs = [P.point_source((0.4, 0.45, 0.52), 1.5e6), P.box_source((0.1, 0.1, 0.1), (0.6, 0.9, 0.5), 2e6),
+            P.volume_watt_source(), P.point_source((0.3, -1.2, 2.0), 2e6), P.box_source((-4, -4, -4), (0, 0, 0))]
+    slots = [0, 0, 0, 1, 1]
+    sizes = [3000, 12000, 700, 5000, 20000, 1500, 9000, 400]
+    rid = 0
+    for k, n in enumerate(sizes):
+        j = k % len(srcs)
+        out.append((rid, slots[j], n, srcs[j]))
+        rid += 7
+    return out
+
+
+def contexts(probs):
+    return [Context(p) for p in probs]
+
+
+def test_requests_same_alone_and_in_mix():
+    probs = meshes()
+    reqs = requests()
+    ctx = contexts(probs)
+    # alone: one request at a time through its own server
+    alone = {}
+    for rid, slot, n, src in reqs:
+        s = Server(ctx, max_inflight=4)
+        s.submit(rid, slot, n, src)
+        alone.update(s.run_all())
+        s.close()
+    # mixed: all at once, small in-flight limit so admission queues them
+    s = Server(ctx, max_inflight=3, bank_bytes=1 << 26)
+    for rid, slot, n, src in reqs:
+        s.submit(rid, slot, n, src)
+    mixed = s.run_all()
+    s.close()
+    assert sorted(alone) == sorted(mixed) == sorted(r[0] for r in reqs)
+    for rid, slot, n, src in reqs:
+        a, b = alone[rid][1], mixed[rid][1]
+        assert a.sum() > 0
+        rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-300)
+        assert rel.max() <= 1e-10, (rid, rel.max())
+        assert alone[rid][0]["segments"] == mixed[rid][0]["segments"]
+        st = mixed[rid][0]
+        assert st["t_submit_ms"] <= st["t_admit_ms"] <= st["t_done_ms"]
+
+
+def test_requests_match_oracle():
+    from oracle.pyoracle import Oracle
+    probs = meshes()
+    reqs = [r for r in requests() if r[2] <= 5000]
+    s = Server(contexts(probs), max_inflight=8)
+    for rid, slot, n, src in reqs:
+        s.submit(rid, slot, n, src)
+    got = s.run_all()
+    s.close()
+    for rid, slot, n, src in reqs:
+        p = probs[slot]
+        p.source = src
+        p.tallies = P.flux_tallies(both=False)
+        o = Oracle(p, master_seed=rid + 1)
+        so = o.run_batch(1, n, threads=4)
+        want = o.tally(1, abi.RESULT_SUM)
+        st, flux = got[rid]
+        assert st["segments"] == so["segments"], rid
+        rel = np.abs(flux - want) / np.maximum(np.abs(want), 1e-300)
+        assert rel.max() <= 1e-10, (rid, rel.max())
