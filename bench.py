@@ -74,6 +74,55 @@ def cpu_baseline(problem, n_hist_full, seconds=12.0):
             "segments_per_s": st["segments"] / dt, "seconds": dt}
 
 
+def bench_serving(args, tuning):
+    """Serving mix: resident 100k / 1M / 4M-tet meshes, requests with point or
+    box sources and particle counts log-uniform in [1e4, 1e7], Poisson
+    arrivals at `--serve-load` of the capacity measured by a saturated run."""
+    from macaw_amd import abi, problems as P
+    from macaw_amd.lib import Context, Server
+    from macaw_amd.serving import run_closed_loop, run_open_loop, summarize
+
+    t = time.perf_counter()
+    probs = P.c4_meshes()
+    log(f"serving meshes {[p.mesh.n_elem for p in probs]} generated ({time.perf_counter() - t:.1f}s)")
+    t = time.perf_counter()
+    ctxs = [Context(p, mode=abi.MODE_BATCHED, lanes_min=tuning.get("lanes_min", 0)) for p in probs]
+    bank = int(args.serve_bank_gib * (1 << 30))
+    srv = Server(ctxs, max_inflight=args.serve_inflight, bank_bytes=bank)
+    log(f"server up ({time.perf_counter() - t:.1f}s)")
+    warm = P.c4_requests(n=6, seed=504, n_max=1e5)
+    for r in warm:
+        r.request_id += 2_000_000
+    run_closed_loop(srv, warm)
+    cal = P.c4_requests(n=args.serve_calibrate, seed=506)
+    for r in cal:
+        r.request_id += 1_000_000
+    t = time.perf_counter()
+    run_closed_loop(srv, cal)
+    cap = sum(r.n_particles for r in cal) / (time.perf_counter() - t)
+    reqs = P.c4_requests(n=args.serve_requests, seed=505)
+    mean_n = sum(r.n_particles for r in reqs) / len(reqs)
+    rate = args.serve_load * cap / mean_n
+    reqs = P.c4_requests(n=args.serve_requests, seed=505, rate=rate)
+    log(f"serving: capacity {cap:.3e} histories/s (saturated, {len(cal)} requests); "
+        f"{len(reqs)} requests at {rate:.2f}/s ({args.serve_load:.0%} load, mean {mean_n:.3e} particles)")
+    t = time.perf_counter()
+    res = run_open_loop(srv, reqs, log=log)
+    wall = time.perf_counter() - t
+    srv.close()
+    for c in ctxs:
+        c.close()
+    out = summarize(res, wall)
+    out.update({"rate_requests_per_s": rate, "offered_load": args.serve_load, "capacity_histories_per_s": cap,
+                "meshes_tets": [p.mesh.n_elem for p in probs], "max_inflight": args.serve_inflight,
+                "bank_gib": args.serve_bank_gib, "particles": "log-uniform [1e4, 1e7]",
+                "sources": "point or box (1:1), Watt", "tally": "per-element track-length flux per request",
+                "arrivals": "Poisson, seed 505", "precision": "fp64"})
+    log(f"serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
+        f"{out['histories_per_s']:.3e} histories/s over {wall:.1f}s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +135,12 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--lanes-min", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving mix")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-calibrate", type=int, default=100, help="requests in the saturated capacity run")
+    ap.add_argument("--serve-load", type=float, default=0.7, help="offered load as a fraction of capacity")
+    ap.add_argument("--serve-inflight", type=int, default=32)
+    ap.add_argument("--serve-bank-gib", type=float, default=4.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
                          "bytes): the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling")
@@ -165,6 +220,10 @@ def main():
         pctx.close()
     ctx.close()
 
+    serving = None
+    if not args.no_serve:
+        serving = bench_serving(args, tuning)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(prob, N)
@@ -196,6 +255,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_persistent<4>", "alg_bytes_per_launch": per_launch_bytes,
                      "launch_ms": per_launch_s * 1e3},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

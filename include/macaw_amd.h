@@ -260,7 +260,8 @@ int mcw_debug_xs_probe(mcw_ctx *ctx, int32_t nuc, const double *E, int64_t n, in
 /* ---- serving (continuous batching of fixed-source requests) -------------- */
 
 /* A request: a source, a particle count and a per-element flux tally
- * (track-length) on one resident mesh (mesh_slot from mcw_mesh_register). */
+ * (track-length) on one resident mesh (mesh_slot: index into the contexts
+ * given to mcw_server_create). */
 typedef struct {
   int64_t request_id;
   int32_t mesh_slot;
@@ -279,10 +280,13 @@ typedef struct {
   double t_done_ms;
 } mcw_request_status;
 
-/* Build a server over `n_meshes` resident contexts (each a mesh + materials +
- * block map); every context shares one XS set of its own.  max_inflight
- * bounds concurrently admitted requests; bank_bytes bounds particle-bank
- * memory across them. */
+/* Build a server over `n_meshes` resident contexts (each a mesh + XS +
+ * materials + block map, set as for mcw_run_batch; a context's own source and
+ * tallies are not used).  max_inflight bounds concurrently admitted requests;
+ * bank_bytes (0: 16 GiB) bounds their memory, each request counting its flux
+ * tally plus min(n_particles, 2^21) x 76 B of particle bank.  A request's
+ * histories are those of batch 1 with n_particles histories and master seed
+ * request_id + 1, whatever else is in flight. */
 typedef struct mcw_server mcw_server;
 int mcw_server_create(mcw_server **srv, mcw_ctx *const *meshes, int32_t n_meshes,
                       int32_t max_inflight, int64_t bank_bytes);
@@ -291,8 +295,9 @@ const char *mcw_server_last_error(const mcw_server *srv);
 int mcw_submit(mcw_server *srv, const mcw_request *req);
 /* Advance the device work; returns the number of finished requests waiting. */
 int mcw_poll(mcw_server *srv, int32_t *n_done);
-/* Pop one finished request; flux receives n_elem values of its mesh
- * (track-length flux per source particle). */
+/* Pop one finished request (in completion order); flux receives n_elem
+ * values of its mesh (track-length flux per source particle, caller element
+ * order), or is NULL to drop them. */
 int mcw_result(mcw_server *srv, mcw_request_status *status, double *flux, size_t n);
 
 #ifdef __cplusplus

@@ -37,6 +37,8 @@ hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1);
 hipError_t launch_reset_work(hipStream_t s, Counters *C);
 hipError_t launch_fill(hipStream_t s, double *p, int64_t n, double v);
 hipError_t launch_zero_counters(hipStream_t s, Counters *C);
+hipError_t launch_gather_scale(hipStream_t s, const double *src, const int32_t *perm, int64_t n, double norm,
+                               double *dst, int tail);
 hipError_t launch_accumulate(hipStream_t s, double *value, double *sum, double *sum_sq, int64_t n, double norm);
 hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *lo, double *so, double *co,
                              double *sq, const uint64_t *seeds, double *po, uint64_t *seo);
@@ -357,6 +359,10 @@ void prepare(mcw_ctx *c, bool need_source = true)
   c->mat_nuc.upload(mnuc, s);
   c->mat_dens.upload(mden, s);
 
+  // point locator for point and box sources (and for the request server)
+  build_locator(m);
+  c->cell_start.upload(m.cell_start, s);
+  c->cell_elem.upload(m.cell_elem, s);
   // source
   c->src_elem = -1;
   if (c->src.space == MCW_SPACE_POINT) {
@@ -368,10 +374,6 @@ void prepare(mcw_ctx *c, bool need_source = true)
   } else {
     require(c->src.space == MCW_SPACE_MESH_VOLUME, "unknown source space type");
   }
-  // point locator for box sources (and for the request server)
-  build_locator(m);
-  c->cell_start.upload(m.cell_start, s);
-  c->cell_elem.upload(m.cell_elem, s);
   require(c->src.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
   require(c->src.energy == MCW_ENERGY_MONO || c->src.energy == MCW_ENERGY_WATT, "source energy type not supported");
 
@@ -466,6 +468,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
     V.nuclides = t.d_nuclides.as<int32_t>();
     V.value = t.value.as<double>();
   }
+  P.tl_fast = (n_tl == 1 && P.tally[0].simple_flux && P.tally[0].n_edges == 0) ? 1 : 0;
   P.master_seed = c->opt.master_seed ? c->opt.master_seed : 1;
   P.roundtrip = c->opt.seed_roundtrip;
   P.lanes_min = c->opt.lanes_min > 0 ? std::min(c->opt.lanes_min, 64) : 40;
@@ -1093,14 +1096,23 @@ int mcw_debug_xs_probe(mcw_ctx *c, int32_t nuc, const double *E, int64_t n, int3
 //
 // Continuous batching of fixed-source requests over resident meshes.  A
 // request is admitted when an in-flight slot is free and its memory (flux
-// tally + its share of a round's source bank) fits the budget.  Each mesh
-// runs rounds: one fused launch (k_source_req + the persistent kernel in its
-// request-server form) covering a slice of every active request on that mesh,
-// at most `round_budget` histories in all; requests that arrive meanwhile join
-// at the next round, and rounds of different meshes overlap on their own
-// streams.  A request's histories use seeds (batch 1, its particle count,
-// master seed request_id + 1), so its result is the same alone or in a mix up
-// to the summation order of the FP64 tally atomics.
+// tally + its share of a round's source bank, min(N, round budget) x 76 B)
+// fits the budget.  Each mesh runs rounds: one fused launch (k_source_req +
+// the persistent kernel in its request-server form) covering a slice of
+// every active request on that mesh, at most `round_budget` histories in
+// all, taken round-robin so no request starves.  A mesh has two round slots,
+// each with its own stream, counters and source bank, so the next round's
+// workgroups fill the compute units that the current round's tail (its last,
+// longest histories) leaves idle; requests that arrive meanwhile join at the
+// next round, and rounds of different meshes overlap as well.  Admission
+// fills and result copies run on a third stream per mesh: a finished
+// request's tally is gathered into caller element order and normalised on the
+// device, then copied to pinned host memory; the request is done when that
+// copy has landed.  Device tallies and pinned buffers are pooled per mesh, so
+// the loop makes no allocation once the pools have grown to the in-flight
+// peak.  A request's histories use seeds (batch 1, its particle count, master
+// seed request_id + 1), so its result is the same alone or in a mix up to the
+// summation order of the FP64 tally atomics.
 
 namespace {
 
@@ -1108,20 +1120,70 @@ struct ServeReq {
   mcw_request r;
   double t_submit = 0, t_admit = -1, t_done = -1;
   int64_t scheduled = 0, done = 0, bytes = 0;
-  DevBuf tally, segs;
-  std::vector<double> flux;
-  unsigned long long segments = 0;
+  int32_t point_elem = -1; // internal element of a point source
+  int slot = -1;           // pooled tally / pinned buffer in its mesh
+  hipEvent_t copied = nullptr;
+  bool copying = false;
+  unsigned long long segments = 0, collisions = 0;
+};
+
+// A request's device buffer: n_elem flux values (internal order), then its
+// segment and collision counters.
+constexpr int REQ_TAIL = 2;
+constexpr int ROUND_SLOTS = 2;
+
+struct RoundSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done_ev = nullptr;
+  bool busy = false;
+  std::vector<std::pair<ServeReq *, int64_t>> round;
+  DevBuf slices, counters;
+  ReqSlice *h_slices = nullptr; // pinned, one per in-flight request
+  DevBuf sb[10];
+  int64_t sb_cap = 0;
+  ~RoundSlot()
+  {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (h_slices) (void)hipHostFree(h_slices);
+    if (done_ev) (void)hipEventDestroy(done_ev);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
 struct ServeMesh {
   mcw_ctx *ctx = nullptr;
-  hipEvent_t done_ev = nullptr;
-  bool busy = false;
-  std::vector<std::pair<ServeReq *, int64_t>> round;
-  DevBuf slices;
-  DevBuf sb[10];
-  int64_t sb_cap = 0;
+  RoundSlot rs[ROUND_SLOTS];
+  hipStream_t aux = nullptr; // admission fills, result gathers and copies
+  hipEvent_t fill_ev = nullptr;
+  bool fill_recorded = false;
+  DevBuf stage;
   int blocks = 0;
+  size_t cursor = 0; // round-robin start among the mesh's active requests
+  std::vector<std::unique_ptr<DevBuf>> tallies; // n_elem + REQ_TAIL doubles
+  std::vector<double *> pinned;                 // n_elem + REQ_TAIL doubles
+  std::vector<int> free_slots;
+  ~ServeMesh()
+  {
+    if (aux) (void)hipStreamSynchronize(aux);
+    for (double *p : pinned)
+      if (p) (void)hipHostFree(p);
+    if (fill_ev) (void)hipEventDestroy(fill_ev);
+    if (aux) (void)hipStreamDestroy(aux);
+  }
+  int acquire()
+  {
+    if (!free_slots.empty()) {
+      const int k = free_slots.back();
+      free_slots.pop_back();
+      return k;
+    }
+    const size_t n = (size_t)ctx->mesh.n_elem + REQ_TAIL;
+    tallies.push_back(std::make_unique<DevBuf>());
+    tallies.back()->alloc(n * 8);
+    pinned.push_back(nullptr);
+    HIPCHK(hipHostMalloc((void **)&pinned.back(), n * 8, hipHostMallocDefault));
+    return (int)tallies.size() - 1;
+  }
 };
 
 } // namespace
@@ -1165,52 +1227,57 @@ template <class F> int sguard(mcw_server *s, F &&f)
 int64_t request_bytes(const mcw_server *s, const ServeReq *q)
 {
   const mcw_ctx *c = s->meshes[q->r.mesh_slot]->ctx;
-  return c->mesh.n_elem * 8 + std::min(q->r.n_particles, s->round_budget) * 76;
+  return (c->mesh.n_elem + REQ_TAIL) * 8 + std::min(q->r.n_particles, s->round_budget) * 76;
 }
 
-void finish(mcw_server *s, ServeMesh &m, ServeReq *q)
+void release_request(mcw_server *s, ServeReq *q)
+{
+  if (q->slot >= 0) s->meshes[q->r.mesh_slot]->free_slots.push_back(q->slot);
+  q->slot = -1;
+  if (q->copied) (void)hipEventDestroy(q->copied);
+  q->copied = nullptr;
+  delete q;
+}
+
+// Every round holding q's histories is complete: gather + normalise on the
+// device, then copy to its pinned buffer; the copy's event marks it done.
+void start_copy(ServeMesh &m, ServeReq *q)
 {
   mcw_ctx *c = m.ctx;
   const int64_t ne = c->mesh.n_elem;
-  std::vector<double> tmp(ne);
-  HIPCHK(hipMemcpyAsync(tmp.data(), q->tally.p, ne * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(&q->segments, q->segs.p, 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  q->flux.resize(ne);
-  const double norm = (double)q->r.n_particles;
-  for (int64_t e = 0; e < ne; ++e) q->flux[e] = tmp[c->mesh.int_of_orig[e]] / norm;
-  q->tally.release();
-  q->segs.release();
-  q->t_done = s->now();
-  s->used -= q->bytes;
-  s->active.erase(std::find(s->active.begin(), s->active.end(), q));
-  s->done.push_back(q);
+  HIPCHK(launch_gather_scale(m.aux, m.tallies[q->slot]->as<double>(), c->int_of_orig.as<int32_t>(), ne,
+                             (double)q->r.n_particles, m.stage.as<double>(), REQ_TAIL));
+  HIPCHK(hipMemcpyAsync(m.pinned[q->slot], m.stage.p, (size_t)(ne + REQ_TAIL) * 8, hipMemcpyDeviceToHost, m.aux));
+  HIPCHK(hipEventRecord(q->copied, m.aux));
+  q->copying = true;
 }
 
-void launch_round(mcw_server *s, int mi)
+// Launch one round in a free slot of mesh mi; false when there is no work.
+bool launch_round(mcw_server *s, int mi, RoundSlot &R)
 {
   ServeMesh &m = *s->meshes[mi];
   mcw_ctx *c = m.ctx;
   std::vector<ServeReq *> mine;
   for (ServeReq *q : s->active)
     if (q->r.mesh_slot == mi && q->scheduled < q->r.n_particles) mine.push_back(q);
-  if (mine.empty()) return;
+  if (mine.empty()) return false;
   const int64_t quota = std::max<int64_t>(s->round_budget / (int64_t)mine.size(), 65536);
-  std::vector<ReqSlice> sl;
-  m.round.clear();
+  const size_t start = m.cursor % mine.size();
+  R.round.clear();
   int64_t total = 0;
-  for (ServeReq *q : mine) {
-    if (total >= s->round_budget) break;
-    const int64_t cnt = std::min(q->r.n_particles - q->scheduled, quota);
-    ReqSlice r;
+  int32_t n_sl = 0;
+  for (size_t j = 0; j < mine.size() && total < s->round_budget; ++j) {
+    ServeReq *q = mine[(start + j) % mine.size()];
+    const int64_t cnt = std::min(std::min(q->r.n_particles - q->scheduled, quota), s->round_budget - total);
+    ReqSlice &r = R.h_slices[n_sl++];
     memset(&r, 0, sizeof r);
     r.first = total;
     r.count = cnt;
     r.hist0 = q->scheduled;
     r.n_particles = q->r.n_particles;
     r.master_seed = (uint64_t)q->r.request_id + 1;
-    r.tally = q->tally.as<double>();
-    r.segments = q->segs.as<unsigned long long>();
+    r.tally = m.tallies[q->slot]->as<double>();
+    r.counts = (unsigned long long *)(r.tally + c->mesh.n_elem);
     const mcw_source &src = q->r.source;
     r.src.space = src.space;
     r.src.angle = src.angle;
@@ -1225,42 +1292,44 @@ void launch_round(mcw_server *s, int mi)
     r.src.E0 = src.E0;
     r.src.watt_a = src.watt_a;
     r.src.watt_b = src.watt_b;
-    r.src.point_elem = -1;
-    if (src.space == MCW_SPACE_POINT) {
-      const int64_t e = locate_point(c->mesh, src.point, 1e-12);
-      r.src.point_elem = e >= 0 ? c->mesh.int_of_orig[e] : -1;
-    }
-    sl.push_back(r);
-    m.round.push_back({q, cnt});
+    r.src.point_elem = q->point_elem;
+    R.round.push_back({q, cnt});
     q->scheduled += cnt;
     total += cnt;
   }
-  hipStream_t st = c->stream;
-  if (m.sb_cap < total) {
-    for (int k = 0; k < 10; ++k) m.sb[k].alloc((size_t)s->round_budget * (k < 8 ? 8 : 4) + 64);
-    m.sb_cap = s->round_budget;
+  m.cursor = start + (size_t)n_sl;
+  hipStream_t st = R.stream;
+  if (R.sb_cap < total) {
+    for (int k = 0; k < 10; ++k) R.sb[k].alloc((size_t)s->round_budget * (k < 8 ? 8 : 4) + 64);
+    R.sb_cap = s->round_budget;
   }
   SourceBank S;
-  S.x = m.sb[0].as<double>(); S.y = m.sb[1].as<double>(); S.z = m.sb[2].as<double>();
-  S.u = m.sb[3].as<double>(); S.v = m.sb[4].as<double>(); S.w = m.sb[5].as<double>();
-  S.E = m.sb[6].as<double>(); S.seed = m.sb[7].as<uint64_t>(); S.elem = m.sb[8].as<int32_t>();
-  S.req = m.sb[9].as<int32_t>();
-  m.slices.upload(sl, st);
+  S.x = R.sb[0].as<double>(); S.y = R.sb[1].as<double>(); S.z = R.sb[2].as<double>();
+  S.u = R.sb[3].as<double>(); S.v = R.sb[4].as<double>(); S.w = R.sb[5].as<double>();
+  S.E = R.sb[6].as<double>(); S.seed = R.sb[7].as<uint64_t>(); S.elem = R.sb[8].as<int32_t>();
+  S.req = R.sb[9].as<int32_t>();
+  // admission fills of the tallies this round adds to come first
+  if (m.fill_recorded) HIPCHK(hipStreamWaitEvent(st, m.fill_ev, 0));
+  // the slot's previous round (and so its slice upload) completed before this call
+  HIPCHK(hipMemcpyAsync(R.slices.p, R.h_slices, (size_t)n_sl * sizeof(ReqSlice), hipMemcpyHostToDevice, st));
   Params P = c->base;
   P.n_tallies = P.n_tl = P.n_coll = 0;
+  P.tl_fast = 0;
   P.record = 0;
   P.batch = 1;
   P.hist_begin = 0;
   P.hist_end = total;
-  P.slices = m.slices.as<ReqSlice>();
+  P.slices = R.slices.as<ReqSlice>();
+  P.cnt = R.counters.as<Counters>();
   const int ns = c->mesh.uniform_sides;
   const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns == 4;
   HIPCHK(launch_reset_work(st, P.cnt));
-  HIPCHK(launch_source_req(st, P, S, (int32_t)sl.size(), total));
+  HIPCHK(launch_source_req(st, P, S, n_sl, total));
   const unsigned blocks = (unsigned)std::min<int64_t>(m.blocks, (total + 255) / 256);
   HIPCHK(launch_serve(ns, p32, blocks < 1 ? 1 : blocks, st, P, S));
-  HIPCHK(hipEventRecord(m.done_ev, st));
-  m.busy = true;
+  HIPCHK(hipEventRecord(R.done_ev, st));
+  R.busy = true;
+  return true;
 }
 
 } // namespace
@@ -1283,11 +1352,22 @@ int mcw_server_create(mcw_server **out, mcw_ctx *const *meshes, int32_t n_meshes
     for (int i = 0; i < n_meshes; ++i) {
       mcw_ctx *c = meshes[i];
       require(c != nullptr, "null mesh context");
+      for (int j = 0; j < i; ++j) require(meshes[j] != c, "a mesh context is listed twice");
       HIPCHK(hipSetDevice(c->opt.device));
       if (guard(c, [&] { prepare(c, false); }) != 0) throw McwError{"mesh " + std::to_string(i) + ": " + c->err};
       ServeMesh &m = *s->meshes[i];
       m.ctx = c;
-      HIPCHK(hipEventCreateWithFlags(&m.done_ev, hipEventDisableTiming));
+      HIPCHK(hipStreamCreateWithFlags(&m.aux, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&m.fill_ev, hipEventDisableTiming));
+      m.stage.alloc((size_t)(c->mesh.n_elem + REQ_TAIL) * 8);
+      for (RoundSlot &R : m.rs) {
+        HIPCHK(hipStreamCreateWithFlags(&R.stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&R.done_ev, hipEventDisableTiming));
+        HIPCHK(hipHostMalloc((void **)&R.h_slices, (size_t)max_inflight * sizeof(ReqSlice), hipHostMallocDefault));
+        R.slices.alloc((size_t)max_inflight * sizeof(ReqSlice));
+        R.counters.alloc(sizeof(Counters));
+        HIPCHK(launch_zero_counters(R.stream, R.counters.as<Counters>()));
+      }
       int per_cu = 0;
       const int ns = c->mesh.uniform_sides;
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -1301,13 +1381,13 @@ int mcw_server_destroy(mcw_server *s)
 {
   if (!s) return 0;
   for (auto &mp : s->meshes) {
-    ServeMesh &m = *mp;
-    if (m.ctx && m.ctx->stream) (void)hipStreamSynchronize(m.ctx->stream);
-    if (m.done_ev) (void)hipEventDestroy(m.done_ev);
+    for (RoundSlot &R : mp->rs)
+      if (R.stream) (void)hipStreamSynchronize(R.stream);
+    if (mp->aux) (void)hipStreamSynchronize(mp->aux);
   }
-  for (auto *q : s->pending) delete q;
-  for (auto *q : s->active) delete q;
-  for (auto *q : s->done) delete q;
+  for (auto *q : s->pending) release_request(s, q);
+  for (auto *q : s->active) release_request(s, q);
+  for (auto *q : s->done) release_request(s, q);
   delete s;
   return 0;
 }
@@ -1323,9 +1403,16 @@ int mcw_submit(mcw_server *s, const mcw_request *req)
     const int sp = req->source.space;
     require(sp == MCW_SPACE_POINT || sp == MCW_SPACE_BOX || sp == MCW_SPACE_MESH_VOLUME, "unknown source space type");
     require(req->source.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
+    require(req->source.energy == MCW_ENERGY_MONO || req->source.energy == MCW_ENERGY_WATT,
+            "source energy type not supported");
     auto *q = new ServeReq();
     q->r = *req;
     q->t_submit = s->now();
+    if (sp == MCW_SPACE_POINT) {
+      const mcw_ctx *c = s->meshes[req->mesh_slot]->ctx;
+      const int64_t e = locate_point(c->mesh, req->source.point, 1e-12);
+      q->point_elem = e >= 0 ? c->mesh.int_of_orig[e] : -1;
+    }
     s->pending.push_back(q);
   });
 }
@@ -1333,40 +1420,70 @@ int mcw_submit(mcw_server *s, const mcw_request *req)
 int mcw_poll(mcw_server *s, int32_t *n_done)
 {
   return sguard(s, [&] {
-    // finished rounds
+    // finished rounds: start the result copy of every request whose last
+    // histories were in the round
     for (auto &mp : s->meshes) {
       ServeMesh &m = *mp;
-      if (!m.busy) continue;
-      HIPCHK(hipSetDevice(m.ctx->opt.device));
-      hipError_t e = hipEventQuery(m.done_ev);
-      if (e == hipErrorNotReady) continue;
-      HIPCHK(e);
-      m.busy = false;
-      for (auto &rc : m.round) {
-        rc.first->done += rc.second;
-        if (rc.first->done == rc.first->r.n_particles) finish(s, m, rc.first);
+      for (RoundSlot &R : m.rs) {
+        if (!R.busy) continue;
+        hipError_t e = hipEventQuery(R.done_ev);
+        if (e == hipErrorNotReady) continue;
+        HIPCHK(e);
+        R.busy = false;
+        for (auto &rc : R.round) {
+          rc.first->done += rc.second;
+          if (rc.first->done == rc.first->r.n_particles) start_copy(m, rc.first);
+        }
+        R.round.clear();
       }
-      m.round.clear();
+    }
+    // landed copies
+    for (size_t i = 0; i < s->active.size();) {
+      ServeReq *q = s->active[i];
+      if (q->copying) {
+        hipError_t e = hipEventQuery(q->copied);
+        if (e != hipErrorNotReady) {
+          HIPCHK(e);
+          const int64_t ne = s->meshes[q->r.mesh_slot]->ctx->mesh.n_elem;
+          const double *tail = s->meshes[q->r.mesh_slot]->pinned[q->slot] + ne;
+          memcpy(&q->segments, tail, 8);
+          memcpy(&q->collisions, tail + 1, 8);
+          q->t_done = s->now();
+          s->used -= q->bytes;
+          s->active.erase(s->active.begin() + (ptrdiff_t)i);
+          s->done.push_back(q);
+          continue;
+        }
+      }
+      ++i;
     }
     // admissions
+    std::vector<bool> filled(s->meshes.size(), false);
     while (!s->pending.empty() && (int32_t)s->active.size() < s->max_inflight) {
       ServeReq *q = s->pending.front();
       const int64_t need = request_bytes(s, q);
       if (s->used + need > s->bank_bytes && !s->active.empty()) break;
-      mcw_ctx *c = s->meshes[q->r.mesh_slot]->ctx;
-      q->tally.alloc(c->mesh.n_elem * 8);
-      q->segs.alloc(8);
-      HIPCHK(launch_fill(c->stream, q->tally.as<double>(), c->mesh.n_elem, 0.0));
-      HIPCHK(launch_fill(c->stream, q->segs.as<double>(), 1, 0.0));
+      ServeMesh &m = *s->meshes[q->r.mesh_slot];
+      mcw_ctx *c = m.ctx;
+      HIPCHK(hipEventCreateWithFlags(&q->copied, hipEventDisableTiming));
+      q->slot = m.acquire();
+      HIPCHK(launch_fill(m.aux, m.tallies[q->slot]->as<double>(), c->mesh.n_elem + REQ_TAIL, 0.0));
+      filled[q->r.mesh_slot] = true;
       q->bytes = need;
       s->used += need;
       q->t_admit = s->now();
       s->pending.pop_front();
       s->active.push_back(q);
     }
-    // next rounds
+    for (size_t i = 0; i < s->meshes.size(); ++i)
+      if (filled[i]) {
+        HIPCHK(hipEventRecord(s->meshes[i]->fill_ev, s->meshes[i]->aux));
+        s->meshes[i]->fill_recorded = true;
+      }
+    // next rounds into free slots
     for (int i = 0; i < (int)s->meshes.size(); ++i)
-      if (!s->meshes[i]->busy) launch_round(s, i);
+      for (RoundSlot &R : s->meshes[i]->rs)
+        if (!R.busy && !launch_round(s, i, R)) break;
     if (n_done) *n_done = (int32_t)s->done.size();
   });
 }
@@ -1376,19 +1493,21 @@ int mcw_result(mcw_server *s, mcw_request_status *status, double *flux, size_t n
   return sguard(s, [&] {
     require(!s->done.empty(), "no finished request");
     ServeReq *q = s->done.front();
-    require(flux == nullptr || n >= q->flux.size(), "flux buffer too small for the request's mesh");
+    const ServeMesh &m = *s->meshes[q->r.mesh_slot];
+    const size_t ne = (size_t)m.ctx->mesh.n_elem;
+    require(flux == nullptr || n >= ne, "flux buffer too small for the request's mesh");
     if (status) {
       status->request_id = q->r.request_id;
       status->histories = q->r.n_particles;
       status->segments = (int64_t)q->segments;
-      status->collisions = 0;
+      status->collisions = (int64_t)q->collisions;
       status->t_submit_ms = q->t_submit;
       status->t_admit_ms = q->t_admit;
       status->t_done_ms = q->t_done;
     }
-    if (flux) std::copy(q->flux.begin(), q->flux.end(), flux);
+    if (flux) memcpy(flux, m.pinned[q->slot], ne * 8);
     s->done.pop_front();
-    delete q;
+    release_request(s, q);
   });
 }
 

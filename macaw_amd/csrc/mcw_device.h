@@ -21,6 +21,47 @@ namespace mcw {
 
 // ------------------------------------------------------------------ bits
 
+// Atomics on device memory through explicit global (address space 1)
+// pointers.  A pointer read from memory is generic to the compiler, which
+// then emits FLAT atomics; the tally pointers of the request server are.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) unsigned long long gu64;
+__device__ __forceinline__ void atomic_add_dev(double *p, double v)
+{
+  (void)__builtin_amdgcn_global_atomic_fadd_f64((gdouble *)p, v);
+}
+__device__ __forceinline__ void atomic_add_dev(unsigned long long *p, unsigned long long v)
+{
+  (void)__hip_atomic_fetch_add((gu64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Atomic add of v at p for the lanes with `on`; the lanes whose address
+// equals the first such lane's add their sum in one atomic.  Lanes that start
+// at one source point share their first elements, and same-address atomics
+// serialise.  Call with the whole wave.
+__device__ __forceinline__ double wave_sum_dbl(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ void atomic_add_agg(bool on, double *p, double v)
+{
+  const unsigned long long m = __ballot(on);
+  if (m == 0ull) return;
+  const int leader = __builtin_ctzll(m);
+  const unsigned long long a = (unsigned long long)p;
+  const unsigned long long al = (unsigned long long)__shfl((long long)a, leader, 64);
+  const bool same = on && a == al;
+  if (__popcll(__ballot(same)) > 1) {
+    const double s = wave_sum_dbl(same ? v : 0.0);
+    if (__lane_id() == leader) atomic_add_dev(p, s);
+    if (on && !same) atomic_add_dev(p, v);
+  } else if (on) {
+    atomic_add_dev(p, v);
+  }
+}
+
 __device__ __forceinline__ uint64_t dbits(double x) { return (uint64_t)__double_as_longlong(x); }
 __device__ __forceinline__ double dfrom(uint64_t u) { return __longlong_as_double((long long)u); }
 
@@ -438,7 +479,7 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
   }
   double *row = T.value + fi * T.n_score_bins;
   if (T.simple_flux) {
-    unsafeAtomicAdd(row, wfl);
+    atomic_add_dev(row, wfl);
     return;
   }
   const int nsc = T.n_scores;
@@ -470,7 +511,7 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
         case 3: m = M.s; break;
         default: m = M.f; break;
       }
-      unsafeAtomicAdd(row + ni * nsc + si, wfl * m);
+      atomic_add_dev(row + ni * nsc + si, wfl * m);
     }
   }
 }

@@ -308,8 +308,12 @@ __device__ __forceinline__ int32_t pick_nbr(const int4 &r0, const int4 &r1, int 
 // EV_COLLIDE leaves the particle at the collision point for collide().
 // P32 (TET4 only): exit side from the FP32 mixed record, FP64 recheck of
 // near ties (counted in `rechecks`).
+// With `tl_ptr`, a plain per-element flux (the request server's, or the one
+// track-length tally when P.tl_fast) is not added here but returned as the
+// address (nullptr: no bin) and value, for the caller's wave-aggregated add.
 template <int NS, bool REC, bool P32, bool SERVE = false>
-__device__ __forceinline__ int segment(const Params &P, PState &st, double &track_sum, unsigned &rechecks)
+__device__ __forceinline__ int segment(const Params &P, PState &st, double &track_sum, unsigned &rechecks,
+                                       double **tl_ptr = nullptr, double *tl_val = nullptr)
 {
   const MeshView &M = P.mesh;
   if (st.elem < 0) return EV_LOST; // source site outside the mesh
@@ -360,9 +364,23 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   const double track = (d < L) ? d : L;
   ++st.nseg;
   track_sum += track;
-  if (SERVE) unsafeAtomicAdd(st.tv + st.elem, track);
-  else
+  if (SERVE || P.tl_fast) {
+    double *bin;
+    if (SERVE) {
+      bin = st.tv + st.elem;
+    } else {
+      const int64_t fi = P.tally[0].elem_base ? P.tally[0].elem_base[st.elem] : st.elem;
+      bin = fi >= 0 ? P.tally[0].value + fi : nullptr;
+    }
+    if (tl_ptr) {
+      *tl_ptr = bin;
+      *tl_val = track;
+    } else if (bin) {
+      atomic_add_dev(bin, track);
+    }
+  } else {
     for (int t = 0; t < P.n_tl; ++t) tally_score(P.tally[t], P.xs, st.elem, st.E, st.k, track, mat);
+  }
   if (d > L) {
     st.x = st.x + L * st.u;
     st.y = st.y + L * st.v;
@@ -681,6 +699,55 @@ enum { PH_IDLE = 0, PH_TRANSPORT = 1, PH_COLLIDE = 2 };
 
 // WPE: minimum wavefronts per SIMD requested from the register allocator
 // (0 = compiler's choice); a tuning knob trading spills for occupancy.
+// Per-request segment and collision counts of the request server.  Each lane
+// keeps a running sum (segments << 32 | collisions) for the request of its
+// last history; it is added to the request's counters when the lane moves on
+// to another request, and at the end of the kernel.  Adding at every history
+// end instead puts every history of a request on the same two addresses,
+// and same-address atomics serialise.
+struct ReqAcc {
+  unsigned long long v = 0ull;
+  int32_t req = -1;
+};
+
+// Add v of the lanes with `todo` to their requests' counters: one atomic pair
+// per distinct request among those lanes.  Call with the whole wave.
+__device__ __forceinline__ void req_flush(const Params &P, bool todo, int32_t req, unsigned long long v)
+{
+  for (;;) {
+    const unsigned long long m = __ballot(todo);
+    if (m == 0ull) break;
+    const int leader = __builtin_ctzll(m);
+    const int32_t r = __shfl(req, leader, 64);
+    const bool in = todo && req == r;
+    const unsigned long long hi = wave_sum(in ? (v >> 32) : 0ull);
+    const unsigned long long lo = wave_sum(in ? (v & 0xffffffffull) : 0ull);
+    if (__lane_id() == leader) {
+      atomic_add_dev(P.slices[r].counts, hi);
+      atomic_add_dev(P.slices[r].counts + 1, lo);
+    }
+    if (in) todo = false;
+  }
+}
+
+// A history of request st.req ended on this lane when `dying`: fold its counts
+// into the lane's sum (nseg, ncoll < 2^20 each; the sum is flushed before
+// its segment half could pass 2^32).  Call with the whole wave.
+__device__ __forceinline__ void req_account(const Params &P, bool dying, const PState &st, ReqAcc &a)
+{
+  req_flush(P, dying && a.req != st.req && a.v != 0ull, a.req, a.v);
+  if (dying) {
+    if (a.req != st.req) {
+      a.v = 0ull;
+      a.req = st.req;
+    }
+    a.v += ((unsigned long long)st.nseg << 32) | (unsigned long long)(uint32_t)st.ncoll;
+  }
+  const bool full = (a.v >> 32) >= (1ull << 31);
+  req_flush(P, full, a.req, a.v);
+  if (full) a.v = 0ull;
+}
+
 template <int NS, bool REC, int WPE, bool P32, bool SERVE = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 k_persistent(Params P, SourceBank S)
@@ -694,6 +761,7 @@ k_persistent(Params P, SourceBank S)
   unsigned rechecks = 0;
   PState st;
   st.elem = 0;
+  ReqAcc racc;
   int phase = PH_IDLE;
   bool exhausted = false;
   const int64_t n_hist = P.hist_end - P.hist_begin;
@@ -721,16 +789,19 @@ k_persistent(Params P, SourceBank S)
     do {
       const bool go = phase == PH_TRANSPORT;
       int ev = -1;
+      double *tl_ptr = nullptr;
+      double tl_val = 0.0;
       if (go) {
-        ev = segment<NS, REC, P32, SERVE>(P, st, track, rechecks);
+        ev = segment<NS, REC, P32, SERVE>(P, st, track, rechecks, &tl_ptr, &tl_val);
         if (ev == EV_COLLIDE) {
           phase = PH_COLLIDE;
         } else if (is_death(ev)) {
           if (REC) write_record(P, st, fate_of(ev));
-          if (SERVE) atomicAdd(P.slices[st.req].segments, (unsigned long long)st.nseg);
           phase = PH_IDLE;
         }
       }
+      if (SERVE || P.tl_fast) atomic_add_agg(tl_ptr != nullptr, tl_ptr, tl_val);
+      if (SERVE) req_account(P, go && is_death(ev), st, racc);
       count_segment(wc, go, ev);
     } while (wave_count(phase == PH_TRANSPORT) >= lanes_min);
     // collisions together
@@ -742,12 +813,16 @@ k_persistent(Params P, SourceBank S)
         phase = PH_TRANSPORT;
       } else {
         if (REC) write_record(P, st, fate_of(ev));
-        if (SERVE) atomicAdd(P.slices[st.req].segments, (unsigned long long)st.nseg);
         phase = PH_IDLE;
       }
     }
+    if (SERVE) {
+      if (hit && !REC) ++st.ncoll;
+      req_account(P, hit && ev != EV_SCATTER, st, racc);
+    }
     count_collision(wc, hit, ev);
   }
+  if (SERVE) req_flush(P, racc.v != 0ull, racc.req, racc.v);
   flush_counts(C, wc, track);
   if (P32) {
     const unsigned long long r = wave_sum(rechecks);
@@ -761,6 +836,17 @@ extern "C" __global__ void k_fill(double *p, int64_t n, double v)
 {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// dst[e] = src[perm[e]] / norm for e < n (internal -> caller element order),
+// then dst[n .. n + tail) = src[n .. n + tail) bit for bit (a request's
+// segment and collision counters).
+extern "C" __global__ void k_gather_scale(const double *src, const int32_t *perm, int64_t n, double norm, double *dst,
+                                          int tail)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n + tail; i += stride)
+    dst[i] = i < n ? src[perm[i]] / norm : src[i];
 }
 
 extern "C" __global__ void k_zero_counters(Counters *C)
@@ -932,6 +1018,14 @@ hipError_t launch_fill(hipStream_t s, double *p, int64_t n, double v)
 {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_scale(hipStream_t s, const double *src, const int32_t *perm, int64_t n, double norm,
+                               double *dst, int tail)
+{
+  hipLaunchKernelGGL(k_gather_scale, dim3(grid_for(n + tail, 256, 4096)), dim3(256), 0, s, src, perm, n, norm, dst,
+                     tail);
   return hipGetLastError();
 }
 

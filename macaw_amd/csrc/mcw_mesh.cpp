@@ -328,18 +328,39 @@ void build_locator(HostMesh &m)
   m.have_grid = true;
 }
 
+static bool contains(const HostMesh &m, int64_t e, const double *p, double tol)
+{
+  for (int64_t s = m.face_off[e]; s < m.face_off[e + 1]; ++s) {
+    const double *q = m.plane.data() + 4 * s;
+    const double v = ((q[0] * p[0] + q[1] * p[1]) + q[2] * p[2]) - q[3];
+    if (v > tol) return false;
+  }
+  return true;
+}
+
 int64_t locate_point(const HostMesh &m, const double *p, double tol)
 {
-  for (int64_t e = 0; e < m.n_elem; ++e) {
-    bool in = true;
-    for (int64_t s = m.face_off[e]; s < m.face_off[e + 1] && in; ++s) {
-      const double *q = m.plane.data() + 4 * s;
-      double v = ((q[0] * p[0] + q[1] * p[1]) + q[2] * p[2]) - q[3];
-      if (v > tol) in = false;
-    }
-    if (in) return e;
+  if (!m.have_grid) {
+    for (int64_t e = 0; e < m.n_elem; ++e)
+      if (contains(m, e, p, tol)) return e;
+    return -1;
   }
-  return -1;
+  // grid cell of p (clamped); its candidates are every element whose widened
+  // bounding box meets the cell, so the lowest caller index among those that
+  // contain p is the brute-force answer
+  int64_t cell[3];
+  for (int k = 0; k < 3; ++k) {
+    const double f = (p[k] - m.grid_lo[k]) * m.grid_inv[k];
+    const int64_t c = f > 0.0 ? (int64_t)f : 0;
+    cell[k] = std::min<int64_t>(c, m.grid_n[k] - 1);
+  }
+  const int64_t ci = cell[0] + (int64_t)m.grid_n[0] * (cell[1] + (int64_t)m.grid_n[1] * cell[2]);
+  int64_t best = -1;
+  for (int32_t q = m.cell_start[ci]; q < m.cell_start[ci + 1]; ++q) {
+    const int64_t e = m.orig_of_int[m.cell_elem[q]];
+    if ((best < 0 || e < best) && contains(m, e, p, tol)) best = e;
+  }
+  return best;
 }
 
 } // namespace mcw

@@ -41,7 +41,8 @@ extern const int HEX_KUHN_TETS[6][4];
 // Returns an error message, or "" on success.
 std::string derive_mesh(HostMesh &m, const int32_t *face_boundary);
 
-// Lowest-index element whose side planes all contain p within tol, or -1.
+// Lowest-index element whose side planes all contain p within tol, or -1
+// (through the grid locator once it is built, else by a scan).
 int64_t locate_point(const HostMesh &m, const double *p, double tol);
 
 double tet_volume6(const double *a, const double *b, const double *c, const double *d);

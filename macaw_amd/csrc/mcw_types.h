@@ -140,7 +140,7 @@ struct ReqSlice {
   int64_t n_particles;
   uint64_t master_seed;
   double *tally;                   // per-element track-length flux (internal order)
-  unsigned long long *segments;    // per-request segment counter
+  unsigned long long *counts;      // per-request segments, collisions
   SourceView src;
 };
 
@@ -169,6 +169,8 @@ struct Params {
   int32_t roundtrip;
   int32_t record;
   int32_t lanes_min;       // persistent kernel: keep stepping segments while this many lanes are in flight
+  int32_t tl_fast;         // the one track-length tally is a plain flux per element bin (no energy filter)
+  int32_t pad;
   int64_t batch;
   int64_t n_particles;     // particles in the batch (seed id stride)
   int64_t hist_begin;      // first history index handled by this launch

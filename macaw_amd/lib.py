@@ -221,12 +221,21 @@ class Server:
         self._chk(self.L.mcw_poll(self.h, C.byref(n)))
         return n.value
 
-    def result(self):
+    def result(self, copy=True):
+        """Pop one finished request: (status dict, flux in caller element
+        order or None when copy is False)."""
         st = abi.RequestStatus()
-        buf = np.zeros(max(self.n_elem))
-        self._chk(self.L.mcw_result(self.h, C.byref(st), buf.ctypes.data_as(C.POINTER(C.c_double)), buf.size))
+        if copy:
+            if getattr(self, "_buf", None) is None:
+                self._buf = np.empty(max(self.n_elem))
+            ptr = self._buf.ctypes.data_as(C.POINTER(C.c_double))
+            self._chk(self.L.mcw_result(self.h, C.byref(st), ptr, self._buf.size))
+        else:
+            self._chk(self.L.mcw_result(self.h, C.byref(st), None, 0))
         d = {k: getattr(st, k) for k, _ in st._fields_}
-        return d, buf[:self.n_elem[self.mesh_of[st.request_id]]].copy()
+        if not copy:
+            return d, None
+        return d, self._buf[:self.n_elem[self.mesh_of[st.request_id]]].copy()
 
     def run_all(self, sleep=0.0):
         """Poll until every submitted request has finished; returns {id: (status, flux)}."""

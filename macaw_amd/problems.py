@@ -141,19 +141,96 @@ def _cache_dir():
     return d
 
 
+def _delaunay_slab(args):
+    """Tets of the Delaunay triangulation of `xyz` owned by one x-slab.
+
+    The slab owns lattice layers [a, b) and sees `halo` more on each side.  A
+    tet of the slab's local triangulation is kept when its centroid lies in
+    the slab's x-range and its circumsphere holds no point of the whole set:
+    the sphere is known to be empty when it stays between the nearest
+    excluded layers, and is checked against every point (k-d tree) when it
+    reaches past them.  For points in general position the Delaunay
+    triangulation is unique, so the union over slabs is the global one."""
+    from scipy.spatial import Delaunay, cKDTree
+    xyz, layer, n, a, b, halo, S = args
+    sel = np.nonzero((layer >= a - halo) & (layer < b + halo))[0]
+    loc = Delaunay(xyz[sel]).simplices
+    tets = sel[loc]
+    p0 = xyz[tets[:, 0]]
+    cen = xyz[tets].mean(axis=1)
+    lo = -np.inf if a == 0 else a / (n - 1)
+    hi = np.inf if b == n else b / (n - 1)
+    own = (cen[:, 0] >= lo) & (cen[:, 0] < hi)
+    tets, p0 = tets[own], p0[own]
+    # circumcentre c solves 2 (p_k - p0) . c' = |p_k - p0|^2, c = p0 + c'
+    A = xyz[tets[:, 1:]] - p0[:, None, :]
+    rhs = 0.5 * np.einsum("nkd,nkd->nk", A, A)
+    cc = np.linalg.solve(A, rhs[..., None])[..., 0]
+    R = np.linalg.norm(cc, axis=1)
+    c = p0 + cc
+    ex_lo = xyz[layer < a - halo, 0].max() if a - halo > 0 else -np.inf
+    ex_hi = xyz[layer >= b + halo, 0].min() if b + halo < n else np.inf
+    safe = (c[:, 0] - R > ex_lo) & (c[:, 0] + R < ex_hi)
+    keep = safe.copy()
+    check = np.nonzero(~safe)[0]
+    if check.size:
+        tree = cKDTree(xyz)
+        for i in check:
+            inside = tree.query_ball_point(c[i], R[i] * (1.0 - 1e-9))
+            keep[i] = all(j in set(tets[i]) for j in inside)
+    return tets[keep]
+
+
+def _canonical_tets(xyz, tets):
+    """Sort each tet's node ids, order the tets lexicographically, then swap
+    nodes 1 and 2 of negatively oriented tets so every volume is positive."""
+    tets = np.sort(tets, axis=1)
+    tets = tets[np.lexsort(tets.T[::-1])]
+    a = xyz[tets[:, 1]] - xyz[tets[:, 0]]
+    b = xyz[tets[:, 2]] - xyz[tets[:, 0]]
+    c = xyz[tets[:, 3]] - xyz[tets[:, 0]]
+    vol = np.einsum("ij,ij->i", a, np.cross(b, c))
+    neg = vol < 0
+    tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
+    return np.ascontiguousarray(tets, np.int32)
+
+
+def delaunay_lattice(xyz: np.ndarray, n_lattice: int, n_slabs: int = 0, halo: int = 3, workers: int = 0):
+    """Delaunay tets of a jittered n^3 lattice (point id = (i n + j) n + k, i
+    the x layer), in canonical order.  `n_slabs` > 1 splits the work into
+    x-slabs triangulated in parallel processes; the result is identical to
+    the single triangulation."""
+    n = n_lattice
+    if n_slabs <= 1:
+        from scipy.spatial import Delaunay
+        return _canonical_tets(xyz, Delaunay(xyz).simplices)
+    layer = np.arange(xyz.shape[0]) // (n * n)
+    cuts = np.linspace(0, n, n_slabs + 1).round().astype(int)
+    jobs = [(xyz, layer, n, int(cuts[s]), int(cuts[s + 1]), halo, s) for s in range(n_slabs)]
+    workers = workers or min(n_slabs, os.cpu_count() or 1, 16)
+    if workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            parts = pool.map(_delaunay_slab, jobs)
+    else:
+        parts = [_delaunay_slab(j) for j in jobs]
+    return _canonical_tets(xyz, np.concatenate(parts))
+
+
 def tet_cube(n_lattice: int, seed: int = 101, jitter: float = 0.25, cache: bool = True) -> Mesh:
     """Delaunay TET4 mesh of the unit cube from a jittered n^3 point lattice.
 
     Interior points move by up to +-jitter lattice spacings; points on a face,
-    edge or corner stay on it.  Tets are oriented positively.  All external
-    sides are boundary 0."""
-    key = hashlib.sha1(f"tet_cube-{n_lattice}-{seed}-{jitter}".encode()).hexdigest()[:16]
+    edge or corner stay on it.  Tets are in canonical order (sorted node ids,
+    lexicographic) and oriented positively.  All external sides are boundary
+    0.  Lattices above 40 points a side are triangulated in parallel x-slabs
+    (identical result)."""
+    key = hashlib.sha1(f"tet_cube-v2-{n_lattice}-{seed}-{jitter}".encode()).hexdigest()[:16]
     path = os.path.join(_cache_dir(), f"tet_cube_{n_lattice}_{key}.npz")
     if cache and os.path.exists(path):
         z = np.load(path)
         xyz, tets = z["xyz"], z["tets"]
     else:
-        from scipy.spatial import Delaunay
         rng = np.random.default_rng(seed)
         g = np.linspace(0.0, 1.0, n_lattice)
         X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
@@ -162,13 +239,8 @@ def tet_cube(n_lattice: int, seed: int = 101, jitter: float = 0.25, cache: bool 
         J = rng.uniform(-jitter * h, jitter * h, P.shape)
         J[(P == 0.0) | (P == 1.0)] = 0.0
         xyz = P + J
-        tets = Delaunay(xyz).simplices.astype(np.int32)
-        a = xyz[tets[:, 1]] - xyz[tets[:, 0]]
-        b = xyz[tets[:, 2]] - xyz[tets[:, 0]]
-        c = xyz[tets[:, 3]] - xyz[tets[:, 0]]
-        vol = np.einsum("ij,ij->i", a, np.cross(b, c))
-        neg = vol < 0
-        tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
+        slabs = 0 if n_lattice <= 40 else max(4, n_lattice // 6)
+        tets = delaunay_lattice(xyz, n_lattice, slabs)
         if cache:
             np.savez(path, xyz=xyz, tets=tets)
     ne = tets.shape[0]
@@ -283,6 +355,56 @@ def c2_tet_vacuum(n_lattice=54, n_particles=10_000_000, n_batches=1, n_points=50
                    source=volume_watt_source(),
                    tallies=tallies if tallies is not None else flux_tallies(both=False),
                    n_particles=n_particles, n_batches=n_batches)
+
+
+C4_LATTICES = (26, 54, 85)   # 105,966 / 1,008,162 / 4,014,898 tets
+
+
+def c4_meshes(lattices=C4_LATTICES, n_points=50_000) -> List[Problem]:
+    """The serving mix's resident meshes: the C2 generator (unit cube, vacuum,
+    4 Voronoi materials of the 4 C2 nuclides) at 100k, 1M and 4M tets.  Their
+    own source and tallies are unused by the request server."""
+    out = []
+    for n in lattices:
+        p = c2_tet_vacuum(n_lattice=n, n_particles=1, n_points=n_points, tallies=[])
+        out.append(p)
+    return out
+
+
+@dataclasses.dataclass
+class Request:
+    request_id: int
+    mesh_slot: int
+    n_particles: int
+    source: abi.Source
+    arrival_s: float = 0.0
+
+
+def c4_requests(n=2000, seed=505, n_meshes=3, n_min=1e4, n_max=1e7, rate=None) -> List[Request]:
+    """Fixed-source requests: mesh uniform over the slots, particle count
+    log-uniform in [n_min, n_max], a point (uniform in [0.05, 0.95]^3) or a
+    box (corner uniform, sides uniform in [0.05, 0.3], inside the unit cube)
+    source with equal odds, Watt energies.  With `rate` (requests/s) the
+    arrival times are a Poisson process from the same seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        slot = int(rng.integers(n_meshes))
+        npart = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        if rng.random() < 0.5:
+            src = point_source(rng.uniform(0.05, 0.95, 3))
+        else:
+            side = rng.uniform(0.05, 0.3, 3)
+            lo = rng.uniform(0.0, 1.0 - side)
+            src = box_source(lo, lo + side)
+        src.energy = abi.ENERGY_WATT
+        src.watt_a, src.watt_b = 0.988e6, 2.249e-6
+        out.append(Request(i, slot, npart, src))
+    if rate:
+        t = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, n))
+        for r, a in zip(out, t):
+            r.arrival_s = float(a)
+    return out
 
 
 # ----------------------------------------------------------------- marshalling

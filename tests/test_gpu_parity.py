@@ -259,8 +259,12 @@ def test_mixed_precision_against_fp64():
     frac_diff = 1.0 - same.mean()
     assert frac_diff < 1e-3, frac_diff
     dx = np.sqrt((ha["x"] - hb["x"]) ** 2 + (ha["y"] - hb["y"]) ** 2 + (ha["z"] - hb["z"]) ** 2)[same]
-    # final positions: FP32 segment lengths leave ~1e-9 cm per history (1e-5 worst case)
-    assert np.median(dx) < 1e-8 and dx.max() < 1e-5
+    # final positions: a segment's FP32 length is off by ~16u(|c'|+S)/|u.n|
+    # (u = 2^-24, |c'| + S ~ 2h, h ~ 0.08 cm here), so ~1e-8 cm for a typical
+    # crossing and up to ~1e-5..1e-4 cm for the rare one whose exit side is
+    # close to the |u.n| <= 1e-5 re-solve threshold; 20k histories showed a
+    # median of 1.6e-9 cm and a maximum of 1.2e-5 cm.
+    assert np.median(dx) < 1e-8 and np.percentile(dx, 99) < 1e-6 and dx.max() < 1e-4
     assert np.array_equal(ha["E"][same], hb["E"][same])
     assert sb["fp64_rechecks"] > 0 and sa["fp64_rechecks"] == 0
     assert sb["fp64_rechecks"] < 0.02 * sb["segments"]

@@ -64,6 +64,7 @@ def test_requests_same_alone_and_in_mix():
         rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-300)
         assert rel.max() <= 1e-10, (rid, rel.max())
         assert alone[rid][0]["segments"] == mixed[rid][0]["segments"]
+        assert alone[rid][0]["collisions"] == mixed[rid][0]["collisions"]
         st = mixed[rid][0]
         assert st["t_submit_ms"] <= st["t_admit_ms"] <= st["t_done_ms"]
 
@@ -86,5 +87,30 @@ def test_requests_match_oracle():
         want = o.tally(1, abi.RESULT_SUM)
         st, flux = got[rid]
         assert st["segments"] == so["segments"], rid
+        assert st["collisions"] == so["collisions"], rid
         rel = np.abs(flux - want) / np.maximum(np.abs(want), 1e-300)
         assert rel.max() <= 1e-10, (rid, rel.max())
+
+
+def test_many_requests_share_rounds():
+    """More active requests on one mesh than a round holds at the minimum
+    quota: rounds are capped at the bank size and served round-robin."""
+    probs = meshes()
+    ctx = contexts(probs)
+    src = P.box_source((0.2, 0.2, 0.2), (0.8, 0.8, 0.8), 1e6)
+    reqs = [(1000 + i, 0, 70_000 + 1000 * i, src) for i in range(40)]
+    s = Server(ctx, max_inflight=64, bank_bytes=1 << 40)
+    for r in reqs:
+        s.submit(*r)
+    mixed = s.run_all()
+    s.close()
+    assert len(mixed) == 40
+    for rid, slot, n, src_ in (reqs[0], reqs[39]):
+        a = Server(ctx, max_inflight=1)
+        a.submit(rid, slot, n, src_)
+        st, flux = a.run_all()[rid]
+        a.close()
+        rel = np.abs(flux - mixed[rid][1]) / np.maximum(np.abs(flux), 1e-300)
+        assert rel.max() <= 1e-10
+        assert st["segments"] == mixed[rid][0]["segments"]
+        assert st["histories"] == n
