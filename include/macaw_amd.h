@@ -94,12 +94,17 @@ typedef struct {
   int64_t bank_capacity;    /* particles resident per chunk, 0 = default */
   int64_t max_segments;     /* per-history guard; beyond it the history is lost; 0 = default */
   uint64_t master_seed;     /* OpenMC settings::seed, default 1 */
-  int32_t sort_cadence;     /* BATCHED: refills between element sorts of the bank, 0 = never */
+  int32_t sort_cadence;     /* order particles by (material, element): BATCHED, > 0 sorts each
+                               chunk's source sites before transport; PLAIN, event cycles
+                               between sorts of the live bank (rounded up to a multiple of
+                               8); 0 = never */
   int32_t lanes_min;        /* BATCHED: a wavefront keeps stepping segments while at least this
                                many of its 64 lanes are in flight (1..64, 0 = default) */
   int32_t waves_per_eu;     /* BATCHED: 5 = kernel variant built for 5 wavefronts per SIMD
                                (fewer registers, some spills), 0 = compiler's choice */
-  int32_t reserved[5];
+  int32_t use_graph;        /* PLAIN: 1 = replay the event cycle (8 segment + collision
+                               cycles) as one captured HIP graph */
+  int32_t reserved[4];
 } mcw_options;
 
 /* One synthetic continuous-energy nuclide: pointwise lin-lin tables on an
@@ -183,7 +188,7 @@ int mcw_create(mcw_ctx **ctx, const mcw_options *opts);
 int mcw_destroy(mcw_ctx *ctx);
 /* Change the schedule and tuning fields of a live context (mode, precision,
  * block_size, bank_capacity, lanes_min, waves_per_eu, sort_cadence,
- * record_histories, max_segments) without re-uploading the problem; device,
+ * use_graph, record_histories, max_segments) without re-uploading the problem; device,
  * master_seed and seed_roundtrip cannot change.  Used by the autotuner. */
 int mcw_options_set(mcw_ctx *ctx, const mcw_options *opts);
 /* Replaces openmc_err_msg. */

@@ -36,7 +36,7 @@ class Options(C.Structure):
                 ("block_size", C.c_int32), ("bank_capacity", C.c_int64),
                 ("max_segments", C.c_int64), ("master_seed", C.c_uint64),
                 ("sort_cadence", C.c_int32), ("lanes_min", C.c_int32),
-                ("waves_per_eu", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("waves_per_eu", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Nuclide(C.Structure):

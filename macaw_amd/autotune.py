@@ -4,9 +4,12 @@ Times candidate schedules on the caller's own problem, on the device, and
 returns the fastest.  Knobs: workgroup size, `lanes_min` (how many of a
 wavefront's 64 lanes must still be stepping segments before the wavefront
 pauses to run its collisions and refills), the register/occupancy variant of
-the kernel (`waves_per_eu`), and the bank chunk (histories resident per
-launch).  Every candidate produces bit-identical histories, so tuning never
-changes results, only speed.
+the kernel (`waves_per_eu`), then, with those fixed, the bank size
+(histories resident per launch) and the sort cadence (source sites ordered by
+material and element, or not).  The search is staged: a full grid over the
+first group, then a grid over the second with the first group's best held.
+Every candidate produces bit-identical histories, so tuning never changes
+results, only speed.
 
     best, table = autotune(ctx, n_particles)
     ctx.set_options(**best)
@@ -21,20 +24,17 @@ DEFAULT_SPACE = {
     "lanes_min": [24, 32, 40, 48, 56],
     "waves_per_eu": [0, 5],
 }
+BANK_SPACE = {
+    "bank_capacity": [1 << 22, 1 << 23, 1 << 24],
+    "sort_cadence": [0, 1],
+}
 
 
-def autotune(ctx, n_particles, space=None, repeats=2, batch0=1000, log=None):
-    """Time every combination in `space` (dict of option -> values) on
-    `n_particles` histories per batch; returns (best option dict, rows)."""
-    from . import abi
-    space = dict(DEFAULT_SPACE if space is None else space)
+def _grid(ctx, n_particles, space, fixed, repeats, b, log):
     keys = sorted(space)
     rows = []
-    saved = {k: getattr(ctx.opts, k) for k in keys + ["mode"]}
-    ctx.set_options(mode=abi.MODE_BATCHED)
-    b = batch0
     for combo in itertools.product(*(space[k] for k in keys)):
-        opts = dict(zip(keys, combo))
+        opts = dict(fixed, **dict(zip(keys, combo)))
         ctx.set_options(**opts)
         ctx.run_batch(b, n_particles)  # warm (first launch of a variant)
         b += 1
@@ -47,9 +47,27 @@ def autotune(ctx, n_particles, space=None, repeats=2, batch0=1000, log=None):
         rows.append(dict(opts, seconds=min(times), kernel_ms=st["kernel_ms"]))
         if log:
             log(f"autotune {opts}: {min(times) * 1e3:.2f} ms")
-    best = min(rows, key=lambda r: r["seconds"])
+    return rows, b
+
+
+def autotune(ctx, n_particles, space=None, bank_space=None, repeats=2, batch0=1000, log=None):
+    """Time every combination of `space` (dict of option -> values), then of
+    `bank_space` with the first stage's best held, on `n_particles`
+    histories per batch; returns (best option dict, rows)."""
+    from . import abi
+    space = dict(DEFAULT_SPACE if space is None else space)
+    bank_space = dict(BANK_SPACE if bank_space is None else bank_space)
+    keys = sorted(space) + sorted(bank_space)
+    saved = {k: getattr(ctx.opts, k) for k in keys + ["mode"]}
+    ctx.set_options(mode=abi.MODE_BATCHED)
+    rows, b = _grid(ctx, n_particles, space, {}, repeats, batch0, log)
+    best = {k: min(rows, key=lambda r: r["seconds"])[k] for k in sorted(space)}
+    if bank_space:
+        rows2, b = _grid(ctx, n_particles, bank_space, best, repeats, b, log)
+        rows += rows2
+        best = {k: min(rows2, key=lambda r: r["seconds"])[k] for k in keys}
     ctx.set_options(**saved)
-    return {k: best[k] for k in keys}, rows
+    return best, rows
 
 
 def cached(path, key, fn):

@@ -46,6 +46,13 @@ hipError_t launch_xs_probe(hipStream_t s, const XSView &X, int32_t nuc, const do
                            double *xs);
 hipError_t launch_source_req(hipStream_t s, const Params &P, const SourceBank &S, int32_t n_slices, int64_t n);
 const void *serve_kernel(int ns, bool p32);
+hipError_t sort_temp_bytes(int64_t n, size_t *bytes);
+hipError_t launch_sort_sites(hipStream_t s, const MeshView &M, const SourceBank &in, const SourceBank &out,
+                             int64_t n, int shift, int bits, uint32_t mat_mask, uint32_t *k0, uint32_t *k1,
+                             uint32_t *v0, uint32_t *v1, void *tmp, size_t tmp_bytes);
+hipError_t launch_sort_bank(hipStream_t s, const MeshView &M, const BankView &in, const BankView &out, int64_t n,
+                            bool rec, Counters *C, int qout, int shift, int bits, uint32_t mat_mask, uint32_t *k0,
+                            uint32_t *k1, uint32_t *v0, uint32_t *v1, void *tmp, size_t tmp_bytes);
 hipError_t launch_serve(int ns, bool p32, unsigned blocks, hipStream_t s, const Params &P, const SourceBank &S);
 } // namespace mcw
 
@@ -152,6 +159,14 @@ struct mcw_ctx {
   // source bank, live/collision banks of the event-based path, records
   DevBuf sb[9];
   int64_t sb_cap = 0;
+  // sorting: (key, slot) pairs, radix-sort scratch, the sorted source bank
+  DevBuf sort_k[2], sort_v[2], sort_tmp;
+  int64_t sort_cap = 0;
+  size_t sort_tmp_bytes = 0;
+  DevBuf sb2[10];
+  int64_t sb2_cap = 0;
+  int sort_shift = 0, sort_bits = 32;
+  uint32_t sort_mat_mask = 0;
   DevBuf bank[3][17];
   int64_t bank_cap = 0;
   DevBuf r_digest, r_seed, r_E, r_x, r_y, r_z, r_nseg, r_ncoll, r_elem, r_fate;
@@ -615,6 +630,58 @@ SourceBank source_bank(mcw_ctx *c, int64_t n)
   return S;
 }
 
+// Key layout of the (material, element) sort: element id in the low bits,
+// material index above it when both fit in 32 bits (else the element alone).
+void sort_layout(mcw_ctx *c)
+{
+  int eb = 1;
+  while (eb < 31 && ((int64_t)1 << eb) <= c->mesh.n_elem) ++eb;
+  int mb = 0;
+  while (((int64_t)1 << mb) < (int64_t)c->mats.size()) ++mb;
+  if (eb + mb <= 32) {
+    c->sort_shift = eb;
+    c->sort_bits = eb + mb;
+    c->sort_mat_mask = mb > 0 ? (uint32_t)(((uint64_t)1 << mb) - 1) : 0u;
+  } else {
+    c->sort_shift = 0;
+    c->sort_bits = eb;
+    c->sort_mat_mask = 0u;
+  }
+}
+
+void ensure_sort(mcw_ctx *c, int64_t n)
+{
+  require(n < ((int64_t)1 << 31), "sorted banks hold fewer than 2^31 particles");
+  sort_layout(c);
+  if (c->sort_cap >= n) return;
+  for (int k = 0; k < 2; ++k) {
+    c->sort_k[k].alloc((size_t)n * 4);
+    c->sort_v[k].alloc((size_t)n * 4);
+  }
+  HIPCHK(sort_temp_bytes(n, &c->sort_tmp_bytes));
+  c->sort_tmp.alloc(c->sort_tmp_bytes);
+  c->sort_cap = n;
+}
+
+// Source sites of a chunk sorted by (material, element) into the second bank.
+SourceBank sort_sites(mcw_ctx *c, const Params &P, const SourceBank &S, int64_t n)
+{
+  ensure_sort(c, n);
+  if (c->sb2_cap < n) {
+    for (int k = 0; k < 10; ++k) c->sb2[k].alloc((size_t)n * (k < 8 ? 8 : 4));
+    c->sb2_cap = n;
+  }
+  SourceBank O;
+  O.x = c->sb2[0].as<double>(); O.y = c->sb2[1].as<double>(); O.z = c->sb2[2].as<double>();
+  O.u = c->sb2[3].as<double>(); O.v = c->sb2[4].as<double>(); O.w = c->sb2[5].as<double>();
+  O.E = c->sb2[6].as<double>(); O.seed = c->sb2[7].as<uint64_t>(); O.elem = c->sb2[8].as<int32_t>();
+  O.slot = c->sb2[9].as<int32_t>();
+  HIPCHK(launch_sort_sites(c->stream, P.mesh, S, O, n, c->sort_shift, c->sort_bits, c->sort_mat_mask,
+                           c->sort_k[0].as<uint32_t>(), c->sort_k[1].as<uint32_t>(), c->sort_v[0].as<uint32_t>(),
+                           c->sort_v[1].as<uint32_t>(), c->sort_tmp.p, c->sort_tmp_bytes));
+  return O;
+}
+
 // Resident workgroups of the persistent kernel over the whole device.
 int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, bool p32, int block)
 {
@@ -626,8 +693,12 @@ int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, bool p32, int block
 
 // Event-based transport of histories hist_begin .. hist_end (sites already
 // in the source bank): segment and collision kernels alternate until no
-// particle is alive.  The live count is read back every 8 cycles to size
-// the grids; kernels read the exact count from device memory.
+// particle is alive.  Cycles go in groups of 8, after which the live count is
+// read back (kernels read the exact count from device memory and loop over
+// it, so a grid only needs to be large enough to fill the device).  With
+// use_graph, a group is one replay of a captured graph (fixed grids); with
+// sort_cadence, the live bank is sorted by (material, element) between
+// groups.
 void run_plain(mcw_ctx *c, Params &P, const SourceBank &S, int64_t n)
 {
   hipStream_t s = c->stream;
@@ -637,21 +708,62 @@ void run_plain(mcw_ctx *c, Params &P, const SourceBank &S, int64_t n)
   BankView B[3] = {bank_view(c, 0), bank_view(c, 1), bank_view(c, 2)};
   HIPCHK(launch_bank_init(s, P, S, B[0], n, 0));
   const int64_t max_blocks = (int64_t)c->n_cu * 8;
+  const bool graph = c->opt.use_graph != 0;
+  const int64_t cadence = c->opt.sort_cadence > 0 ? ((int64_t)c->opt.sort_cadence + 7) / 8 * 8 : 0;
+  if (cadence) ensure_sort(c, n);
+  hipGraphExec_t exec[2] = {nullptr, nullptr}; // 8 cycles starting from live bank 0 / 1
+  struct ExecGuard {
+    hipGraphExec_t *e;
+    ~ExecGuard()
+    {
+      for (int k = 0; k < 2; ++k)
+        if (e[k]) (void)hipGraphExecDestroy(e[k]);
+    }
+  } exec_guard{exec};
+  auto group = [&](int first, int64_t blocks) {
+    int cur = first;
+    for (int k = 0; k < 8; ++k) {
+      const int nxt = 1 - cur;
+      HIPCHK(launch_reset_queue(s, P.cnt, nxt, 2));
+      HIPCHK(launch_segment(ns, (unsigned)blocks, s, P, B[cur], cur, B[nxt], nxt, B[2], 2));
+      HIPCHK(launch_collide((unsigned)blocks, s, P, B[2], 2, B[nxt], nxt));
+      cur = nxt;
+    }
+  };
   int64_t bound = n;
   int cur = 0;
-  for (int64_t it = 0;; ++it) {
-    const int nxt = 1 - cur;
-    int64_t blocks = std::min<int64_t>((bound + block - 1) / block, max_blocks);
-    if (blocks < 1) blocks = 1;
-    HIPCHK(launch_reset_queue(s, P.cnt, nxt, 2));
-    HIPCHK(launch_segment(ns, (unsigned)blocks, s, P, B[cur], cur, B[nxt], nxt, B[2], 2));
-    HIPCHK(launch_collide((unsigned)blocks, s, P, B[2], 2, B[nxt], nxt));
-    cur = nxt;
-    if ((it & 7) == 7) {
-      HIPCHK(hipMemcpyAsync(&c->h_counters->q_count[cur], &P.cnt->q_count[cur], 8, hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      bound = (int64_t)c->h_counters->q_count[cur];
-      if (bound == 0) break;
+  for (int64_t it = 8;; it += 8) {
+    if (graph) {
+      if (!exec[cur]) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          group(cur, max_blocks);
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        HIPCHK(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&exec[cur], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(e);
+      }
+      HIPCHK(hipGraphLaunch(exec[cur], s));
+    } else {
+      group(cur, std::max<int64_t>(1, std::min<int64_t>((bound + block - 1) / block, max_blocks)));
+    }
+    HIPCHK(hipMemcpyAsync(&c->h_counters->q_count[cur], &P.cnt->q_count[cur], 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    bound = (int64_t)c->h_counters->q_count[cur];
+    if (bound == 0) break;
+    if (cadence && it % cadence == 0) {
+      const int nxt = 1 - cur;
+      HIPCHK(launch_sort_bank(s, P.mesh, B[cur], B[nxt], bound, P.record != 0, P.cnt, nxt, c->sort_shift,
+                              c->sort_bits, c->sort_mat_mask, c->sort_k[0].as<uint32_t>(),
+                              c->sort_k[1].as<uint32_t>(), c->sort_v[0].as<uint32_t>(), c->sort_v[1].as<uint32_t>(),
+                              c->sort_tmp.p, c->sort_tmp_bytes));
+      cur = nxt;
     }
   }
 }
@@ -682,6 +794,8 @@ int mcw_create(mcw_ctx **out, const mcw_options *opts)
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     require(c->opt.device >= 0 && c->opt.device < ndev, "no such HIP device " + std::to_string(c->opt.device));
+    require(c->opt.sort_cadence >= 0, "sort_cadence must not be negative");
+    require(c->opt.use_graph == 0 || c->opt.use_graph == 1, "use_graph must be 0 or 1");
     HIPCHK(hipSetDevice(c->opt.device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
@@ -721,6 +835,8 @@ int mcw_options_set(mcw_ctx *c, const mcw_options *o)
     require(o->lanes_min >= 0 && o->lanes_min <= 64, "lanes_min must be in 0..64");
     require(o->block_size == 0 || (o->block_size % 64 == 0 && o->block_size <= 256),
             "block_size must be 64, 128, 192 or 256");
+    require(o->sort_cadence >= 0, "sort_cadence must not be negative");
+    require(o->use_graph == 0 || o->use_graph == 1, "use_graph must be 0 or 1");
     const bool reprep = o->max_segments != c->opt.max_segments || o->lanes_min != c->opt.lanes_min;
     c->opt = *o;
     if (c->opt.master_seed == 0) c->opt.master_seed = 1;
@@ -928,11 +1044,12 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       const int64_t h1 = std::min(n_particles, h0 + cap);
       P.hist_begin = h0;
       P.hist_end = h1;
-      const SourceBank S = source_bank(c, h1 - h0);
+      SourceBank S = source_bank(c, h1 - h0);
       HIPCHK(launch_source_bank(s, P, S, h1 - h0));
       if (c->opt.mode == MCW_MODE_PLAIN) {
         run_plain(c, P, S, h1 - h0);
       } else {
+        if (c->opt.sort_cadence > 0) S = sort_sites(c, P, S, h1 - h0);
         const int block = c->opt.block_size > 0 ? c->opt.block_size : 256;
         const int wpe = c->opt.waves_per_eu == 5 ? 5 : 0;
         const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns == 4;

@@ -773,7 +773,7 @@ k_persistent(Params P, SourceBank S)
     if (need) {
       if (slot < n_hist) {
         load_site(S, slot, st);
-        start_history(st, P.hist_begin + slot);
+        start_history(st, P.hist_begin + (S.slot ? (int64_t)S.slot[slot] : slot));
         if (SERVE) {
           st.req = S.req[slot];
           st.tv = P.slices[st.req].tally;

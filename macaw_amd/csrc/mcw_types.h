@@ -122,10 +122,11 @@ struct BankView {
 
 // Source sites sampled ahead of transport (one per history of a chunk).
 struct SourceBank {
-  double *x, *y, *z, *u, *v, *w, *E;
-  uint64_t *seed;    // tracking-stream seed
-  int32_t *elem;
-  int32_t *req;      // request-server rounds: slice index of each site, else nullptr
+  double *x = nullptr, *y = nullptr, *z = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *E = nullptr;
+  uint64_t *seed = nullptr; // tracking-stream seed
+  int32_t *elem = nullptr;
+  int32_t *req = nullptr;   // request-server rounds: slice index of each site, else nullptr
+  int32_t *slot = nullptr;  // sorted banks: the site's slot before sorting, else nullptr
 };
 
 // One request's slice of a request-server round: histories hist0 ..
