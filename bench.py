@@ -1,19 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark: histories/s and segments/s of the particle-tracking hot path.
 
-Workload (one "step" = one batch): unit-cube TET4 mesh from a Delaunay
-triangulation of a jittered 54^3 lattice (1,008,162 tets), vacuum
-boundaries, 4 Voronoi materials of 4 synthetic nuclides (50k-point grids),
-uniform volumetric Watt source, 1e7 histories per batch, FP64, a per-element
-track-length flux tally.  Inputs are synthetic and resident in HBM before
-the timed region.
+Workloads (one "step" = one batch of fixed-source histories):
+  c2 (default) unit-cube TET4 mesh from a Delaunay triangulation of a
+     jittered 54^3 lattice (1,008,162 tets), vacuum boundaries, 4 Voronoi
+     materials of 4 synthetic nuclides (50k-point grids), uniform volumetric
+     Watt source, 1e7 histories per batch, per-element track-length flux;
+  c3 the same generator at 8M tets (107^3 lattice), 16 materials of 4 of 32
+     nuclides on the unionized grid, 1e8 histories per batch in bank chunks.
+Inputs are synthetic and resident in HBM before the timed region.
 
-Reported beside the batched (persistent, fused) path: the plain event-based
-GPU path on the same batch, the dominant kernel's roofline (algorithmic bytes
-per launch over its HIP-event duration) and a CPU baseline (the FP64 oracle,
-std::thread on the host cores, on a bounded sample of the same histories).
+`value` is the batched (persistent, fused) FP64 path, whose histories are
+bit-identical to the CPU oracle's.  Reported beside it: the mixed-precision
+form (FP32 exit distances with FP64 re-solve of near ties) with its tally
+difference from the FP64 run, the plain event-based GPU path on the same
+batch, the dominant kernel's roofline (algorithmic bytes per launch over its
+HIP-event duration), the serving mix (c2 runs only: 2000 requests over
+resident 100k / 1M / 4M-tet meshes, Poisson arrivals at 70% of measured
+capacity, p50/p99 latency) and a CPU baseline (the FP64 oracle, std::thread
+on the host cores, on a bounded sample of the same histories).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--histories N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3]
 """
 import argparse
 import json
@@ -123,14 +130,73 @@ def bench_serving(args, tuning):
     return out
 
 
+WORKLOADS = {
+    # name: (generator, default histories per step, description)
+    "c2": ("c2_tet_vacuum", 10_000_000, "1M TET4 unit cube, vacuum, 4 materials / 4 nuclides x 50k points"),
+    "c3": ("c3_tet_vacuum", 100_000_000, "8M TET4 unit cube, vacuum, 16 materials / 32 nuclides x 50k points "
+                                          "(unionized grid), chunked bank"),
+}
+
+
+def tally_error(fp64_ctx, mixed_ctx, n_batches):
+    """Per-element flux of the mixed-precision run against the FP64 run over
+    the same batches: relative differences and differences in units of the
+    FP64 estimate's standard error (from SUM and SUM_SQ over the batches)."""
+    import numpy as np
+    from macaw_amd import abi
+    f, m = fp64_ctx.tally(1, abi.RESULT_SUM), mixed_ctx.tally(1, abi.RESULT_SUM)
+    f2 = fp64_ctx.tally(1, abi.RESULT_SUM_SQ)
+    nz = f > 0
+    rel = np.abs(m[nz] - f[nz]) / f[nz]
+    mean = f / n_batches
+    var = np.maximum(f2 / n_batches - mean * mean, 0.0) / max(n_batches - 1, 1)
+    sig = np.sqrt(var)
+    ok = sig > 0
+    z = np.abs(m[ok] - f[ok]) / n_batches / sig[ok]
+    return {"total_rel": float(abs(m.sum() - f.sum()) / f.sum()), "elem_rel_max": float(rel.max()),
+            "elem_rel_median": float(np.median(rel)), "elem_rel_p99": float(np.percentile(rel, 99)),
+            "max_sigma": float(z.max()), "frac_beyond_3sigma": float((z > 3).mean()), "batches": n_batches}
+
+
+def timed_steps(ctx, N, steps, warmup, label):
+    for w in range(warmup):
+        st = ctx.run_batch(100 + w, N)
+        log(f"{label} warmup {w}: {st['time_ms']:.1f} ms, {st['segments']} segments")
+    out = []
+    t = time.perf_counter()
+    for k in range(steps):
+        out.append(ctx.run_batch(k + 1, N))
+    wall = time.perf_counter() - t
+    for k, st in enumerate(out):
+        log(f"{label} step {k}: {st['time_ms']:.1f} ms device, {st['segments']} segments, "
+            f"{st['collisions']} collisions, leaks {st['leaks']}, lost {st['lost']}, rechecks {st['fp64_rechecks']}")
+    return out, wall
+
+
+def roofline_of(steps, N, n_nuc, b_seg):
+    segs = sum(s["segments"] for s in steps)
+    colls = sum(s["collisions"] for s in steps)
+    kern_ms = sum(s["kernel_ms"] for s in steps)
+    launches = sum(s["kernel_launches"] for s in steps)
+    b_alg = algorithmic_bytes({"histories": N * len(steps), "segments": segs, "collisions": colls}, n_nuc, b_seg)
+    per_launch_bytes = b_alg / launches
+    per_launch_s = kern_ms / launches / 1e3
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_launch": per_launch_bytes,
+            "launch_ms": per_launch_s * 1e3, "launches_per_step": launches / len(steps)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--histories", type=int, default=10_000_000)
-    ap.add_argument("--lattice", type=int, default=54)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2")
+    ap.add_argument("--histories", type=int, default=0, help="histories per step (default: the workload's)")
+    ap.add_argument("--lattice", type=int, default=0, help="Delaunay lattice points per side (default: the workload's)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--lanes-min", type=int, default=0)
@@ -163,65 +229,71 @@ def main():
         print(json.dumps({"calibration": "k_math_probe", "n": n, "read_bytes": 16 * n, "write_bytes": 48 * n}))
         return
 
+    gen_name, n_default, desc = WORKLOADS[args.workload]
+    N = args.histories or n_default
     t0 = time.perf_counter()
-    log(f"generating workload: Delaunay lattice {args.lattice}^3")
-    prob = P.c2_tet_vacuum(n_lattice=args.lattice, n_particles=args.histories)
+    gen_kw = {"n_particles": N}
+    if args.lattice:
+        gen_kw["n_lattice"] = args.lattice
+    prob = getattr(P, gen_name)(**gen_kw)
     n_elem = prob.mesh.n_elem
-    workload = f"c2_tet4_{n_elem // 1000}k_vacuum"
-    log(f"mesh {n_elem} tets, {prob.mesh.n_nodes} nodes ({time.perf_counter() - t0:.1f}s)")
+    n_mat, n_nuc = len(prob.materials), len(prob.nuclides)
+    nuc_per_mat = max(len(m.nuclides) for m in prob.materials)
+    workload = f"{args.workload}_tet4_{n_elem // 1000}k_vacuum"
+    log(f"{workload}: {n_elem} tets, {prob.mesh.n_nodes} nodes, {n_mat} materials, {n_nuc} nuclides "
+        f"({time.perf_counter() - t0:.1f}s)")
 
     ctx = Context(prob, mode=abi.MODE_BATCHED, block_size=args.block, lanes_min=args.lanes_min)
-    N = args.histories
     tuning = {"block_size": ctx.opts.block_size, "lanes_min": ctx.opts.lanes_min,
               "waves_per_eu": ctx.opts.waves_per_eu}
     if not args.no_autotune:
         from macaw_amd.autotune import autotune
         t = time.perf_counter()
-        tuning, rows = autotune(ctx, N, log=log)
+        tuning, rows = autotune(ctx, min(N, 10_000_000), log=log)
         ctx.set_options(**tuning)
         log(f"autotune: best {tuning} ({time.perf_counter() - t:.1f}s)")
-    for w in range(args.warmup):
-        st = ctx.run_batch(100 + w, N)
-        log(f"warmup {w}: {st['time_ms']:.1f} ms, {st['segments']} segments")
-    steps = []
-    t = time.perf_counter()
-    for k in range(args.steps):
-        steps.append(ctx.run_batch(k + 1, N))
-    wall = time.perf_counter() - t
-    for k, st in enumerate(steps):
-        log(f"step {k}: {st['time_ms']:.1f} ms device, {st['segments']} segments, {st['collisions']} collisions, "
-            f"leaks {st['leaks']}, lost {st['lost']}")
+    ctx.tally_reset()  # tallies then hold exactly the warmup + timed batches
+    steps, wall = timed_steps(ctx, N, args.steps, args.warmup, "fp64")
     segs = sum(s["segments"] for s in steps)
     colls = sum(s["collisions"] for s in steps)
-    kern_ms = sum(s["kernel_ms"] for s in steps)
-    launches = sum(s["kernel_launches"] for s in steps)
     hist_s = N * args.steps / wall
     seg_s = segs / wall
+    roof = roofline_of(steps, N, nuc_per_mat, 276.0)
+    roof.update({"traffic": read_traffic(workload), "kernel": "k_persistent<4>"})
 
-    # dominant kernel: k_persistent, one launch per batch
-    agg = {"histories": N * args.steps, "segments": segs, "collisions": colls}
-    b_alg = algorithmic_bytes(agg, 4)
-    per_launch_bytes = b_alg / launches
-    per_launch_s = kern_ms / launches / 1e3
-    achieved = per_launch_bytes / per_launch_s / 1e9
-    traffic = read_traffic(workload)
+    mixed = None
+    if not args.no_mixed:
+        mctx = Context(prob, mode=abi.MODE_BATCHED, precision=abi.PREC_MIXED)
+        mctx.set_options(**tuning)
+        msteps, mwall = timed_steps(mctx, N, args.steps, args.warmup, "mixed")
+        mroof = roofline_of(msteps, N, nuc_per_mat, 212.0)
+        mroof["kernel"] = "k_persistent<4, mixed>"
+        mixed = {"histories_per_s": N * args.steps / mwall,
+                 "segments_per_s": sum(s["segments"] for s in msteps) / mwall,
+                 "ms_per_step": mwall / args.steps * 1e3,
+                 "fp64_rechecks_per_segment": sum(s["fp64_rechecks"] for s in msteps)
+                 / max(1, sum(s["segments"] for s in msteps)),
+                 "roofline": mroof,
+                 "tally_error_vs_fp64": tally_error(ctx, mctx, args.steps + args.warmup)}
+        log(f"mixed: {mixed['histories_per_s']:.3e} histories/s; tally vs fp64 {mixed['tally_error_vs_fp64']}")
+        mctx.close()
 
     plain = None
     if not args.no_plain:
         pctx = Context(prob, mode=abi.MODE_PLAIN)
-        pctx.run_batch(200, N)  # warm
+        pN = min(N, 10_000_000)
+        pctx.run_batch(200, pN)  # warm
         t = time.perf_counter()
-        pst = pctx.run_batch(1, N)
+        pst = pctx.run_batch(1, pN)
         pw = time.perf_counter() - t
-        plain = {"histories_per_s": N / pw, "segments_per_s": pst["segments"] / pw, "ms_per_step": pw * 1e3}
-        if pst["segments"] != steps[0]["segments"]:
-            log("WARNING: plain and batched segment counts differ")
-        log(f"plain: {pw * 1e3:.1f} ms per batch")
+        plain = {"histories_per_s": pN / pw, "segments_per_s": pst["segments"] / pw, "ms_per_step": pw * 1e3,
+                 "histories": pN}
+        log(f"plain: {pw * 1e3:.1f} ms for {pN} histories")
         pctx.close()
     ctx.close()
 
     serving = None
-    if not args.no_serve:
+    if not args.no_serve and args.workload == "c2":
         serving = bench_serving(args, tuning)
 
     cpu = None
@@ -242,19 +314,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": workload, "n_elem": n_elem, "element": "TET4", "histories_per_step": N,
-                   "materials": 4, "nuclides": 4, "grid_points": 50000, "source": "volume Watt",
-                   "boundary": "vacuum", "tally": "per-element track-length flux", "mode": "batched",
-                   "precision": "fp64", "parallelism": "1 GPU", "tuning": tuning},
+        "config": {"workload": workload, "description": desc, "n_elem": n_elem, "element": "TET4",
+                   "histories_per_step": N, "materials": n_mat, "nuclides": n_nuc, "grid_points": 50000,
+                   "source": "volume Watt", "boundary": "vacuum", "tally": "per-element track-length flux",
+                   "mode": "batched", "precision": "fp64", "parallelism": "1 GPU", "tuning": tuning},
         "segments_per_s": seg_s,
         "segments_per_history": segs / (N * args.steps),
         "collisions_per_history": colls / (N * args.steps),
         "plain_gpu": plain,
         "speedup_vs_plain": (hist_s / plain["histories_per_s"]) if plain else None,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_persistent<4>", "alg_bytes_per_launch": per_launch_bytes,
-                     "launch_ms": per_launch_s * 1e3},
+        "mixed_precision": mixed,
+        "mixed_speedup_vs_plain": (mixed["histories_per_s"] / plain["histories_per_s"]) if (plain and mixed) else None,
+        "roofline": roof,
         "serving": serving,
         "cpu_baseline": cpu,
     }

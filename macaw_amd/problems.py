@@ -357,6 +357,14 @@ def c2_tet_vacuum(n_lattice=54, n_particles=10_000_000, n_batches=1, n_points=50
                    n_particles=n_particles, n_batches=n_batches)
 
 
+def c3_tet_vacuum(n_lattice=107, n_particles=100_000_000, n_batches=1) -> Problem:
+    """C2's generator at 8M tets (107^3 lattice -> 8.07M tets) with 16
+    Voronoi materials mixing 4 of 32 synthetic nuclides (50k-point grids,
+    unionized on the device)."""
+    return c2_tet_vacuum(n_lattice=n_lattice, n_particles=n_particles, n_batches=n_batches,
+                         n_materials=16, n_nuclides=32)
+
+
 C4_LATTICES = (26, 54, 85)   # 105,966 / 1,008,162 / 4,014,898 tets
 
 
