@@ -151,7 +151,7 @@ struct mcw_ctx {
   bool dirty = true;
 
   // device problem
-  DevBuf plane, erec, mtet, face_off, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
+  DevBuf plane, erec, mtet, split, face_off, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf cell_start, cell_elem;
   DevBuf grid_off, npts, rec, fission, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
@@ -204,6 +204,18 @@ template <class F> int guard(mcw_ctx *c, F &&f)
 void require(bool ok, const std::string &m)
 {
   if (!ok) throw McwError{m};
+}
+
+// Isotropic, or linear in mu about a unit axis with |beta| <= 1.
+void check_angle(const mcw_source &src)
+{
+  if (src.angle == MCW_ANGLE_LINEAR_MU) {
+    const double n2 = (src.axis[0] * src.axis[0] + src.axis[1] * src.axis[1]) + src.axis[2] * src.axis[2];
+    require(std::fabs(n2 - 1.0) <= 1e-12, "source axis must be a unit vector");
+    require(std::fabs(src.beta) <= 1.0, "source beta must be in [-1, 1]");
+  } else {
+    require(src.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
+  }
 }
 
 // Upload everything the kernels read; validate the problem as
@@ -264,7 +276,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
       const double *q = &m.plane[4 * (f0 + k)];
       planes[slot + k] = make_double4(q[0], q[1], q[2], q[3]);
       const int32_t nb = m.nbr[f0 + k];
-      code[k] = nb >= 0 ? m.int_of_orig[nb >> 3] * 8 + (nb & 7) : nb;
+      code[k] = nb < 0 || (nb & 7) == 7 ? nb : m.int_of_orig[nb >> 3] * 8 + (nb & 7);
     }
     slot += ns;
     erec[2 * i] = make_int4(code[0], code[1], code[2], code[3]);
@@ -272,17 +284,30 @@ void prepare(mcw_ctx *c, bool need_source = true)
   }
   c->plane.upload(planes, s);
   c->erec.upload(erec, s);
-  // mixed-precision records of TET4 meshes: FP32 planes relative to vertex 0
-  if (m.uniform_sides == 4) {
+  // split interfaces, with internal codes
+  std::vector<SplitRec> sp(m.splits.size());
+  for (size_t k = 0; k < m.splits.size(); ++k) {
+    const HostMesh::Split &h = m.splits[k];
+    sp[k].pl = make_double4(h.m[0], h.m[1], h.m[2], h.m[3]);
+    for (int q = 0; q < 2; ++q) sp[k].code[q] = m.int_of_orig[h.elem[q]] * 8 + h.side[q];
+    sp[k].pad[0] = sp[k].pad[1] = 0;
+  }
+  c->split.upload(sp, s);
+  // mixed-precision records of TET4 elements: FP32 planes relative to vertex
+  // 0 (a mixed mesh's HEX8 entries only say nsides = 6)
+  if (m.uniform_sides == 4 || m.uniform_sides == 0) {
     std::vector<MixedTet> mt(ne);
     for (int64_t i = 0; i < ne; ++i) {
       const int64_t o = m.orig_of_int[i];
       const double *v0 = &m.xyz[3 * (int64_t)m.conn[m.conn_off[o]]];
       MixedTet &R = mt[i];
       memset(&R, 0, sizeof R);
+      R.nsides = erec[2 * i + 1].w;
+      if (R.nsides != 4) continue;
+      const int64_t s0 = m.uniform_sides == 4 ? 4 * i : fo[i];
       for (int k = 0; k < 3; ++k) R.v0[k] = v0[k];
       for (int sd = 0; sd < 4; ++sd) {
-        const double4 q = planes[4 * i + sd];
+        const double4 q = planes[s0 + sd];
         const double rel = q.w - ((q.x * v0[0] + q.y * v0[1]) + q.z * v0[2]);
         R.pl[sd] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)rel);
       }
@@ -389,7 +414,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
   } else {
     require(c->src.space == MCW_SPACE_MESH_VOLUME, "unknown source space type");
   }
-  require(c->src.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
+  check_angle(c->src);
   require(c->src.energy == MCW_ENERGY_MONO || c->src.energy == MCW_ENERGY_WATT, "source energy type not supported");
 
   c->counters.alloc(sizeof(Counters));
@@ -412,7 +437,8 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P.mesh.type = c->type.as<uint8_t>();
   P.mesh.int_of_orig = c->int_of_orig.as<int32_t>();
   P.mesh.orig_of_int = c->orig_of_int.as<int32_t>();
-  P.mesh.mtet = m.uniform_sides == 4 ? c->mtet.as<MixedTet>() : nullptr;
+  P.mesh.mtet = m.uniform_sides != 6 ? c->mtet.as<MixedTet>() : nullptr;
+  P.mesh.split = m.splits.empty() ? nullptr : c->split.as<SplitRec>();
   for (int k = 0; k < 3; ++k) {
     P.mesh.grid_lo[k] = m.grid_lo[k];
     P.mesh.grid_inv[k] = m.grid_inv[k];
@@ -1052,7 +1078,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
         if (c->opt.sort_cadence > 0) S = sort_sites(c, P, S, h1 - h0);
         const int block = c->opt.block_size > 0 ? c->opt.block_size : 256;
         const int wpe = c->opt.waves_per_eu == 5 ? 5 : 0;
-        const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns == 4;
+        const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns != 6;
         require(block % 64 == 0 && block <= 256, "block_size must be 64, 128, 192 or 256");
         int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, p32, block), (h1 - h0 + block - 1) / block);
         if (blocks < 1) blocks = 1;
@@ -1439,7 +1465,7 @@ bool launch_round(mcw_server *s, int mi, RoundSlot &R)
   P.slices = R.slices.as<ReqSlice>();
   P.cnt = R.counters.as<Counters>();
   const int ns = c->mesh.uniform_sides;
-  const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns == 4;
+  const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns != 6;
   HIPCHK(launch_reset_work(st, P.cnt));
   HIPCHK(launch_source_req(st, P, S, n_sl, total));
   const unsigned blocks = (unsigned)std::min<int64_t>(m.blocks, (total + 255) / 256);
@@ -1488,7 +1514,7 @@ int mcw_server_create(mcw_server **out, mcw_ctx *const *meshes, int32_t n_meshes
       int per_cu = 0;
       const int ns = c->mesh.uniform_sides;
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, serve_kernel(ns, c->opt.precision == MCW_PREC_MIXED && ns == 4), 256, 0));
+          &per_cu, serve_kernel(ns, c->opt.precision == MCW_PREC_MIXED && ns != 6), 256, 0));
       m.blocks = c->n_cu * std::max(1, per_cu);
     }
   });
@@ -1519,7 +1545,7 @@ int mcw_submit(mcw_server *s, const mcw_request *req)
     require(req->n_particles >= 1, "a request needs at least one particle");
     const int sp = req->source.space;
     require(sp == MCW_SPACE_POINT || sp == MCW_SPACE_BOX || sp == MCW_SPACE_MESH_VOLUME, "unknown source space type");
-    require(req->source.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
+    check_angle(req->source);
     require(req->source.energy == MCW_ENERGY_MONO || req->source.energy == MCW_ENERGY_WATT,
             "source energy type not supported");
     auto *q = new ServeReq();

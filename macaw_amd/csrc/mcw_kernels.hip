@@ -251,7 +251,7 @@ __device__ void sample_site(const Params &P, const SourceView &S, uint64_t maste
                           X + 3 * (int64_t)c[c_kuhn[k][2]], X + 3 * (int64_t)c[c_kuhn[k][3]], ss, p);
     }
   }
-  {
+  if (S.angle == 0) {
     double xi_phi = prn(ss);
     double mu = 2.0 * prn(ss) - 1.0;
     double s, c;
@@ -260,6 +260,18 @@ __device__ void sample_site(const Params &P, const SourceView &S, uint64_t maste
     dir[0] = mu;
     dir[1] = r * c;
     dir[2] = r * s;
+  } else {
+    // p(mu) ~ 1 + beta mu about the axis: mu = -2c / (1 + sqrt(1 - 2 beta c)),
+    // c = (1 - beta/2) - 2 xi, the root of F(mu) = xi without cancellation;
+    // azimuth 2 pi xi' about the axis
+    const double cq = (1.0 - 0.5 * S.beta) - 2.0 * prn(ss);
+    double mu = -2.0 * cq / (1.0 + sqrt(1.0 - 2.0 * S.beta * cq));
+    if (mu > 1.0) mu = 1.0;
+    if (mu < -1.0) mu = -1.0;
+    dir[0] = S.axis[0];
+    dir[1] = S.axis[1];
+    dir[2] = S.axis[2];
+    rotate_angle(dir[0], dir[1], dir[2], mu, prn(ss));
   }
   if (S.energy == 0) {
     E = S.E0;
@@ -326,20 +338,27 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   double L;
   int fx;
   int64_t slot0;
+  bool fp32 = false;
   if (P32) {
+    // FP32 record of a TET4 (every element of a TET4 mesh; in a mixed mesh
+    // the record says which elements are TET4, the rest take the FP64 path)
     const MixedTet &R = M.mtet[st.elem];
-    const int4 code = *(const int4 *)R.code;
-    r0 = code;
-    r1 = make_int4(-1, -1, R.mat, 4);
-    mat = R.mat;
-    slot0 = (int64_t)st.elem * 4;
-    bool recheck;
-    fx = exit_side_p32(R, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L, recheck);
-    if (recheck) {
-      fx = exit_side<4>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
-      ++rechecks;
+    if (NS == 4 || R.nsides == 4) {
+      fp32 = true;
+      const int4 code = *(const int4 *)R.code;
+      r0 = code;
+      r1 = make_int4(-1, -1, R.mat, 4);
+      mat = R.mat;
+      slot0 = NS == 4 ? (int64_t)st.elem * 4 : (int64_t)M.face_off[st.elem];
+      bool recheck;
+      fx = exit_side_p32(R, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L, recheck);
+      if (recheck) {
+        fx = exit_side<4>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+        ++rechecks;
+      }
     }
-  } else {
+  }
+  if (!fp32) {
     r0 = M.erec[2 * (int64_t)st.elem];
     r1 = M.erec[2 * (int64_t)st.elem + 1];
     mat = r1.z;
@@ -385,7 +404,13 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
     st.x = st.x + L * st.u;
     st.y = st.y + L * st.v;
     st.z = st.z + L * st.w;
-    const int32_t nb = pick_nbr(r0, r1, fx);
+    int32_t nb = pick_nbr(r0, r1, fx);
+    if (NS == 0 && nb >= 0 && (nb & 7) == 7) {
+      // split interface: the tet on the exit point's side of the diagonal
+      const SplitRec &R = M.split[nb >> 3];
+      const double sg = ((R.pl.x * st.x + R.pl.y * st.y) + R.pl.z * st.z) - R.pl.w;
+      nb = sg <= 0.0 ? R.code[0] : R.code[1];
+    }
     int ev;
     if (nb >= 0) {
       ev = EV_CROSS;
@@ -958,6 +983,7 @@ hipError_t launch_collide(unsigned blocks, hipStream_t s, const Params &P, const
 template <int WPE> const void *persistent_kernel_w(int ns, bool rec, bool p32)
 {
   if (p32 && ns == 4) return rec ? (const void *)k_persistent<4, true, WPE, true> : (const void *)k_persistent<4, false, WPE, true>;
+  if (p32 && ns == 0) return rec ? (const void *)k_persistent<0, true, WPE, true> : (const void *)k_persistent<0, false, WPE, true>;
   if (rec) {
     if (ns == 4) return (const void *)k_persistent<4, true, WPE, false>;
     if (ns == 6) return (const void *)k_persistent<6, true, WPE, false>;
@@ -993,7 +1019,7 @@ const void *serve_kernel(int ns, bool p32)
 {
   if (ns == 4) return p32 ? (const void *)k_persistent<4, false, 0, true, true> : (const void *)k_persistent<4, false, 0, false, true>;
   if (ns == 6) return (const void *)k_persistent<6, false, 0, false, true>;
-  return (const void *)k_persistent<0, false, 0, false, true>;
+  return p32 ? (const void *)k_persistent<0, false, 0, true, true> : (const void *)k_persistent<0, false, 0, false, true>;
 }
 
 hipError_t launch_serve(int ns, bool p32, unsigned blocks, hipStream_t s, const Params &P, const SourceBank &S)

@@ -7,7 +7,9 @@
 //     elements gets one canonical plane (through its three lowest-numbered
 //     nodes) used with opposite signs on the two sides, so a point that
 //     leaves one element through the plane is exactly on the neighbour's;
-//   - face neighbours by matching sorted side-node tuples;
+//   - face neighbours by matching sorted side-node tuples; in mixed meshes a
+//     HEX8 side covered by two TET4 sides (split along a diagonal) becomes a
+//     split interface resolved by the exit point's side of the diagonal;
 //   - boundary ids of external sides;
 //   - the element-volume CDF of volumetric sources (sequential sum).
 // Planar sides are required (warn_non_planar in the reference's inputs,
@@ -16,7 +18,9 @@
 #include "mcw_mesh.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
 #include <numeric>
 #include <thread>
 
@@ -111,8 +115,9 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
     if (a.n[3] != b.n[3]) return a.n[3] < b.n[3];
     return a.slot < b.slot;
   });
-  m.nbr.assign(nf, 0);
+  m.nbr.assign(nf, -1);
   m.boundary_ids.clear();
+  std::vector<int32_t> open; // sides with no side of equal node set
   for (size_t i = 0; i < keys.size();) {
     size_t j = i + 1;
     while (j < keys.size() && keys[j].n[0] == keys[i].n[0] && keys[j].n[1] == keys[i].n[1] &&
@@ -125,16 +130,64 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
       m.nbr[a] = eb * 8 + (int32_t)(b - m.face_off[eb]);
       m.nbr[b] = ea * 8 + (int32_t)(a - m.face_off[ea]);
     } else {
-      int32_t a = keys[i].slot;
-      int32_t bid = face_boundary ? face_boundary[a] : 0;
-      if (bid < 0) return "external side without a boundary id (element " + std::to_string(slot_elem[a]) + ")";
-      m.nbr[a] = -1 - bid;
-      m.boundary_ids.insert(bid);
+      open.push_back(keys[i].slot);
     }
     i = j;
   }
   keys.clear();
   keys.shrink_to_fit();
+
+  // -- split interfaces (mixed meshes): a HEX8 side q0 q1 q2 q3 whose two
+  // halves along the diagonal q0-q2 or q1-q3 are open TET4 sides.  The hex
+  // side's code is split*8 + 7; the two triangles' code is the hex side.
+  m.splits.clear();
+  std::vector<std::array<int32_t, 3>> split_slots; // hex side, triangle, triangle
+  std::vector<std::array<int32_t, 2>> split_diag;  // diagonal nodes, ascending
+  if (m.uniform_sides == 0) {
+    std::map<std::array<int32_t, 3>, int32_t> tri;
+    auto key3 = [](int32_t a, int32_t b, int32_t c) {
+      std::array<int32_t, 3> k = {a, b, c};
+      std::sort(k.begin(), k.end());
+      return k;
+    };
+    for (int32_t a : open)
+      if (m.type[slot_elem[a]] == 4) {
+        int32_t fn[4];
+        int nn;
+        side_nodes(m, slot_elem[a], (int)(a - m.face_off[slot_elem[a]]), fn, nn);
+        tri[key3(fn[0], fn[1], fn[2])] = a;
+      }
+    for (int32_t a : open) {
+      const int32_t e = slot_elem[a];
+      if (m.type[e] != 8) continue;
+      int32_t q[4];
+      int nn;
+      side_nodes(m, e, (int)(a - m.face_off[e]), q, nn);
+      for (int d = 0; d < 2; ++d) {
+        auto t0 = tri.find(key3(q[d], q[d + 1], q[d + 2]));
+        auto t1 = tri.find(key3(q[d + 2], q[(d + 3) & 3], q[d]));
+        if (t0 == tri.end() || t1 == tri.end()) continue;
+        const int32_t s0 = t0->second, s1 = t1->second;
+        tri.erase(t0);
+        tri.erase(tri.find(key3(q[d + 2], q[(d + 3) & 3], q[d])));
+        if (split_slots.size() >= ((size_t)1 << 28)) return "mesh too large: more than 2^28 split sides";
+        m.nbr[a] = (int32_t)split_slots.size() * 8 + 7;
+        split_slots.push_back({a, s0, s1});
+        split_diag.push_back({std::min(q[d], q[d + 2]), std::max(q[d], q[d + 2])});
+        m.nbr[s0] = e * 8 + (int32_t)(a - m.face_off[e]);
+        m.nbr[s1] = m.nbr[s0];
+        break;
+      }
+    }
+  }
+  for (int32_t a : open) {
+    if (m.nbr[a] >= 0) continue; // matched to a split interface
+    int32_t bid = face_boundary ? face_boundary[a] : 0;
+    if (bid < 0) return "external side without a boundary id (element " + std::to_string(slot_elem[a]) + ")";
+    m.nbr[a] = -1 - bid;
+    m.boundary_ids.insert(bid);
+  }
+  std::vector<int32_t>().swap(open);
 
   // -- outward planes
   m.plane.assign(4 * nf, 0.0);
@@ -209,6 +262,45 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
   for (auto &th : pool) th.join();
   for (auto &e : errs)
     if (!e.empty()) return e;
+
+  // -- split planes: through the diagonal (lower node id first), normal to
+  // the hex side's outward normal n: m = (n x d)/|n x d|, c = m.x_a.  The tet
+  // whose off-diagonal vertex has m.v - c <= 0 takes exit points with
+  // m.x - c <= 0, the other the rest.
+  for (size_t k = 0; k < split_slots.size(); ++k) {
+    const double *n = m.plane.data() + 4 * (int64_t)split_slots[k][0];
+    const double *xa = X + 3 * (int64_t)split_diag[k][0];
+    const double *xb = X + 3 * (int64_t)split_diag[k][1];
+    const double dx = xb[0] - xa[0], dy = xb[1] - xa[1], dz = xb[2] - xa[2];
+    double mx = n[1] * dz - n[2] * dy;
+    double my = n[2] * dx - n[0] * dz;
+    double mz = n[0] * dy - n[1] * dx;
+    const double len = std::sqrt((mx * mx + my * my) + mz * mz);
+    if (!(len > 0.0)) return "degenerate split side of element " + std::to_string(slot_elem[split_slots[k][0]]);
+    mx = mx / len;
+    my = my / len;
+    mz = mz / len;
+    HostMesh::Split S;
+    S.m[0] = mx;
+    S.m[1] = my;
+    S.m[2] = mz;
+    S.m[3] = (mx * xa[0] + my * xa[1]) + mz * xa[2];
+    const int32_t t0 = split_slots[k][1], t1 = split_slots[k][2];
+    int32_t fn[4];
+    int nn;
+    side_nodes(m, slot_elem[t0], (int)(t0 - m.face_off[slot_elem[t0]]), fn, nn);
+    int32_t v = fn[0];
+    for (int q = 0; q < 3; ++q)
+      if (fn[q] != split_diag[k][0] && fn[q] != split_diag[k][1]) v = fn[q];
+    const double *xv = X + 3 * (int64_t)v;
+    const double sv = ((mx * xv[0] + my * xv[1]) + mz * xv[2]) - S.m[3];
+    const int32_t lo = sv <= 0.0 ? t0 : t1, hi = sv <= 0.0 ? t1 : t0;
+    S.elem[0] = slot_elem[lo];
+    S.side[0] = (int32_t)(lo - m.face_off[slot_elem[lo]]);
+    S.elem[1] = slot_elem[hi];
+    S.side[1] = (int32_t)(hi - m.face_off[slot_elem[hi]]);
+    m.splits.push_back(S);
+  }
 
   // -- volume CDF, sequential in element order
   m.cdf.assign(ne, 0.0);

@@ -20,7 +20,13 @@ struct HostMesh {
   int uniform_sides = 0;            // 4 all-TET4, 6 all-HEX8, 0 mixed
   std::vector<int64_t> face_off;    // n_elem + 1
   std::vector<double> plane;        // 4 per side slot
-  std::vector<int32_t> nbr;         // neighbour*8 + side, or -1 - boundary id
+  std::vector<int32_t> nbr;         // neighbour*8 + side, split*8 + 7, or -1 - boundary id
+  struct Split {                    // HEX8 side covered by two TET4 sides
+    double m[4];                    // plane through the diagonal: m.x - c
+    int32_t elem[2];                // tet taking m.x - c <= 0, tet taking the rest
+    int32_t side[2];                // their side on the hex side
+  };
+  std::vector<Split> splits;
   std::set<int32_t> boundary_ids;   // ids of external sides
   std::vector<double> cdf;          // element volume CDF (6x volumes)
   // device order: elements sorted along a Morton curve of their centroids

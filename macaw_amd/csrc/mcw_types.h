@@ -3,7 +3,9 @@
 // HBM layout (all arrays device-resident, built once per problem):
 //   mesh   plane[slot]   double4 (nx, ny, nz, c), outward unit normal; a
 //                        TET4 element's 4 planes are one 128-byte record
-//          nbr[slot]     int32: neighbour*8 + neighbour side, or -1 - boundary
+//          nbr[slot]     int32: neighbour*8 + neighbour side, split*8 + 7 (mixed
+//                        meshes: a HEX8 side covered by two TET4 sides), or
+//                        -1 - boundary
 //          elem_mat[e]   int32 material index (-1 void)
 //          face_off[e]   int32 first slot (mixed meshes only)
 //   xs     energy[]      per-nuclide grids, packed (grid_off[n] .. +n_points[n])
@@ -28,7 +30,17 @@ struct __align__(128) MixedTet {
   double v0[3];
   int32_t code[4];   // neighbour codes, as in erec
   int32_t mat;
-  int32_t pad[5];
+  int32_t nsides;    // 4; in mixed meshes a HEX8's entry holds 6 and nothing else
+  int32_t pad[4];
+};
+
+// Split interface of a mixed mesh: a HEX8 side covered by two TET4 sides
+// that cut it along a diagonal.  An exit point x through the hex side enters
+// code[0] when pl.xyz . x - pl.w <= 0, else code[1] (internal id * 8 + side).
+struct SplitRec {
+  double4 pl;
+  int32_t code[2];
+  int32_t pad[2];
 };
 
 // Elements live on the device in Morton order of their centroids
@@ -51,7 +63,8 @@ struct MeshView {
   const uint8_t *type;
   const int32_t *int_of_orig; // caller element -> internal id
   const int32_t *orig_of_int; // internal id -> caller element
-  const MixedTet *mtet;       // mixed-precision records (TET4 meshes, precision MIXED), else nullptr
+  const MixedTet *mtet;       // mixed-precision records (meshes with TET4, precision MIXED), else nullptr
+  const SplitRec *split;      // split interfaces of mixed meshes (neighbour code split * 8 + 7)
   // point location: uniform grid over the mesh bounding box, CSR lists of
   // the (internal) elements whose bounding box meets each cell
   double grid_lo[3], grid_inv[3];

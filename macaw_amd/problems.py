@@ -86,13 +86,15 @@ class Problem:
 
 def synth_nuclide(n_points: int, awr: float, seed: int, sigma_a0: float = 1.0,
                   sigma_p: float = 4.7, n_res: int = 50, e_min: float = 1e-5,
-                  e_max: float = 2e7) -> NuclideData:
+                  e_max: float = 2e7, jitter: float = 0.0) -> NuclideData:
     """A pointwise nuclide: sigma_a = sigma_a0 sqrt(0.0253/E) + SLBW capture,
     sigma_s = sigma_p + SLBW elastic, on a log grid of n_points in [e_min, e_max] eV.
 
     Resonances: E_r log-uniform in [1, 1e5] eV, Gamma log-uniform in
     [1e-2, 10] eV, peak log-uniform in [10, 1e4] b, split Gamma_n/Gamma_gamma
-    by a uniform fraction in [0.2, 0.8]."""
+    by a uniform fraction in [0.2, 0.8].  With jitter > 0 (below 0.5) every
+    interior grid point moves by U(-jitter, jitter) log-steps, so nuclides of
+    different seeds have different grids (a union grid of all their points)."""
     rng = np.random.default_rng(seed)
     E = np.geomspace(e_min, e_max, n_points)
     E[0], E[-1] = e_min, e_max
@@ -100,6 +102,10 @@ def synth_nuclide(n_points: int, awr: float, seed: int, sigma_a0: float = 1.0,
     G = np.exp(rng.uniform(np.log(1e-2), np.log(10.0), n_res))
     peak = np.exp(rng.uniform(np.log(10.0), np.log(1e4), n_res))
     fg = rng.uniform(0.2, 0.8, n_res)
+    if jitter > 0.0:
+        assert jitter < 0.5
+        step = (np.log(e_max) - np.log(e_min)) / (n_points - 1)
+        E[1:-1] = np.exp(np.log(E[1:-1]) + rng.uniform(-jitter, jitter, n_points - 2) * step)
     sa = sigma_a0 * np.sqrt(0.0253 / E)
     ss = np.full_like(E, sigma_p)
     for r in range(n_res):
@@ -250,19 +256,71 @@ def tet_cube(n_lattice: int, seed: int = 101, jitter: float = 0.25, cache: bool 
                 subdomain=np.zeros(ne, np.int32), face_boundary=None)
 
 
+def graded_mixed_box(n: int, m_tet: int, seed: int = 606, gamma=(1.4, 1.8)) -> Mesh:
+    """Unit cube of n^3 tensor-product cells graded toward the corner
+    (0, 0, 0): node i of axis d sits at (i/n)^gamma_d, gamma_d drawn from
+    U(gamma) with `seed`.  Cells with i, j, k < m_tet (the refined corner) are
+    each split into 6 Kuhn TET4 (the paths 000 -> 111 through the three axis
+    orders, conforming between cells); the others are HEX8 in libMesh node
+    order.  Hex and tet regions meet through split interfaces (a hex side
+    covered by two tet sides).  Tets are oriented positively; all external
+    sides are boundary 0."""
+    rng = np.random.default_rng(seed)
+    g = [np.linspace(0.0, 1.0, n + 1) ** rng.uniform(*gamma) for _ in range(3)]
+    X, Y, Z = np.meshgrid(g[0], g[1], g[2], indexing="ij")
+    xyz = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    m1 = n + 1
+    I, J, K = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+
+    def node(di, dj, dk, sel):
+        return ((I[sel] + di) * m1 + (J[sel] + dj)) * m1 + (K[sel] + dk)
+
+    is_tet = (I < m_tet) & (J < m_tet) & (K < m_tet)
+    hx = ~is_tet
+    hexes = np.stack([node(0, 0, 0, hx), node(1, 0, 0, hx), node(1, 1, 0, hx), node(0, 1, 0, hx),
+                      node(0, 0, 1, hx), node(1, 0, 1, hx), node(1, 1, 1, hx), node(0, 1, 1, hx)], 1)
+    tets = []
+    for a, b, c in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        step = np.zeros((4, 3), int)
+        step[1, a] = 1
+        step[2] = step[1]
+        step[2, b] = 1
+        step[3] = 1
+        tets.append(np.stack([node(*step[q], is_tet) for q in range(4)], 1))
+    tets = np.stack(tets, 1).reshape(-1, 4)
+    v = xyz[tets]
+    vol = np.einsum("ij,ij->i", v[:, 1] - v[:, 0], np.cross(v[:, 2] - v[:, 0], v[:, 3] - v[:, 0]))
+    neg = vol < 0
+    tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
+    nh, nt = hexes.shape[0], tets.shape[0]
+    conn = np.concatenate([hexes.ravel(), tets.ravel()]).astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(np.r_[np.full(nh, 8), np.full(nt, 4)])]).astype(np.int64)
+    etype = np.r_[np.full(nh, abi.HEX8, np.uint8), np.full(nt, abi.TET4, np.uint8)]
+    return Mesh(xyz=np.ascontiguousarray(xyz, np.float64), elem_type=etype, conn_offsets=off, conn=conn,
+                subdomain=np.zeros(nh + nt, np.int32), face_boundary=None)
+
+
+def centroids(mesh: Mesh) -> np.ndarray:
+    """Element centroids (mean of the nodes), any mix of TET4 and HEX8."""
+    nn = np.diff(mesh.conn_offsets)
+    cen = np.zeros((mesh.n_elem, 3))
+    for k in np.unique(nn):
+        sel = np.nonzero(nn == k)[0]
+        idx = mesh.conn_offsets[sel][:, None] + np.arange(k)[None, :]
+        cen[sel] = mesh.xyz[mesh.conn[idx]].mean(1)
+    return cen
+
+
 def voronoi_blocks(mesh: Mesh, n_regions: int, seed: int) -> np.ndarray:
     """Subdomain per element: index of the nearest of n_regions seeded centres
-    to the element centroid."""
+    to the element centroid (ties to the lower index)."""
+    from scipy.spatial import cKDTree
     rng = np.random.default_rng(seed)
     lo, hi = mesh.xyz.min(0), mesh.xyz.max(0)
     centres = rng.uniform(lo, hi, (n_regions, 3))
-    nn = np.diff(mesh.conn_offsets)
-    cen = np.zeros((mesh.n_elem, 3))
-    # all elements of one type here
-    k = int(nn[0])
-    cen = mesh.xyz[mesh.conn.reshape(-1, k)].mean(1)
-    d = ((cen[:, None, :] - centres[None, :, :]) ** 2).sum(-1)
-    return d.argmin(1).astype(np.int32)
+    _, idx = cKDTree(centres).query(centroids(mesh))
+    return idx.astype(np.int32)
 
 
 # ----------------------------------------------------------------- sources
@@ -289,6 +347,17 @@ def box_source(lo, hi, E0=2.0e6) -> abi.Source:
     s.box_hi[:] = list(hi)
     s.E0 = E0
     return s
+
+
+def linear_mu(src: abi.Source, axis, beta) -> abi.Source:
+    """Make `src` anisotropic: p(mu) ~ 1 + beta mu, mu the cosine to `axis`
+    (normalised here), azimuth uniform."""
+    a = np.asarray(axis, float)
+    a = a / np.linalg.norm(a)
+    src.angle = abi.ANGLE_LINEAR_MU
+    src.axis[:] = list(a)
+    src.beta = float(beta)
+    return src
 
 
 # ----------------------------------------------------------------- configs
@@ -322,15 +391,15 @@ C2_NUCLIDES = [  # (A, sigma_p [b], sigma_a0 [b])
 ]
 
 
-def c2_nuclides(n_points=50_000, n_nuclides=4):
+def c2_nuclides(n_points=50_000, n_nuclides=4, jitter=0.0):
     out = []
     for i in range(n_nuclides):
         A, sp, sa = C2_NUCLIDES[i % len(C2_NUCLIDES)]
-        out.append(synth_nuclide(n_points, A, seed=1000 + i, sigma_a0=sa, sigma_p=sp))
+        out.append(synth_nuclide(n_points, A, seed=1000 + i, sigma_a0=sa, sigma_p=sp, jitter=jitter))
     return out
 
 
-def c2_materials(n_materials=4, n_nuclides=4, seed=303):
+def c2_materials(n_materials=4, n_nuclides=4, seed=303, dens=(0.01, 0.1)):
     rng = np.random.default_rng(seed + 1)
     mats = []
     for m in range(n_materials):
@@ -338,17 +407,17 @@ def c2_materials(n_materials=4, n_nuclides=4, seed=303):
         if n_nuclides > 4:
             nuc = np.sort(rng.choice(n_nuclides, size=4, replace=False)).astype(np.int32)
         mats.append(MaterialData(id=m + 1, nuclides=nuc,
-                                 densities=rng.uniform(0.01, 0.1, nuc.size)))
+                                 densities=rng.uniform(dens[0], dens[1], nuc.size)))
     return mats
 
 
 def c2_tet_vacuum(n_lattice=54, n_particles=10_000_000, n_batches=1, n_points=50_000,
-                  n_materials=4, n_nuclides=4, tallies=None) -> Problem:
+                  n_materials=4, n_nuclides=4, tallies=None, grid_jitter=0.0) -> Problem:
     """Unit cube, vacuum, Delaunay TET4 (54^3 lattice -> 1.0M tets), Voronoi
     materials from 4 nuclides, uniform volumetric Watt source."""
     mesh = tet_cube(n_lattice, seed=101)
     mesh.subdomain = voronoi_blocks(mesh, n_materials, seed=303)
-    nucs = c2_nuclides(n_points, n_nuclides)
+    nucs = c2_nuclides(n_points, n_nuclides, jitter=grid_jitter)
     mats = c2_materials(n_materials, n_nuclides)
     return Problem(name=f"c2_tet4_vacuum_{mesh.n_elem}", mesh=mesh, nuclides=nucs, materials=mats,
                    block_material={b: b + 1 for b in range(n_materials)}, bcs={0: abi.BC_VACUUM},
@@ -360,9 +429,31 @@ def c2_tet_vacuum(n_lattice=54, n_particles=10_000_000, n_batches=1, n_points=50
 def c3_tet_vacuum(n_lattice=107, n_particles=100_000_000, n_batches=1) -> Problem:
     """C2's generator at 8M tets (107^3 lattice -> 8.07M tets) with 16
     Voronoi materials mixing 4 of 32 synthetic nuclides (50k-point grids,
-    unionized on the device)."""
+    each jittered, so the device's union grid holds 1.6M energies)."""
     return c2_tet_vacuum(n_lattice=n_lattice, n_particles=n_particles, n_batches=n_batches,
-                         n_materials=16, n_nuclides=32)
+                         n_materials=16, n_nuclides=32, grid_jitter=0.4)
+
+
+C5_MESH = (190, 138)   # 19,999,360 elements: 15,768,432 TET4 + 4,230,928 HEX8
+
+
+def c5_mixed(n=C5_MESH[0], m_tet=C5_MESH[1], n_particles=10_000_000, n_batches=1, n_points=100_000,
+             n_nuclides=64, n_materials=128, seed=707) -> Problem:
+    """Stress case: graded mixed TET4/HEX8 unit cube (190^3 cells, the
+    138^3-cell refined corner as Kuhn tets: 20.0M elements), vacuum; 128
+    Voronoi materials mixing 4 of 64 nuclides (100k-point jittered grids: a
+    6.4M-point union) at low densities U(0.001, 0.01) (long tracks); uniform
+    volumetric Watt source with directions linear in mu about a seeded axis
+    (beta seeded in [0.3, 0.9])."""
+    mesh = graded_mixed_box(n, m_tet, seed=606)
+    mesh.subdomain = voronoi_blocks(mesh, n_materials, seed=seed)
+    nucs = c2_nuclides(n_points, n_nuclides, jitter=0.4)
+    mats = c2_materials(n_materials, n_nuclides, seed=seed, dens=(0.001, 0.01))
+    rng = np.random.default_rng(seed + 2)
+    src = linear_mu(volume_watt_source(), rng.normal(size=3), rng.uniform(0.3, 0.9))
+    return Problem(name=f"c5_mixed_{mesh.n_elem}", mesh=mesh, nuclides=nucs, materials=mats,
+                   block_material={b: b + 1 for b in range(n_materials)}, bcs={0: abi.BC_VACUUM},
+                   source=src, tallies=flux_tallies(both=False), n_particles=n_particles, n_batches=n_batches)
 
 
 C4_LATTICES = (26, 54, 85)   # 105,966 / 1,008,162 / 4,014,898 tets

@@ -31,6 +31,7 @@
  * the bit.
  */
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -104,6 +105,15 @@ struct Oracle {
   std::vector<double> plane;       // 4 per face slot
   std::vector<int64_t> nbr;        // neighbour element, or -1 - boundary id
   std::vector<int32_t> nbr_side;   // neighbour's side index
+  // split interfaces: a HEX8 quad side covered by two TET4 triangle sides
+  // that cut it along a diagonal; split_of[slot] indexes `splits` (-1: none)
+  struct Split {
+    double m[4];        // plane through the diagonal, normal to the quad: m.x - c
+    int64_t elem[2];    // tet on the side m.x - c <= 0, tet on the side > 0
+    int32_t side[2];    // their side on the quad
+  };
+  std::vector<int64_t> split_of;
+  std::vector<Split> splits;
   std::vector<double> cdf;         // element volume CDF
   std::map<int, int> bc;           // boundary id -> kind
   // physics
@@ -231,6 +241,7 @@ bool build_mesh(Oracle *o, const int32_t *face_boundary)
   std::vector<int64_t> slot_elem(nf);
   for (int64_t e = 0; e < ne; ++e)
     for (int64_t s = o->face_off[e]; s < o->face_off[e + 1]; ++s) slot_elem[s] = e;
+  std::vector<int64_t> open; // sides with no side of equal node set
   for (size_t i = 0; i < sorted.size();) {
     size_t j = i + 1;
     while (j < sorted.size() && same(sorted[i], sorted[j])) ++j;
@@ -243,12 +254,57 @@ bool build_mesh(Oracle *o, const int32_t *face_boundary)
       o->nbr[b] = ea;
       o->nbr_side[b] = (int32_t)(a - o->face_off[ea]);
     } else {
-      int64_t a = sorted[i].slot;
-      int32_t bid = face_boundary ? face_boundary[a] : 0;
-      if (bid < 0) return fail(o, "external side without a boundary id");
-      o->nbr[a] = -1 - (int64_t)bid;
+      open.push_back(sorted[i].slot);
     }
     i = j;
+  }
+  // split interfaces: a quad side (q0 q1 q2 q3, libMesh order) whose two
+  // halves along diagonal q0-q2 or q1-q3 are open triangle sides of tets
+  o->split_of.assign(nf, -1);
+  o->splits.clear();
+  std::map<std::array<int32_t, 3>, int64_t> tri;
+  auto key3 = [](int32_t a, int32_t b, int32_t c) {
+    std::array<int32_t, 3> k = {a, b, c};
+    std::sort(k.begin(), k.end());
+    return k;
+  };
+  for (int64_t a : open)
+    if (o->type[slot_elem[a]] == MCW_TET4) {
+      int32_t fn[4];
+      int nn;
+      face_nodes(*o, slot_elem[a], (int)(a - o->face_off[slot_elem[a]]), fn, &nn);
+      tri[key3(fn[0], fn[1], fn[2])] = a;
+    }
+  std::vector<std::array<int32_t, 2>> split_diag;
+  std::vector<std::array<int64_t, 3>> split_slots; // quad, triangle 0, triangle 1
+  for (int64_t a : open) {
+    const int64_t e = slot_elem[a];
+    if (o->type[e] != MCW_HEX8) continue;
+    int32_t q[4];
+    int nn;
+    face_nodes(*o, e, (int)(a - o->face_off[e]), q, &nn);
+    for (int d = 0; d < 2; ++d) {
+      auto t0 = tri.find(key3(q[d], q[d + 1], q[d + 2]));
+      auto t1 = tri.find(key3(q[d + 2], q[(d + 3) & 3], q[d]));
+      if (t0 == tri.end() || t1 == tri.end()) continue;
+      const int64_t s0 = t0->second, s1 = t1->second;
+      tri.erase(t0);
+      tri.erase(tri.find(key3(q[d + 2], q[(d + 3) & 3], q[d])));
+      o->split_of[a] = (int64_t)split_slots.size();
+      split_slots.push_back({a, s0, s1});
+      split_diag.push_back({std::min(q[d], q[d + 2]), std::max(q[d], q[d + 2])});
+      for (int64_t ts : {s0, s1}) {
+        o->nbr[ts] = e;
+        o->nbr_side[ts] = (int32_t)(a - o->face_off[e]);
+      }
+      break;
+    }
+  }
+  for (int64_t a : open) {
+    if (o->split_of[a] >= 0 || o->nbr[a] >= 0) continue;
+    int32_t bid = face_boundary ? face_boundary[a] : 0;
+    if (bid < 0) return fail(o, "external side without a boundary id");
+    o->nbr[a] = -1 - (int64_t)bid;
   }
   // outward planes: canonical plane, flipped to point away from the centroid
   for (int64_t e = 0; e < ne; ++e) {
@@ -277,6 +333,42 @@ bool build_mesh(Oracle *o, const int32_t *face_boundary)
         p[3] = -p[3];
       }
     }
+  }
+  // split planes: through the diagonal (lower node id first), normal to the
+  // quad's outward normal n: m = (n x d)/|n x d|, c = m.x_a; the tet whose
+  // off-diagonal vertex has m.v - c <= 0 takes exit points with m.x - c <= 0
+  for (size_t k = 0; k < split_slots.size(); ++k) {
+    const double *n = &o->plane[4 * split_slots[k][0]];
+    const double *xa = &o->xyz[3 * (int64_t)split_diag[k][0]];
+    const double *xb = &o->xyz[3 * (int64_t)split_diag[k][1]];
+    const double dx = xb[0] - xa[0], dy = xb[1] - xa[1], dz = xb[2] - xa[2];
+    double mx = n[1] * dz - n[2] * dy;
+    double my = n[2] * dx - n[0] * dz;
+    double mz = n[0] * dy - n[1] * dx;
+    const double len = std::sqrt((mx * mx + my * my) + mz * mz);
+    mx = mx / len;
+    my = my / len;
+    mz = mz / len;
+    Oracle::Split S;
+    S.m[0] = mx;
+    S.m[1] = my;
+    S.m[2] = mz;
+    S.m[3] = (mx * xa[0] + my * xa[1]) + mz * xa[2];
+    const int64_t t0 = split_slots[k][1], t1 = split_slots[k][2];
+    int32_t fn[4];
+    int nn;
+    face_nodes(*o, slot_elem[t0], (int)(t0 - o->face_off[slot_elem[t0]]), fn, &nn);
+    int32_t v = fn[0];
+    for (int q = 0; q < 3; ++q)
+      if (fn[q] != split_diag[k][0] && fn[q] != split_diag[k][1]) v = fn[q];
+    const double *xv = &o->xyz[3 * (int64_t)v];
+    const double sv = ((mx * xv[0] + my * xv[1]) + mz * xv[2]) - S.m[3];
+    const int64_t lo = sv <= 0.0 ? t0 : t1, hi = sv <= 0.0 ? t1 : t0;
+    S.elem[0] = slot_elem[lo];
+    S.side[0] = (int32_t)(lo - o->face_off[slot_elem[lo]]);
+    S.elem[1] = slot_elem[hi];
+    S.side[1] = (int32_t)(hi - o->face_off[slot_elem[hi]]);
+    o->splits.push_back(S);
   }
   // volume CDF (sequential sum in element order)
   o->cdf.assign(ne, 0.0);
@@ -487,6 +579,20 @@ bool sample_source(const Oracle &o, uint64_t *seed, Site *st)
     st->u = mu;
     st->v = r * c;
     st->w = r * s;
+  } else if (S.angle == MCW_ANGLE_LINEAR_MU) {
+    // p(mu) ~ 1 + beta mu about the axis: mu solves F(mu) = xi, i.e.
+    // (beta/2) mu^2 + mu + c = 0 with c = (1 - beta/2) - 2 xi, taken as the
+    // root -2c / (1 + sqrt(1 - 2 beta c)) (no cancellation, beta = 0 included);
+    // the azimuth is 2 pi xi' about the axis (rotate_angle)
+    const double cq = (1.0 - 0.5 * S.beta) - 2.0 * orc_prn(seed);
+    double mu = -2.0 * cq / (1.0 + std::sqrt(1.0 - 2.0 * S.beta * cq));
+    if (mu > 1.0) mu = 1.0;
+    if (mu < -1.0) mu = -1.0;
+    double u = S.axis[0], v = S.axis[1], w = S.axis[2];
+    rotate_angle(&u, &v, &w, mu, orc_prn(seed));
+    st->u = u;
+    st->v = v;
+    st->w = w;
   } else {
     return false;
   }
@@ -665,10 +771,19 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
       y = y + L * v;
       z = z + L * w;
       int64_t nb = o.nbr[f0 + fx];
+      int nside = o.nbr_side[f0 + fx];
+      const int64_t sp = o.split_of[f0 + fx];
+      if (sp >= 0) { // split interface: the tet on the exit point's side of the diagonal
+        const Oracle::Split &S = o.splits[sp];
+        const double sg = ((S.m[0] * x + S.m[1] * y) + S.m[2] * z) - S.m[3];
+        const int k = sg <= 0.0 ? 0 : 1;
+        nb = S.elem[k];
+        nside = S.side[k];
+      }
       if (nb >= 0) {
         ev = EV_CROSS;
         h = digest_step(h, elem, ev);
-        inface = o.nbr_side[f0 + fx];
+        inface = nside;
         elem = nb;
       } else {
         int bid = (int)(-1 - nb);
@@ -882,6 +997,13 @@ int orc_source_set(void *h, const mcw_source *s)
     o->src_elem = locate(*o, s->point, 1e-12);
     if (o->src_elem < 0) return fail(o, "point source outside the mesh") ? 0 : 1;
   }
+  if (s->angle == MCW_ANGLE_LINEAR_MU) {
+    const double n2 = (s->axis[0] * s->axis[0] + s->axis[1] * s->axis[1]) + s->axis[2] * s->axis[2];
+    if (!(std::fabs(n2 - 1.0) <= 1e-12)) return fail(o, "source axis must be a unit vector") ? 0 : 1;
+    if (!(std::fabs(s->beta) <= 1.0)) return fail(o, "source beta must be in [-1, 1]") ? 0 : 1;
+  } else if (s->angle != MCW_ANGLE_ISOTROPIC) {
+    return fail(o, "source angle type not supported") ? 0 : 1;
+  }
   return 0;
 }
 
@@ -1068,6 +1190,21 @@ void orc_mesh_neighbors(void *h, int64_t *nbr, int32_t *side)
   Oracle *o = (Oracle *)h;
   std::copy(o->nbr.begin(), o->nbr.end(), nbr);
   std::copy(o->nbr_side.begin(), o->nbr_side.end(), side);
+}
+/* Split interfaces: per side slot the split index (-1: none); per split the
+ * plane (4 doubles) and the two (element, side) pairs. */
+int64_t orc_mesh_n_splits(void *h) { return (int64_t)((Oracle *)h)->splits.size(); }
+void orc_mesh_splits(void *h, int64_t *split_of, double *planes, int64_t *elems, int32_t *sides)
+{
+  Oracle *o = (Oracle *)h;
+  std::copy(o->split_of.begin(), o->split_of.end(), split_of);
+  for (size_t k = 0; k < o->splits.size(); ++k) {
+    for (int q = 0; q < 4; ++q) planes[4 * k + q] = o->splits[k].m[q];
+    for (int q = 0; q < 2; ++q) {
+      elems[2 * k + q] = o->splits[k].elem[q];
+      sides[2 * k + q] = o->splits[k].side[q];
+    }
+  }
 }
 void orc_mesh_cdf(void *h, double *out)
 {

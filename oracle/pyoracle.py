@@ -54,6 +54,9 @@ def lib():
         L.orc_mesh_n_faces.argtypes = [vp]
         L.orc_mesh_planes.argtypes = [vp, dp]
         L.orc_mesh_neighbors.argtypes = [vp, C.POINTER(i64), C.POINTER(i32)]
+        L.orc_mesh_n_splits.argtypes = [vp]
+        L.orc_mesh_n_splits.restype = i64
+        L.orc_mesh_splits.argtypes = [vp, C.POINTER(i64), C.POINTER(C.c_double), C.POINTER(i64), C.POINTER(i32)]
         L.orc_mesh_cdf.argtypes = [vp, dp]
         L.orc_locate.restype = i64
         L.orc_locate.argtypes = [vp, dp, C.c_double]
@@ -179,6 +182,19 @@ class Oracle:
         self.L.orc_mesh_neighbors(self.h, nb.ctypes.data_as(C.POINTER(C.c_int64)),
                                   sd.ctypes.data_as(C.POINTER(C.c_int32)))
         return nb, sd
+
+    def splits(self):
+        """(split index per side slot, -1 = none; planes (k, 4); elements (k, 2); sides (k, 2))."""
+        nf = self.L.orc_mesh_n_faces(self.h)
+        k = self.L.orc_mesh_n_splits(self.h)
+        so = np.zeros(nf, np.int64)
+        pl = np.zeros((max(k, 1), 4))
+        el = np.zeros((max(k, 1), 2), np.int64)
+        sd = np.zeros((max(k, 1), 2), np.int32)
+        self.L.orc_mesh_splits(self.h, so.ctypes.data_as(C.POINTER(C.c_int64)),
+                               pl.ctypes.data_as(C.POINTER(C.c_double)), el.ctypes.data_as(C.POINTER(C.c_int64)),
+                               sd.ctypes.data_as(C.POINTER(C.c_int32)))
+        return so, pl[:k], el[:k], sd[:k]
 
     def cdf(self):
         out = np.zeros(self.n_elem)

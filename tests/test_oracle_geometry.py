@@ -188,3 +188,58 @@ def test_reflection_keeps_unit_direction(oracle_lib):
     assert st["lost"] == 50 and st["leaks"] == 0 and st["reflections"] > 0
     assert np.all(h["n_segments"] == 400)
     assert np.all(np.abs([h["x"], h["y"], h["z"]]) <= 1.0 + 1e-12)
+
+
+def test_mixed_mesh_split_interfaces(oracle_lib):
+    """Graded TET4/HEX8 mesh: the hex sides on the tet region's surface are
+    split interfaces (two tet sides each, on either side of the diagonal
+    plane); the remaining open sides lie on the cube's surface; straight rays
+    visit, segment by segment, an element that contains the segment's
+    midpoint (brute force over its planes), cross hex/tet interfaces both
+    ways, and their lengths add up to the distance to the surface."""
+    n, m = 6, 4
+    mesh = P.graded_mixed_box(n, m, seed=3)
+    is_tet = mesh.elem_type == abi.TET4
+    fo = np.concatenate([[0], np.cumsum(np.where(is_tet, 4, 6))])
+    X = mesh.xyz
+    crossings = 0
+    for src in ((0.02, 0.03, 0.025), (0.7, 0.4, 0.55)):  # tet region; hex region
+        prob = void_problem(mesh, src, {0: abi.BC_VACUUM})
+        o = oracle_lib.Oracle(prob)
+        so, spl, sel, ssd = o.splits()
+        assert len(spl) == 3 * m * m
+        nb, _ = o.neighbors()
+        planes = o.planes()
+        for k in range(len(spl)):
+            quad = np.nonzero(so == k)[0]
+            assert quad.size == 1
+            e = int(np.searchsorted(fo, quad[0], side="right") - 1)
+            qn = set(elem_nodes(mesh, e)[list(HEX_SIDES[quad[0] - fo[e]])].tolist())
+            for q in range(2):
+                t = int(sel[k, q])
+                assert is_tet[t]
+                tn = set(elem_nodes(mesh, t)[list(TET_SIDES[ssd[k, q]])].tolist())
+                assert tn < qn and nb[fo[t] + ssd[k, q]] == e
+                # the side's vertices: on the diagonal (0) or on the tet's own side of the plane
+                sg = X[sorted(tn)] @ spl[k, :3] - spl[k, 3]
+                assert (sg.min() if q == 1 else -sg.max()) >= -1e-15
+        for e in range(mesh.n_elem):
+            sides = TET_SIDES if is_tet[e] else HEX_SIDES
+            for s, sn in enumerate(sides):
+                if so[fo[e] + s] < 0 and nb[fo[e] + s] < 0:
+                    pts = X[elem_nodes(mesh, e)[list(sn)]]
+                    assert any(np.all(pts[:, d] == v) for d in range(3) for v in (0.0, 1.0))
+        sites = o.sources(1, prob.n_particles, 60)
+        for i in range(60):
+            el, ev, le = o.trace(1, prob.n_particles, i)
+            p, u = sites[i, :3], sites[i, 3:6]
+            t = 0.0
+            for e, ln in zip(el, le):
+                mid = p + (t + 0.5 * ln) * u
+                q = planes[fo[e]:fo[e + 1]]
+                assert (q[:, :3] @ mid - q[:, 3]).max() <= 1e-12, (i, e)
+                t += ln
+            crossings += int(np.sum(is_tet[el[1:]] != is_tet[el[:-1]]))
+            tb = min(((1.0 - p[k]) / u[k]) if u[k] > 0 else (-p[k] / u[k]) for k in range(3) if u[k] != 0)
+            assert abs(t - tb) < 1e-12 and ev[-1] == 2
+    assert crossings > 20

@@ -103,3 +103,42 @@ def test_c1_estimators_agree(oracle_lib):
     s2 = o.tally(1, abi.RESULT_SUM_SQ)
     assert np.allclose(s.sum(), tls.sum(), rtol=1e-12)
     assert np.all(s2 >= 0)
+
+
+@pytest.mark.parametrize("beta", [0.6, -0.9, 0.0])
+def test_linear_mu_source_directions(oracle_lib, beta):
+    """Source directions p(mu) ~ 1 + beta mu about the axis: unit vectors,
+    mu distributed by F(mu) = ((mu + 1) + beta (mu^2 - 1)/2)/2 (KS), mean
+    beta/3, azimuth about the axis uniform (KS)."""
+    axis = np.array([1.0, 2.0, -2.0]) / 3.0
+    p = flat_box()
+    p.source = P.linear_mu(P.point_source((0.1, 0.2, 0.3), 1e6), axis, beta)
+    o = oracle_lib.Oracle(p)
+    n = 100_000
+    s = o.sources(1, n, n)
+    u = s[:, 3:6]
+    assert np.abs(np.linalg.norm(u, axis=1) - 1.0).max() < 1e-14
+    mu = u @ axis
+    cdf = lambda x: ((x + 1.0) + beta * (x * x - 1.0) / 2.0) / 2.0
+    assert stats.kstest(mu, cdf).pvalue > 1e-3
+    var = 1.0 / 3.0 - (beta / 3.0) ** 2
+    assert abs(mu.mean() - beta / 3.0) < 4.0 * np.sqrt(var / n)
+    e1 = np.cross(axis, [1.0, 0.0, 0.0])
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(axis, e1)
+    phi = np.arctan2(u @ e2, u @ e1)
+    assert stats.kstest(phi, stats.uniform(-np.pi, 2 * np.pi).cdf).pvalue > 1e-3
+
+
+def test_linear_mu_source_validation(oracle_lib):
+    p = flat_box()
+    src = P.point_source((0.1, 0.2, 0.3), 1e6)
+    src.angle = abi.ANGLE_LINEAR_MU
+    src.axis[:] = [1.0, 1.0, 0.0]
+    src.beta = 0.5
+    p.source = src
+    with pytest.raises(Exception, match="unit vector"):
+        oracle_lib.Oracle(p)
+    p.source = P.linear_mu(P.point_source((0.1, 0.2, 0.3), 1e6), (0, 0, 1), 1.5)
+    with pytest.raises(Exception, match="beta"):
+        oracle_lib.Oracle(p)
