@@ -7,7 +7,11 @@ Workloads (one "step" = one batch of fixed-source histories):
      materials of 4 synthetic nuclides (50k-point grids), uniform volumetric
      Watt source, 1e7 histories per batch, per-element track-length flux;
   c3 the same generator at 8M tets (107^3 lattice), 16 materials of 4 of 32
-     nuclides on the unionized grid, 1e8 histories per batch in bank chunks.
+     nuclides on the unionized grid, 1e8 histories per batch in bank chunks;
+  c5 a 20M-element mixed TET4/HEX8 mesh graded toward a corner (tets meet
+     hexes through split interfaces), 128 materials of 4 of 64 nuclides with
+     100k-point grids, low densities, a volumetric Watt source linear in mu,
+     1e7 histories per batch.
 Inputs are synthetic and resident in HBM before the timed region.
 
 `value` is the batched (persistent, fused) FP64 path, whose histories are
@@ -20,7 +24,7 @@ resident 100k / 1M / 4M-tet meshes, Poisson arrivals at 70% of measured
 capacity, p50/p99 latency) and a CPU baseline (the FP64 oracle, std::thread
 on the host cores, on a bounded sample of the same histories).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c5]
 """
 import argparse
 import json
@@ -134,7 +138,10 @@ WORKLOADS = {
     # name: (generator, default histories per step, description)
     "c2": ("c2_tet_vacuum", 10_000_000, "1M TET4 unit cube, vacuum, 4 materials / 4 nuclides x 50k points"),
     "c3": ("c3_tet_vacuum", 100_000_000, "8M TET4 unit cube, vacuum, 16 materials / 32 nuclides x 50k points "
-                                          "(unionized grid), chunked bank"),
+                                          "(jittered grids, 1.6M-point union), chunked bank"),
+    "c5": ("c5_mixed", 10_000_000, "20M mixed TET4/HEX8 unit cube graded toward a corner, vacuum, 128 materials / "
+                                   "64 nuclides x 100k points (6.4M-point union), low densities, volumetric Watt "
+                                   "source linear in mu"),
 }
 
 
@@ -153,9 +160,11 @@ def tally_error(fp64_ctx, mixed_ctx, n_batches):
     sig = np.sqrt(var)
     ok = sig > 0
     z = np.abs(m[ok] - f[ok]) / n_batches / sig[ok]
+    # one batch gives no standard error: the sigma figures need two or more
     return {"total_rel": float(abs(m.sum() - f.sum()) / f.sum()), "elem_rel_max": float(rel.max()),
             "elem_rel_median": float(np.median(rel)), "elem_rel_p99": float(np.percentile(rel, 99)),
-            "max_sigma": float(z.max()), "frac_beyond_3sigma": float((z > 3).mean()), "batches": n_batches}
+            "max_sigma": float(z.max()) if z.size else None,
+            "frac_beyond_3sigma": float((z > 3).mean()) if z.size else None, "batches": n_batches}
 
 
 def timed_steps(ctx, N, steps, warmup, label):
@@ -173,12 +182,15 @@ def timed_steps(ctx, N, steps, warmup, label):
     return out, wall
 
 
-def roofline_of(steps, N, n_nuc, b_seg):
+def roofline_of(steps, N, n_nuc, b_seg, b_hex=348.0):
+    """B_seg applies to TET4 segments, b_hex to HEX8 segments (FP64 planes)."""
     segs = sum(s["segments"] for s in steps)
+    hexs = sum(s["hex_segments"] for s in steps)
     colls = sum(s["collisions"] for s in steps)
     kern_ms = sum(s["kernel_ms"] for s in steps)
     launches = sum(s["kernel_launches"] for s in steps)
-    b_alg = algorithmic_bytes({"histories": N * len(steps), "segments": segs, "collisions": colls}, n_nuc, b_seg)
+    b_alg = algorithmic_bytes({"histories": N * len(steps), "segments": segs - hexs, "collisions": colls},
+                              n_nuc, b_seg) + b_hex * hexs
     per_launch_bytes = b_alg / launches
     per_launch_s = kern_ms / launches / 1e3
     achieved = per_launch_bytes / per_launch_s / 1e9
@@ -239,7 +251,9 @@ def main():
     n_elem = prob.mesh.n_elem
     n_mat, n_nuc = len(prob.materials), len(prob.nuclides)
     nuc_per_mat = max(len(m.nuclides) for m in prob.materials)
-    workload = f"{args.workload}_tet4_{n_elem // 1000}k_vacuum"
+    all_tet = bool((prob.mesh.elem_type == abi.TET4).all())
+    element = "TET4" if all_tet else "TET4+HEX8"
+    workload = f"{args.workload}_{'tet4' if all_tet else 'mixed'}_{n_elem // 1000}k_vacuum"
     log(f"{workload}: {n_elem} tets, {prob.mesh.n_nodes} nodes, {n_mat} materials, {n_nuc} nuclides "
         f"({time.perf_counter() - t0:.1f}s)")
 
@@ -259,7 +273,7 @@ def main():
     hist_s = N * args.steps / wall
     seg_s = segs / wall
     roof = roofline_of(steps, N, nuc_per_mat, 276.0)
-    roof.update({"traffic": read_traffic(workload), "kernel": "k_persistent<4>"})
+    roof.update({"traffic": read_traffic(workload), "kernel": "k_persistent<4>" if all_tet else "k_persistent<0>"})
 
     mixed = None
     if not args.no_mixed:
@@ -267,7 +281,7 @@ def main():
         mctx.set_options(**tuning)
         msteps, mwall = timed_steps(mctx, N, args.steps, args.warmup, "mixed")
         mroof = roofline_of(msteps, N, nuc_per_mat, 212.0)
-        mroof["kernel"] = "k_persistent<4, mixed>"
+        mroof["kernel"] = "k_persistent<4, mixed>" if all_tet else "k_persistent<0, mixed>"
         mixed = {"histories_per_s": N * args.steps / mwall,
                  "segments_per_s": sum(s["segments"] for s in msteps) / mwall,
                  "ms_per_step": mwall / args.steps * 1e3,
@@ -314,12 +328,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": workload, "description": desc, "n_elem": n_elem, "element": "TET4",
-                   "histories_per_step": N, "materials": n_mat, "nuclides": n_nuc, "grid_points": 50000,
-                   "source": "volume Watt", "boundary": "vacuum", "tally": "per-element track-length flux",
+        "config": {"workload": workload, "description": desc, "n_elem": n_elem, "element": element,
+                   "histories_per_step": N, "materials": n_mat, "nuclides": n_nuc,
+                   "grid_points": int(prob.nuclides[0].energy.size),
+                   "source": "volume Watt" + ("" if prob.source.angle == abi.ANGLE_ISOTROPIC else ", linear in mu"),
+                   "boundary": "vacuum", "tally": "per-element track-length flux",
                    "mode": "batched", "precision": "fp64", "parallelism": "1 GPU", "tuning": tuning},
         "segments_per_s": seg_s,
         "segments_per_history": segs / (N * args.steps),
+        "hex_segment_fraction": sum(s["hex_segments"] for s in steps) / max(1, segs),
         "collisions_per_history": colls / (N * args.steps),
         "plain_gpu": plain,
         "speedup_vs_plain": (hist_s / plain["histories_per_s"]) if plain else None,

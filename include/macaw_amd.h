@@ -165,6 +165,7 @@ typedef struct {
   double kernel_ms;     /* device time of the dominant kernel(s) */
   int64_t kernel_launches;
   int64_t fp64_rechecks; /* precision MIXED: segments whose exit side was re-solved in FP64 */
+  int64_t hex_segments;  /* segments in HEX8 elements (the rest are in TET4) */
 } mcw_batch_stats;
 
 /* Per-history record, for parity checks against a history-based reference. */

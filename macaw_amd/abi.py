@@ -65,7 +65,7 @@ class BatchStats(C.Structure):
                 ("absorptions", C.c_int64), ("leaks", C.c_int64), ("cutoffs", C.c_int64),
                 ("lost", C.c_int64), ("reflections", C.c_int64), ("track_length", C.c_double),
                 ("time_ms", C.c_double), ("kernel_ms", C.c_double),
-                ("kernel_launches", C.c_int64), ("fp64_rechecks", C.c_int64)]
+                ("kernel_launches", C.c_int64), ("fp64_rechecks", C.c_int64), ("hex_segments", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

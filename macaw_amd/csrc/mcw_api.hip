@@ -151,7 +151,7 @@ struct mcw_ctx {
   bool dirty = true;
 
   // device problem
-  DevBuf plane, erec, mtet, split, face_off, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
+  DevBuf plane, erec, mtet, split, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf cell_start, cell_elem;
   DevBuf grid_off, npts, rec, fission, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
@@ -260,17 +260,17 @@ void prepare(mcw_ctx *c, bool need_source = true)
 
   // mesh in internal (Morton) order: planes per side slot, and per element
   // one 32-byte record of neighbour codes, material and side count
+  // (a mixed mesh gives every element 6 plane slots, so that an element's
+  // planes are found from its id alone; its TET4 leave the last two empty)
   const int64_t ne = m.n_elem;
-  std::vector<double4> planes(m.plane.size() / 4);
+  const int64_t per = m.uniform_sides == 4 ? 4 : 6;
+  std::vector<double4> planes((size_t)(ne * per), make_double4(0.0, 0.0, 0.0, 0.0));
   std::vector<int4> erec(2 * ne);
-  std::vector<int32_t> fo;
-  if (m.uniform_sides == 0) fo.resize(ne);
-  int64_t slot = 0;
   for (int64_t i = 0; i < ne; ++i) {
     const int64_t o = m.orig_of_int[i];
     const int64_t f0 = m.face_off[o];
     const int ns = (int)(m.face_off[o + 1] - f0);
-    if (m.uniform_sides == 0) fo[i] = (int32_t)slot;
+    const int64_t slot = i * per;
     int32_t code[6] = {-1, -1, -1, -1, -1, -1};
     for (int k = 0; k < ns; ++k) {
       const double *q = &m.plane[4 * (f0 + k)];
@@ -278,7 +278,6 @@ void prepare(mcw_ctx *c, bool need_source = true)
       const int32_t nb = m.nbr[f0 + k];
       code[k] = nb < 0 || (nb & 7) == 7 ? nb : m.int_of_orig[nb >> 3] * 8 + (nb & 7);
     }
-    slot += ns;
     erec[2 * i] = make_int4(code[0], code[1], code[2], code[3]);
     erec[2 * i + 1] = make_int4(code[4], code[5], emat[o], ns);
   }
@@ -293,18 +292,16 @@ void prepare(mcw_ctx *c, bool need_source = true)
     sp[k].pad[0] = sp[k].pad[1] = 0;
   }
   c->split.upload(sp, s);
-  // mixed-precision records of TET4 elements: FP32 planes relative to vertex
-  // 0 (a mixed mesh's HEX8 entries only say nsides = 6)
-  if (m.uniform_sides == 4 || m.uniform_sides == 0) {
-    std::vector<MixedTet> mt(ne);
-    for (int64_t i = 0; i < ne; ++i) {
+  // mixed-precision records of the TET4 (internal ids 0 .. n_tet-1): FP32
+  // planes relative to vertex 0
+  if (m.n_tet > 0) {
+    std::vector<MixedTet> mt(m.n_tet);
+    for (int64_t i = 0; i < m.n_tet; ++i) {
       const int64_t o = m.orig_of_int[i];
       const double *v0 = &m.xyz[3 * (int64_t)m.conn[m.conn_off[o]]];
       MixedTet &R = mt[i];
       memset(&R, 0, sizeof R);
-      R.nsides = erec[2 * i + 1].w;
-      if (R.nsides != 4) continue;
-      const int64_t s0 = m.uniform_sides == 4 ? 4 * i : fo[i];
+      const int64_t s0 = i * per;
       for (int k = 0; k < 3; ++k) R.v0[k] = v0[k];
       for (int sd = 0; sd < 4; ++sd) {
         const double4 q = planes[s0 + sd];
@@ -319,7 +316,6 @@ void prepare(mcw_ctx *c, bool need_source = true)
     }
     c->mtet.upload(mt, s);
   }
-  if (m.uniform_sides == 0) c->face_off.upload(fo, s);
   c->bc_kind.upload(kinds, s);
   c->cdf.upload(m.cdf, s);
   c->xyz.upload(m.xyz, s);
@@ -425,7 +421,6 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P = Params{};
   P.mesh.plane = c->plane.as<double4>();
   P.mesh.erec = c->erec.as<int4>();
-  P.mesh.face_off = m.uniform_sides ? nullptr : c->face_off.as<int32_t>();
   P.mesh.stride = m.uniform_sides;
   P.mesh.n_bc = c->n_bc;
   P.mesh.bc_kind = c->bc_kind.as<int8_t>();
@@ -437,7 +432,8 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P.mesh.type = c->type.as<uint8_t>();
   P.mesh.int_of_orig = c->int_of_orig.as<int32_t>();
   P.mesh.orig_of_int = c->orig_of_int.as<int32_t>();
-  P.mesh.mtet = m.uniform_sides != 6 ? c->mtet.as<MixedTet>() : nullptr;
+  P.mesh.mtet = m.n_tet > 0 ? c->mtet.as<MixedTet>() : nullptr;
+  P.mesh.n_tet = m.n_tet;
   P.mesh.split = m.splits.empty() ? nullptr : c->split.as<SplitRec>();
   for (int k = 0; k < 3; ++k) {
     P.mesh.grid_lo[k] = m.grid_lo[k];
@@ -1121,6 +1117,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       out->kernel_ms = c->opt.mode == MCW_MODE_PLAIN ? total_ms : kernel_ms;
       out->kernel_launches = launches;
       out->fp64_rechecks = (int64_t)k.rechecks;
+      out->hex_segments = (int64_t)k.hex_segments;
     }
   });
 }

@@ -48,8 +48,8 @@ struct PState {
 // Event counts of one wavefront.  Every field is wavefront-uniform (sums of
 // ballots), so the compiler keeps them in scalar registers.
 struct WaveCount {
-  unsigned long long seg, coll, absn, leak, cut, lost, refl;
-  __device__ void zero() { seg = coll = absn = leak = cut = lost = refl = 0; }
+  unsigned long long seg, coll, absn, leak, cut, lost, refl, hex;
+  __device__ void zero() { seg = coll = absn = leak = cut = lost = refl = hex = 0; }
 };
 
 // ------------------------------------------------------------------ wave helpers
@@ -71,9 +71,10 @@ __device__ __forceinline__ double wave_sum_d(double v)
 __device__ __forceinline__ unsigned long long wave_pop(bool p) { return (unsigned long long)__popcll(__ballot(p)); }
 
 // Count the outcome of one segment step (every lane of the wavefront calls it).
-__device__ __forceinline__ void count_segment(WaveCount &w, bool stepped, int ev)
+__device__ __forceinline__ void count_segment(WaveCount &w, bool stepped, int ev, bool hex = false)
 {
   w.seg += wave_pop(stepped && ev != 6); // a history lost before its segment adds none
+  w.hex += wave_pop(stepped && ev != 6 && hex);
   w.leak += wave_pop(stepped && ev == 2);
   w.refl += wave_pop(stepped && ev == 1);
   w.lost += wave_pop(stepped && ev == 6);
@@ -98,6 +99,7 @@ __device__ void flush_counts(Counters *c, const WaveCount &l, double track)
     if (l.cut) atomicAdd(&c->cutoffs, l.cut);
     if (l.lost) atomicAdd(&c->lost, l.lost);
     if (l.refl) atomicAdd(&c->reflections, l.refl);
+    if (l.hex) atomicAdd(&c->hex_segments, l.hex);
     if (tr != 0.0) unsafeAtomicAdd(&c->track, tr);
   }
 }
@@ -177,7 +179,7 @@ __device__ int32_t locate_point_d(const MeshView &M, const double *p, double tol
       s0 = (int64_t)e * M.stride;
       ns = M.stride;
     } else {
-      s0 = M.face_off[e];
+      s0 = (int64_t)e * 6;
       ns = M.erec[2 * (int64_t)e + 1].w;
     }
     bool in = true;
@@ -325,7 +327,7 @@ __device__ __forceinline__ int32_t pick_nbr(const int4 &r0, const int4 &r1, int 
 // address (nullptr: no bin) and value, for the caller's wave-aggregated add.
 template <int NS, bool REC, bool P32, bool SERVE = false>
 __device__ __forceinline__ int segment(const Params &P, PState &st, double &track_sum, unsigned &rechecks,
-                                       double **tl_ptr = nullptr, double *tl_val = nullptr)
+                                       double **tl_ptr = nullptr, double *tl_val = nullptr, bool *in_hex = nullptr)
 {
   const MeshView &M = P.mesh;
   if (st.elem < 0) return EV_LOST; // source site outside the mesh
@@ -341,15 +343,15 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   bool fp32 = false;
   if (P32) {
     // FP32 record of a TET4 (every element of a TET4 mesh; in a mixed mesh
-    // the record says which elements are TET4, the rest take the FP64 path)
-    const MixedTet &R = M.mtet[st.elem];
-    if (NS == 4 || R.nsides == 4) {
+    // the ids below n_tet; HEX8 take the FP64 path)
+    if (NS == 4 || st.elem < M.n_tet) {
+      const MixedTet &R = M.mtet[st.elem];
       fp32 = true;
       const int4 code = *(const int4 *)R.code;
       r0 = code;
       r1 = make_int4(-1, -1, R.mat, 4);
       mat = R.mat;
-      slot0 = NS == 4 ? (int64_t)st.elem * 4 : (int64_t)M.face_off[st.elem];
+      slot0 = (int64_t)st.elem * (NS == 4 ? 4 : 6);
       bool recheck;
       fx = exit_side_p32(R, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L, recheck);
       if (recheck) {
@@ -361,12 +363,13 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   if (!fp32) {
     r0 = M.erec[2 * (int64_t)st.elem];
     r1 = M.erec[2 * (int64_t)st.elem + 1];
+    if (NS == 0 && in_hex) *in_hex = st.elem >= M.n_tet;
     mat = r1.z;
     if (NS > 0) {
       slot0 = (int64_t)st.elem * NS;
       fx = exit_side<NS>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
     } else {
-      slot0 = M.face_off[st.elem];
+      slot0 = (int64_t)st.elem * 6;
       fx = exit_side_dyn(M.plane + slot0, r1.w, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
     }
   }
@@ -662,13 +665,14 @@ __global__ void __launch_bounds__(256) k_segment(Params P, BankView in, int qin,
     const int64_t i = base + threadIdx.x;
     const bool valid = i < n;
     int ev = -1;
+    bool hx = NS == 6;
     PState st;
     if (valid) {
       load<REC>(in, i, st);
-      ev = segment<NS, REC, false>(P, st, track, rechecks);
+      ev = segment<NS, REC, false>(P, st, track, rechecks, nullptr, nullptr, &hx);
       if (REC && is_death(ev)) write_record(P, st, fate_of(ev));
     }
-    count_segment(wc, valid, ev);
+    count_segment(wc, valid, ev, hx);
     const bool to_coll = valid && ev == EV_COLLIDE;
     const bool to_live = valid && (ev == EV_CROSS || ev == EV_REFLECT);
     const int64_t pc = wave_append(&C->q_count[qcoll], to_coll);
@@ -816,8 +820,9 @@ k_persistent(Params P, SourceBank S)
       int ev = -1;
       double *tl_ptr = nullptr;
       double tl_val = 0.0;
+      bool hx = NS == 6;
       if (go) {
-        ev = segment<NS, REC, P32, SERVE>(P, st, track, rechecks, &tl_ptr, &tl_val);
+        ev = segment<NS, REC, P32, SERVE>(P, st, track, rechecks, &tl_ptr, &tl_val, &hx);
         if (ev == EV_COLLIDE) {
           phase = PH_COLLIDE;
         } else if (is_death(ev)) {
@@ -827,7 +832,7 @@ k_persistent(Params P, SourceBank S)
       }
       if (SERVE || P.tl_fast) atomic_add_agg(tl_ptr != nullptr, tl_ptr, tl_val);
       if (SERVE) req_account(P, go && is_death(ev), st, racc);
-      count_segment(wc, go, ev);
+      count_segment(wc, go, ev, hx);
     } while (wave_count(phase == PH_TRANSPORT) >= lanes_min);
     // collisions together
     const bool hit = phase == PH_COLLIDE;

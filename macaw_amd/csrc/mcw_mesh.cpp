@@ -319,7 +319,7 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
     m.cdf[e] = acc;
   }
 
-  // -- Morton order of element centroids (21 bits per axis), ties by index
+  // -- Morton order of element centroids (21 bits per axis), ties by index;
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int64_t i = 0; i < m.n_nodes; ++i)
     for (int k = 0; k < 3; ++k) {
@@ -335,7 +335,20 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
     v = (v | (v << 2)) & 0x1249249249249249ull;
     return v;
   };
-  std::vector<std::pair<uint64_t, int32_t>> key(ne);
+  // TET4 first, then HEX8, each group along the curve, so that an element's
+  // type follows from its internal id (id < n_tet)
+  struct MKey {
+    int32_t hex;
+    uint64_t code;
+    int32_t e;
+    bool operator<(const MKey &b) const
+    {
+      if (hex != b.hex) return hex < b.hex;
+      if (code != b.code) return code < b.code;
+      return e < b.e;
+    }
+  };
+  std::vector<MKey> key(ne);
   for (int64_t e = 0; e < ne; ++e) {
     const int32_t *c = m.conn.data() + m.conn_off[e];
     const int nn = m.type[e];
@@ -348,14 +361,16 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
       uint64_t q = (uint64_t)std::min(2097151.0, std::max(0.0, t * 2097152.0));
       code |= spread(q) << k;
     }
-    key[e] = {code, (int32_t)e};
+    key[e] = {m.type[e] == 8 ? 1 : 0, code, (int32_t)e};
   }
   std::sort(key.begin(), key.end());
   m.orig_of_int.resize(ne);
   m.int_of_orig.resize(ne);
+  m.n_tet = 0;
   for (int64_t i = 0; i < ne; ++i) {
-    m.orig_of_int[i] = key[i].second;
-    m.int_of_orig[key[i].second] = (int32_t)i;
+    m.orig_of_int[i] = key[i].e;
+    m.int_of_orig[key[i].e] = (int32_t)i;
+    m.n_tet += key[i].hex == 0;
   }
   return "";
 }

@@ -29,7 +29,8 @@ struct HostMesh {
   std::vector<Split> splits;
   std::set<int32_t> boundary_ids;   // ids of external sides
   std::vector<double> cdf;          // element volume CDF (6x volumes)
-  // device order: elements sorted along a Morton curve of their centroids
+  // device order: TET4 then HEX8, each along a Morton curve of the centroids
+  int64_t n_tet = 0;                // internal ids below n_tet are TET4
   std::vector<int32_t> orig_of_int; // internal position -> caller element
   std::vector<int32_t> int_of_orig; // caller element -> internal position
   // point locator (built on demand): uniform grid, CSR of internal ids

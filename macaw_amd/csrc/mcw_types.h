@@ -7,7 +7,6 @@
 //                        meshes: a HEX8 side covered by two TET4 sides), or
 //                        -1 - boundary
 //          elem_mat[e]   int32 material index (-1 void)
-//          face_off[e]   int32 first slot (mixed meshes only)
 //   xs     energy[]      per-nuclide grids, packed (grid_off[n] .. +n_points[n])
 //          values[]      double4 (total, absorption, scatter, fission) per point
 //   bank   SoA particle state (x,y,z,u,v,w,E,sigt f64; seed u64; elem, inface,
@@ -30,8 +29,7 @@ struct __align__(128) MixedTet {
   double v0[3];
   int32_t code[4];   // neighbour codes, as in erec
   int32_t mat;
-  int32_t nsides;    // 4; in mixed meshes a HEX8's entry holds 6 and nothing else
-  int32_t pad[4];
+  int32_t pad[5];
 };
 
 // Split interface of a mixed mesh: a HEX8 side covered by two TET4 sides
@@ -43,14 +41,15 @@ struct SplitRec {
   int32_t pad[2];
 };
 
-// Elements live on the device in Morton order of their centroids
-// ("internal" ids); the caller's ids appear only in source sampling (the
+// Elements live on the device with the TET4 first and the HEX8 after, each
+// group in Morton order of their centroids ("internal" ids); the caller's ids appear only in source sampling (the
 // volume CDF is in caller order), records and tally read-out.
 struct MeshView {
-  const double4 *plane;      // per side slot (internal order): outward unit normal, offset
+  const double4 *plane;      // per side slot (internal order): outward unit normal, offset;
+                             // element e's sides start at slot 4e (TET4 meshes) or 6e
+                             // (HEX8 and mixed meshes, where a TET4 uses 4 of its 6)
   const int4 *erec;          // per element, 2 x int4: neighbour code of sides 0..5
                              // (internal id * 8 + side, or -1 - boundary id), material, side count
-  const int32_t *face_off;   // first side slot per element, mixed meshes only
   int32_t stride;            // sides per element for single-type meshes (4 or 6), 0 = mixed
   int32_t n_bc;
   const int8_t *bc_kind;     // per boundary id
@@ -63,7 +62,8 @@ struct MeshView {
   const uint8_t *type;
   const int32_t *int_of_orig; // caller element -> internal id
   const int32_t *orig_of_int; // internal id -> caller element
-  const MixedTet *mtet;       // mixed-precision records (meshes with TET4, precision MIXED), else nullptr
+  const MixedTet *mtet;       // mixed-precision records of the TET4 (internal ids < n_tet)
+  int64_t n_tet;              // internal ids below n_tet are TET4, the rest HEX8
   const SplitRec *split;      // split interfaces of mixed meshes (neighbour code split * 8 + 7)
   // point location: uniform grid over the mesh bounding box, CSR lists of
   // the (internal) elements whose bounding box meets each cell
@@ -167,6 +167,7 @@ struct RecordView {
 // Global counters, accumulated with one atomic per wavefront.
 struct Counters {
   unsigned long long segments, collisions, absorptions, leaks, cutoffs, lost, reflections, rechecks;
+  unsigned long long hex_segments; // segments in HEX8 elements
   double track;
   unsigned long long next_hist;   // persistent kernel work queue
   unsigned long long q_count[4];  // event queue lengths

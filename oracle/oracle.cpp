@@ -684,7 +684,7 @@ void score(const Oracle &o, const Tally &t, std::vector<double> &val, int64_t el
 
 struct Counters {
   int64_t segments = 0, collisions = 0, absorptions = 0, leaks = 0, cutoffs = 0, lost = 0,
-          reflections = 0;
+          reflections = 0, hex_segments = 0;
   double track = 0.0;
 };
 
@@ -761,6 +761,7 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
     double d = sigt > 0.0 ? -orc_log(xi) / sigt : INFINITY;
     double track = (d < L) ? d : L;
     ++nseg;
+    if (o.type[elem] == MCW_HEX8) ++cnt.hex_segments;
     cnt.track += wgt * track;
     for (size_t k = 0; k < T.size(); ++k)
       if (T[k].estimator == MCW_EST_TRACKLENGTH) score(o, T[k], sc.value[k], elem, E, wgt * track, mat);
@@ -1113,6 +1114,7 @@ int orc_run_batch(void *h, int64_t batch, int64_t n_particles, int32_t threads, 
     tot.cutoffs += cnt[t].cutoffs;
     tot.lost += cnt[t].lost;
     tot.reflections += cnt[t].reflections;
+    tot.hex_segments += cnt[t].hex_segments;
     tot.track += cnt[t].track;
   }
   // batch close: VALUE / total weight -> SUM, SUM_SQ (OpenMC accumulate)
@@ -1138,6 +1140,7 @@ int orc_run_batch(void *h, int64_t batch, int64_t n_particles, int32_t threads, 
     out->cutoffs = tot.cutoffs;
     out->lost = tot.lost;
     out->reflections = tot.reflections;
+    out->hex_segments = tot.hex_segments;
     out->track_length = tot.track;
   }
   return 0;

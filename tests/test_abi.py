@@ -33,7 +33,7 @@ def test_struct_sizes_match_header():
     assert C.sizeof(abi.Options) == 80
     assert C.sizeof(abi.Source) == 144
     assert C.sizeof(abi.History) == 72
-    assert C.sizeof(abi.BatchStats) == 104
+    assert C.sizeof(abi.BatchStats) == 112
 
 
 def test_default_options():

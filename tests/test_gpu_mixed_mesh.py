@@ -31,6 +31,7 @@ def test_mixed_mesh_volume_source_parity(mode):
     for sg, so, hg, ho in res:
         assert_records_equal(hg, ho)
         assert sg["segments"] == so["segments"] and sg["collisions"] == so["collisions"]
+        assert sg["hex_segments"] == so["hex_segments"] and 0 < sg["hex_segments"] < sg["segments"]
         assert sg["lost"] == 0 and sg["leaks"] > 0
     for tid in (1, 2):
         assert_tally_close(g.tally(tid), o.tally(tid))
