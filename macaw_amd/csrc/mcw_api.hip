@@ -260,17 +260,21 @@ void prepare(mcw_ctx *c, bool need_source = true)
 
   // mesh in internal (Morton) order: planes per side slot, and per element
   // one 32-byte record of neighbour codes, material and side count
-  // (a mixed mesh gives every element 6 plane slots, so that an element's
-  // planes are found from its id alone; its TET4 leave the last two empty)
+  // (a mixed mesh lays out its TET4 first in 4-slot, 128-byte blocks and its
+  // HEX8 after in 8-slot, 256-byte blocks, so that an element's planes are
+  // found from its id alone and start on a cache line)
   const int64_t ne = m.n_elem;
-  const int64_t per = m.uniform_sides == 4 ? 4 : 6;
-  std::vector<double4> planes((size_t)(ne * per), make_double4(0.0, 0.0, 0.0, 0.0));
+  auto slot_of = [&](int64_t i) -> int64_t {
+    if (m.uniform_sides) return i * m.uniform_sides;
+    return i < m.n_tet ? 4 * i : 4 * m.n_tet + 8 * (i - m.n_tet);
+  };
+  std::vector<double4> planes((size_t)slot_of(ne), make_double4(0.0, 0.0, 0.0, 0.0));
   std::vector<int4> erec(2 * ne);
   for (int64_t i = 0; i < ne; ++i) {
     const int64_t o = m.orig_of_int[i];
     const int64_t f0 = m.face_off[o];
     const int ns = (int)(m.face_off[o + 1] - f0);
-    const int64_t slot = i * per;
+    const int64_t slot = slot_of(i);
     int32_t code[6] = {-1, -1, -1, -1, -1, -1};
     for (int k = 0; k < ns; ++k) {
       const double *q = &m.plane[4 * (f0 + k)];
@@ -301,7 +305,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
       const double *v0 = &m.xyz[3 * (int64_t)m.conn[m.conn_off[o]]];
       MixedTet &R = mt[i];
       memset(&R, 0, sizeof R);
-      const int64_t s0 = i * per;
+      const int64_t s0 = slot_of(i);
       for (int k = 0; k < 3; ++k) R.v0[k] = v0[k];
       for (int sd = 0; sd < 4; ++sd) {
         const double4 q = planes[s0 + sd];

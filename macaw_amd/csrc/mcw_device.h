@@ -21,6 +21,14 @@ namespace mcw {
 
 // ------------------------------------------------------------------ bits
 
+// First plane slot of element e (internal id); NS is the mesh's side count
+// (4, 6) or 0 for a mixed mesh.
+template <int NS> __device__ __forceinline__ int64_t slot_of(const MeshView &M, int64_t e)
+{
+  if (NS > 0) return e * NS;
+  return e < M.n_tet ? 4 * e : 4 * M.n_tet + 8 * (e - M.n_tet);
+}
+
 // Atomics on device memory through explicit global (address space 1)
 // pointers.  A pointer read from memory is generic to the compiler, which
 // then emits FLAT atomics; the tally pointers of the request server are.

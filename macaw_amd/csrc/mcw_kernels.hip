@@ -179,7 +179,7 @@ __device__ int32_t locate_point_d(const MeshView &M, const double *p, double tol
       s0 = (int64_t)e * M.stride;
       ns = M.stride;
     } else {
-      s0 = (int64_t)e * 6;
+      s0 = slot_of<0>(M, e);
       ns = M.erec[2 * (int64_t)e + 1].w;
     }
     bool in = true;
@@ -351,7 +351,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
       r0 = code;
       r1 = make_int4(-1, -1, R.mat, 4);
       mat = R.mat;
-      slot0 = (int64_t)st.elem * (NS == 4 ? 4 : 6);
+      slot0 = (int64_t)st.elem * 4; // TET4 are first in a mixed mesh
       bool recheck;
       fx = exit_side_p32(R, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L, recheck);
       if (recheck) {
@@ -369,7 +369,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
       slot0 = (int64_t)st.elem * NS;
       fx = exit_side<NS>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
     } else {
-      slot0 = (int64_t)st.elem * 6;
+      slot0 = slot_of<0>(M, st.elem);
       fx = exit_side_dyn(M.plane + slot0, r1.w, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
     }
   }

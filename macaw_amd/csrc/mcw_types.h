@@ -46,8 +46,9 @@ struct SplitRec {
 // volume CDF is in caller order), records and tally read-out.
 struct MeshView {
   const double4 *plane;      // per side slot (internal order): outward unit normal, offset;
-                             // element e's sides start at slot 4e (TET4 meshes) or 6e
-                             // (HEX8 and mixed meshes, where a TET4 uses 4 of its 6)
+                             // element e's sides start at slot_of(M, e): 4e (TET4 meshes),
+                             // 6e (HEX8 meshes); mixed meshes 4e for the TET4 (e < n_tet),
+                             // 4 n_tet + 8 (e - n_tet) for the HEX8 (cache-line aligned)
   const int4 *erec;          // per element, 2 x int4: neighbour code of sides 0..5
                              // (internal id * 8 + side, or -1 - boundary id), material, side count
   int32_t stride;            // sides per element for single-type meshes (4 or 6), 0 = mixed
