@@ -43,9 +43,11 @@ __device__ __forceinline__ void atomic_add_dev(unsigned long long *p, unsigned l
   (void)__hip_atomic_fetch_add((gu64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Atomic add of v at p for the lanes with `on`; the lanes whose address
-// equals the first such lane's add their sum in one atomic.  Lanes that start
-// at one source point share their first elements, and same-address atomics
+// Atomic add of v at p for the lanes with `on`.  Lanes that share an address
+// add their sum in one atomic: the first pending lane's address is matched
+// against the others, repeatedly while a match joins at least two lanes (at
+// most 4 groups); the remaining lanes add one by one.  Lanes that start at one
+// source point share their first elements, and same-address atomics
 // serialise.  Call with the whole wave.
 __device__ __forceinline__ double wave_sum_dbl(double v)
 {
@@ -55,19 +57,20 @@ __device__ __forceinline__ double wave_sum_dbl(double v)
 }
 __device__ __forceinline__ void atomic_add_agg(bool on, double *p, double v)
 {
-  const unsigned long long m = __ballot(on);
-  if (m == 0ull) return;
-  const int leader = __builtin_ctzll(m);
   const unsigned long long a = (unsigned long long)p;
-  const unsigned long long al = (unsigned long long)__shfl((long long)a, leader, 64);
-  const bool same = on && a == al;
-  if (__popcll(__ballot(same)) > 1) {
+  bool todo = on;
+  for (int g = 0; g < 4; ++g) {
+    const unsigned long long m = __ballot(todo);
+    if (m == 0ull) return;
+    const int leader = __builtin_ctzll(m);
+    const unsigned long long al = (unsigned long long)__shfl((long long)a, leader, 64);
+    const bool same = todo && a == al;
+    if (__popcll(__ballot(same)) < 2) break;
     const double s = wave_sum_dbl(same ? v : 0.0);
     if (__lane_id() == leader) atomic_add_dev(p, s);
-    if (on && !same) atomic_add_dev(p, v);
-  } else if (on) {
-    atomic_add_dev(p, v);
+    if (same) todo = false;
   }
+  if (todo) atomic_add_dev(p, v);
 }
 
 __device__ __forceinline__ uint64_t dbits(double x) { return (uint64_t)__double_as_longlong(x); }
