@@ -104,7 +104,10 @@ typedef struct {
                                (fewer registers, some spills), 0 = compiler's choice */
   int32_t use_graph;        /* PLAIN: 1 = replay the event cycle (8 segment + collision
                                cycles) as one captured HIP graph */
-  int32_t reserved[4];
+  int32_t trace_histories;  /* BATCHED with record_histories: keep one record per element
+                               visit of the first trace_histories histories of a batch
+                               (mcw_trace_get), as RayTracingExodus writes them */
+  int32_t reserved[3];
 } mcw_options;
 
 /* One synthetic continuous-energy nuclide: pointwise lin-lin tables on an
@@ -166,6 +169,7 @@ typedef struct {
   int64_t kernel_launches;
   int64_t fp64_rechecks; /* precision MIXED: segments whose exit side was re-solved in FP64 */
   int64_t hex_segments;  /* segments in HEX8 elements (the rest are in TET4) */
+  int64_t trace_records; /* visit records emitted (trace_histories); beyond the buffer they are dropped */
 } mcw_batch_stats;
 
 /* Per-history record, for parity checks against a history-based reference. */
@@ -189,7 +193,7 @@ int mcw_create(mcw_ctx **ctx, const mcw_options *opts);
 int mcw_destroy(mcw_ctx *ctx);
 /* Change the schedule and tuning fields of a live context (mode, precision,
  * block_size, bank_capacity, lanes_min, waves_per_eu, sort_cadence,
- * use_graph, record_histories, max_segments) without re-uploading the problem; device,
+ * use_graph, trace_histories, record_histories, max_segments) without re-uploading the problem; device,
  * master_seed and seed_roundtrip cannot change.  Used by the autotuner. */
 int mcw_options_set(mcw_ctx *ctx, const mcw_options *opts);
 /* Replaces openmc_err_msg. */
@@ -256,6 +260,29 @@ int mcw_history_get(mcw_ctx *ctx, mcw_history *out, size_t n);
 int mcw_debug_math_probe(mcw_ctx *ctx, const double *x, int64_t n, double *log_out, double *sin_out,
                          double *cos_out, double *sqrt_out, const uint64_t *seeds, double *prn_out,
                          uint64_t *seed_out);
+
+/* One element visit of a traced history, as a record of MOOSE's
+ * RayTracingExodus output (test/tests/study/openmc_study.i:73-81): the
+ * segment from where the history entered the element (or started, or was
+ * reflected) to where it left it (or died), with the energy and tracking seed
+ * at the visit's end.  event: MCW_EVENT_*. */
+enum { MCW_EVENT_CROSS = 0, MCW_EVENT_REFLECT = 1, MCW_EVENT_LEAK = 2, MCW_EVENT_ABSORB = 4,
+       MCW_EVENT_CUTOFF = 5, MCW_EVENT_LOST = 6 };
+typedef struct {
+  int64_t history; /* index within the batch, from 0 */
+  int32_t visit;   /* visit index within the history, from 0 */
+  int32_t element; /* caller element id */
+  double start[3];
+  double end[3];
+  double energy;
+  uint64_t seed_tracking;
+  int32_t event;
+  int32_t pad;
+} mcw_trace_record;
+
+/* Visit records of the last batch, ordered by (history, visit).  *n_out gets
+ * the number available; at most `max` are copied (out may be NULL). */
+int mcw_trace_get(mcw_ctx *ctx, mcw_trace_record *out, size_t max, int64_t *n_out);
 
 /* Diagnostics (no reference counterpart): the device's union-grid lookup of
  * nuclide `nuc` at n host energies: union index and (total, absorption,

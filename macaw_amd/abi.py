@@ -36,7 +36,8 @@ class Options(C.Structure):
                 ("block_size", C.c_int32), ("bank_capacity", C.c_int64),
                 ("max_segments", C.c_int64), ("master_seed", C.c_uint64),
                 ("sort_cadence", C.c_int32), ("lanes_min", C.c_int32),
-                ("waves_per_eu", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("waves_per_eu", C.c_int32), ("use_graph", C.c_int32), ("trace_histories", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class Nuclide(C.Structure):
@@ -65,10 +66,28 @@ class BatchStats(C.Structure):
                 ("absorptions", C.c_int64), ("leaks", C.c_int64), ("cutoffs", C.c_int64),
                 ("lost", C.c_int64), ("reflections", C.c_int64), ("track_length", C.c_double),
                 ("time_ms", C.c_double), ("kernel_ms", C.c_double),
-                ("kernel_launches", C.c_int64), ("fp64_rechecks", C.c_int64), ("hex_segments", C.c_int64)]
+                ("kernel_launches", C.c_int64), ("fp64_rechecks", C.c_int64), ("hex_segments", C.c_int64),
+                ("trace_records", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TraceRecord(C.Structure):
+    _fields_ = [("history", C.c_int64), ("visit", C.c_int32), ("element", C.c_int32),
+                ("start", C.c_double * 3), ("end", C.c_double * 3), ("energy", C.c_double),
+                ("seed_tracking", C.c_uint64), ("event", C.c_int32), ("pad", C.c_int32)]
+
+
+def trace_dtype():
+    """numpy dtype matching mcw_trace_record."""
+    import numpy as np
+    return np.dtype([("history", np.int64), ("visit", np.int32), ("element", np.int32),
+                     ("start", np.float64, 3), ("end", np.float64, 3), ("energy", np.float64),
+                     ("seed_tracking", np.uint64), ("event", np.int32), ("pad", np.int32)])
+
+
+EVENT_CROSS, EVENT_REFLECT, EVENT_LEAK, EVENT_ABSORB, EVENT_CUTOFF, EVENT_LOST = 0, 1, 2, 4, 5, 6
 
 
 class History(C.Structure):

@@ -165,6 +165,9 @@ struct mcw_ctx {
   size_t sort_tmp_bytes = 0;
   DevBuf sb2[10];
   int64_t sb2_cap = 0;
+  // visit records of traced histories (record mode)
+  DevBuf trace;
+  int64_t trace_cap = 0, trace_count = 0;
   int sort_shift = 0, sort_bits = 32;
   uint32_t sort_mat_mask = 0;
   DevBuf bank[3][17];
@@ -510,6 +513,9 @@ void prepare(mcw_ctx *c, bool need_source = true)
     V.value = t.value.as<double>();
   }
   P.tl_fast = (n_tl == 1 && P.tally[0].simple_flux && P.tally[0].n_edges == 0) ? 1 : 0;
+  P.trace = nullptr;
+  P.trace_n = 0;
+  P.trace_cap = 0;
   P.master_seed = c->opt.master_seed ? c->opt.master_seed : 1;
   P.roundtrip = c->opt.seed_roundtrip;
   P.lanes_min = c->opt.lanes_min > 0 ? std::min(c->opt.lanes_min, 64) : 40;
@@ -863,6 +869,7 @@ int mcw_options_set(mcw_ctx *c, const mcw_options *o)
             "block_size must be 64, 128, 192 or 256");
     require(o->sort_cadence >= 0, "sort_cadence must not be negative");
     require(o->use_graph == 0 || o->use_graph == 1, "use_graph must be 0 or 1");
+    require(o->trace_histories >= 0, "trace_histories must not be negative");
     const bool reprep = o->max_segments != c->opt.max_segments || o->lanes_min != c->opt.lanes_min;
     c->opt = *o;
     if (c->opt.master_seed == 0) c->opt.master_seed = 1;
@@ -1058,6 +1065,19 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       P.rec.elem = c->r_elem.as<int32_t>();
       P.rec.fate = c->r_fate.as<int32_t>();
     }
+    if (c->opt.trace_histories > 0) {
+      require(c->opt.mode == MCW_MODE_BATCHED && P.record,
+              "trace_histories needs mode BATCHED and record_histories");
+      const int64_t tn = std::min<int64_t>(c->opt.trace_histories, n_particles);
+      const int64_t cap = std::min<int64_t>(std::max<int64_t>(tn * 256, (int64_t)1 << 16), (int64_t)1 << 24);
+      if (c->trace_cap < cap) {
+        c->trace.alloc((size_t)cap * sizeof(TraceRec));
+        c->trace_cap = cap;
+      }
+      P.trace = c->trace.as<TraceRec>();
+      P.trace_n = tn;
+      P.trace_cap = c->trace_cap;
+    }
     for (auto &t : c->tallies) HIPCHK(launch_fill(s, t->value.as<double>(), t->n_filter_bins * t->n_score_bins, 0.0));
     HIPCHK(launch_zero_counters(s, P.cnt));
     const int64_t cap_default = (int64_t)1 << 24;
@@ -1105,6 +1125,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
     HIPCHK(hipEventElapsedTime(&total_ms, c->ev[0], c->ev[1]));
     c->last_batch_n = n_particles;
     c->last_batch = batch;
+    c->trace_count = P.trace_n > 0 ? std::min<int64_t>((int64_t)c->h_counters->trace_count, c->trace_cap) : 0;
     if (out) {
       const Counters &k = *c->h_counters;
       memset(out, 0, sizeof *out);
@@ -1122,6 +1143,40 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       out->kernel_launches = launches;
       out->fp64_rechecks = (int64_t)k.rechecks;
       out->hex_segments = (int64_t)k.hex_segments;
+      out->trace_records = (int64_t)k.trace_count;
+    }
+  });
+}
+
+int mcw_trace_get(mcw_ctx *c, mcw_trace_record *out, size_t max, int64_t *n_out)
+{
+  return guard(c, [&] {
+    const int64_t n = c->trace_count;
+    if (n_out) *n_out = n;
+    if (!out || max == 0 || n == 0) return;
+    std::vector<TraceRec> tmp((size_t)n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), c->trace.p, (size_t)n * sizeof(TraceRec), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::sort(tmp.begin(), tmp.end(), [](const TraceRec &a, const TraceRec &b) {
+      return a.hid != b.hid ? a.hid < b.hid : a.visit < b.visit;
+    });
+    const size_t m = std::min<size_t>(max, (size_t)n);
+    for (size_t i = 0; i < m; ++i) {
+      const TraceRec &r = tmp[i];
+      mcw_trace_record &o = out[i];
+      o.history = r.hid;
+      o.visit = r.visit;
+      o.element = r.elem;
+      o.start[0] = r.x0;
+      o.start[1] = r.y0;
+      o.start[2] = r.z0;
+      o.end[0] = r.x1;
+      o.end[1] = r.y1;
+      o.end[2] = r.z1;
+      o.energy = r.E;
+      o.seed_tracking = r.seed;
+      o.event = r.event;
+      o.pad = 0;
     }
   });
 }

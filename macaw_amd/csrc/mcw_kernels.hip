@@ -43,6 +43,8 @@ struct PState {
   int32_t nseg, ncoll;
   double *tv;   // request server: the request's flux tally
   int32_t req;  // request server: slice of the round
+  double x0, y0, z0; // record mode: where the current element visit began
+  int32_t visit;     // record mode: index of the current visit
 };
 
 // Event counts of one wavefront.  Every field is wavefront-uniform (sums of
@@ -303,6 +305,39 @@ __device__ __forceinline__ void start_history(PState &st, int64_t hid)
   st.h = 0xcbf29ce484222325ull;
   st.nseg = 0;
   st.ncoll = 0;
+  st.x0 = st.x;
+  st.y0 = st.y;
+  st.z0 = st.z;
+  st.visit = 0;
+}
+
+// Record mode: close the current element visit of a traced history with
+// event `ev` (the visit was in internal element `e`) and start the next one
+// at the current position.
+template <bool REC> __device__ __forceinline__ void trace_visit(const Params &P, PState &st, int32_t e, int ev)
+{
+  if (!REC || st.hid >= P.trace_n) return;
+  const unsigned long long k = atomicAdd(&P.cnt->trace_count, 1ull);
+  if ((int64_t)k < P.trace_cap) {
+    TraceRec &R = P.trace[k];
+    R.x0 = st.x0;
+    R.y0 = st.y0;
+    R.z0 = st.z0;
+    R.x1 = st.x;
+    R.y1 = st.y;
+    R.z1 = st.z;
+    R.E = st.E;
+    R.seed = st.seed;
+    R.hid = st.hid;
+    R.elem = e < 0 ? -1 : P.mesh.orig_of_int[e];
+    R.visit = st.visit;
+    R.event = ev;
+    R.pad = 0;
+  }
+  ++st.visit;
+  st.x0 = st.x;
+  st.y0 = st.y;
+  st.z0 = st.z;
 }
 
 // ------------------------------------------------------------------ one segment
@@ -333,6 +368,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   if (st.elem < 0) return EV_LOST; // source site outside the mesh
   if (st.nseg >= P.max_seg) {
     if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
+    trace_visit<REC>(P, st, st.elem, EV_LOST);
     return EV_LOST;
   }
   int4 r0, r1;
@@ -379,6 +415,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   }
   if (fx < 0) {
     if (REC) st.h = digest_step(st.h, orig_elem(M, st.elem), EV_LOST);
+    trace_visit<REC>(P, st, st.elem, EV_LOST);
     return EV_LOST;
   }
   const double xi = prn(st.seed);
@@ -404,6 +441,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
     for (int t = 0; t < P.n_tl; ++t) tally_score(P.tally[t], P.xs, st.elem, st.E, st.k, track, mat);
   }
   if (d > L) {
+    const int32_t e_visit = st.elem;
     st.x = st.x + L * st.u;
     st.y = st.y + L * st.v;
     st.z = st.z + L * st.w;
@@ -438,6 +476,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
       }
     }
     if (P.roundtrip) st.seed = seed_roundtrip(st.seed);
+    trace_visit<REC>(P, st, e_visit, ev);
     return ev;
   }
   // collision: back-step from the exit point (CollisionKernel.C:172-175)
@@ -498,6 +537,7 @@ __device__ __forceinline__ int collide(const Params &P, PState &st, const int32_
   if (REC) st.h = digest_step(st.h, orig_elem(P.mesh, st.elem), ev);
   st.inface = -1;
   if (P.roundtrip) st.seed = seed_roundtrip(st.seed);
+  if (ev != EV_SCATTER) trace_visit<REC>(P, st, st.elem, ev);
   return ev;
 }
 

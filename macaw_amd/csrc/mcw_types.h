@@ -159,6 +159,21 @@ struct ReqSlice {
   SourceView src;
 };
 
+// One element visit of a traced history (record mode of the persistent
+// kernel): from entering the element (or the start, or a reflection) to
+// leaving it (crossing, reflection, leak, loss) or dying in it (absorption,
+// energy cutoff).  Energy and tracking seed are the values at the visit's end.
+struct TraceRec {
+  double x0, y0, z0, x1, y1, z1;
+  double E;
+  uint64_t seed;
+  int64_t hid;    // history index within the batch
+  int32_t elem;   // caller element id
+  int32_t visit;  // visit index within the history, from 0
+  int32_t event;  // event that ended the visit (EV_*)
+  int32_t pad;
+};
+
 struct RecordView {
   uint64_t *digest, *seed;
   double *E, *x, *y, *z;
@@ -169,6 +184,7 @@ struct RecordView {
 struct Counters {
   unsigned long long segments, collisions, absorptions, leaks, cutoffs, lost, reflections, rechecks;
   unsigned long long hex_segments; // segments in HEX8 elements
+  unsigned long long trace_count;  // trace records emitted (may exceed the buffer)
   double track;
   unsigned long long next_hist;   // persistent kernel work queue
   unsigned long long q_count[4];  // event queue lengths
@@ -196,6 +212,9 @@ struct Params {
   Counters *cnt;
   RecordView rec;
   const ReqSlice *slices;  // request-server rounds, else nullptr
+  TraceRec *trace;         // record mode: visit records of histories hid < trace_n
+  int64_t trace_n;
+  int64_t trace_cap;
 };
 
 } // namespace mcw

@@ -22,7 +22,7 @@ EXPORTS = (
     "mcw_xs_set", "mcw_materials_set", "mcw_block_material_map", "mcw_source_set", "mcw_tally_add",
     "mcw_tally_shape", "mcw_run_batch", "mcw_tally_get", "mcw_tally_reset", "mcw_history_get",
     "mcw_server_create", "mcw_server_destroy", "mcw_server_last_error", "mcw_submit", "mcw_poll",
-    "mcw_result", "mcw_debug_math_probe", "mcw_debug_xs_probe",
+    "mcw_result", "mcw_trace_get", "mcw_debug_math_probe", "mcw_debug_xs_probe",
 )
 
 
@@ -63,6 +63,7 @@ def lib():
         "mcw_submit": ([vp, C.POINTER(abi.Request)], C.c_int),
         "mcw_poll": ([vp, i32p], C.c_int),
         "mcw_result": ([vp, C.POINTER(abi.RequestStatus), dp, C.c_size_t], C.c_int),
+        "mcw_trace_get": ([vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int64)], C.c_int),
         "mcw_debug_math_probe": ([vp, dp, i64, dp, dp, dp, dp, u64p, dp, u64p], C.c_int),
         "mcw_debug_xs_probe": ([vp, i32, dp, i64, i32p, dp], C.c_int),
     }
@@ -141,6 +142,16 @@ class Context:
         nf, ns = self.tally_shape(tally_id)
         out = np.zeros(nf * ns)
         self._chk(self.L.mcw_tally_get(self.h, tally_id, kind, out.ctypes.data_as(C.POINTER(C.c_double)), out.size))
+        return out
+
+    def trace(self):
+        """Visit records of the last batch (mcw_trace_get), ordered by
+        (history, visit), as a numpy array of abi.trace_dtype()."""
+        n = C.c_int64()
+        self._chk(self.L.mcw_trace_get(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, abi.trace_dtype())
+        if n.value:
+            self._chk(self.L.mcw_trace_get(self.h, out.ctypes.data, n.value, C.byref(n)))
         return out
 
     def tally_reset(self):

@@ -8,7 +8,9 @@ registered in src/userobjects/OpenMCStudy.C:190-197.
 
 Output: per record the ray id, the visit index (`intersections`) and the aux
 data (energy, weight, id, n_progeny, particle type and the three seeds as
-exact integers).  Only data is written; nothing of the reference is run.
+exact integers) in gold_rays.json, and the file's layout (dimensions,
+variables, element-variable names, attributes) in gold_rays_layout.json.
+Only data is written; nothing of the reference is run.
 
 Usage: python tests/golden/extract_gold_seeds.py /path/to/openmc_study_rays.e
 """
@@ -50,6 +52,31 @@ def main(path):
     with open(dst, "w") as fh:
         json.dump(out, fh, separators=(",", ":"))
     print(f"wrote {dst}: {len(rec)} records")
+    # the file's layout (dimensions, variables, element-variable names,
+    # global attributes) for the trace writer's format test
+    fixed = {k: v for k, v in f.dimensions.items() if v is not None}
+    layout = {
+        "dimensions_fixed": {k: v for k, v in fixed.items()
+                             if k in ("len_string", "len_line", "four", "len_name", "num_dim", "num_el_blk",
+                                      "num_nod_per_el1")},
+        "dimensions_sized": {"num_nodes": "num_elem + number of rays", "num_elem": "visits",
+                             "num_el_in_blk1": "visits", "num_elem_var": len(names)},
+        "unlimited": [k for k, v in f.dimensions.items() if v is None],
+        "variables": {k: [list(v.dimensions), v.typecode()] for k, v in f.variables.items()},
+        "variable_attributes": {k: {a: (b.decode() if isinstance(b, bytes) else b) for a, b in v._attributes.items()}
+                                for k, v in f.variables.items() if v._attributes},
+        "time_whole": float(f.variables["time_whole"][0]),
+        "connect1_elem_type": f.variables["connect1"].elem_type.decode(),
+        "elem_var_names": names,
+        "attributes": {k: (v.decode() if isinstance(v, bytes) else float(v) if isinstance(v, np.floating)
+                           else int(v)) for k, v in f._attributes.items() if k != "title"},
+        "counts": {"num_nodes": f.dimensions["num_nodes"], "num_elem": f.dimensions["num_elem"],
+                   "rays": int(len(set(cols["ray_id"].tolist())))},
+    }
+    dst2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gold_rays_layout.json")
+    with open(dst2, "w") as fh:
+        json.dump(layout, fh, indent=1)
+    print(f"wrote {dst2}")
 
 
 if __name__ == "__main__":

@@ -33,7 +33,8 @@ def test_struct_sizes_match_header():
     assert C.sizeof(abi.Options) == 80
     assert C.sizeof(abi.Source) == 144
     assert C.sizeof(abi.History) == 72
-    assert C.sizeof(abi.BatchStats) == 112
+    assert C.sizeof(abi.BatchStats) == 120
+    assert C.sizeof(abi.TraceRecord) == 88 == abi.trace_dtype().itemsize
 
 
 def test_default_options():
