@@ -209,6 +209,21 @@ void require(bool ok, const std::string &m)
   if (!ok) throw McwError{m};
 }
 
+// A source's numbers: finite positions, positive finite energy parameters.
+void check_source_values(const mcw_source &src)
+{
+  for (int k = 0; k < 3; ++k) {
+    if (src.space == MCW_SPACE_POINT) require(std::isfinite(src.point[k]), "point source coordinates must be finite");
+    if (src.space == MCW_SPACE_BOX)
+      require(std::isfinite(src.box_lo[k]) && std::isfinite(src.box_hi[k]), "box source bounds must be finite");
+  }
+  if (src.energy == MCW_ENERGY_MONO)
+    require(std::isfinite(src.E0) && src.E0 > 0.0, "source energy must be positive and finite");
+  if (src.energy == MCW_ENERGY_WATT)
+    require(std::isfinite(src.watt_a) && src.watt_a > 0.0 && std::isfinite(src.watt_b) && src.watt_b > 0.0,
+            "Watt spectrum parameters a and b must be positive and finite");
+}
+
 // Isotropic, or linear in mu about a unit axis with |beta| <= 1.
 void check_angle(const mcw_source &src)
 {
@@ -407,6 +422,8 @@ void prepare(mcw_ctx *c, bool need_source = true)
   c->cell_start.upload(m.cell_start, s);
   c->cell_elem.upload(m.cell_elem, s);
   // source
+  check_angle(c->src);
+  check_source_values(c->src);
   c->src_elem = -1;
   if (c->src.space == MCW_SPACE_POINT) {
     c->src_elem = locate_point(m, c->src.point, 1e-12);
@@ -417,7 +434,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
   } else {
     require(c->src.space == MCW_SPACE_MESH_VOLUME, "unknown source space type");
   }
-  check_angle(c->src);
+
   require(c->src.energy == MCW_ENERGY_MONO || c->src.energy == MCW_ENERGY_WATT, "source energy type not supported");
 
   c->counters.alloc(sizeof(Counters));
@@ -1602,6 +1619,7 @@ int mcw_submit(mcw_server *s, const mcw_request *req)
     const int sp = req->source.space;
     require(sp == MCW_SPACE_POINT || sp == MCW_SPACE_BOX || sp == MCW_SPACE_MESH_VOLUME, "unknown source space type");
     check_angle(req->source);
+    check_source_values(req->source);
     require(req->source.energy == MCW_ENERGY_MONO || req->source.energy == MCW_ENERGY_WATT,
             "source energy type not supported");
     auto *q = new ServeReq();

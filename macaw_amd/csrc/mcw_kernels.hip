@@ -167,9 +167,9 @@ __device__ int32_t locate_point_d(const MeshView &M, const double *p, double tol
   int32_t cell[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double f = (p[k] - M.grid_lo[k]) * M.grid_inv[k];
-    int32_t c = f > 0.0 ? (int32_t)f : 0;
-    cell[k] = c < M.grid_n[k] ? c : M.grid_n[k] - 1;
+    double f = (p[k] - M.grid_lo[k]) * M.grid_inv[k];
+    f = f < (double)(M.grid_n[k] - 1) ? f : (double)(M.grid_n[k] - 1);
+    cell[k] = f > 0.0 ? (int32_t)f : 0; // NaN and below the box: cell 0
   }
   const int64_t ci = (int64_t)cell[0] + (int64_t)M.grid_n[0] * ((int64_t)cell[1] + (int64_t)M.grid_n[1] * cell[2]);
   int32_t best = -1, best_orig = 0x7fffffff;

@@ -89,6 +89,8 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
   if (ne >= ((int64_t)1 << 28)) return "mesh too large: more than 2^28 elements";
   for (int64_t i = 0; i < m.conn_off[ne]; ++i)
     if (m.conn[i] < 0 || m.conn[i] >= m.n_nodes) return "connectivity refers to a node out of range";
+  for (int64_t i = 0; i < 3 * m.n_nodes; ++i)
+    if (!std::isfinite(m.xyz[i])) return "node coordinates must be finite (node " + std::to_string(i / 3) + ")";
 
   // -- neighbours: sort side keys
   struct Key {
@@ -315,6 +317,7 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
   }
   double acc = 0.0;
   for (int64_t e = 0; e < ne; ++e) {
+    if (!(vol[e] > 0.0)) return "degenerate element " + std::to_string(e) + " (zero volume)";
     acc = acc + vol[e];
     m.cdf[e] = acc;
   }
@@ -457,9 +460,8 @@ int64_t locate_point(const HostMesh &m, const double *p, double tol)
   // contain p is the brute-force answer
   int64_t cell[3];
   for (int k = 0; k < 3; ++k) {
-    const double f = (p[k] - m.grid_lo[k]) * m.grid_inv[k];
-    const int64_t c = f > 0.0 ? (int64_t)f : 0;
-    cell[k] = std::min<int64_t>(c, m.grid_n[k] - 1);
+    const double f = std::min((p[k] - m.grid_lo[k]) * m.grid_inv[k], (double)(m.grid_n[k] - 1));
+    cell[k] = f > 0.0 ? (int64_t)f : 0; // NaN and below the box: cell 0
   }
   const int64_t ci = cell[0] + (int64_t)m.grid_n[0] * (cell[1] + (int64_t)m.grid_n[1] * cell[2]);
   int64_t best = -1;

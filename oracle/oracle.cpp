@@ -374,7 +374,9 @@ bool build_mesh(Oracle *o, const int32_t *face_boundary)
   o->cdf.assign(ne, 0.0);
   double acc = 0.0;
   for (int64_t e = 0; e < ne; ++e) {
-    acc = acc + elem_volume6(*o, e);
+    const double v = elem_volume6(*o, e);
+    if (!(v > 0.0)) return fail(o, "degenerate element (zero volume)");
+    acc = acc + v;
     o->cdf[e] = acc;
   }
   return true;

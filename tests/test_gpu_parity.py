@@ -77,7 +77,7 @@ def test_device_math_bit_identical(ctx_probe, oracle_lib):
 def run_both(problem, mode, batches=(1,), roundtrip=False, **opt):
     from oracle.pyoracle import Oracle
     g = Context(problem, mode=mode, record_histories=1, seed_roundtrip=int(roundtrip), **opt)
-    o = Oracle(problem, seed_roundtrip=roundtrip)
+    o = Oracle(problem, seed_roundtrip=roundtrip, max_segments=opt.get("max_segments", 0))
     out = []
     for b in batches:
         sg = g.run_batch(b, problem.n_particles)
