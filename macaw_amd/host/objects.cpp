@@ -1,6 +1,8 @@
 // objects.cpp -- MOOSE-shaped host objects (see objects.h).
 #include "objects.h"
 
+#include "exodus.h"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -666,6 +668,18 @@ Problem::Problem(std::unique_ptr<HitNode> root, const std::string &input) : _roo
     InputParameters p(o);
     csv = p.getBool("csv", true);
     if (p.isParamValid("file_base")) file_base = p.getString("file_base");
+    for (auto &c : o->children) {
+      if (!active(*o, *c)) continue;
+      InputParameters q(c.get());
+      const std::string type = q.getString("type", "");
+      if (type != "RayTracingExodus")
+        paramError(q, "type", "output type '" + type + "' is not supported (RayTracingExodus is)");
+      if (q.isParamValid("study") && q.getString("study") != "study" && !R.find("UserObjects/" + q.getString("study")))
+        paramError(q, "study", "no study '" + q.getString("study") + "'");
+      for (auto &w : q.getWords("execute_on"))
+        if (w != "final") paramError(q, "execute_on", "only execute_on = final is supported for ray output");
+      ray_outputs.push_back(c->name);
+    }
   }
 }
 
@@ -769,6 +783,14 @@ void Problem::run()
   std::vector<std::vector<double>> rows;
   std::map<int, std::vector<double>> sums;
   for (int step = 1; step <= n_batches; ++step) {
+    if (step == n_batches && !ray_outputs.empty()) {
+      // RayTracingExodus at execute_on = final: trace every ray of the last batch
+      mcw_options o = options;
+      o.mode = MCW_MODE_BATCHED;
+      o.record_histories = 1;
+      o.trace_histories = (int32_t)std::min<int64_t>(particles, INT32_MAX);
+      check(ctx, mcw_options_set(ctx, &o), "mcw_options_set");
+    }
     study->generateRays();
     study->execute();
     study->postExecuteStudy();
@@ -786,6 +808,20 @@ void Problem::run()
       row.push_back(tot);
     }
     rows.push_back(row);
+  }
+  if (!ray_outputs.empty()) {
+    int64_t n = 0;
+    check(ctx, mcw_trace_get(ctx, nullptr, 0, &n), "mcw_trace_get");
+    std::vector<mcw_trace_record> recs((size_t)n);
+    if (n) check(ctx, mcw_trace_get(ctx, recs.data(), recs.size(), &n), "mcw_trace_get");
+    const mcw_batch_stats &st = study->lastStats();
+    if (st.trace_records > n)
+      std::cerr << "*** Warning: ray output keeps " << n << " of " << st.trace_records << " visit records" << std::endl;
+    for (auto &name : ray_outputs) {
+      const std::string path = file_base + "_" + name + ".e";
+      write_rays_exodus(path, recs, n_batches, particles, options.master_seed ? options.master_seed : 1,
+                        path.substr(path.rfind('/') + 1));
+    }
   }
   if (csv) {
     write_outputs(rows);

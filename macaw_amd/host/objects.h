@@ -196,6 +196,7 @@ public:
   std::vector<std::unique_ptr<RayBC>> raybcs;
   std::unique_ptr<OpenMCStudy> study;
   bool csv = true;
+  std::vector<std::string> ray_outputs; // [Outputs/<name>] type = RayTracingExodus (execute_on = final)
   const OpenMCTally *tally(int id) const;
 
 private:

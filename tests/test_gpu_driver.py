@@ -67,3 +67,39 @@ def test_tally_driver_matches_oracle(tmp_path, oracle_lib):
     t2 = o.tally(2).reshape(-1, 3, 2)       # cell, nuclide bin, score
     want2 = t2[:, 0, 1]                     # nuclide H1, score total
     assert close(read_col(base + "_cell_flux.csv", "cell_flux"), want2)
+
+
+def test_driver_ray_output_matches_gold_layout_and_source_seeds(tmp_path):
+    """[Outputs/rays] type = RayTracingExodus, 30 particles x 3 batches as the
+    reference's study test: the file has the gold file's layout, rays
+    60..89 of batch 3, and every ray's source seed equals the gold's for the
+    same ray (source seeds depend only on batch, particle count and id)."""
+    import json
+    from scipy.io import netcdf_file
+    base = str(tmp_path / "out")
+    args = [build.DRIVER, "-i", os.path.join(INPUTS, "c1_hex8_reflective.i"), "Outputs/file_base=" + base,
+            "MonteCarlo/particles=30", "MonteCarlo/batches=3", "Outputs/rays/type=RayTracingExodus",
+            "Outputs/rays/study=study", "Outputs/rays/execute_on=final"]
+    r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    gold_dir = os.path.join(ROOT, "tests", "golden")
+    lay = json.load(open(os.path.join(gold_dir, "gold_rays_layout.json")))
+    gold = json.load(open(os.path.join(gold_dir, "gold_rays.json")))["records"]
+    src_gold = {g["ray_id"]: g["seed_source"] for g in gold if g["intersections"] == 0}
+    f = netcdf_file(base + "_rays.e", "r", mmap=False)
+    for k, v in lay["dimensions_fixed"].items():
+        assert f.dimensions[k] == v
+    assert set(f.variables) == set(lay["variables"])
+    names = [b"".join(x).decode().strip("\x00 ") for x in f.variables["name_elem_var"][:]]
+    assert names == lay["elem_var_names"]
+    vals = {n: f.variables[f"vals_elem_var{i + 1}eb1"][0] for i, n in enumerate(names)}
+    ray = vals["ray_id"].astype(int)
+    assert sorted(set(ray)) == list(range(60, 90))
+    assert f.dimensions["num_nodes"] == f.dimensions["num_elem"] + 30
+    assert f.variables["time_whole"][0] == 3.0
+    for rid in range(60, 90):
+        sel = ray == rid
+        assert list(vals["intersections"][sel]) == list(range(sel.sum()))
+        assert np.all(vals["aux_seed_source"][sel].astype(np.uint64) == np.uint64(src_gold[rid]))
+        assert np.all(vals["aux_id"][sel] == rid - 59)
+    f.close()

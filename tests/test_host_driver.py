@@ -57,6 +57,25 @@ def test_reference_parameter_errors(name, override, message):
     assert message in r.stderr, r.stderr
 
 
+RAY_OUTPUT = ["Outputs/rays/type=RayTracingExodus", "Outputs/rays/study=study", "Outputs/rays/execute_on=final"]
+
+
+def test_ray_output_block_validates():
+    r = run(["-i", os.path.join(INPUTS, "c1_hex8_reflective.i")] + RAY_OUTPUT + ["--check-input"])
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("override,message", [
+    ("Outputs/rays/type=Exodus", "output type 'Exodus' is not supported"),
+    ("Outputs/rays/execute_on=timestep_end", "only execute_on = final is supported for ray output"),
+    ("Outputs/rays/study=other", "no study 'other'"),
+])
+def test_ray_output_block_errors(override, message):
+    r = run(["-i", os.path.join(INPUTS, "c1_hex8_reflective.i")] + RAY_OUTPUT + [override, "--check-input"])
+    assert r.returncode == 1
+    assert message in r.stderr, r.stderr
+
+
 def test_missing_material_is_reported_at_initial_setup(tmp_path):
     # CollisionKernel::initialSetup (CollisionKernel.C:84-90): a material id
     # that does not exist; found before the device is touched.

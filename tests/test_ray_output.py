@@ -75,3 +75,38 @@ def test_writer_layout_matches_gold(tmp_path):
     assert np.all(vals["aux_weight"] == 1.0) and np.all(vals["aux_particle_type"] == 0.0)
     assert vals["aux_seed_source"][0] == float(init_seed(61, 1))
     f.close()
+
+
+def test_cpp_writer_equals_python_writer(tmp_path):
+    """The driver's C++ writer and the Python writer produce the same file
+    contents (variables, dimensions, attributes) for the same records."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "harness")
+    r = subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(root, "include"),
+                        os.path.join(root, "tests", "exodus_harness.cpp"),
+                        os.path.join(root, "macaw_amd", "host", "exodus.cpp"), "-o", exe],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+    t = synthetic_trace()
+    rec = str(tmp_path / "rec.bin")
+    with open(rec, "wb") as fh:
+        fh.write(np.int64(t.size).tobytes())
+        fh.write(t.tobytes())
+    a, b = str(tmp_path / "cpp.e"), str(tmp_path / "py.e")
+    r = subprocess.run([exe, rec, a, "3", "30", "1"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+    write_rays_exodus(b, t, batch=3, n_particles=30, master_seed=1)
+    fa, fb = netcdf_file(a, "r", mmap=False), netcdf_file(b, "r", mmap=False)
+    assert dict(fa.dimensions) == dict(fb.dimensions)
+    assert set(fa.variables) == set(fb.variables)
+    for k in fa.variables:
+        va, vb = fa.variables[k], fb.variables[k]
+        assert va.dimensions == vb.dimensions and va.typecode() == vb.typecode(), k
+        assert np.array_equal(va[:], vb[:]), k
+        assert va._attributes == vb._attributes, k
+    for k, v in fb._attributes.items():
+        if k != "title":
+            assert fa._attributes[k] == v, k
+    fa.close()
+    fb.close()
