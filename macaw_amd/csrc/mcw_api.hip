@@ -1314,9 +1314,11 @@ int mcw_debug_xs_probe(mcw_ctx *c, int32_t nuc, const double *E, int64_t n, int3
 // request is admitted when an in-flight slot is free and its memory (flux
 // tally + its share of a round's source bank, min(N, round budget) x 76 B)
 // fits the budget.  Each mesh runs rounds: one fused launch (k_source_req +
-// the persistent kernel in its request-server form) covering a slice of
-// every active request on that mesh, at most `round_budget` histories in
-// all, taken round-robin so no request starves.  A mesh has two round slots,
+// the persistent kernel in its request-server form) over at most
+// `round_budget` histories, filled from the mesh's active requests by
+// shortest remaining work first (a request admitted more than `age_ms` ago
+// goes first, so none starves).  Few requests per round keep few tallies
+// hot: a round spread over many requests' tallies runs markedly slower.  A mesh has two round slots,
 // each with its own stream, counters and source bank, so the next round's
 // workgroups fill the compute units that the current round's tail (its last,
 // longest histories) leaves idle; requests that arrive meanwhile join at the
@@ -1374,7 +1376,6 @@ struct ServeMesh {
   bool fill_recorded = false;
   DevBuf stage;
   int blocks = 0;
-  size_t cursor = 0; // round-robin start among the mesh's active requests
   std::vector<std::unique_ptr<DevBuf>> tallies; // n_elem + REQ_TAIL doubles
   std::vector<double *> pinned;                 // n_elem + REQ_TAIL doubles
   std::vector<int> free_slots;
@@ -1412,6 +1413,7 @@ struct mcw_server {
   int32_t max_inflight = 8;
   int64_t bank_bytes = 0, used = 0;
   int64_t round_budget = (int64_t)1 << 21;
+  double age_ms = 1000.0; // admitted longer ago than this: served first
   std::chrono::steady_clock::time_point t0;
   double now() const
   {
@@ -1477,14 +1479,21 @@ bool launch_round(mcw_server *s, int mi, RoundSlot &R)
   for (ServeReq *q : s->active)
     if (q->r.mesh_slot == mi && q->scheduled < q->r.n_particles) mine.push_back(q);
   if (mine.empty()) return false;
-  const int64_t quota = std::max<int64_t>(s->round_budget / (int64_t)mine.size(), 65536);
-  const size_t start = m.cursor % mine.size();
+  // shortest remaining work first; a request admitted more than age_ms ago
+  // goes ahead of the others (oldest first), so none starves
+  const double now = s->now();
+  std::stable_sort(mine.begin(), mine.end(), [&](const ServeReq *a, const ServeReq *b) {
+    const bool oa = now - a->t_admit > s->age_ms, ob = now - b->t_admit > s->age_ms;
+    if (oa != ob) return oa;
+    if (oa) return a->t_admit < b->t_admit;
+    return a->r.n_particles - a->scheduled < b->r.n_particles - b->scheduled;
+  });
   R.round.clear();
   int64_t total = 0;
   int32_t n_sl = 0;
   for (size_t j = 0; j < mine.size() && total < s->round_budget; ++j) {
-    ServeReq *q = mine[(start + j) % mine.size()];
-    const int64_t cnt = std::min(std::min(q->r.n_particles - q->scheduled, quota), s->round_budget - total);
+    ServeReq *q = mine[j];
+    const int64_t cnt = std::min(q->r.n_particles - q->scheduled, s->round_budget - total);
     ReqSlice &r = R.h_slices[n_sl++];
     memset(&r, 0, sizeof r);
     r.first = total;
@@ -1513,7 +1522,6 @@ bool launch_round(mcw_server *s, int mi, RoundSlot &R)
     q->scheduled += cnt;
     total += cnt;
   }
-  m.cursor = start + (size_t)n_sl;
   hipStream_t st = R.stream;
   if (R.sb_cap < total) {
     for (int k = 0; k < 10; ++k) R.sb[k].alloc((size_t)s->round_budget * (k < 8 ? 8 : 4) + 64);

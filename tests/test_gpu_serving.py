@@ -93,8 +93,9 @@ def test_requests_match_oracle():
 
 
 def test_many_requests_share_rounds():
-    """More active requests on one mesh than a round holds at the minimum
-    quota: rounds are capped at the bank size and served round-robin."""
+    """More active requests on one mesh than one round holds: rounds are
+    capped at the bank size, and every request completes with the result it
+    has alone."""
     probs = meshes()
     ctx = contexts(probs)
     src = P.box_source((0.2, 0.2, 0.2), (0.8, 0.8, 0.8), 1e6)
@@ -114,3 +115,21 @@ def test_many_requests_share_rounds():
         assert rel.max() <= 1e-10
         assert st["segments"] == mixed[rid][0]["segments"]
         assert st["histories"] == n
+
+
+def test_shortest_remaining_work_first():
+    """A large request and small ones admitted together: the small ones
+    finish first, the large one still completes with its own result."""
+    probs = meshes()
+    ctx = contexts(probs)
+    src = P.box_source((0.2, 0.2, 0.2), (0.8, 0.8, 0.8), 1e6)
+    s = Server(ctx, max_inflight=16)
+    s.submit(500, 0, 3_000_000, src)
+    for i in range(5):
+        s.submit(501 + i, 0, 20_000, src)
+    got = s.run_all()
+    s.close()
+    big = got[500][0]
+    for i in range(5):
+        assert got[501 + i][0]["t_done_ms"] <= big["t_done_ms"]
+    assert big["histories"] == 3_000_000 and big["segments"] > 0
