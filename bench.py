@@ -160,11 +160,18 @@ def tally_error(fp64_ctx, mixed_ctx, n_batches):
     sig = np.sqrt(var)
     ok = sig > 0
     z = np.abs(m[ok] - f[ok]) / n_batches / sig[ok]
+    # elements whose standard error is itself well estimated (relative
+    # standard error below 10%): with few batches an element can show a
+    # near-zero sample variance by chance
+    good = ok & (sig < 0.1 * np.abs(mean))
+    zg = np.abs(m[good] - f[good]) / n_batches / sig[good]
     # one batch gives no standard error: the sigma figures need two or more
     return {"total_rel": float(abs(m.sum() - f.sum()) / f.sum()), "elem_rel_max": float(rel.max()),
             "elem_rel_median": float(np.median(rel)), "elem_rel_p99": float(np.percentile(rel, 99)),
             "max_sigma": float(z.max()) if z.size else None,
-            "frac_beyond_3sigma": float((z > 3).mean()) if z.size else None, "batches": n_batches}
+            "frac_beyond_3sigma": float((z > 3).mean()) if z.size else None,
+            "max_sigma_rse_below_0.1": float(zg.max()) if zg.size else None,
+            "elements_rse_below_0.1": int(good.sum()), "batches": n_batches}
 
 
 def timed_steps(ctx, N, steps, warmup, label):
