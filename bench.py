@@ -21,8 +21,11 @@ difference from the FP64 run, the plain event-based GPU path on the same
 batch, the dominant kernel's roofline (algorithmic bytes per launch over its
 HIP-event duration), the serving mix (c2 runs only: 2000 requests over
 resident 100k / 1M / 4M-tet meshes, Poisson arrivals at 70% of measured
-capacity, p50/p99 latency) and a CPU baseline (the FP64 oracle, std::thread
-on the host cores, on a bounded sample of the same histories).
+capacity, p50/p99 latency), an eigenvalue run (c2 runs only: C2's mesh with
+a fissile nuclide, 1e7 histories per generation, 2 inactive + 3 active
+batches, each with its fission-bank sort and resampling) and a CPU baseline
+(the FP64 oracle, std::thread on the host cores, on a bounded sample of the
+same histories).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c5]
 """
@@ -134,6 +137,44 @@ def bench_serving(args, tuning):
     return out
 
 
+def bench_eigen(args, tuning):
+    """Eigenvalue leg: C2's mesh and materials with a fissile nuclide added
+    (keff near 1), n histories per generation, 2 inactive + 3 active
+    batches; each batch is transport plus the batch close (keff reduction,
+    fission-bank sort, resampling into the next source)."""
+    from macaw_amd import abi, problems as P
+    from macaw_amd.lib import Context
+
+    n = args.eigen_histories
+    t = time.perf_counter()
+    prob = P.k2_tet_eigen(n_particles=n)
+    ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=2,
+                  lanes_min=tuning.get("lanes_min", 0), block_size=tuning.get("block_size", 0))
+    log(f"eigen: {prob.name} ready ({time.perf_counter() - t:.1f}s)")
+    ctx.run_batch(1, n)  # warm: allocations and code objects
+    rows = []
+    t = time.perf_counter()
+    for b in range(1, 6):
+        st = ctx.run_batch(b, n)
+        k = ctx.keff()
+        rows.append((st, k))
+        log(f"eigen batch {b}: {st['time_ms']:.1f} ms transport + {k['close_ms']:.1f} ms close, "
+            f"{k['fission_sites']} sites, k_gen {k['k_generation']:.5f}, keff {k['keff']:.5f} +- {k['keff_std']:.5f}")
+    wall = time.perf_counter() - t
+    ctx.close()
+    segs = sum(st["segments"] for st, _ in rows)
+    k = rows[-1][1]
+    return {"histories_per_s": 5 * n / wall, "segments_per_s": segs / wall, "ms_per_batch": wall / 5 * 1e3,
+            "transport_ms_per_batch": sum(st["time_ms"] for st, _ in rows) / 5,
+            "close_ms_per_batch": sum(kk["close_ms"] for _, kk in rows) / 5,
+            "keff": k["keff"], "keff_std": k["keff_std"], "k_generation": [kk["k_generation"] for _, kk in rows],
+            "fission_sites_per_history": sum(kk["fission_sites"] for _, kk in rows) / (5 * n),
+            "segments_per_history": segs / (5 * n), "histories_per_batch": n, "batches": 5, "inactive": 2,
+            "workload": f"k2_tet4_{prob.mesh.n_elem // 1000}k_eigen",
+            "description": "C2 mesh and materials plus a fissile nuclide (A=235, 85% of absorption is fission, "
+                           "nu 2.43, Watt spectrum), vacuum, batch 1 from a volumetric Watt source"}
+
+
 WORKLOADS = {
     # name: (generator, default histories per step, description)
     "c2": ("c2_tet_vacuum", 10_000_000, "1M TET4 unit cube, vacuum, 4 materials / 4 nuclides x 50k points"),
@@ -226,6 +267,8 @@ def main():
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered load as a fraction of capacity")
     ap.add_argument("--serve-inflight", type=int, default=32)
     ap.add_argument("--serve-bank-gib", type=float, default=4.0)
+    ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
+    ap.add_argument("--eigen-histories", type=int, default=10_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
                          "bytes): the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling")
@@ -313,6 +356,10 @@ def main():
         pctx.close()
     ctx.close()
 
+    eigen = None
+    if not args.no_eigen and args.workload == "c2":
+        eigen = bench_eigen(args, tuning)
+
     serving = None
     if not args.no_serve and args.workload == "c2":
         serving = bench_serving(args, tuning)
@@ -351,6 +398,7 @@ def main():
         "mixed_speedup_vs_plain": (mixed["histories_per_s"] / plain["histories_per_s"]) if (plain and mixed) else None,
         "roofline": roof,
         "serving": serving,
+        "eigenvalue": eigen,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

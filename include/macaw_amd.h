@@ -74,6 +74,13 @@ enum { MCW_MODE_PLAIN = 0, MCW_MODE_BATCHED = 1 };
  * tallies stay FP64.  Other meshes and the PLAIN mode run FP64. */
 enum { MCW_PREC_FP64 = 0, MCW_PREC_MIXED = 1 };
 
+/* Run modes (settings.xml <run_mode>).  EIGENVALUE: batch 1 starts from the
+ * external source, each later batch from the fission sites of the one before
+ * it, resampled to n_particles as OpenMCStudy::synchronizeBanks does
+ * (src/userobjects/OpenMCStudy.C:454-607); keff from the track-length
+ * estimator; tallies score in active batches only. */
+enum { MCW_RUN_FIXED_SOURCE = 0, MCW_RUN_EIGENVALUE = 1 };
+
 /* Per-history fates. */
 enum { MCW_FATE_ALIVE = 0, MCW_FATE_LEAKED = 1, MCW_FATE_ABSORBED = 2, MCW_FATE_CUTOFF = 3,
        MCW_FATE_LOST = 4 };
@@ -107,13 +114,19 @@ typedef struct {
   int32_t trace_histories;  /* BATCHED with record_histories: keep one record per element
                                visit of the first trace_histories histories of a batch
                                (mcw_trace_get), as RayTracingExodus writes them */
-  int32_t reserved[3];
+  int32_t run_mode;         /* MCW_RUN_* (EIGENVALUE needs mode BATCHED) */
+  int32_t n_inactive;       /* EIGENVALUE: batches 1..n_inactive are inactive (settings
+                               <inactive>): no tally scores, keff not averaged */
+  int32_t reserved;
 } mcw_options;
 
 /* One synthetic continuous-energy nuclide: pointwise lin-lin tables on an
  * increasing energy grid [eV]; microscopic cross sections in barns; elastic
  * scattering isotropic in the centre-of-mass frame off a target at rest of
- * mass ratio awr. */
+ * mass ratio awr.  A nuclide with a fission table is fissionable: fission is
+ * part of absorption, and in EIGENVALUE runs a collision with it banks
+ * fission sites (nu(E) = nu[0] + nu[1] E neutrons per fission, isotropic,
+ * energies from a Watt spectrum with a = watt[0] eV, b = watt[1] 1/eV). */
 typedef struct {
   int32_t n_points;
   double awr;
@@ -121,7 +134,9 @@ typedef struct {
   const double *total;
   const double *absorption;
   const double *scatter;
-  const double *fission; /* may be NULL (no fission) */
+  const double *fission; /* may be NULL (not fissionable) */
+  double nu[2];
+  double watt[2];
 } mcw_nuclide;
 
 /* A material: id (as in materials.xml) and atom densities [1/(b cm)]. */
@@ -241,8 +256,54 @@ int mcw_tally_shape(mcw_ctx *ctx, int32_t id, int64_t *n_filter_bins, int64_t *n
 
 /* One batch: generateRays + execute + postExecuteStudy (OpenMCStudy.C:218-402)
  * for particles 1..n_particles of `batch` (1-based), then VALUE -> SUM,
- * SUM_SQ as OpenMC's finalize_batch. */
+ * SUM_SQ as OpenMC's finalize_batch.  EIGENVALUE runs go batch 1, 2, ... with
+ * one n_particles (batch 1 restarts the run); each batch ends with the
+ * fission bank sorted and resampled into the next batch's source. */
 int mcw_run_batch(mcw_ctx *ctx, int64_t batch, int64_t n_particles, mcw_batch_stats *out);
+
+/* ---- eigenvalue runs ------------------------------------------------------ */
+
+/* A fission source site (OpenMC SourceSite): position, direction, energy,
+ * the history that banked it (index within its batch, from 0), its index
+ * among that history's sites, and the caller element it lies in. */
+typedef struct {
+  double r[3];
+  double u[3];
+  double E;
+  int64_t parent;
+  int32_t progeny;
+  int32_t element;
+} mcw_site;
+
+/* The keff state after the last EIGENVALUE batch (OpenMC
+ * calculate_generation_keff / calculate_average_keff, called from
+ * OpenMCStudy::finalizeGeneration, OpenMCStudy.C:404-452). */
+typedef struct {
+  int64_t batch;          /* last batch run */
+  int64_t fission_sites;  /* sites banked in it (the fission bank holds at most 3 n_particles) */
+  int64_t sites_dropped;  /* sites that found the bank full (OpenMC warns and drops them) */
+  int32_t n_active;       /* active batches so far */
+  int32_t pad;
+  double k_generation;    /* the batch's track-length estimate: sum w l nuSigma_f / n_particles */
+  double k_collision;     /* the batch's collision estimate: sum w nuSigma_f / Sigma_t / n */
+  double k_absorption;    /* the batch's absorption estimate: sum w nu sigma_f / sigma_a / n */
+  double keff;            /* mean of k_generation over active batches (inactive: the last
+                             k_generation); used for the next batch's fission yield */
+  double keff_std;        /* standard deviation of that mean (0 below two active batches) */
+  double keff_used;       /* the keff the batch divided the fission yield by */
+  double close_ms;        /* wall time of the batch close: keff reduction, bank sort, resampling */
+} mcw_keff_result;
+
+int mcw_keff_get(mcw_ctx *ctx, mcw_keff_result *out);
+
+/* The last EIGENVALUE batch's fission bank in OpenMC's sorted order (parent,
+ * then progeny).  *n_out gets the number of sites; at most `max` are copied
+ * (out may be NULL). */
+int mcw_fission_bank_get(mcw_ctx *ctx, mcw_site *out, size_t max, int64_t *n_out);
+
+/* The source of the next EIGENVALUE batch: n_particles sites resampled from
+ * the fission bank, each with the parent and progeny of the site it copies. */
+int mcw_source_bank_get(mcw_ctx *ctx, mcw_site *out, size_t max, int64_t *n_out);
 
 /* Read a tally's results kind (MCW_RESULT_*) into out[n_filter_bins*n_score_bins]. */
 int mcw_tally_get(mcw_ctx *ctx, int32_t id, int32_t kind, double *out, size_t n);

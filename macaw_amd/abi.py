@@ -17,6 +17,7 @@ ENERGY_MONO, ENERGY_WATT = 0, 1
 MODE_PLAIN, MODE_BATCHED = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 FATE_ALIVE, FATE_LEAKED, FATE_ABSORBED, FATE_CUTOFF, FATE_LOST = 0, 1, 2, 3, 4
+RUN_FIXED_SOURCE, RUN_EIGENVALUE = 0, 1
 
 SCORE_NAMES = {"flux": SCORE_FLUX, "total": SCORE_TOTAL, "absorption": SCORE_ABSORPTION,
                "scatter": SCORE_SCATTER, "fission": SCORE_FISSION}
@@ -37,12 +38,13 @@ class Options(C.Structure):
                 ("max_segments", C.c_int64), ("master_seed", C.c_uint64),
                 ("sort_cadence", C.c_int32), ("lanes_min", C.c_int32),
                 ("waves_per_eu", C.c_int32), ("use_graph", C.c_int32), ("trace_histories", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("run_mode", C.c_int32), ("n_inactive", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Nuclide(C.Structure):
     _fields_ = [("n_points", C.c_int32), ("awr", C.c_double), ("energy", _dp), ("total", _dp),
-                ("absorption", _dp), ("scatter", _dp), ("fission", _dp)]
+                ("absorption", _dp), ("scatter", _dp), ("fission", _dp), ("nu", C.c_double * 2),
+                ("watt", C.c_double * 2)]
 
 
 class Material(C.Structure):
@@ -88,6 +90,28 @@ def trace_dtype():
 
 
 EVENT_CROSS, EVENT_REFLECT, EVENT_LEAK, EVENT_ABSORB, EVENT_CUTOFF, EVENT_LOST = 0, 1, 2, 4, 5, 6
+
+
+class Site(C.Structure):
+    _fields_ = [("r", C.c_double * 3), ("u", C.c_double * 3), ("E", C.c_double),
+                ("parent", C.c_int64), ("progeny", C.c_int32), ("element", C.c_int32)]
+
+
+def site_dtype():
+    """numpy dtype matching mcw_site."""
+    import numpy as np
+    return np.dtype([("r", np.float64, 3), ("u", np.float64, 3), ("E", np.float64),
+                     ("parent", np.int64), ("progeny", np.int32), ("element", np.int32)])
+
+
+class KeffResult(C.Structure):
+    _fields_ = [("batch", C.c_int64), ("fission_sites", C.c_int64), ("sites_dropped", C.c_int64),
+                ("n_active", C.c_int32), ("pad", C.c_int32), ("k_generation", C.c_double),
+                ("k_collision", C.c_double), ("k_absorption", C.c_double), ("keff", C.c_double),
+                ("keff_std", C.c_double), ("keff_used", C.c_double), ("close_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
 class History(C.Structure):

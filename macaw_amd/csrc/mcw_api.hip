@@ -32,7 +32,22 @@ hipError_t launch_collide(unsigned blocks, hipStream_t s, const Params &P, const
                           const BankView &out, int qout);
 hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int block, hipStream_t s, const Params &P,
                              const SourceBank &S);
-const void *persistent_kernel(int ns, int wpe, bool rec, bool p32);
+const void *persistent_kernel(int ns, int wpe, bool rec, bool p32, bool eig);
+hipError_t launch_tree_sum(hipStream_t s, const double *in, int64_t n, double *scratch, double *out);
+hipError_t fission_sort_temp_bytes(int64_t n, size_t *bytes);
+hipError_t launch_fission_sort(hipStream_t s, const FissionView &F, int64_t n, int end_bit, uint64_t *keys_tmp,
+                               uint32_t *perm, uint32_t *idx, void *tmp, size_t tmp_bytes);
+hipError_t launch_sync_count(hipStream_t s, int64_t total, int64_t base, double p_sample, uint64_t seed0,
+                             int64_t *counts);
+hipError_t scan_temp_bytes(int64_t n, size_t *bytes);
+hipError_t launch_scan(hipStream_t s, const int64_t *in, int64_t *out, int64_t n, void *tmp, size_t tmp_bytes);
+hipError_t launch_sync_scatter(hipStream_t s, const FissionView &F, const uint32_t *perm, const int64_t *counts,
+                               const int64_t *offsets, int64_t total, int64_t n, const SourceBank &out,
+                               uint64_t *out_key, int64_t next_batch, uint64_t master, int roundtrip);
+hipError_t launch_sites_out(hipStream_t s, const double *x, const double *y, const double *z, const double *u,
+                            const double *v, const double *w, const double *E, const int32_t *elem,
+                            const uint64_t *key, const uint32_t *perm, int64_t n, const int32_t *orig_of_int,
+                            mcw_site *out);
 hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1);
 hipError_t launch_reset_work(hipStream_t s, Counters *C);
 hipError_t launch_fill(hipStream_t s, double *p, int64_t n, double v);
@@ -105,6 +120,7 @@ struct HostNuclide {
   double awr;
   std::vector<double> E;
   std::vector<double> t, a, s, f;
+  double nu[2] = {0.0, 0.0}, watt[2] = {0.0, 0.0};
 };
 
 struct HostMaterial {
@@ -153,7 +169,7 @@ struct mcw_ctx {
   // device problem
   DevBuf plane, erec, mtet, split, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf cell_start, cell_elem;
-  DevBuf grid_off, npts, rec, fission, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
+  DevBuf grid_off, npts, rec, fission, fis, fissionable, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
   Counters *h_counters = nullptr; // pinned
   // source bank, live/collision banks of the event-based path, records
@@ -178,6 +194,18 @@ struct mcw_ctx {
   int64_t last_batch = 0;
   double e_min = 0, e_max = 0;
   int32_t n_bc = 0;
+
+  // eigenvalue runs: the fission bank (9 arrays: x y z u v w E elem key), its
+  // sort (keys, slot permutation), per-history keff scores, the resampling's
+  // counts and offsets, and the next batch's source (9 arrays + key)
+  DevBuf fb[9], fb_keys2, fb_perm, fb_perm2, fb_tmp, k_hist, k_red, sync_cnt, sync_off, scan_tmp, gen[10];
+  int64_t fb_cap = 0, gen_cap = 0, khist_cap = 0, sync_cap = 0;
+  size_t fb_tmp_bytes = 0, scan_tmp_bytes = 0;
+  int64_t eig_batch = 0, eig_n = 0, eig_sites = 0;
+  bool eig_src_ready = false;
+  double keff = 1.0, k_sum = 0.0, k_sum_sq = 0.0;
+  int32_t n_active = 0;
+  mcw_keff_result kres{};
 
   Params base{};
 };
@@ -234,6 +262,24 @@ void check_angle(const mcw_source &src)
   } else {
     require(src.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
   }
+}
+
+// OpenMC init_seed on the host: id * 152917 steps of the LCG from master +
+// stream (Brown's arbitrary-stride skip-ahead).
+uint64_t mcwhost_init_seed(int64_t id, uint64_t stream, uint64_t master)
+{
+  const uint64_t M = 6364136223846793005ull, C = 1442695040888963407ull;
+  uint64_t n = (uint64_t)id * 152917ull, g = M, cc = C, g_new = 1, c_new = 0;
+  while (n) {
+    if (n & 1) {
+      g_new *= g;
+      c_new = c_new * g + cc;
+    }
+    cc = (g + 1) * cc;
+    g *= g;
+    n >>= 1;
+  }
+  return g_new * (master + stream) + c_new;
 }
 
 // Upload everything the kernels read; validate the problem as
@@ -374,6 +420,15 @@ void prepare(mcw_ctx *c, bool need_source = true)
   c->rec.upload(recs, s);
   if (any_fission) c->fission.upload(fis, s);
   c->awr.upload(aw, s);
+  // fission yield and spectrum per nuclide, and which nuclides are fissionable
+  std::vector<double4> fpar;
+  std::vector<int32_t> fable;
+  for (auto &n : c->nucs) {
+    fpar.push_back(make_double4(n.nu[0], n.nu[1], n.watt[0], n.watt[1]));
+    fable.push_back(n.f.empty() ? 0 : 1);
+  }
+  c->fis.upload(fpar, s);
+  c->fissionable.upload(fable, s);
   // unionized grid, per-nuclide index map and log-bucket table
   std::sort(uni.begin(), uni.end());
   uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
@@ -470,6 +525,8 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P.xs.n_points = c->npts.as<int32_t>();
   P.xs.rec = c->rec.as<double4>();
   P.xs.fission = any_fission ? c->fission.as<double>() : nullptr;
+  P.xs.fis = c->fis.as<double4>();
+  P.xs.fissionable = c->fissionable.as<int32_t>();
   P.xs.awr = c->awr.as<double>();
   P.xs.U = c->uni.as<double>();
   P.xs.uidx = c->uidx.as<int32_t>();
@@ -732,12 +789,134 @@ SourceBank sort_sites(mcw_ctx *c, const Params &P, const SourceBank &S, int64_t 
 }
 
 // Resident workgroups of the persistent kernel over the whole device.
-int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, bool p32, int block)
+int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, bool p32, int block, bool eig = false)
 {
   int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns, wpe, rec, p32), block, 0));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns, wpe, rec, p32, eig), block, 0));
   if (per_cu < 1) per_cu = 1;
   return c->n_cu * per_cu;
+}
+
+// Eigenvalue batches of n histories: the fission bank (3 n sites, as OpenMC
+// sizes it) with its sort and resampling scratch, the per-history keff
+// scores, and the next batch's source.
+void ensure_fission(mcw_ctx *c, int64_t n)
+{
+  const int64_t cap = 3 * n;
+  require(cap < ((int64_t)1 << 31), "eigenvalue batches hold at most 715827882 particles");
+  if (c->fb_cap < cap) {
+    for (int k = 0; k < 9; ++k) c->fb[k].alloc((size_t)cap * (k == 7 ? 4 : 8));
+    c->fb_keys2.alloc((size_t)cap * 8);
+    c->fb_perm.alloc((size_t)cap * 4);
+    c->fb_perm2.alloc((size_t)cap * 4);
+    HIPCHK(fission_sort_temp_bytes(cap, &c->fb_tmp_bytes));
+    c->fb_tmp.alloc(c->fb_tmp_bytes);
+    c->sync_cnt.alloc((size_t)cap * 8);
+    c->sync_off.alloc((size_t)cap * 8);
+    HIPCHK(scan_temp_bytes(cap, &c->scan_tmp_bytes));
+    c->scan_tmp.alloc(c->scan_tmp_bytes);
+    c->fb_cap = cap;
+  }
+  if (c->khist_cap < n) {
+    c->k_hist.alloc((size_t)n * 3 * 8);
+    c->k_red.alloc(((size_t)(n + 1023) / 1024 * 2 + 8) * 8);
+    c->khist_cap = n;
+  }
+  if (c->gen_cap < n) {
+    for (int k = 0; k < 10; ++k) c->gen[k].alloc((size_t)n * (k == 8 ? 4 : 8));
+    c->gen_cap = n;
+  }
+}
+
+// The fission bank of batches of n histories holds 3 n sites.
+FissionView fission_view(mcw_ctx *c, int64_t n)
+{
+  FissionView F;
+  F.x = c->fb[0].as<double>(); F.y = c->fb[1].as<double>(); F.z = c->fb[2].as<double>();
+  F.u = c->fb[3].as<double>(); F.v = c->fb[4].as<double>(); F.w = c->fb[5].as<double>();
+  F.E = c->fb[6].as<double>();
+  F.elem = c->fb[7].as<int32_t>();
+  F.key = c->fb[8].as<uint64_t>();
+  F.cap = std::min<int64_t>(3 * n, c->fb_cap);
+  F.k_hist = c->k_hist.as<double>();
+  F.keff = c->keff;
+  return F;
+}
+
+// The next batch's source (histories h0 ..) as a source bank.
+SourceBank gen_bank(mcw_ctx *c, int64_t h0)
+{
+  SourceBank S;
+  S.x = c->gen[0].as<double>() + h0; S.y = c->gen[1].as<double>() + h0; S.z = c->gen[2].as<double>() + h0;
+  S.u = c->gen[3].as<double>() + h0; S.v = c->gen[4].as<double>() + h0; S.w = c->gen[5].as<double>() + h0;
+  S.E = c->gen[6].as<double>() + h0;
+  S.seed = c->gen[7].as<uint64_t>() + h0;
+  S.elem = c->gen[8].as<int32_t>() + h0;
+  return S;
+}
+
+// End of an eigenvalue batch (OpenMCStudy::postExecuteStudy ->
+// finalize_generation, OpenMCStudy.C:374-452): reduce the keff scores, update
+// keff as calculate_generation_keff / calculate_average_keff do, sort the
+// fission bank and resample it into the next batch's source.
+void eigen_close(mcw_ctx *c, const Params &P, int64_t batch, int64_t n, bool active)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = c->stream;
+  double *kr = c->k_red.as<double>();
+  double *scratch = kr + 8;
+  for (int k = 0; k < 3; ++k) HIPCHK(launch_tree_sum(s, P.fb.k_hist + k * n, n, scratch, kr + k));
+  double ks[3];
+  HIPCHK(hipMemcpyAsync(ks, kr, sizeof ks, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(c->h_counters, P.cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const int64_t requested = (int64_t)c->h_counters->fission_sites;
+  const int64_t total = std::min<int64_t>(requested, 3 * n);
+  mcw_keff_result &K = c->kres;
+  memset(&K, 0, sizeof K);
+  K.batch = batch;
+  K.fission_sites = total;
+  K.sites_dropped = requested - total;
+  K.keff_used = c->keff;
+  K.k_generation = ks[0] / (double)n;
+  K.k_collision = ks[1] / (double)n;
+  K.k_absorption = ks[2] / (double)n;
+  if (active) {
+    ++c->n_active;
+    c->k_sum += K.k_generation;
+    c->k_sum_sq += K.k_generation * K.k_generation;
+    const double na = (double)c->n_active;
+    c->keff = c->k_sum / na;
+    K.keff_std = c->n_active > 1 ? std::sqrt((c->k_sum_sq / na - c->keff * c->keff) / (na - 1.0)) : 0.0;
+  } else {
+    c->keff = K.k_generation;
+  }
+  K.keff = c->keff;
+  K.n_active = c->n_active;
+  c->eig_batch = batch;
+  c->eig_n = n;
+  c->eig_sites = total;
+  c->eig_src_ready = false;
+  require(total > 0, "No fission sites sampled on any process, cannot sample a new batch");
+  // OpenMC order, then synchronizeBanks
+  int hb = 0;
+  while (((int64_t)1 << hb) < n) ++hb;
+  const FissionView F = fission_view(c, n);
+  HIPCHK(launch_fission_sort(s, F, total, PROGENY_BITS + hb, c->fb_keys2.as<uint64_t>(), c->fb_perm.as<uint32_t>(),
+                             c->fb_perm2.as<uint32_t>(), c->fb_tmp.p, c->fb_tmp_bytes));
+  const int64_t base = total < n ? n / total : 0;
+  const int64_t needed = total < n ? n % total : n;
+  const double p_sample = (double)needed / (double)total;
+  uint64_t seed0 = mcwhost_init_seed(batch, 0, c->opt.master_seed);
+  HIPCHK(launch_sync_count(s, total, base, p_sample, seed0, c->sync_cnt.as<int64_t>()));
+  HIPCHK(launch_scan(s, c->sync_cnt.as<int64_t>(), c->sync_off.as<int64_t>(), total, c->scan_tmp.p,
+                     c->scan_tmp_bytes));
+  HIPCHK(launch_sync_scatter(s, F, c->fb_perm.as<uint32_t>(), c->sync_cnt.as<int64_t>(), c->sync_off.as<int64_t>(),
+                             total, n, gen_bank(c, 0), c->gen[9].as<uint64_t>(), batch + 1, c->opt.master_seed,
+                             c->opt.seed_roundtrip));
+  HIPCHK(hipStreamSynchronize(s));
+  c->eig_src_ready = true;
+  K.close_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // Event-based transport of histories hist_begin .. hist_end (sites already
@@ -845,6 +1024,8 @@ int mcw_create(mcw_ctx **out, const mcw_options *opts)
     require(c->opt.device >= 0 && c->opt.device < ndev, "no such HIP device " + std::to_string(c->opt.device));
     require(c->opt.sort_cadence >= 0, "sort_cadence must not be negative");
     require(c->opt.use_graph == 0 || c->opt.use_graph == 1, "use_graph must be 0 or 1");
+    require(c->opt.run_mode == MCW_RUN_FIXED_SOURCE || c->opt.run_mode == MCW_RUN_EIGENVALUE, "unknown run_mode");
+    require(c->opt.n_inactive >= 0, "n_inactive must not be negative");
     HIPCHK(hipSetDevice(c->opt.device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
@@ -887,6 +1068,8 @@ int mcw_options_set(mcw_ctx *c, const mcw_options *o)
     require(o->sort_cadence >= 0, "sort_cadence must not be negative");
     require(o->use_graph == 0 || o->use_graph == 1, "use_graph must be 0 or 1");
     require(o->trace_histories >= 0, "trace_histories must not be negative");
+    require(o->run_mode == MCW_RUN_FIXED_SOURCE || o->run_mode == MCW_RUN_EIGENVALUE, "unknown run_mode");
+    require(o->n_inactive >= 0, "n_inactive must not be negative");
     const bool reprep = o->max_segments != c->opt.max_segments || o->lanes_min != c->opt.lanes_min;
     c->opt = *o;
     if (c->opt.master_seed == 0) c->opt.master_seed = 1;
@@ -950,7 +1133,17 @@ int mcw_xs_set(mcw_ctx *c, int32_t n, const mcw_nuclide *nuc)
       h.t.assign(q.total, q.total + q.n_points);
       h.a.assign(q.absorption, q.absorption + q.n_points);
       h.s.assign(q.scatter, q.scatter + q.n_points);
-      if (q.fission) h.f.assign(q.fission, q.fission + q.n_points);
+      if (q.fission) {
+        h.f.assign(q.fission, q.fission + q.n_points);
+        for (int k = 0; k < 2; ++k) {
+          require(std::isfinite(q.nu[k]) && std::isfinite(q.watt[k]),
+                  "nuclide " + std::to_string(i) + ": nu and Watt parameters must be finite");
+          h.nu[k] = q.nu[k];
+          h.watt[k] = q.watt[k];
+        }
+        require(q.watt[0] > 0.0 && q.watt[1] >= 0.0,
+                "nuclide " + std::to_string(i) + ": the fission spectrum needs Watt a > 0 and b >= 0");
+      }
       c->nucs.push_back(std::move(h));
     }
     c->dirty = true;
@@ -1064,10 +1257,37 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
     require(batch >= 1, "batch numbers start at 1");
     require(n_particles >= 1, "No Rays were moved to _rays in defineRays()");
     prepare(c);
+    const bool eig = c->opt.run_mode == MCW_RUN_EIGENVALUE;
+    bool active = true;
+    if (eig) {
+      require(c->opt.mode == MCW_MODE_BATCHED, "eigenvalue runs need mode BATCHED");
+      if (batch == 1) {
+        c->keff = 1.0;
+        c->k_sum = c->k_sum_sq = 0.0;
+        c->n_active = 0;
+        c->eig_src_ready = false;
+      } else {
+        require(batch == c->eig_batch + 1 && n_particles == c->eig_n && c->eig_src_ready,
+                "eigenvalue batches run in order 1, 2, ... with one particle count");
+      }
+      require(c->keff > 0.0 && std::isfinite(c->keff), "keff must be positive and finite");
+      active = batch > c->opt.n_inactive;
+      ensure_fission(c, n_particles);
+    }
     hipStream_t s = c->stream;
     Params P = c->base;
     P.batch = batch;
     P.n_particles = n_particles;
+    if (eig) {
+      P.eigen = 1;
+      P.fb = fission_view(c, n_particles);
+      HIPCHK(launch_fill(s, P.fb.k_hist, 3 * n_particles, 0.0));
+      if (!active) { // inactive batches score no tallies
+        P.n_tl = 0;
+        P.n_coll = 0;
+        P.tl_fast = 0;
+      }
+    }
     P.record = c->opt.record_histories ? 1 : 0;
     if (P.record) {
       ensure_records(c, n_particles);
@@ -1107,8 +1327,13 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       const int64_t h1 = std::min(n_particles, h0 + cap);
       P.hist_begin = h0;
       P.hist_end = h1;
-      SourceBank S = source_bank(c, h1 - h0);
-      HIPCHK(launch_source_bank(s, P, S, h1 - h0));
+      SourceBank S;
+      if (eig && batch > 1) {
+        S = gen_bank(c, h0); // resampled fission sites of the previous batch
+      } else {
+        S = source_bank(c, h1 - h0);
+        HIPCHK(launch_source_bank(s, P, S, h1 - h0));
+      }
       if (c->opt.mode == MCW_MODE_PLAIN) {
         run_plain(c, P, S, h1 - h0);
       } else {
@@ -1117,7 +1342,8 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
         const int wpe = c->opt.waves_per_eu == 5 ? 5 : 0;
         const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns != 6;
         require(block % 64 == 0 && block <= 256, "block_size must be 64, 128, 192 or 256");
-        int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, p32, block), (h1 - h0 + block - 1) / block);
+        int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, p32, block, eig),
+                                           (h1 - h0 + block - 1) / block);
         if (blocks < 1) blocks = 1;
         // the work counter restarts for every chunk
         HIPCHK(launch_reset_work(s, P.cnt));
@@ -1132,10 +1358,12 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       }
     }
     HIPCHK(hipEventRecord(c->ev[1], s));
-    // batch close: VALUE / total source weight -> SUM, SUM_SQ
-    for (auto &t : c->tallies)
-      HIPCHK(launch_accumulate(s, t->value.as<double>(), t->sum.as<double>(), t->sum_sq.as<double>(),
-                               t->n_filter_bins * t->n_score_bins, (double)n_particles));
+    // batch close: VALUE / total source weight -> SUM, SUM_SQ (active batches)
+    if (active)
+      for (auto &t : c->tallies)
+        HIPCHK(launch_accumulate(s, t->value.as<double>(), t->sum.as<double>(), t->sum_sq.as<double>(),
+                                 t->n_filter_bins * t->n_score_bins, (double)n_particles));
+    if (eig) eigen_close(c, P, batch, n_particles, active);
     HIPCHK(hipMemcpyAsync(c->h_counters, P.cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     float total_ms = 0.0f;
@@ -1162,6 +1390,49 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       out->hex_segments = (int64_t)k.hex_segments;
       out->trace_records = (int64_t)k.trace_count;
     }
+  });
+}
+
+int mcw_keff_get(mcw_ctx *c, mcw_keff_result *out)
+{
+  return guard(c, [&] {
+    require(out != nullptr, "null output");
+    require(c->eig_batch > 0, "no eigenvalue batch has run");
+    *out = c->kres;
+  });
+}
+
+static void copy_sites(mcw_ctx *c, const SourceBank &S, const uint64_t *key, const uint32_t *perm, int64_t n,
+                       mcw_site *out, size_t max)
+{
+  const int64_t m = std::min<int64_t>(n, (int64_t)max);
+  if (!out || m <= 0) return;
+  DevBuf tmp;
+  tmp.alloc((size_t)m * sizeof(mcw_site));
+  HIPCHK(launch_sites_out(c->stream, S.x, S.y, S.z, S.u, S.v, S.w, S.E, S.elem, key, perm, m,
+                          c->orig_of_int.as<int32_t>(), tmp.as<mcw_site>()));
+  HIPCHK(hipMemcpyAsync(out, tmp.p, (size_t)m * sizeof(mcw_site), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+int mcw_fission_bank_get(mcw_ctx *c, mcw_site *out, size_t max, int64_t *n_out)
+{
+  return guard(c, [&] {
+    require(c->eig_batch > 0, "no eigenvalue batch has run");
+    if (n_out) *n_out = c->eig_sites;
+    const FissionView F = fission_view(c, c->eig_n);
+    SourceBank S;
+    S.x = F.x; S.y = F.y; S.z = F.z; S.u = F.u; S.v = F.v; S.w = F.w; S.E = F.E; S.elem = F.elem;
+    copy_sites(c, S, F.key, c->fb_perm.as<uint32_t>(), c->eig_sites, out, max);
+  });
+}
+
+int mcw_source_bank_get(mcw_ctx *c, mcw_site *out, size_t max, int64_t *n_out)
+{
+  return guard(c, [&] {
+    require(c->eig_src_ready, "no resampled source: run an eigenvalue batch first");
+    if (n_out) *n_out = c->eig_n;
+    copy_sites(c, gen_bank(c, 0), c->gen[9].as<uint64_t>(), nullptr, c->eig_n, out, max);
   });
 }
 

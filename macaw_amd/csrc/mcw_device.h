@@ -327,6 +327,27 @@ __device__ __forceinline__ double macro_total(const XSView &X, int mat, int32_t 
   return t;
 }
 
+// Macroscopic total and nu-fission at (mat, E) in one pass over the material.
+__device__ __forceinline__ double macro_total_nf(const XSView &X, int mat, int32_t k, double E, double &nusf)
+{
+  nusf = 0.0;
+  if (mat < 0) return 0.0;
+  double t = 0.0, nf = 0.0;
+  const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
+  for (int32_t q = b; q < e; ++q) {
+    const int32_t n = X.mat_nuc[q];
+    Micro u;
+    micro_at(X, n, k, E, u);
+    t = t + X.mat_dens[q] * u.t;
+    if (X.fis) {
+      const double4 fp = X.fis[n];
+      nf = nf + X.mat_dens[q] * ((fp.x + fp.y * E) * u.f);
+    }
+  }
+  nusf = nf;
+  return t;
+}
+
 struct Macro {
   double t, a, s, f;
 };

@@ -45,6 +45,10 @@ struct PState {
   int32_t req;  // request server: slice of the round
   double x0, y0, z0; // record mode: where the current element visit began
   int32_t visit;     // record mode: index of the current visit
+  // eigenvalue batches: macroscopic nu-fission (cached with sigt), the
+  // history's keff scores and its fission-site count
+  double nusf, ktl, kcol, kabs;
+  int32_t prog;
 };
 
 // Event counts of one wavefront.  Every field is wavefront-uniform (sums of
@@ -309,6 +313,11 @@ __device__ __forceinline__ void start_history(PState &st, int64_t hid)
   st.y0 = st.y;
   st.z0 = st.z;
   st.visit = 0;
+  st.nusf = 0.0;
+  st.ktl = 0.0;
+  st.kcol = 0.0;
+  st.kabs = 0.0;
+  st.prog = 0;
 }
 
 // Record mode: close the current element visit of a traced history with
@@ -360,7 +369,7 @@ __device__ __forceinline__ int32_t pick_nbr(const int4 &r0, const int4 &r1, int 
 // With `tl_ptr`, a plain per-element flux (the request server's, or the one
 // track-length tally when P.tl_fast) is not added here but returned as the
 // address (nullptr: no bin) and value, for the caller's wave-aggregated add.
-template <int NS, bool REC, bool P32, bool SERVE = false>
+template <int NS, bool REC, bool P32, bool SERVE = false, bool EIG = false>
 __device__ __forceinline__ int segment(const Params &P, PState &st, double &track_sum, unsigned &rechecks,
                                        double **tl_ptr = nullptr, double *tl_val = nullptr, bool *in_hex = nullptr)
 {
@@ -410,7 +419,8 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
     }
   }
   if (mat != st.mat) {
-    st.sigt = macro_total(P.xs, mat, st.k, st.E);
+    if (EIG) st.sigt = macro_total_nf(P.xs, mat, st.k, st.E, st.nusf);
+    else st.sigt = macro_total(P.xs, mat, st.k, st.E);
     st.mat = mat;
   }
   if (fx < 0) {
@@ -423,6 +433,8 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   const double track = (d < L) ? d : L;
   ++st.nseg;
   track_sum += track;
+  // keff track-length estimator, weight 1 (CollisionKernel.C:152-154)
+  if (EIG) st.ktl = st.ktl + track * st.nusf;
   if (SERVE || P.tl_fast) {
     double *bin;
     if (SERVE) {
@@ -487,18 +499,32 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   return EV_COLLIDE;
 }
 
-// Collision at the current point: collision-estimator tallies, sample the
-// nuclide (OpenMC sample_nuclide), absorb (analog) or scatter elastically
-// (isotropic in CM, target at rest) with an energy cutoff at the grid minimum.
-// A scatter looks up the new energy's union-grid index through `hash`.
-template <bool REC>
-__device__ __forceinline__ int collide(const Params &P, PState &st, const int32_t *hash)
+// Collision at the current point, in two parts around the fission bank's
+// slot reservation (which the whole wavefront makes together):
+//   collide_pick: collision-estimator tallies (and the keff collision
+//     estimator), sample the nuclide (OpenMC sample_nuclide) and, in an
+//     eigenvalue batch with a fissionable nuclide, the number of fission
+//     neutrons (create_fission_sites: floor(nu_t) plus one with probability
+//     frac(nu_t), nu_t = w / keff * nu sigma_f / sigma_t);
+//   collide_finish: bank those sites (isotropic: mu, then phi; a prompt /
+//     delayed draw, always prompt; a Watt energy), then absorb (analog, with
+//     the keff absorption estimator) or scatter elastically (isotropic in CM,
+//     target at rest) with an energy cutoff at the grid minimum.  A scatter
+//     looks up the new energy's union-grid index through `hash`.
+struct CollPick {
+  int32_t nuc;
+  double t, a, nuf; // picked nuclide's total, absorption and nu-fission at E
+};
+
+template <bool REC, bool EIG>
+__device__ __forceinline__ int64_t collide_pick(const Params &P, PState &st, CollPick &cp)
 {
   const XSView &X = P.xs;
   const int32_t mat = st.mat;
   if (REC) ++st.ncoll;
   for (int t = P.n_tl; t < P.n_tl + P.n_coll; ++t)
     tally_score(P.tally[t], X, st.elem, st.E, st.k, 1.0 / st.sigt, mat);
+  if (EIG) st.kcol = st.kcol + st.nusf / st.sigt;
   const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
   const double cutoff = prn(st.seed) * st.sigt;
   double prob = 0.0;
@@ -512,12 +538,78 @@ __device__ __forceinline__ int collide(const Params &P, PState &st, const int32_
       break;
     }
   }
-  const int32_t nuc = X.mat_nuc[pick];
+  cp.nuc = X.mat_nuc[pick];
+  cp.t = mu.t;
+  cp.a = mu.a;
+  cp.nuf = 0.0;
+  int64_t nu = 0;
+  if (EIG) {
+    if (X.fis) {
+      const double4 fp = X.fis[cp.nuc];
+      cp.nuf = (fp.x + fp.y * st.E) * mu.f;
+    }
+    if (X.fissionable && X.fissionable[cp.nuc] && mu.t > 0.0) {
+      const double nu_t = ((1.0 / P.fb.keff) * cp.nuf) / mu.t;
+      nu = nu_t < 9.0e15 ? (int64_t)nu_t : (int64_t)9.0e15;
+      if (prn(st.seed) <= nu_t - (double)nu) ++nu;
+      // a history's progeny index has PROGENY_BITS bits: every reserved slot
+      // below the bank's capacity is then written
+      const int64_t room = ((int64_t)1 << PROGENY_BITS) - st.prog;
+      if (nu > room) nu = room;
+    }
+  }
+  return nu;
+}
+
+// Fission sites go to slots base, base + 1, ...; from the first slot past the
+// bank's capacity on, the collision's sites are not sampled, as OpenMC stops
+// at a full bank.
+template <bool REC, bool EIG>
+__device__ __forceinline__ int collide_finish(const Params &P, PState &st, const CollPick &cp, int64_t nu,
+                                              int64_t base, const int32_t *hash)
+{
+  const XSView &X = P.xs;
+  if (EIG) {
+    const FissionView &F = P.fb;
+    const double4 fp = X.fis[cp.nuc];
+    for (int64_t q = 0; q < nu; ++q) {
+      const int64_t slot = base + q;
+      if (slot >= F.cap) break;
+      const double fmu = 2.0 * prn(st.seed) - 1.0;
+      double sphi, cphi;
+      sincos_2pi_d(prn(st.seed), &sphi, &cphi);
+      const double sn = sqrt(1.0 - fmu * fmu);
+      (void)prn(st.seed); // prompt or delayed: no delayed data, always prompt
+      const double a = fp.z, bw = fp.w;
+      double Ef;
+      for (int it = 0;; ++it) {
+        const double r1 = prn(st.seed), r2 = prn(st.seed), r3 = prn(st.seed);
+        const double c = cos_halfpi_d(r3);
+        const double wmax = -a * (log_d(r1) + log_d(r2) * c * c);
+        const double Ew = wmax + 0.25 * a * a * bw + (-1.0 + 2.0 * prn(st.seed)) * sqrt(a * a * bw * wmax);
+        if (Ew < X.e_max || it >= 1000) {
+          Ef = Ew < X.e_max ? Ew : X.e_min;
+          break;
+        }
+      }
+      F.x[slot] = st.x;
+      F.y[slot] = st.y;
+      F.z[slot] = st.z;
+      F.u[slot] = fmu;
+      F.v[slot] = sn * cphi;
+      F.w[slot] = sn * sphi;
+      F.E[slot] = Ef;
+      F.elem[slot] = st.elem;
+      F.key[slot] = ((uint64_t)st.hid << PROGENY_BITS) | (uint64_t)st.prog;
+      ++st.prog;
+    }
+  }
   int ev;
-  if (mu.a > prn(st.seed) * mu.t) {
+  if (cp.a > prn(st.seed) * cp.t) {
     ev = EV_ABSORB;
+    if (EIG) st.kabs = st.kabs + cp.nuf / cp.a;
   } else {
-    const double A = X.awr[nuc];
+    const double A = X.awr[cp.nuc];
     const double mu_cm = 2.0 * prn(st.seed) - 1.0;
     const double q2 = (A * A + 2.0 * A * mu_cm) + 1.0;
     const double Ep = st.E * q2 / ((A + 1.0) * (A + 1.0));
@@ -539,6 +631,42 @@ __device__ __forceinline__ int collide(const Params &P, PState &st, const int32_
   if (P.roundtrip) st.seed = seed_roundtrip(st.seed);
   if (ev != EV_SCATTER) trace_visit<REC>(P, st, st.elem, ev);
   return ev;
+}
+
+// A whole collision of a fixed-source history.
+template <bool REC>
+__device__ __forceinline__ int collide(const Params &P, PState &st, const int32_t *hash)
+{
+  CollPick cp;
+  collide_pick<REC, false>(P, st, cp);
+  return collide_finish<REC, false>(P, st, cp, 0, 0, hash);
+}
+
+// Slots for n items per lane, consecutive in lane order, with one atomic per
+// wavefront: returns this lane's first slot.  Every lane must call it.
+__device__ __forceinline__ int64_t wave_reserve(unsigned long long *count, int64_t n)
+{
+  const int lane = __lane_id();
+  int64_t incl = n;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int64_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  const int64_t total = __shfl(incl, 63, 64);
+  unsigned long long base = 0;
+  if (total > 0 && lane == 0) base = atomicAdd(count, (unsigned long long)total);
+  base = (unsigned long long)__shfl((long long)base, 0, 64);
+  return (int64_t)base + (incl - n);
+}
+
+// Per-history keff scores of a history that ended.
+__device__ __forceinline__ void store_keff(const Params &P, const PState &st)
+{
+  const int64_t n = P.n_particles;
+  P.fb.k_hist[st.hid] = st.ktl;
+  P.fb.k_hist[n + st.hid] = st.kcol;
+  P.fb.k_hist[2 * n + st.hid] = st.kabs;
 }
 
 __device__ __forceinline__ int fate_of(int ev)
@@ -817,7 +945,7 @@ __device__ __forceinline__ void req_account(const Params &P, bool dying, const P
   if (full) a.v = 0ull;
 }
 
-template <int NS, bool REC, int WPE, bool P32, bool SERVE = false>
+template <int NS, bool REC, int WPE, bool P32, bool SERVE = false, bool EIG = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
 k_persistent(Params P, SourceBank S)
 {
@@ -862,11 +990,12 @@ k_persistent(Params P, SourceBank S)
       double tl_val = 0.0;
       bool hx = NS == 6;
       if (go) {
-        ev = segment<NS, REC, P32, SERVE>(P, st, track, rechecks, &tl_ptr, &tl_val, &hx);
+        ev = segment<NS, REC, P32, SERVE, EIG>(P, st, track, rechecks, &tl_ptr, &tl_val, &hx);
         if (ev == EV_COLLIDE) {
           phase = PH_COLLIDE;
         } else if (is_death(ev)) {
           if (REC) write_record(P, st, fate_of(ev));
+          if (EIG) store_keff(P, st);
           phase = PH_IDLE;
         }
       }
@@ -874,15 +1003,25 @@ k_persistent(Params P, SourceBank S)
       if (SERVE) req_account(P, go && is_death(ev), st, racc);
       count_segment(wc, go, ev, hx);
     } while (wave_count(phase == PH_TRANSPORT) >= lanes_min);
-    // collisions together
+    // collisions together (in an eigenvalue batch the wavefront reserves its
+    // fission-bank slots between sampling the yields and the sites)
     const bool hit = phase == PH_COLLIDE;
     int ev = -1;
-    if (hit) {
+    if (EIG) {
+      CollPick cp;
+      int64_t nu = 0;
+      if (hit) nu = collide_pick<REC, true>(P, st, cp);
+      const int64_t base = wave_reserve(&C->fission_sites, nu);
+      if (hit) ev = collide_finish<REC, true>(P, st, cp, nu, base, s_hash);
+    } else if (hit) {
       ev = collide<REC>(P, st, s_hash);
+    }
+    if (hit) {
       if (ev == EV_SCATTER) {
         phase = PH_TRANSPORT;
       } else {
         if (REC) write_record(P, st, fate_of(ev));
+        if (EIG) store_keff(P, st);
         phase = PH_IDLE;
       }
     }
@@ -1039,16 +1178,28 @@ template <int WPE> const void *persistent_kernel_w(int ns, bool rec, bool p32)
   return (const void *)k_persistent<0, false, WPE, false>;
 }
 
-// The kernel variant for side count, record mode, occupancy hint and precision.
-const void *persistent_kernel(int ns, int wpe, bool rec, bool p32)
+// Eigenvalue-batch variants (compiler-chosen occupancy).
+template <bool REC> const void *eigen_kernel_r(int ns, bool p32)
 {
+  if (ns == 4) return p32 ? (const void *)k_persistent<4, REC, 0, true, false, true>
+                          : (const void *)k_persistent<4, REC, 0, false, false, true>;
+  if (ns == 6) return (const void *)k_persistent<6, REC, 0, false, false, true>;
+  return p32 ? (const void *)k_persistent<0, REC, 0, true, false, true>
+             : (const void *)k_persistent<0, REC, 0, false, false, true>;
+}
+
+// The kernel variant for side count, record mode, occupancy hint, precision
+// and run mode.
+const void *persistent_kernel(int ns, int wpe, bool rec, bool p32, bool eig)
+{
+  if (eig) return rec ? eigen_kernel_r<true>(ns, p32) : eigen_kernel_r<false>(ns, p32);
   return wpe == 5 ? persistent_kernel_w<5>(ns, rec, p32) : persistent_kernel_w<0>(ns, rec, p32);
 }
 
 hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int block, hipStream_t s, const Params &P,
                              const SourceBank &S)
 {
-  const void *k = persistent_kernel(ns, wpe, P.record != 0, p32);
+  const void *k = persistent_kernel(ns, wpe, P.record != 0, p32, P.eigen != 0);
   void *args[] = {(void *)&P, (void *)&S};
   return hipLaunchKernel(k, dim3(blocks), dim3(block), args, 0, s);
 }

@@ -80,6 +80,8 @@ struct XSView {
   const int32_t *n_points;
   const double4 *rec;        // per point: (E, total, absorption, scatter)
   const double *fission;     // per point, nullptr when no nuclide has fission data
+  const double4 *fis;        // per nuclide: (nu0, nu1, Watt a, Watt b), nu(E) = nu0 + nu1 E
+  const int32_t *fissionable; // per nuclide: 1 when it has fission data
   const double *awr;
   // unionized grid: every nuclide's points merged, sorted, unique
   const double *U;
@@ -180,11 +182,26 @@ struct RecordView {
   int32_t *nseg, *ncoll, *elem, *fate;
 };
 
+// Fission bank of an eigenvalue batch (structure of arrays, unsorted: slots
+// are taken in arrival order; `key` gives OpenMC's order) and the
+// per-history keff scores, reduced in a fixed order after the batch.
+struct FissionView {
+  double *x, *y, *z, *u, *v, *w, *E;
+  int32_t *elem;           // internal element id
+  uint64_t *key;           // history << PROGENY_BITS | progeny
+  int64_t cap;             // sites the bank holds
+  double *k_hist;          // 3 x n_particles: track-length, collision, absorption scores
+  double keff;             // divisor of the fission yield
+};
+
+constexpr int PROGENY_BITS = 24;
+
 // Global counters, accumulated with one atomic per wavefront.
 struct Counters {
   unsigned long long segments, collisions, absorptions, leaks, cutoffs, lost, reflections, rechecks;
   unsigned long long hex_segments; // segments in HEX8 elements
   unsigned long long trace_count;  // trace records emitted (may exceed the buffer)
+  unsigned long long fission_sites; // sites requested (those beyond the bank are dropped)
   double track;
   unsigned long long next_hist;   // persistent kernel work queue
   unsigned long long q_count[4];  // event queue lengths
@@ -202,7 +219,7 @@ struct Params {
   int32_t record;
   int32_t lanes_min;       // persistent kernel: keep stepping segments while this many lanes are in flight
   int32_t tl_fast;         // the one track-length tally is a plain flux per element bin (no energy filter)
-  int32_t pad;
+  int32_t eigen;           // eigenvalue batch: keff scores and fission sites (fb)
   int64_t batch;
   int64_t n_particles;     // particles in the batch (seed id stride)
   int64_t hist_begin;      // first history index handled by this launch
@@ -215,6 +232,7 @@ struct Params {
   TraceRec *trace;         // record mode: visit records of histories hid < trace_n
   int64_t trace_n;
   int64_t trace_cap;
+  FissionView fb;
 };
 
 } // namespace mcw

@@ -205,8 +205,19 @@ static NuclideData synthetic_nuclide(const InputParameters &p)
   const long long np = p.getInt("n_points", 20000);
   const double e0 = p.getReal("e_min", 1e-5), e1 = p.getReal("e_max", 2e7);
   const double sa0 = p.getReal("sigma_a0", 1.0), sp = p.getReal("sigma_p", 4.7);
+  const double sa_flat = p.getReal("sigma_a_flat", 0.0);
+  const double ff = p.getReal("fission_fraction", 0.0);
   const long long nres = p.getInt("resonances", 50);
   if (np < 2) paramError(p, "n_points", "a grid needs at least 2 points");
+  if (!(ff >= 0.0 && ff <= 1.0)) paramError(p, "fission_fraction", "the fission fraction must be in [0, 1]");
+  if (ff > 0.0) {
+    const std::vector<double> nu = p.isParamValid("nu") ? p.getReals("nu") : std::vector<double>{2.43};
+    if (nu.empty() || nu.size() > 2) paramError(p, "nu", "nu takes one or two coefficients: nu(E) = nu0 + nu1 E");
+    n.nu[0] = nu[0];
+    n.nu[1] = nu.size() > 1 ? nu[1] : 0.0;
+    n.watt[0] = p.getReal("watt_a", 0.988e6);
+    n.watt[1] = p.getReal("watt_b", 2.249e-6);
+  }
   uint64_t s = (uint64_t)p.getInt("seed", 1) * 152917ull;
   auto prn = [&]() {
     s = 6364136223846793005ull * s + 1442695040888963407ull;
@@ -226,7 +237,7 @@ static NuclideData synthetic_nuclide(const InputParameters &p)
   n.s.resize(np);
   for (long long j = 0; j < np; ++j) {
     double E = j == 0 ? e0 : (j == np - 1 ? e1 : e0 * std::pow(e1 / e0, (double)j / (np - 1)));
-    double a = sa0 * std::sqrt(0.0253 / E), sc = sp;
+    double a = sa0 * std::sqrt(0.0253 / E) + sa_flat, sc = sp;
     for (long long r = 0; r < nres; ++r) {
       double x = 2.0 * (E - Er[r]) / G[r];
       double shape = pk[r] / (1.0 + x * x);
@@ -237,6 +248,10 @@ static NuclideData synthetic_nuclide(const InputParameters &p)
     n.a[j] = a;
     n.s[j] = sc;
     n.t[j] = a + sc;
+  }
+  if (ff > 0.0) {
+    n.f.resize(np);
+    for (long long j = 0; j < np; ++j) n.f[j] = ff * n.a[j];
   }
   return n;
 }
@@ -544,7 +559,8 @@ void OpenMCStudy::initialSetup()
         "mcw_mesh_set");
   std::vector<mcw_nuclide> nd;
   for (auto &n : fe.nuclides)
-    nd.push_back(mcw_nuclide{(int32_t)n.E.size(), n.awr, n.E.data(), n.t.data(), n.a.data(), n.s.data(), nullptr});
+    nd.push_back(mcw_nuclide{(int32_t)n.E.size(), n.awr, n.E.data(), n.t.data(), n.a.data(), n.s.data(),
+                             n.f.empty() ? nullptr : n.f.data(), {n.nu[0], n.nu[1]}, {n.watt[0], n.watt[1]}});
   check(_ctx, mcw_xs_set(_ctx, (int32_t)nd.size(), nd.data()), "mcw_xs_set");
   std::vector<mcw_material> md;
   for (auto &mt : fe.materials)
@@ -579,6 +595,12 @@ void OpenMCStudy::postExecuteStudy()
     std::cout << "Batch " << _t_step << ": " << _stats.histories << " histories, " << _stats.segments << " segments, "
               << _stats.collisions << " collisions, leakage " << (double)_stats.leaks / _stats.histories << ", "
               << _stats.time_ms << " ms" << std::endl;
+  if (_fe.options.run_mode == MCW_RUN_EIGENVALUE) {
+    check(_ctx, mcw_keff_get(_ctx, &_keff), "mcw_keff_get");
+    // OpenMC's output is silenced, so keff is printed (OpenMCStudy.C:397-400)
+    std::cout << "Keff " << _keff.keff << " (" << _keff.keff_std << ") Generation: " << _keff.k_generation
+              << std::endl;
+  }
 }
 
 // ------------------------------------------------------------------ Problem
@@ -705,6 +727,16 @@ void Problem::read_monte_carlo(const HitNode &mcb)
   else if (mode == "plain") options.mode = MCW_MODE_PLAIN;
   else paramError(p, "mode", "mode must be batched or plain");
   options.lanes_min = (int32_t)p.getInt("lanes_min", 0);
+  const std::string run_mode = p.getString("run_mode", "fixed source");
+  if (run_mode == "eigenvalue") options.run_mode = MCW_RUN_EIGENVALUE;
+  else if (run_mode == "fixed source" || run_mode == "fixed_source") options.run_mode = MCW_RUN_FIXED_SOURCE;
+  else paramError(p, "run_mode", "run_mode must be eigenvalue or fixed source");
+  options.n_inactive = (int32_t)p.getInt("inactive", 0);
+  if (options.n_inactive < 0) paramError(p, "inactive", "the number of inactive batches must not be negative");
+  if (options.run_mode == MCW_RUN_EIGENVALUE && options.n_inactive >= n_batches)
+    paramError(p, "inactive", "an eigenvalue run needs more batches than inactive batches");
+  if (options.run_mode == MCW_RUN_EIGENVALUE && options.mode != MCW_MODE_BATCHED)
+    paramError(p, "mode", "eigenvalue runs need mode = batched");
   HitNode *nb = mcb.child("Nuclides");
   HitNode *mb = mcb.child("Materials");
   HitNode *sb = mcb.child("source");
@@ -739,8 +771,19 @@ void Problem::read_monte_carlo(const HitNode &mcb)
     for (int k = 0; k < 3; ++k) source.point[k] = pt[k];
   } else if (space == "volume") {
     source.space = MCW_SPACE_MESH_VOLUME;
+  } else if (space == "box") {
+    // settings.xml <space type="box"><parameters>lower_left upper_right</parameters>
+    source.space = MCW_SPACE_BOX;
+    auto lo = s.getReals("lower_left"), hi = s.getReals("upper_right");
+    if (lo.size() != 3) paramError(s, "lower_left", "a box corner needs 3 coordinates");
+    if (hi.size() != 3) paramError(s, "upper_right", "a box corner needs 3 coordinates");
+    for (int k = 0; k < 3; ++k) {
+      if (!(hi[k] > lo[k])) paramError(s, "upper_right", "the box's upper corner must lie above its lower corner");
+      source.box_lo[k] = lo[k];
+      source.box_hi[k] = hi[k];
+    }
   } else {
-    paramError(s, "space", "space must be point or volume");
+    paramError(s, "space", "space must be point, box or volume");
   }
   source.angle = MCW_ANGLE_ISOTROPIC;
   if (s.isParamValid("watt_a")) {
@@ -797,6 +840,10 @@ void Problem::run()
     const mcw_batch_stats &st = study->lastStats();
     std::vector<double> row = {(double)step, (double)st.segments, (double)st.collisions,
                                (double)st.leaks / (double)st.histories, st.histories / (st.time_ms * 1e-3)};
+    if (options.run_mode == MCW_RUN_EIGENVALUE) {
+      const mcw_keff_result &k = study->lastKeff();
+      row.insert(row.end(), {k.k_generation, k.keff, k.keff_std});
+    }
     for (auto &a : auxkernels) {
       int64_t nf = 0, ns = 0;
       check(ctx, mcw_tally_shape(ctx, a->tallyId(), &nf, &ns), "mcw_tally_shape");
@@ -838,6 +885,7 @@ void Problem::write_outputs(const std::vector<std::vector<double>> &rows)
 {
   std::ofstream f(file_base + ".csv");
   f << "time,segments,collisions,leakage,histories_per_s";
+  if (options.run_mode == MCW_RUN_EIGENVALUE) f << ",k_generation,keff,keff_std";
   for (auto &a : auxkernels) f << "," << a->variable() << "_total";
   f << "\n";
   f.precision(17);
@@ -854,7 +902,7 @@ void Problem::dump(const std::string &path) const
   std::ofstream f(path, std::ios::binary);
   auto w64 = [&](int64_t v) { f.write((const char *)&v, 8); };
   auto wd = [&](const void *p, size_t n) { f.write((const char *)p, n); };
-  f.write("MCWD0001", 8);
+  f.write("MCWD0002", 8);
   w64(mesh.n_nodes());
   wd(mesh.xyz.data(), mesh.xyz.size() * 8);
   w64(mesh.n_elem());
@@ -887,6 +935,10 @@ void Problem::dump(const std::string &path) const
     wd(n.t.data(), n.t.size() * 8);
     wd(n.a.data(), n.a.size() * 8);
     wd(n.s.data(), n.s.size() * 8);
+    w64(n.f.empty() ? 0 : 1);
+    if (!n.f.empty()) wd(n.f.data(), n.f.size() * 8);
+    wd(n.nu, 16);
+    wd(n.watt, 16);
   }
   w64((int64_t)materials.size());
   for (auto &m : materials) {
@@ -900,6 +952,8 @@ void Problem::dump(const std::string &path) const
   w64(n_batches);
   w64(options.seed_roundtrip);
   w64((int64_t)options.master_seed);
+  w64(options.run_mode);
+  w64(options.n_inactive);
 }
 
 } // namespace mcwhost

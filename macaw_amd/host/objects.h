@@ -66,6 +66,9 @@ struct NuclideData {
   std::string name;
   double awr;
   std::vector<double> E, t, a, s;
+  std::vector<double> f;              // fission (part of absorption); empty: not fissionable
+  double nu[2] = {0.0, 0.0};          // nu(E) = nu[0] + nu[1] E
+  double watt[2] = {0.988e6, 2.249e-6}; // fission spectrum a [eV], b [1/eV]
 };
 
 struct MaterialData {
@@ -161,12 +164,14 @@ public:
   void postExecuteStudy();
   mcw_ctx *ctx() const { return _ctx; }
   const mcw_batch_stats &lastStats() const { return _stats; }
+  const mcw_keff_result &lastKeff() const { return _keff; }
 
 private:
   InputParameters _params;
   Problem &_fe;
   mcw_ctx *_ctx = nullptr;
   mcw_batch_stats _stats{};
+  mcw_keff_result _keff{};
   bool _verbose;
   int64_t _t_step = 0;
 };

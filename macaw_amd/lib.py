@@ -23,6 +23,7 @@ EXPORTS = (
     "mcw_tally_shape", "mcw_run_batch", "mcw_tally_get", "mcw_tally_reset", "mcw_history_get",
     "mcw_server_create", "mcw_server_destroy", "mcw_server_last_error", "mcw_submit", "mcw_poll",
     "mcw_result", "mcw_trace_get", "mcw_debug_math_probe", "mcw_debug_xs_probe",
+    "mcw_keff_get", "mcw_fission_bank_get", "mcw_source_bank_get",
 )
 
 
@@ -66,6 +67,9 @@ def lib():
         "mcw_trace_get": ([vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int64)], C.c_int),
         "mcw_debug_math_probe": ([vp, dp, i64, dp, dp, dp, dp, u64p, dp, u64p], C.c_int),
         "mcw_debug_xs_probe": ([vp, i32, dp, i64, i32p, dp], C.c_int),
+        "mcw_keff_get": ([vp, C.POINTER(abi.KeffResult)], C.c_int),
+        "mcw_fission_bank_get": ([vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int64)], C.c_int),
+        "mcw_source_bank_get": ([vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int64)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -156,6 +160,28 @@ class Context:
 
     def tally_reset(self):
         self._chk(self.L.mcw_tally_reset(self.h))
+
+    def keff(self):
+        """keff state after the last eigenvalue batch (mcw_keff_get) as a dict."""
+        k = abi.KeffResult()
+        self._chk(self.L.mcw_keff_get(self.h, C.byref(k)))
+        return k.as_dict()
+
+    def _sites(self, fn):
+        n = C.c_int64()
+        self._chk(fn(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, abi.site_dtype())
+        if n.value:
+            self._chk(fn(self.h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
+    def fission_bank(self):
+        """The last eigenvalue batch's fission bank in (parent, progeny) order."""
+        return self._sites(self.L.mcw_fission_bank_get)
+
+    def source_bank(self):
+        """The next eigenvalue batch's resampled source."""
+        return self._sites(self.L.mcw_source_bank_get)
 
     def histories(self, n):
         out = np.zeros(n, abi.history_dtype())

@@ -17,7 +17,7 @@ from __future__ import annotations
 import dataclasses
 import hashlib
 import os
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
@@ -49,7 +49,9 @@ class NuclideData:
     total: np.ndarray
     absorption: np.ndarray
     scatter: np.ndarray
-    fission: Optional[np.ndarray] = None
+    fission: Optional[np.ndarray] = None   # part of absorption; makes the nuclide fissionable
+    nu: Tuple[float, float] = (0.0, 0.0)   # nu(E) = nu[0] + nu[1] E
+    watt: Tuple[float, float] = (0.988e6, 2.249e-6)  # fission spectrum a [eV], b [1/eV]
 
 
 @dataclasses.dataclass
@@ -115,6 +117,26 @@ def synth_nuclide(n_points: int, awr: float, seed: int, sigma_a0: float = 1.0,
         ss += shape * (1.0 - fg[r])
     st = sa + ss
     return NuclideData(awr=float(awr), energy=E, total=st, absorption=sa, scatter=ss)
+
+
+def flat_nuclide(awr: float, total: float, absorption: float, fission: float = 0.0,
+                 nu=(0.0, 0.0), watt=(0.988e6, 2.249e-6), e_min: float = 1e-5,
+                 e_max: float = 2e7, n_points: int = 2) -> NuclideData:
+    """Energy-independent cross sections [b] on a log grid: the case where
+    infinite-medium results have closed forms (k_inf = nu sigma_f / sigma_a)."""
+    E = np.geomspace(e_min, e_max, n_points)
+    E[0], E[-1] = e_min, e_max
+    one = np.ones_like(E)
+    return NuclideData(awr=float(awr), energy=E, total=total * one, absorption=absorption * one,
+                       scatter=(total - absorption) * one,
+                       fission=None if fission <= 0.0 else fission * one,
+                       nu=(float(nu[0]), float(nu[1])), watt=(float(watt[0]), float(watt[1])))
+
+
+def fissile(n: NuclideData, fraction: float, nu=(2.43, 0.0), watt=(0.988e6, 2.249e-6)) -> NuclideData:
+    """The nuclide with a share `fraction` of its absorption made fission."""
+    return dataclasses.replace(n, fission=fraction * np.asarray(n.absorption, np.float64),
+                               nu=(float(nu[0]), float(nu[1])), watt=(float(watt[0]), float(watt[1])))
 
 
 # ----------------------------------------------------------------- meshes
@@ -383,6 +405,19 @@ def c1_hex_reflective(n_particles=10_000, n_batches=10, n=8, n_points=20_000) ->
                    n_particles=n_particles, n_batches=n_batches)
 
 
+def eigen_box(sig_t=10.0, sig_a=3.0, sig_f=1.2, nu=2.5, awr=235.0, dens=0.05, n=4, half=1.0,
+              reflective=True, n_particles=2000, n_batches=5) -> Problem:
+    """A cube of one flat fissile nuclide.  Reflective: an infinite medium,
+    k_inf = nu sigma_f / sigma_a; every history ends in absorption."""
+    nuc = flat_nuclide(awr, sig_t, sig_a, sig_f, nu=(nu, 0.0))
+    mat = MaterialData(id=1, nuclides=np.array([0], np.int32), densities=np.array([dens]))
+    return Problem(name="eigen_box", mesh=hex_box(n, -half, half), nuclides=[nuc], materials=[mat],
+                   block_material={0: 1},
+                   bcs={b: (abi.BC_REFLECT if reflective else abi.BC_VACUUM) for b in range(6)},
+                   source=point_source((0.0, 0.0, 0.0), 2.0e6), tallies=flux_tallies(),
+                   n_particles=n_particles, n_batches=n_batches)
+
+
 C2_NUCLIDES = [  # (A, sigma_p [b], sigma_a0 [b])
     (1.0, 20.0, 0.33),
     (12.0, 4.7, 0.0035),
@@ -423,6 +458,35 @@ def c2_tet_vacuum(n_lattice=54, n_particles=10_000_000, n_batches=1, n_points=50
                    block_material={b: b + 1 for b in range(n_materials)}, bcs={0: abi.BC_VACUUM},
                    source=volume_watt_source(),
                    tallies=tallies if tallies is not None else flux_tallies(both=False),
+                   n_particles=n_particles, n_batches=n_batches)
+
+
+def fissile_nuclide(n_points=50_000, seed=1235, jitter=0.0) -> NuclideData:
+    """A synthetic heavy fissile nuclide: A = 235, 1/v absorption of 680 b at
+    0.0253 eV plus SLBW resonances plus a flat 1.5 b (fast absorption), 85% of
+    absorption is fission, nu = 2.43, Watt fission spectrum (a = 0.988 MeV,
+    b = 2.249 /MeV)."""
+    n = synth_nuclide(n_points, 235.0, seed=seed, sigma_a0=680.0, sigma_p=10.0, jitter=jitter)
+    n = dataclasses.replace(n, absorption=n.absorption + 1.5, total=n.total + 1.5)
+    return fissile(n, 0.85, nu=(2.43, 0.0))
+
+
+def k2_tet_eigen(n_lattice=54, n_particles=10_000_000, n_batches=5, n_points=50_000, dens=(0.5, 1.0),
+                 fuel=0.4, n_materials=4) -> Problem:
+    """C2's unit cube and Voronoi materials, made a critical-ish system for
+    eigenvalue runs: the 4 C2 nuclides at densities U(dens) plus the fissile
+    nuclide at `fuel` /b-cm in every material, vacuum boundary; batch 1
+    starts from a uniform volumetric Watt source."""
+    mesh = tet_cube(n_lattice, seed=101, cache=n_lattice > 20)
+    mesh.subdomain = voronoi_blocks(mesh, n_materials, seed=303)
+    nucs = c2_nuclides(n_points, 4) + [fissile_nuclide(n_points)]
+    mats = c2_materials(n_materials, 4, dens=dens)
+    for m in mats:
+        m.nuclides = np.append(m.nuclides, np.int32(4)).astype(np.int32)
+        m.densities = np.append(m.densities, fuel)
+    return Problem(name=f"k2_tet4_eigen_{mesh.n_elem}", mesh=mesh, nuclides=nucs, materials=mats,
+                   block_material={b: b + 1 for b in range(n_materials)}, bcs={0: abi.BC_VACUUM},
+                   source=volume_watt_source(), tallies=flux_tallies(both=False),
                    n_particles=n_particles, n_batches=n_batches)
 
 
@@ -520,7 +584,8 @@ class Marshal:
             fis = None if n.fission is None else np.ascontiguousarray(n.fission, np.float64)
             self.keep += arrs + [fis]
             self.nuclides[i] = abi.Nuclide(n.energy.size, n.awr, *[abi.ptr(a, abi.C.c_double) for a in arrs],
-                                           abi.ptr(fis, abi.C.c_double))
+                                           abi.ptr(fis, abi.C.c_double), (abi.C.c_double * 2)(*n.nu),
+                                           (abi.C.c_double * 2)(*n.watt))
         self.materials = (abi.Material * len(p.materials))()
         for i, m in enumerate(p.materials):
             nu = np.ascontiguousarray(m.nuclides, np.int32)
@@ -580,8 +645,9 @@ def load_dump(path) -> Problem:
     """Problem written by `macaw-amd-opt --dump-problem` (the exact bytes the
     driver handed to the library), for replay on a CPU reference."""
     raw = open(path, "rb").read()
-    if raw[:8] != b"MCWD0001":
+    if raw[:8] not in (b"MCWD0001", b"MCWD0002"):
         raise ValueError(f"{path}: not a problem dump")
+    v2 = raw[:8] == b"MCWD0002"
     pos = [8]
 
     def i64():
@@ -617,7 +683,13 @@ def load_dump(path) -> Problem:
         awr = float(arr("<f8", 1)[0])
         npt = i64()
         E, t, a, s = (arr("<f8", npt) for _ in range(4))
-        nucs.append(NuclideData(awr=awr, energy=E, total=t, absorption=a, scatter=s))
+        n = NuclideData(awr=awr, energy=E, total=t, absorption=a, scatter=s)
+        if v2:
+            fis = arr("<f8", npt) if i64() else None
+            nu, watt = arr("<f8", 2), arr("<f8", 2)
+            n = dataclasses.replace(n, fission=fis, nu=(float(nu[0]), float(nu[1])),
+                                    watt=(float(watt[0]), float(watt[1])))
+        nucs.append(n)
     mats = []
     for _ in range(i64()):
         mid = i64()
@@ -626,10 +698,13 @@ def load_dump(path) -> Problem:
     src = abi.Source.from_buffer_copy(raw[pos[0]:pos[0] + abi.C.sizeof(abi.Source)])
     pos[0] += abi.C.sizeof(abi.Source)
     particles, batches, roundtrip, seed = i64(), i64(), i64(), i64()
+    run_mode, n_inactive = (i64(), i64()) if v2 else (abi.RUN_FIXED_SOURCE, 0)
     mesh = Mesh(xyz=np.ascontiguousarray(xyz), elem_type=etype, conn_offsets=off, conn=conn, subdomain=sub,
                 face_boundary=fb if nfb else None)
     p = Problem(name=os.path.basename(path), mesh=mesh, nuclides=nucs, materials=mats, block_material=bm,
                 bcs=bcs, source=src, tallies=[], n_particles=particles, n_batches=batches)
     p.seed_roundtrip = bool(roundtrip)
     p.master_seed = seed
+    p.run_mode = run_mode
+    p.n_inactive = n_inactive
     return p

@@ -18,6 +18,13 @@
  *       absorption, elastic scatter off a target at rest, rotate_angle)
  *   - tallies and batch close ................... src/userobjects/OpenMCTally.C:103-225,
  *       src/auxkernels/OpenMCTallyAux.C:110-203, src/userobjects/OpenMCStudy.C:374-402
+ *   - eigenvalue runs ........................... keff track-length estimator
+ *       (CollisionKernel.C:152-154), OpenMC's collision/absorption estimators and
+ *       create_fission_sites / sample_fission_neutron (called through event_collide,
+ *       CollisionKernel.C:185), fission bank resampling OpenMCStudy::synchronizeBanks
+ *       (src/userobjects/OpenMCStudy.C:454-607), generation and average keff
+ *       (OpenMCStudy.C:404-452, OpenMC calculate_generation_keff /
+ *       calculate_average_keff)
  *
  * Synthetic nuclides are pointwise tables: lin-lin interpolation at the
  * bracketing grid index (largest j with E_j <= E).  Elastic scattering is
@@ -64,6 +71,9 @@ inline uint64_t digest_step(uint64_t h, int64_t elem, int ev)
 struct Nuclide {
   double awr;
   std::vector<double> E, t, a, s, f;
+  bool fissionable = false;
+  double nu0 = 0.0, nu1 = 0.0;      // nu(E) = nu0 + nu1 E
+  double watt_a = 0.0, watt_b = 0.0; // fission spectrum
 };
 
 struct Material {
@@ -127,6 +137,13 @@ struct Oracle {
   double energy_max = 0, energy_min = 0;
   std::vector<Tally> tallies;
   std::vector<mcw_history> hist;
+  // eigenvalue runs
+  double keff = 1.0, k_sum = 0.0, k_sum_sq = 0.0;
+  int32_t n_active = 0;
+  int64_t eig_batch = 0, eig_n = 0;
+  std::vector<mcw_site> fbank;    // last batch's fission bank, (parent, progeny) order
+  std::vector<mcw_site> next_src; // next batch's source
+  mcw_keff_result kres{};
 };
 
 bool fail(Oracle *o, const std::string &m)
@@ -523,6 +540,20 @@ void sample_point_in_tet(const double *p0, const double *p1, const double *p2, c
   for (int k = 0; k < 3; ++k) out[k] = ((a * p0[k] + s * p1[k]) + t * p2[k]) + u * p3[k];
 }
 
+// OpenMC watt_spectrum: a Maxwellian w = -a (ln r1 + ln r2 cos^2(pi r3 / 2))
+// plus a^2 b / 4 + (2 r4 - 1) sqrt(a^2 b w), resampled while at or above the
+// top of the energy grid (after 1000 tries the grid minimum is taken).
+double sample_watt(const Oracle &o, double a, double b, uint64_t *seed)
+{
+  for (int it = 0;; ++it) {
+    double r1 = orc_prn(seed), r2 = orc_prn(seed), r3 = orc_prn(seed);
+    double c = orc_cos_halfpi(r3);
+    double wmax = -a * (orc_log(r1) + orc_log(r2) * c * c);
+    double E = wmax + 0.25 * a * a * b + (-1.0 + 2.0 * orc_prn(seed)) * std::sqrt(a * a * b * wmax);
+    if (E < o.energy_max || it >= 1000) return E < o.energy_max ? E : o.energy_min;
+  }
+}
+
 bool sample_source(const Oracle &o, uint64_t *seed, Site *st)
 {
   const mcw_source &S = o.src;
@@ -599,21 +630,7 @@ bool sample_source(const Oracle &o, uint64_t *seed, Site *st)
     return false;
   }
 
-  if (S.energy == MCW_ENERGY_MONO) {
-    st->E = S.E0;
-  } else {
-    const double a = S.watt_a, b = S.watt_b;
-    for (int it = 0;; ++it) {
-      double r1 = orc_prn(seed), r2 = orc_prn(seed), r3 = orc_prn(seed);
-      double c = orc_cos_halfpi(r3);
-      double wmax = -a * (orc_log(r1) + orc_log(r2) * c * c);
-      double E = wmax + 0.25 * a * a * b + (-1.0 + 2.0 * orc_prn(seed)) * std::sqrt(a * a * b * wmax);
-      if (E < o.energy_max || it >= 1000) {
-        st->E = E < o.energy_max ? E : o.energy_min;
-        break;
-      }
-    }
-  }
+  st->E = S.energy == MCW_ENERGY_MONO ? S.E0 : sample_watt(o, S.watt_a, S.watt_b, seed);
   return true;
 }
 
@@ -696,18 +713,59 @@ struct TraceSink {
   std::vector<double> *lengths = nullptr;
 };
 
+// One history's part of an eigenvalue batch: the keff its fission yield is
+// divided by, where its fission sites go, and its three keff estimator sums.
+struct EigenHistory {
+  double keff = 1.0;
+  bool tally = true;              // inactive batches score no tallies
+  const mcw_site *source = nullptr; // batch > 1: the history's source site
+  std::vector<mcw_site> *sites = nullptr;
+  double k_tl = 0.0, k_col = 0.0, k_abs = 0.0;
+};
+
+// Macroscopic total and nu-fission at (mat, E).
+double macro_total_nf(const Oracle &o, int mat, double E, double *nusf)
+{
+  *nusf = 0.0;
+  if (mat < 0) return 0.0;
+  const Material &m = o.mats[mat];
+  double t = 0.0, nf = 0.0;
+  for (size_t i = 0; i < m.nuc.size(); ++i) {
+    const Nuclide &n = o.nucs[m.nuc[i]];
+    Micro u;
+    micro_xs(n, E, &u);
+    t = t + m.dens[i] * u.t;
+    nf = nf + m.dens[i] * ((n.nu0 + n.nu1 * E) * u.f);
+  }
+  *nusf = nf;
+  return t;
+}
+
 void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int64_t i, Scratch &sc,
-                       Counters &cnt, mcw_history *rec, TraceSink *trace)
+                       Counters &cnt, mcw_history *rec, TraceSink *trace, EigenHistory *eg = nullptr)
 {
   const int64_t id = (batch - 1) * n_particles + i + 1;
   const uint64_t master = o.opt.master_seed;
   const bool rt = o.opt.seed_roundtrip != 0;
   const int64_t max_seg = o.opt.max_segments > 0 ? o.opt.max_segments : (int64_t)1 << 20;
   uint64_t seed = orc_init_seed(id, 0, master);
-  uint64_t sseed = orc_init_seed(id, 1, master);
   Site st;
-  sample_source(o, &sseed, &st);
+  if (eg && eg->source) {
+    // a fission site of the previous batch (OpenMC initialize_history takes
+    // source_bank[i]; only the tracking stream is seeded here)
+    const mcw_site &q = *eg->source;
+    st.x = q.r[0]; st.y = q.r[1]; st.z = q.r[2];
+    st.u = q.u[0]; st.v = q.u[1]; st.w = q.u[2];
+    st.E = q.E;
+    st.elem = q.element;
+  } else {
+    uint64_t sseed = orc_init_seed(id, 1, master);
+    sample_source(o, &sseed, &st);
+  }
   if (rt) seed = orc_seed_roundtrip(seed);
+  const bool tally_on = !eg || eg->tally;
+  double nusf = 0.0;
+  int32_t progeny = 0;
 
   double x = st.x, y = st.y, z = st.z, u = st.u, v = st.v, w = st.w, E = st.E;
   const double wgt = 1.0;
@@ -730,7 +788,7 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
     const int64_t seg_elem = elem;
     const int mat = o.elem_mat[elem];
     if (mat != last_mat || E != last_E) {
-      sigt = macro_total(o, mat, E);
+      sigt = eg ? macro_total_nf(o, mat, E, &nusf) : macro_total(o, mat, E);
       last_mat = mat;
       last_E = E;
     }
@@ -765,8 +823,11 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
     ++nseg;
     if (o.type[elem] == MCW_HEX8) ++cnt.hex_segments;
     cnt.track += wgt * track;
-    for (size_t k = 0; k < T.size(); ++k)
-      if (T[k].estimator == MCW_EST_TRACKLENGTH) score(o, T[k], sc.value[k], elem, E, wgt * track, mat);
+    // keff track-length estimator (CollisionKernel.C:152-154)
+    if (eg) eg->k_tl = eg->k_tl + (wgt * track) * nusf;
+    if (tally_on)
+      for (size_t k = 0; k < T.size(); ++k)
+        if (T[k].estimator == MCW_EST_TRACKLENGTH) score(o, T[k], sc.value[k], elem, E, wgt * track, mat);
 
     int ev;
     if (d > L) {
@@ -816,8 +877,11 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
       y = (y + L * v) - back * v;
       z = (z + L * w) - back * w;
       ++ncoll;
-      for (size_t k = 0; k < T.size(); ++k)
-        if (T[k].estimator == MCW_EST_COLLISION) score(o, T[k], sc.value[k], elem, E, wgt / sigt, mat);
+      if (tally_on)
+        for (size_t k = 0; k < T.size(); ++k)
+          if (T[k].estimator == MCW_EST_COLLISION) score(o, T[k], sc.value[k], elem, E, wgt / sigt, mat);
+      // keff collision estimator (OpenMC Particle::event_collide)
+      if (eg) eg->k_col = eg->k_col + (wgt * nusf) / sigt;
       // sample nuclide
       const Material &m = o.mats[mat];
       double cutoff = orc_prn(&seed) * sigt;
@@ -834,10 +898,41 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
       }
       const Nuclide &nuc = o.nucs[m.nuc[pick]];
       micro_xs(nuc, E, &mu);
+      const double nuf = (nuc.nu0 + nuc.nu1 * E) * mu.f;
+      if (eg && nuc.fissionable && mu.t > 0.0) {
+        // OpenMC create_fission_sites: nu_t = w / keff * nu sigma_f / sigma_t
+        // neutrons expected, floor(nu_t) plus one with probability frac(nu_t)
+        const double nu_t = (((wgt / eg->keff) * 1.0) * nuf) / mu.t;
+        int64_t nu = nu_t < 9.0e15 ? (int64_t)nu_t : (int64_t)9.0e15;
+        if (orc_prn(&seed) <= nu_t - (double)nu) ++nu;
+        nu = std::min<int64_t>(nu, ((int64_t)1 << 24) - progeny); // progeny index: 24 bits
+        // more than the bank holds (3 n_particles) cannot be kept
+        for (int64_t q = 0; q < nu && q < 3 * n_particles; ++q) {
+          // sample_fission_neutron: isotropic (mu, then phi), the prompt /
+          // delayed choice (no delayed data: always prompt), then the energy
+          mcw_site f;
+          f.r[0] = x; f.r[1] = y; f.r[2] = z;
+          const double fmu = 2.0 * orc_prn(&seed) - 1.0;
+          double sphi, cphi;
+          orc_sincos_2pi(orc_prn(&seed), &sphi, &cphi);
+          const double sn = std::sqrt(1.0 - fmu * fmu);
+          f.u[0] = fmu;
+          f.u[1] = sn * cphi;
+          f.u[2] = sn * sphi;
+          (void)orc_prn(&seed);
+          f.E = sample_watt(o, nuc.watt_a, nuc.watt_b, &seed);
+          f.parent = i;
+          f.progeny = progeny++;
+          f.element = (int32_t)elem;
+          eg->sites->push_back(f);
+        }
+      }
       if (mu.a > orc_prn(&seed) * mu.t) {
         ev = EV_ABSORB;
         fate = MCW_FATE_ABSORBED;
         ++cnt.absorptions;
+        // keff absorption estimator (OpenMC absorption())
+        if (eg) eg->k_abs = eg->k_abs + (wgt * nuf) / mu.a;
       } else {
         double mu_cm = 2.0 * orc_prn(&seed) - 1.0;
         double Ep, mu_lab;
@@ -880,6 +975,63 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
     rec->elem = (int32_t)elem;
     rec->fate = fate;
   }
+}
+
+// Sum of v in a fixed order that a GPU reduction can follow: blocks of 1024
+// consecutive values (missing ones 0), each folded by 256 lanes as
+// ((v[t] + v[t+256]) + v[t+512]) + v[t+768], then halved pairwise
+// a[t] += a[t+s] for s = 128, 64, ..., 1; repeated over the block sums until
+// one value is left.
+double tree_sum(std::vector<double> v)
+{
+  if (v.empty()) return 0.0;
+  for (;;) {
+    const int64_t n = (int64_t)v.size();
+    const int64_t nb = (n + 1023) / 1024;
+    std::vector<double> out((size_t)nb);
+    for (int64_t b = 0; b < nb; ++b) {
+      auto at = [&](int64_t i) { return i < n ? v[(size_t)i] : 0.0; };
+      double a[256];
+      for (int t = 0; t < 256; ++t) {
+        const int64_t i = b * 1024 + t;
+        a[t] = ((at(i) + at(i + 256)) + at(i + 512)) + at(i + 768);
+      }
+      for (int s2 = 128; s2 > 0; s2 >>= 1)
+        for (int t = 0; t < s2; ++t) a[t] = a[t] + a[t + s2];
+      out[(size_t)b] = a[0];
+    }
+    if (nb == 1) return out[0];
+    v.swap(out);
+  }
+}
+
+// OpenMCStudy::synchronizeBanks on one rank (OpenMCStudy.C:454-607): every
+// banked site is copied floor(n / total) times when fewer than n were banked,
+// and once more with probability p = (n mod total, or n) / total, one draw per
+// site from the tracking stream of seed id `batch`; then the list is cut to n,
+// or filled up with the last sites of the bank.
+std::vector<mcw_site> synchronize_bank(const std::vector<mcw_site> &bank, int64_t batch, int64_t n,
+                                       uint64_t master)
+{
+  const int64_t total = (int64_t)bank.size();
+  uint64_t seed = orc_init_seed(batch, 0, master);
+  const int64_t needed = total < n ? n % total : n;
+  const double p_sample = (double)needed / (double)total;
+  std::vector<mcw_site> out;
+  out.reserve((size_t)n);
+  for (int64_t i = 0; i < total; ++i) {
+    if (total < n)
+      for (int64_t j = 1; j <= n / total; ++j) out.push_back(bank[(size_t)i]);
+    if (orc_prn(&seed) < p_sample) out.push_back(bank[(size_t)i]);
+  }
+  const int64_t fin = (int64_t)out.size();
+  if (fin > n) {
+    out.resize((size_t)n);
+  } else if (fin < n) {
+    const int64_t more = n - fin;
+    for (int64_t i = 0; i < more; ++i) out.push_back(bank[(size_t)std::max<int64_t>(0, total - more + i)]);
+  }
+  return out;
 }
 
 bool check_ready(Oracle *o)
@@ -962,7 +1114,14 @@ int orc_xs_set(void *h, int32_t n, const mcw_nuclide *nuc)
     q.t.assign(nuc[i].total, nuc[i].total + np);
     q.a.assign(nuc[i].absorption, nuc[i].absorption + np);
     q.s.assign(nuc[i].scatter, nuc[i].scatter + np);
-    if (nuc[i].fission) q.f.assign(nuc[i].fission, nuc[i].fission + np);
+    if (nuc[i].fission) {
+      q.f.assign(nuc[i].fission, nuc[i].fission + np);
+      q.fissionable = true;
+    }
+    q.nu0 = nuc[i].nu[0];
+    q.nu1 = nuc[i].nu[1];
+    q.watt_a = nuc[i].watt[0];
+    q.watt_b = nuc[i].watt[1];
     o->nucs.push_back(std::move(q));
   }
   return 0;
@@ -1087,6 +1246,20 @@ int orc_run_batch(void *h, int64_t batch, int64_t n_particles, int32_t threads, 
 {
   Oracle *o = (Oracle *)h;
   if (!check_ready(o)) return 1;
+  const bool eig = o->opt.run_mode == MCW_RUN_EIGENVALUE;
+  bool active = true;
+  if (eig) {
+    if (batch == 1) {
+      o->keff = 1.0;
+      o->k_sum = o->k_sum_sq = 0.0;
+      o->n_active = 0;
+      o->next_src.clear();
+    } else if (batch != o->eig_batch + 1 || n_particles != o->eig_n ||
+               (int64_t)o->next_src.size() != n_particles) {
+      return fail(o, "eigenvalue batches run in order 1, 2, ... with one particle count") ? 0 : 1;
+    }
+    active = batch > o->opt.n_inactive;
+  }
   int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
   if (nt > n_particles) nt = (int)std::max<int64_t>(1, n_particles);
   if (record) o->hist.assign(n_particles, mcw_history());
@@ -1097,10 +1270,25 @@ int orc_run_batch(void *h, int64_t batch, int64_t n_particles, int32_t threads, 
     s.value.resize(o->tallies.size());
     for (size_t k = 0; k < o->tallies.size(); ++k) s.value[k].assign(o->tallies[k].value.size(), 0.0);
   }
+  std::vector<std::vector<mcw_site>> tsites(eig ? nt : 0);
+  std::vector<double> k_tl(eig ? n_particles : 0), k_col(eig ? n_particles : 0), k_abs(eig ? n_particles : 0);
   auto work = [&](int t) {
     int64_t lo = n_particles * t / nt, hi = n_particles * (t + 1) / nt;
-    for (int64_t i = lo; i < hi; ++i)
-      transport_history(*o, batch, n_particles, i, scr[t], cnt[t], record ? &o->hist[i] : nullptr, nullptr);
+    for (int64_t i = lo; i < hi; ++i) {
+      if (!eig) {
+        transport_history(*o, batch, n_particles, i, scr[t], cnt[t], record ? &o->hist[i] : nullptr, nullptr);
+        continue;
+      }
+      EigenHistory eh;
+      eh.keff = o->keff;
+      eh.tally = active;
+      eh.source = batch > 1 ? &o->next_src[(size_t)i] : nullptr;
+      eh.sites = &tsites[t];
+      transport_history(*o, batch, n_particles, i, scr[t], cnt[t], record ? &o->hist[i] : nullptr, nullptr, &eh);
+      k_tl[(size_t)i] = eh.k_tl;
+      k_col[(size_t)i] = eh.k_col;
+      k_abs[(size_t)i] = eh.k_abs;
+    }
   };
   std::vector<std::thread> pool;
   for (int t = 1; t < nt; ++t) pool.emplace_back(work, t);
@@ -1119,7 +1307,8 @@ int orc_run_batch(void *h, int64_t batch, int64_t n_particles, int32_t threads, 
     tot.hex_segments += cnt[t].hex_segments;
     tot.track += cnt[t].track;
   }
-  // batch close: VALUE / total weight -> SUM, SUM_SQ (OpenMC accumulate)
+  // batch close: VALUE / total weight -> SUM, SUM_SQ (OpenMC accumulate);
+  // an inactive batch scores nothing
   const double norm = (double)n_particles;
   for (size_t k = 0; k < o->tallies.size(); ++k) {
     Tally &T = o->tallies[k];
@@ -1127,10 +1316,44 @@ int orc_run_batch(void *h, int64_t batch, int64_t n_particles, int32_t threads, 
       double v = 0.0;
       for (int t = 0; t < nt; ++t) v += scr[t].value[k][b];
       T.value[b] = v;
+      if (!active) continue;
       double val = v / norm;
       T.sum[b] += val;
       T.sum_sq[b] += val * val;
     }
+  }
+  if (eig) {
+    o->fbank.clear();
+    for (auto &v : tsites) o->fbank.insert(o->fbank.end(), v.begin(), v.end());
+    mcw_keff_result &K = o->kres;
+    memset(&K, 0, sizeof K);
+    K.batch = batch;
+    K.fission_sites = (int64_t)o->fbank.size();
+    K.keff_used = o->keff;
+    K.k_generation = tree_sum(k_tl) / norm;
+    K.k_collision = tree_sum(k_col) / norm;
+    K.k_absorption = tree_sum(k_abs) / norm;
+    // calculate_average_keff: the generation value while inactive, else the
+    // mean over active batches and the standard deviation of that mean
+    if (active) {
+      ++o->n_active;
+      o->k_sum += K.k_generation;
+      o->k_sum_sq += K.k_generation * K.k_generation;
+      const double n = (double)o->n_active;
+      o->keff = o->k_sum / n;
+      K.keff_std = o->n_active > 1 ? std::sqrt((o->k_sum_sq / n - o->keff * o->keff) / (n - 1.0)) : 0.0;
+    } else {
+      o->keff = K.k_generation;
+    }
+    K.keff = o->keff;
+    K.n_active = o->n_active;
+    o->eig_batch = batch;
+    o->eig_n = n_particles;
+    if (o->fbank.empty()) {
+      o->next_src.clear();
+      return fail(o, "No fission sites sampled on any process, cannot sample a new batch") ? 0 : 1;
+    }
+    o->next_src = synchronize_bank(o->fbank, batch, n_particles, o->opt.master_seed);
   }
   if (out) {
     memset(out, 0, sizeof *out);
@@ -1145,6 +1368,43 @@ int orc_run_batch(void *h, int64_t batch, int64_t n_particles, int32_t threads, 
     out->hex_segments = tot.hex_segments;
     out->track_length = tot.track;
   }
+  return 0;
+}
+
+int orc_keff_get(void *h, mcw_keff_result *out)
+{
+  *out = ((Oracle *)h)->kres;
+  return 0;
+}
+
+static int copy_sites(const std::vector<mcw_site> &v, mcw_site *out, size_t max, int64_t *n_out)
+{
+  if (n_out) *n_out = (int64_t)v.size();
+  if (out) std::copy(v.begin(), v.begin() + std::min(max, v.size()), out);
+  return 0;
+}
+
+int orc_fission_bank_get(void *h, mcw_site *out, size_t max, int64_t *n_out)
+{
+  return copy_sites(((Oracle *)h)->fbank, out, max, n_out);
+}
+
+int orc_source_bank_get(void *h, mcw_site *out, size_t max, int64_t *n_out)
+{
+  return copy_sites(((Oracle *)h)->next_src, out, max, n_out);
+}
+
+/* The fixed-order sum the GPU reduces per-history keff scores with. */
+double orc_tree_sum(const double *v, int64_t n) { return tree_sum(std::vector<double>(v, v + n)); }
+
+/* synchronize_bank on caller sites, for tests. */
+int orc_synchronize_bank(const mcw_site *bank, int64_t total, int64_t batch, int64_t n, uint64_t master,
+                         mcw_site *out)
+{
+  if (total < 1 || n < 1) return 1;
+  std::vector<mcw_site> b(bank, bank + total);
+  std::vector<mcw_site> r = synchronize_bank(b, batch, n, master);
+  std::copy(r.begin(), r.end(), out);
   return 0;
 }
 

@@ -79,6 +79,15 @@ def lib():
         L.orc_seed_roundtrip_x.argtypes = [u64]
         L.orc_rotate_angle_x.argtypes = [dp, C.c_double, C.c_double]
         L.orc_elastic_x.argtypes = [C.c_double, C.c_double, C.c_double, dp, dp]
+        sp = C.POINTER(abi.Site)
+        for name in ("orc_keff_get", "orc_fission_bank_get", "orc_source_bank_get", "orc_synchronize_bank"):
+            getattr(L, name).restype = C.c_int
+        L.orc_keff_get.argtypes = [vp, C.POINTER(abi.KeffResult)]
+        L.orc_fission_bank_get.argtypes = [vp, C.c_void_p, C.c_size_t, C.POINTER(i64)]
+        L.orc_source_bank_get.argtypes = [vp, C.c_void_p, C.c_size_t, C.POINTER(i64)]
+        L.orc_tree_sum.restype = C.c_double
+        L.orc_tree_sum.argtypes = [dp, i64]
+        L.orc_synchronize_bank.argtypes = [C.c_void_p, i64, i64, i64, u64, C.c_void_p]
         _lib = L
     return _lib
 
@@ -87,15 +96,33 @@ class OracleError(RuntimeError):
     pass
 
 
+def tree_sum(v):
+    """The fixed-order sum of the per-history keff scores."""
+    v = np.ascontiguousarray(v, np.float64)
+    return lib().orc_tree_sum(v.ctypes.data_as(C.POINTER(C.c_double)), v.size)
+
+
+def synchronize_bank(bank, batch, n, master_seed=1):
+    """OpenMCStudy::synchronizeBanks of `bank` (abi.site_dtype()) to n sites."""
+    bank = np.ascontiguousarray(bank, abi.site_dtype())
+    out = np.zeros(n, abi.site_dtype())
+    if lib().orc_synchronize_bank(bank.ctypes.data, bank.size, batch, n, master_seed, out.ctypes.data) != 0:
+        raise OracleError("synchronize_bank: empty bank or n < 1")
+    return out
+
+
 class Oracle:
     """History-based CPU transport of a Problem."""
 
-    def __init__(self, problem: Problem = None, seed_roundtrip=False, max_segments=0, master_seed=1):
+    def __init__(self, problem: Problem = None, seed_roundtrip=False, max_segments=0, master_seed=1,
+                 eigenvalue=False, n_inactive=0):
         L = lib()
         o = abi.Options()
         o.seed_roundtrip = int(seed_roundtrip)
         o.max_segments = max_segments
         o.master_seed = master_seed
+        o.run_mode = abi.RUN_EIGENVALUE if eigenvalue else abi.RUN_FIXED_SOURCE
+        o.n_inactive = n_inactive
         self.h = L.orc_create(C.byref(o))
         self.L = L
         self.problem = None
@@ -132,6 +159,24 @@ class Oracle:
         out = np.zeros(n, abi.history_dtype())
         self._chk(self.L.orc_history_get(self.h, out.ctypes.data, n))
         return out
+
+    def keff(self):
+        k = abi.KeffResult()
+        self._chk(self.L.orc_keff_get(self.h, C.byref(k)))
+        return k.as_dict()
+
+    def _sites(self, fn):
+        n = C.c_int64(0)
+        self._chk(fn(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, abi.site_dtype())
+        self._chk(fn(self.h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
+    def fission_bank(self):
+        return self._sites(self.L.orc_fission_bank_get)
+
+    def source_bank(self):
+        return self._sites(self.L.orc_source_bank_get)
 
     def tally(self, tally_id, kind=abi.RESULT_SUM, size=None):
         t = [t for t in self.problem.tallies if t.id == tally_id][0]

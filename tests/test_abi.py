@@ -35,6 +35,38 @@ def test_struct_sizes_match_header():
     assert C.sizeof(abi.History) == 72
     assert C.sizeof(abi.BatchStats) == 120
     assert C.sizeof(abi.TraceRecord) == 88 == abi.trace_dtype().itemsize
+    assert C.sizeof(abi.Site) == 72 == abi.site_dtype().itemsize
+
+
+STRUCTS = {"mcw_options": abi.Options, "mcw_nuclide": abi.Nuclide, "mcw_material": abi.Material,
+           "mcw_source": abi.Source, "mcw_filter": abi.Filter, "mcw_batch_stats": abi.BatchStats,
+           "mcw_history": abi.History, "mcw_trace_record": abi.TraceRecord, "mcw_site": abi.Site,
+           "mcw_keff_result": abi.KeffResult, "mcw_request": abi.Request,
+           "mcw_request_status": abi.RequestStatus}
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """Every ctypes mirror has the C compiler's size and field offsets."""
+    import ctypes as C
+    import subprocess
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "macaw_amd.h"', "int main(void) {"]
+    for cname, py in STRUCTS.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("  return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        c, f, v = line.split()
+        got[(c, f)] = int(v)
+    for cname, py in STRUCTS.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
 
 
 def test_default_options():

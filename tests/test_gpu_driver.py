@@ -103,3 +103,23 @@ def test_driver_ray_output_matches_gold_layout_and_source_seeds(tmp_path):
         assert np.all(vals["aux_seed_source"][sel].astype(np.uint64) == np.uint64(src_gold[rid]))
         assert np.all(vals["aux_id"][sel] == rid - 59)
     f.close()
+
+
+def test_eigen_driver_matches_oracle(tmp_path, oracle_lib):
+    """run_mode = eigenvalue with one inactive batch (the reference study's
+    settings): per-batch keff and the fission-power aux values equal the
+    oracle's."""
+    base, p = run_driver("openmc_study_eigen.i", tmp_path, "MonteCarlo/particles=400")
+    p.tallies = [P.TallySpec(id=1, estimator=abi.EST_TRACKLENGTH, filters=[{"type": abi.FILTER_CELL}],
+                             scores=[abi.SCORE_FISSION])]
+    o = oracle_lib.Oracle(p, eigenvalue=True, n_inactive=p.n_inactive)
+    kg, kf = [], []
+    for b in (1, 2, 3):
+        o.run_batch(b, p.n_particles, threads=4)
+        k = o.keff()
+        kg.append(k["k_generation"])
+        kf.append(k["keff"])
+    assert np.array_equal(read_col(base + ".csv", "k_generation"), np.array(kg))
+    assert np.array_equal(read_col(base + ".csv", "keff"), np.array(kf))
+    power = read_col(base + "_power.csv", "power")
+    assert power.sum() > 0 and close(power, o.tally(1))

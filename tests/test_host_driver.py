@@ -26,7 +26,7 @@ def run(args, cwd=None):
                           timeout=120)
 
 
-@pytest.mark.parametrize("name", ["c1_hex8_reflective.i", "openmc_tally.i"])
+@pytest.mark.parametrize("name", ["c1_hex8_reflective.i", "openmc_tally.i", "openmc_study_eigen.i"])
 def test_inputs_validate(name):
     r = run(["-i", os.path.join(INPUTS, name), "--check-input"])
     assert r.returncode == 0, r.stderr
@@ -47,6 +47,12 @@ ERRORS = [
      "Only CONST MONOMIAL temperatures (current order: 1) are supported."),
     ("c1_hex8_reflective.i", "RayBCs/reflect/boundary=back nowhere", "the boundary 'nowhere' was not found"),
     ("c1_hex8_reflective.i", "MonteCarlo/Materials/graphite/nuclides=U235", "unknown nuclide 'U235'"),
+    ("openmc_study_eigen.i", "MonteCarlo/run_mode=criticality", "run_mode must be eigenvalue or fixed source"),
+    ("openmc_study_eigen.i", "MonteCarlo/inactive=3", "an eigenvalue run needs more batches than inactive batches"),
+    ("openmc_study_eigen.i", "MonteCarlo/mode=plain", "eigenvalue runs need mode = batched"),
+    ("openmc_study_eigen.i", "MonteCarlo/Nuclides/U235/fission_fraction=1.5", "the fission fraction must be in [0, 1]"),
+    ("openmc_study_eigen.i", "MonteCarlo/Nuclides/U235/nu=1 2 3", "nu takes one or two coefficients"),
+    ("openmc_study_eigen.i", "MonteCarlo/source/upper_right=0 0", "a box corner needs 3 coordinates"),
 ]
 
 
@@ -98,6 +104,24 @@ def test_hit_syntax(tmp_path):
     bad.write_text("[Mesh]\n  [gmg]\n    type = GeneratedMeshGenerator\n  []\n")
     r = run(["-i", str(bad), "--check-input"])
     assert r.returncode == 1 and "is not closed" in r.stderr
+
+
+def test_eigen_dump_replays_on_the_oracle(tmp_path, oracle_lib):
+    d = tmp_path / "eig.dump"
+    r = run(["-i", os.path.join(INPUTS, "openmc_study_eigen.i"), "MonteCarlo/particles=200", "--dump-problem", str(d)])
+    assert r.returncode == 0, r.stderr
+    p = P.load_dump(str(d))
+    assert p.run_mode == abi.RUN_EIGENVALUE and p.n_inactive == 1 and p.n_batches == 3
+    u235 = p.nuclides[0]
+    assert u235.fission is not None and u235.nu == (2.43, 0.0) and u235.watt == (0.988e6, 2.249e-6)
+    assert np.allclose(u235.fission, 0.85 * u235.absorption, rtol=1e-15)
+    assert all(n.fission is None for n in p.nuclides[1:])
+    assert p.source.space == abi.SPACE_BOX and list(p.source.box_hi) == [0.62992] * 3
+    o = oracle_lib.Oracle(p, eigenvalue=True, n_inactive=p.n_inactive)
+    for b in (1, 2, 3):
+        o.run_batch(b, 200, threads=2)
+    k = o.keff()
+    assert k["n_active"] == 2 and k["fission_sites"] > 0 and 0.5 < k["keff"] < 3.0
 
 
 def test_dump_replays_on_the_oracle(tmp_path, oracle_lib):
