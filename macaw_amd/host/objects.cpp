@@ -597,7 +597,7 @@ void OpenMCStudy::postExecuteStudy()
               << _stats.time_ms << " ms" << std::endl;
   if (_fe.options.run_mode == MCW_RUN_EIGENVALUE) {
     check(_ctx, mcw_keff_get(_ctx, &_keff), "mcw_keff_get");
-    // OpenMC's output is silenced, so keff is printed (OpenMCStudy.C:397-400)
+    // OpenMC's output is silenced, so keff is printed (OpenMCStudy.C:398-401)
     std::cout << "Keff " << _keff.keff << " (" << _keff.keff_std << ") Generation: " << _keff.k_generation
               << std::endl;
   }
