@@ -231,7 +231,10 @@ def timed_steps(ctx, N, steps, warmup, label):
 
 
 def roofline_of(steps, N, n_nuc, b_seg, b_hex=348.0):
-    """B_seg applies to TET4 segments, b_hex to HEX8 segments (FP64 planes)."""
+    """B_seg applies to TET4 segments, b_hex to HEX8 segments: 348 B with FP64
+    planes (particle 76 + 36, six planes 192 + six neighbours and material 28,
+    tally 16), 252 B from the FP32 record (planes 96 + neighbours and
+    material 28)."""
     segs = sum(s["segments"] for s in steps)
     hexs = sum(s["hex_segments"] for s in steps)
     colls = sum(s["collisions"] for s in steps)
@@ -330,7 +333,7 @@ def main():
         mctx = Context(prob, mode=abi.MODE_BATCHED, precision=abi.PREC_MIXED)
         mctx.set_options(**tuning)
         msteps, mwall = timed_steps(mctx, N, args.steps, args.warmup, "mixed")
-        mroof = roofline_of(msteps, N, nuc_per_mat, 212.0)
+        mroof = roofline_of(msteps, N, nuc_per_mat, 212.0, b_hex=252.0)
         mroof["kernel"] = "k_persistent<4, mixed>" if all_tet else "k_persistent<0, mixed>"
         mixed = {"histories_per_s": N * args.steps / mwall,
                  "segments_per_s": sum(s["segments"] for s in msteps) / mwall,

@@ -167,7 +167,7 @@ struct mcw_ctx {
   bool dirty = true;
 
   // device problem
-  DevBuf plane, erec, mtet, split, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
+  DevBuf plane, erec, mtet, mhex, split, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf cell_start, cell_elem;
   DevBuf grid_off, npts, rec, fission, fis, fissionable, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
@@ -348,6 +348,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
     }
     erec[2 * i] = make_int4(code[0], code[1], code[2], code[3]);
     erec[2 * i + 1] = make_int4(code[4], code[5], emat[o], ns);
+    if (!m.uniform_sides && i >= m.n_tet) memcpy(&planes[slot + 6], &erec[2 * i], 2 * sizeof(int4));
   }
   c->plane.upload(planes, s);
   c->erec.upload(erec, s);
@@ -383,6 +384,24 @@ void prepare(mcw_ctx *c, bool need_source = true)
       R.mat = erec[2 * i + 1].z;
     }
     c->mtet.upload(mt, s);
+  }
+  // mixed meshes: mixed-precision records of the HEX8 (internal ids n_tet ..)
+  if (!m.uniform_sides && ne > m.n_tet) {
+    std::vector<MixedHex> mh(ne - m.n_tet);
+    for (int64_t i = m.n_tet; i < ne; ++i) {
+      MixedHex &R = mh[i - m.n_tet];
+      memset(&R, 0, sizeof R);
+      const int64_t s0 = slot_of(i);
+      for (int sd = 0; sd < 6; ++sd) {
+        const double4 q = planes[s0 + sd];
+        R.pl[sd] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
+      }
+      const int4 a = erec[2 * i], b = erec[2 * i + 1];
+      R.code[0] = a.x; R.code[1] = a.y; R.code[2] = a.z; R.code[3] = a.w;
+      R.code[4] = b.x; R.code[5] = b.y;
+      R.mat = b.z;
+    }
+    c->mhex.upload(mh, s);
   }
   c->bc_kind.upload(kinds, s);
   c->cdf.upload(m.cdf, s);
@@ -512,6 +531,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P.mesh.int_of_orig = c->int_of_orig.as<int32_t>();
   P.mesh.orig_of_int = c->orig_of_int.as<int32_t>();
   P.mesh.mtet = m.n_tet > 0 ? c->mtet.as<MixedTet>() : nullptr;
+  P.mesh.mhex = !m.uniform_sides && m.n_elem > m.n_tet ? c->mhex.as<MixedHex>() : nullptr;
   P.mesh.n_tet = m.n_tet;
   P.mesh.split = m.splits.empty() ? nullptr : c->split.as<SplitRec>();
   for (int k = 0; k < 3; ++k) {

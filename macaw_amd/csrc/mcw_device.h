@@ -395,47 +395,26 @@ __device__ __forceinline__ int exit_side(const double4 *__restrict__ pl, double 
   return fx;
 }
 
-__device__ __forceinline__ int exit_side_dyn(const double4 *__restrict__ pl, int ns, double x, double y,
-                                             double z, double u, double v, double w, int inface,
-                                             double &L)
-{
-  L = __builtin_inf();
-  int fx = -1;
-  for (int s = 0; s < ns; ++s) {
-    const double4 q = pl[s];
-    double den = (q.x * u + q.y * v) + q.z * w;
-    if (s == inface || !(den > 0.0)) continue;
-    double num = q.w - ((q.x * x + q.y * y) + q.z * z);
-    double t = num / den;
-    if (!(t > 0.0)) t = 0.0;
-    if (t < L) {
-      L = t;
-      fx = s;
-    }
-  }
-  return fx;
-}
-
-// FP32 exit side of a TET4 from a mixed-precision record.  Each candidate
+// FP32 exit side from NSD FP32 planes, the position given relative to the
+// planes' origin (dx, dy, dz) with S = |dx| + |dy| + |dz|.  Each candidate
 // distance t carries an error bound e (evaluation and rounding of FP32
-// operands: |dt| <= (16u(|c'| + |dx|+|dy|+|dz|) + 16u|t|)/den + 2u|t|,
-// u = 2^-24).  When the best candidate's interval overlaps another's, or a
-// side is within 1e-5 of grazing, `recheck` is set and the caller must redo
-// the selection in FP64; otherwise the chosen side is the one FP64 would
-// choose, and L is the FP32 distance.
-__device__ __forceinline__ int exit_side_p32(const MixedTet &R, double x, double y, double z, double u, double v,
-                                             double w, int inface, double &L, bool &recheck)
+// operands: |dt| <= (16u(|c| + S) + 16u|t|)/den + 2u|t|, u = 2^-24).  When
+// the best candidate's interval overlaps another's, or a side is within 1e-5
+// of grazing, `recheck` is set and the caller must redo the selection in
+// FP64; otherwise the chosen side is the one FP64 would choose, and L is the
+// FP32 distance.
+template <int NSD>
+__device__ __forceinline__ int exit_side_f32(const float4 *__restrict__ pl, float dx, float dy, float dz, float S,
+                                             double u, double v, double w, int inface, double &L, bool &recheck)
 {
   const float uu = 0x1p-24f;
-  const float dx = (float)(x - R.v0[0]), dy = (float)(y - R.v0[1]), dz = (float)(z - R.v0[2]);
   const float ux = (float)u, uy = (float)v, uz = (float)w;
-  const float S = fabsf(dx) + fabsf(dy) + fabsf(dz);
-  float t[4], e[4];
-  bool ok[4];
+  float t[NSD], e[NSD];
+  bool ok[NSD];
   bool graze = false;
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const float4 q = R.pl[s];
+  for (int s = 0; s < NSD; ++s) {
+    const float4 q = pl[s];
     const float den = (q.x * ux + q.y * uy) + q.z * uz;
     const float num = q.w - ((q.x * dx + q.y * dy) + q.z * dz);
     float ts = num / den;
@@ -448,7 +427,7 @@ __device__ __forceinline__ int exit_side_p32(const MixedTet &R, double x, double
   int fx = -1;
   float best = __builtin_inff();
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
+  for (int s = 0; s < NSD; ++s)
     if (ok[s] && t[s] < best) {
       best = t[s];
       fx = s;
@@ -457,12 +436,30 @@ __device__ __forceinline__ int exit_side_p32(const MixedTet &R, double x, double
   if (fx >= 0) {
     const float hi = best + e[fx];
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < NSD; ++s)
       if (ok[s] && s != fx && t[s] - e[s] <= hi) overlap = true;
   }
   recheck = graze || overlap || fx < 0;
   L = (double)best;
   return fx;
+}
+
+// FP32 exit side of a TET4 (planes relative to its vertex v0).
+__device__ __forceinline__ int exit_side_p32(const MixedTet &R, double x, double y, double z, double u, double v,
+                                             double w, int inface, double &L, bool &recheck)
+{
+  const float dx = (float)(x - R.v0[0]), dy = (float)(y - R.v0[1]), dz = (float)(z - R.v0[2]);
+  const float S = fabsf(dx) + fabsf(dy) + fabsf(dz);
+  return exit_side_f32<4>(R.pl, dx, dy, dz, S, u, v, w, inface, L, recheck);
+}
+
+// FP32 exit side of a HEX8 (absolute planes).
+__device__ __forceinline__ int exit_side_p32(const MixedHex &R, double x, double y, double z, double u, double v,
+                                             double w, int inface, double &L, bool &recheck)
+{
+  const float dx = (float)x, dy = (float)y, dz = (float)z;
+  const float S = fabsf(dx) + fabsf(dy) + fabsf(dz);
+  return exit_side_f32<6>(R.pl, dx, dy, dz, S, u, v, w, inface, L, recheck);
 }
 
 // OpenMC rotate_angle with phi = 2 pi xi.

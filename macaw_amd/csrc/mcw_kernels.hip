@@ -388,7 +388,7 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
   bool fp32 = false;
   if (P32) {
     // FP32 record of a TET4 (every element of a TET4 mesh; in a mixed mesh
-    // the ids below n_tet; HEX8 take the FP64 path)
+    // the ids below n_tet) or of a HEX8 of a mixed mesh
     if (NS == 4 || st.elem < M.n_tet) {
       const MixedTet &R = M.mtet[st.elem];
       fp32 = true;
@@ -403,19 +403,41 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
         fx = exit_side<4>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
         ++rechecks;
       }
+    } else if (NS == 0) {
+      const MixedHex &R = M.mhex[st.elem - M.n_tet];
+      fp32 = true;
+      if (in_hex) *in_hex = true;
+      r0 = *(const int4 *)R.code;
+      const int2 c45 = *(const int2 *)(R.code + 4);
+      r1 = make_int4(c45.x, c45.y, R.mat, 6);
+      mat = R.mat;
+      slot0 = slot_of<0>(M, st.elem);
+      bool recheck;
+      fx = exit_side_p32(R, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L, recheck);
+      if (recheck) {
+        fx = exit_side<6>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+        ++rechecks;
+      }
     }
   }
   if (!fp32) {
-    r0 = M.erec[2 * (int64_t)st.elem];
-    r1 = M.erec[2 * (int64_t)st.elem + 1];
-    if (NS == 0 && in_hex) *in_hex = st.elem >= M.n_tet;
-    mat = r1.z;
-    if (NS > 0) {
-      slot0 = (int64_t)st.elem * NS;
-      fx = exit_side<NS>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
-    } else {
+    if (NS == 0 && st.elem >= M.n_tet) {
+      // HEX8 of a mixed mesh: its neighbour record sits in slot 6 of its
+      // plane block (one cache line fewer per segment)
       slot0 = slot_of<0>(M, st.elem);
-      fx = exit_side_dyn(M.plane + slot0, r1.w, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+      const int4 *er = (const int4 *)(M.plane + slot0 + 6);
+      r0 = er[0];
+      r1 = er[1];
+      if (in_hex) *in_hex = true;
+      mat = r1.z;
+      fx = exit_side<6>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+    } else {
+      r0 = M.erec[2 * (int64_t)st.elem];
+      r1 = M.erec[2 * (int64_t)st.elem + 1];
+      mat = r1.z;
+      slot0 = (int64_t)st.elem * (NS > 0 ? NS : 4); // TET4 are first in a mixed mesh
+      if (NS == 6) fx = exit_side<6>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
+      else fx = exit_side<4>(M.plane + slot0, st.x, st.y, st.z, st.u, st.v, st.w, st.inface, L);
     }
   }
   if (mat != st.mat) {

@@ -32,6 +32,17 @@ struct __align__(128) MixedTet {
   int32_t pad[5];
 };
 
+// Mixed-precision HEX8 record of a mixed mesh: one 128-byte line per element.
+// The six planes are FP32 with absolute offsets (there is no room for a
+// reference vertex), so FP32 distances carry errors of order 2^-24 of the
+// domain size; the FP64 re-solve of near ties keeps the exit side exact.
+struct __align__(128) MixedHex {
+  float4 pl[6];      // nx, ny, nz, c
+  int32_t code[6];   // neighbour codes, as in erec
+  int32_t mat;
+  int32_t pad;
+};
+
 // Split interface of a mixed mesh: a HEX8 side covered by two TET4 sides
 // that cut it along a diagonal.  An exit point x through the hex side enters
 // code[0] when pl.xyz . x - pl.w <= 0, else code[1] (internal id * 8 + side).
@@ -48,7 +59,9 @@ struct MeshView {
   const double4 *plane;      // per side slot (internal order): outward unit normal, offset;
                              // element e's sides start at slot_of(M, e): 4e (TET4 meshes),
                              // 6e (HEX8 meshes); mixed meshes 4e for the TET4 (e < n_tet),
-                             // 4 n_tet + 8 (e - n_tet) for the HEX8 (cache-line aligned)
+                             // 4 n_tet + 8 (e - n_tet) for the HEX8 (cache-line aligned; the
+                             // HEX8 block's slot 6 holds its erec record, so a segment in a
+                             // hex reads two lines, not three)
   const int4 *erec;          // per element, 2 x int4: neighbour code of sides 0..5
                              // (internal id * 8 + side, or -1 - boundary id), material, side count
   int32_t stride;            // sides per element for single-type meshes (4 or 6), 0 = mixed
@@ -64,6 +77,7 @@ struct MeshView {
   const int32_t *int_of_orig; // caller element -> internal id
   const int32_t *orig_of_int; // internal id -> caller element
   const MixedTet *mtet;       // mixed-precision records of the TET4 (internal ids < n_tet)
+  const MixedHex *mhex;       // mixed meshes: mixed-precision records of the HEX8 (id - n_tet)
   int64_t n_tet;              // internal ids below n_tet are TET4, the rest HEX8
   const SplitRec *split;      // split interfaces of mixed meshes (neighbour code split * 8 + 7)
   // point location: uniform grid over the mesh bounding box, CSR lists of
