@@ -34,6 +34,8 @@ hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int blo
                              const SourceBank &S);
 const void *persistent_kernel(int ns, int wpe, bool rec, bool p32, bool eig);
 hipError_t launch_tree_sum(hipStream_t s, const double *in, int64_t n, double *scratch, double *out);
+hipError_t launch_fission_sites(hipStream_t s, const XSView &X, const FissionView &F, int64_t n);
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, int64_t n, unsigned long long v);
 hipError_t fission_sort_temp_bytes(int64_t n, size_t *bytes);
 hipError_t launch_fission_sort(hipStream_t s, const FissionView &F, int64_t n, int end_bit, uint64_t *keys_tmp,
                                uint32_t *perm, uint32_t *idx, void *tmp, size_t tmp_bytes);
@@ -169,7 +171,7 @@ struct mcw_ctx {
   // device problem
   DevBuf plane, erec, mtet, mhex, split, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf cell_start, cell_elem;
-  DevBuf grid_off, npts, rec, fission, fis, fissionable, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
+  DevBuf grid_off, npts, rec, scatter, fis, fissionable, watt_t, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
   Counters *h_counters = nullptr; // pinned
   // source bank, live/collision banks of the event-based path, records
@@ -195,10 +197,11 @@ struct mcw_ctx {
   double e_min = 0, e_max = 0;
   int32_t n_bc = 0;
 
-  // eigenvalue runs: the fission bank (9 arrays: x y z u v w E elem key), its
+  // eigenvalue runs: the fission bank (11 arrays: x y z u v w E elem key seed
+  // nuc), its
   // sort (keys, slot permutation), per-history keff scores, the resampling's
   // counts and offsets, and the next batch's source (9 arrays + key)
-  DevBuf fb[9], fb_keys2, fb_perm, fb_perm2, fb_tmp, k_hist, k_red, sync_cnt, sync_off, scan_tmp, gen[10];
+  DevBuf fb[11], fb_keys2, fb_perm, fb_perm2, fb_tmp, k_hist, k_red, sync_cnt, sync_off, scan_tmp, gen[10];
   int64_t fb_cap = 0, gen_cap = 0, khist_cap = 0, sync_cap = 0;
   size_t fb_tmp_bytes = 0, scan_tmp_bytes = 0;
   int64_t eig_batch = 0, eig_n = 0, eig_sites = 0;
@@ -412,13 +415,13 @@ void prepare(mcw_ctx *c, bool need_source = true)
   c->int_of_orig.upload(m.int_of_orig, s);
   c->orig_of_int.upload(m.orig_of_int, s);
 
-  // nuclear data: per-point records (E, total, absorption, scatter), fission apart
+  // nuclear data: per-point records (E, total, absorption, fission), scatter
+  // apart (collisions and eigenvalue scores read fission; only scatter
+  // tally scores read scatter)
   std::vector<int64_t> goff;
   std::vector<int32_t> np;
-  std::vector<double> aw, fis;
+  std::vector<double> aw, sct;
   std::vector<double4> recs;
-  bool any_fission = false;
-  for (auto &n : c->nucs) any_fission |= !n.f.empty();
   c->e_min = INFINITY;
   c->e_max = -INFINITY;
   std::vector<double> uni;
@@ -427,8 +430,8 @@ void prepare(mcw_ctx *c, bool need_source = true)
     np.push_back((int32_t)n.E.size());
     aw.push_back(n.awr);
     for (size_t j = 0; j < n.E.size(); ++j) {
-      recs.push_back(make_double4(n.E[j], n.t[j], n.a[j], n.s[j]));
-      if (any_fission) fis.push_back(n.f.empty() ? 0.0 : n.f[j]);
+      recs.push_back(make_double4(n.E[j], n.t[j], n.a[j], n.f.empty() ? 0.0 : n.f[j]));
+      sct.push_back(n.s[j]);
     }
     uni.insert(uni.end(), n.E.begin(), n.E.end());
     c->e_min = std::min(c->e_min, n.E.front());
@@ -437,7 +440,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
   c->grid_off.upload(goff, s);
   c->npts.upload(np, s);
   c->rec.upload(recs, s);
-  if (any_fission) c->fission.upload(fis, s);
+  c->scatter.upload(sct, s);
   c->awr.upload(aw, s);
   // fission yield and spectrum per nuclide, and which nuclides are fissionable
   std::vector<double4> fpar;
@@ -448,6 +451,16 @@ void prepare(mcw_ctx *c, bool need_source = true)
   }
   c->fis.upload(fpar, s);
   c->fissionable.upload(fable, s);
+  // Watt first-attempt test: Ew <= (sqrt(w) + a sqrt(b) / 2)^2 and
+  // w <= -a ln(r1 r2), so r1 r2 > exp(-w* / a) with w* = (sqrt(e_max) -
+  // a sqrt(b) / 2)^2 keeps Ew below e_max; 1e-6 of w* is margin for rounding
+  std::vector<double> wt;
+  for (auto &n : c->nucs) {
+    const double a = n.watt[0], b = n.watt[1];
+    const double r = std::sqrt(c->e_max) - 0.5 * a * std::sqrt(b);
+    wt.push_back(n.f.empty() || !(a > 0.0) || !(r > 0.0) ? 2.0 : std::exp(-(r * r) * (1.0 - 1e-6) / a));
+  }
+  c->watt_t.upload(wt, s);
   // unionized grid, per-nuclide index map and log-bucket table
   std::sort(uni.begin(), uni.end());
   uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
@@ -544,9 +557,10 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P.xs.grid_off = c->grid_off.as<int64_t>();
   P.xs.n_points = c->npts.as<int32_t>();
   P.xs.rec = c->rec.as<double4>();
-  P.xs.fission = any_fission ? c->fission.as<double>() : nullptr;
+  P.xs.scatter = c->scatter.as<double>();
   P.xs.fis = c->fis.as<double4>();
   P.xs.fissionable = c->fissionable.as<int32_t>();
+  P.xs.watt_t = c->watt_t.as<double>();
   P.xs.awr = c->awr.as<double>();
   P.xs.U = c->uni.as<double>();
   P.xs.uidx = c->uidx.as<int32_t>();
@@ -817,15 +831,20 @@ int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, bool p32, int block
   return c->n_cu * per_cu;
 }
 
-// Eigenvalue batches of n histories: the fission bank (3 n sites, as OpenMC
-// sizes it) with its sort and resampling scratch, the per-history keff
-// scores, and the next batch's source.
+// Slots of the fission bank for batches of n histories: the 3 n sites OpenMC
+// keeps, plus two chunks (FB_CHUNK = 256 slots) per resident wavefront that
+// partly used chunks may leave empty.
+int64_t fission_slots(mcw_ctx *c, int64_t n) { return 3 * n + 2 * 256 * (int64_t)c->n_cu * 32; }
+
+// Eigenvalue batches of n histories: the fission bank with its sort and
+// resampling scratch, the per-history keff scores, and the next batch's
+// source.
 void ensure_fission(mcw_ctx *c, int64_t n)
 {
-  const int64_t cap = 3 * n;
-  require(cap < ((int64_t)1 << 31), "eigenvalue batches hold at most 715827882 particles");
+  const int64_t cap = fission_slots(c, n);
+  require(cap < ((int64_t)1 << 31), "eigenvalue batches hold at most about 714 million particles");
   if (c->fb_cap < cap) {
-    for (int k = 0; k < 9; ++k) c->fb[k].alloc((size_t)cap * (k == 7 ? 4 : 8));
+    for (int k = 0; k < 11; ++k) c->fb[k].alloc((size_t)cap * (k == 7 || k == 10 ? 4 : 8));
     c->fb_keys2.alloc((size_t)cap * 8);
     c->fb_perm.alloc((size_t)cap * 4);
     c->fb_perm2.alloc((size_t)cap * 4);
@@ -848,7 +867,8 @@ void ensure_fission(mcw_ctx *c, int64_t n)
   }
 }
 
-// The fission bank of batches of n histories holds 3 n sites.
+// The fission bank of batches of n histories (3 n sites kept; the slots
+// include room for partly used chunks).
 FissionView fission_view(mcw_ctx *c, int64_t n)
 {
   FissionView F;
@@ -857,7 +877,9 @@ FissionView fission_view(mcw_ctx *c, int64_t n)
   F.E = c->fb[6].as<double>();
   F.elem = c->fb[7].as<int32_t>();
   F.key = c->fb[8].as<uint64_t>();
-  F.cap = std::min<int64_t>(3 * n, c->fb_cap);
+  F.seed = c->fb[9].as<uint64_t>();
+  F.nuc = c->fb[10].as<int32_t>();
+  F.cap = std::min<int64_t>(fission_slots(c, n), c->fb_cap);
   F.k_hist = c->k_hist.as<double>();
   F.keff = c->keff;
   return F;
@@ -890,8 +912,11 @@ void eigen_close(mcw_ctx *c, const Params &P, int64_t batch, int64_t n, bool act
   HIPCHK(hipMemcpyAsync(ks, kr, sizeof ks, hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(c->h_counters, P.cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  const int64_t requested = (int64_t)c->h_counters->fission_sites;
-  const int64_t total = std::min<int64_t>(requested, 3 * n);
+  const int64_t requested = (int64_t)c->h_counters->fission_requested;
+  const int64_t written = (int64_t)c->h_counters->fission_sites;
+  const int64_t total = std::min<int64_t>(written, 3 * n);
+  const FissionView F = fission_view(c, n);
+  const int64_t slots = std::min<int64_t>((int64_t)c->h_counters->fission_slots, F.cap);
   mcw_keff_result &K = c->kres;
   memset(&K, 0, sizeof K);
   K.batch = batch;
@@ -918,12 +943,14 @@ void eigen_close(mcw_ctx *c, const Params &P, int64_t batch, int64_t n, bool act
   c->eig_sites = total;
   c->eig_src_ready = false;
   require(total > 0, "No fission sites sampled on any process, cannot sample a new batch");
-  // OpenMC order, then synchronizeBanks
+  // the sites' directions and energies; OpenMC order (empty slots, key ~0,
+  // sort after every site: one key bit above the largest history); past
+  // 3 n sites the bank keeps the first in that order; then synchronizeBanks
+  HIPCHK(launch_fission_sites(s, P.xs, F, slots));
   int hb = 0;
   while (((int64_t)1 << hb) < n) ++hb;
-  const FissionView F = fission_view(c, n);
-  HIPCHK(launch_fission_sort(s, F, total, PROGENY_BITS + hb, c->fb_keys2.as<uint64_t>(), c->fb_perm.as<uint32_t>(),
-                             c->fb_perm2.as<uint32_t>(), c->fb_tmp.p, c->fb_tmp_bytes));
+  HIPCHK(launch_fission_sort(s, F, slots, PROGENY_BITS + hb + 1, c->fb_keys2.as<uint64_t>(),
+                             c->fb_perm.as<uint32_t>(), c->fb_perm2.as<uint32_t>(), c->fb_tmp.p, c->fb_tmp_bytes));
   const int64_t base = total < n ? n / total : 0;
   const int64_t needed = total < n ? n % total : n;
   const double p_sample = (double)needed / (double)total;
@@ -1302,7 +1329,8 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       P.eigen = 1;
       P.fb = fission_view(c, n_particles);
       HIPCHK(launch_fill(s, P.fb.k_hist, 3 * n_particles, 0.0));
-      if (!active) { // inactive batches score no tallies
+      HIPCHK(launch_fill_u64(s, (unsigned long long *)P.fb.key, P.fb.cap, ~0ull));
+    if (!active) { // inactive batches score no tallies
         P.n_tl = 0;
         P.n_coll = 0;
         P.tl_fast = 0;

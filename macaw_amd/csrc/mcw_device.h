@@ -310,8 +310,8 @@ __device__ __forceinline__ void micro_at(const XSView &X, int32_t nuc, int32_t k
   const double g = 1.0 - f;
   m.t = g * r0.y + f * r1.y;
   m.a = g * r0.z + f * r1.z;
-  m.s = g * r0.w + f * r1.w;
-  m.f = X.fission ? g * X.fission[base + j] + f * X.fission[base + j + 1] : 0.0;
+  m.f = g * r0.w + f * r1.w;
+  m.s = g * X.scatter[base + j] + f * X.scatter[base + j + 1];
 }
 
 __device__ __forceinline__ double macro_total(const XSView &X, int mat, int32_t k, double E)

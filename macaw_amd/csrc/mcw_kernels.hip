@@ -583,46 +583,81 @@ __device__ __forceinline__ int64_t collide_pick(const Params &P, PState &st, Col
   return nu;
 }
 
+// One fission neutron (OpenMC sample_fission_neutron) from tracking-stream
+// state `s`: mu = 2 xi - 1, then phi = 2 pi xi (isotropic), a prompt /
+// delayed draw (no delayed data: always prompt), then a Watt energy
+// resampled while at or above the grid's top (after 1000 tries the grid
+// minimum).  Advances `s` past the site's draws.
+__device__ __forceinline__ void sample_fission_site(const XSView &X, int32_t nuc, uint64_t &s, double &u, double &v,
+                                                    double &w, double &E)
+{
+  const double4 fp = X.fis[nuc];
+  const double fmu = 2.0 * prn(s) - 1.0;
+  double sphi, cphi;
+  sincos_2pi_d(prn(s), &sphi, &cphi);
+  const double sn = sqrt(1.0 - fmu * fmu);
+  u = fmu;
+  v = sn * cphi;
+  w = sn * sphi;
+  (void)prn(s);
+  const double a = fp.z, bw = fp.w;
+  for (int it = 0;; ++it) {
+    const double r1 = prn(s), r2 = prn(s), r3 = prn(s);
+    const double c = cos_halfpi_d(r3);
+    const double wmax = -a * (log_d(r1) + log_d(r2) * c * c);
+    const double Ew = wmax + 0.25 * a * a * bw + (-1.0 + 2.0 * prn(s)) * sqrt(a * a * bw * wmax);
+    if (Ew < X.e_max || it >= 1000) {
+      E = Ew < X.e_max ? Ew : X.e_min;
+      return;
+    }
+  }
+}
+
+__device__ __forceinline__ void lcg_skip(uint64_t &s) { s = PRN_MULT * s + PRN_ADD; }
+
 // Fission sites go to slots base, base + 1, ...; from the first slot past the
 // bank's capacity on, the collision's sites are not sampled, as OpenMC stops
-// at a full bank.
+// at a full bank.  A site's directions and energy are not computed here: the
+// slot keeps the stream state before its draws and the nuclide, and
+// k_fission_sites evaluates them after the batch, for all sites at once.
+// Here only the stream advances past the site's draws: three, then a Watt
+// attempt of four.  When r1 r2 > watt_t the first attempt is certainly below
+// the grid's top (Ew <= (sqrt(w) + a sqrt(b)/2)^2 with w <= -a ln(r1 r2)), so
+// the site takes exactly seven draws; otherwise (about 1e-5 of sites) the site
+// is sampled in full to find where its draws end.
 template <bool REC, bool EIG>
 __device__ __forceinline__ int collide_finish(const Params &P, PState &st, const CollPick &cp, int64_t nu,
-                                              int64_t base, const int32_t *hash)
+                                              int64_t base, const int32_t *hash, int32_t *written = nullptr)
 {
   const XSView &X = P.xs;
   if (EIG) {
     const FissionView &F = P.fb;
-    const double4 fp = X.fis[cp.nuc];
     for (int64_t q = 0; q < nu; ++q) {
       const int64_t slot = base + q;
       if (slot >= F.cap) break;
-      const double fmu = 2.0 * prn(st.seed) - 1.0;
-      double sphi, cphi;
-      sincos_2pi_d(prn(st.seed), &sphi, &cphi);
-      const double sn = sqrt(1.0 - fmu * fmu);
-      (void)prn(st.seed); // prompt or delayed: no delayed data, always prompt
-      const double a = fp.z, bw = fp.w;
-      double Ef;
-      for (int it = 0;; ++it) {
-        const double r1 = prn(st.seed), r2 = prn(st.seed), r3 = prn(st.seed);
-        const double c = cos_halfpi_d(r3);
-        const double wmax = -a * (log_d(r1) + log_d(r2) * c * c);
-        const double Ew = wmax + 0.25 * a * a * bw + (-1.0 + 2.0 * prn(st.seed)) * sqrt(a * a * bw * wmax);
-        if (Ew < X.e_max || it >= 1000) {
-          Ef = Ew < X.e_max ? Ew : X.e_min;
-          break;
-        }
+      ++*written;
+      const uint64_t s0 = st.seed;
+      uint64_t s = s0;
+      lcg_skip(s); // mu
+      lcg_skip(s); // phi
+      lcg_skip(s); // prompt or delayed
+      const double r1 = prn(s), r2 = prn(s);
+      if (r1 * r2 > X.watt_t[cp.nuc]) {
+        lcg_skip(s); // r3
+        lcg_skip(s); // r4
+        st.seed = s;
+      } else {
+        double u, v, w, E;
+        st.seed = s0;
+        sample_fission_site(X, cp.nuc, st.seed, u, v, w, E);
       }
       F.x[slot] = st.x;
       F.y[slot] = st.y;
       F.z[slot] = st.z;
-      F.u[slot] = fmu;
-      F.v[slot] = sn * cphi;
-      F.w[slot] = sn * sphi;
-      F.E[slot] = Ef;
       F.elem[slot] = st.elem;
       F.key[slot] = ((uint64_t)st.hid << PROGENY_BITS) | (uint64_t)st.prog;
+      F.seed[slot] = s0;
+      F.nuc[slot] = cp.nuc;
       ++st.prog;
     }
   }
@@ -664,22 +699,37 @@ __device__ __forceinline__ int collide(const Params &P, PState &st, const int32_
   return collide_finish<REC, false>(P, st, cp, 0, 0, hash);
 }
 
-// Slots for n items per lane, consecutive in lane order, with one atomic per
-// wavefront: returns this lane's first slot.  Every lane must call it.
-__device__ __forceinline__ int64_t wave_reserve(unsigned long long *count, int64_t n)
+// Slots for n items per lane (n <= 2^24, so the wavefront's sum fits 32
+// bits), consecutive in lane order, from the wavefront's current chunk of
+// fission-bank slots [next, end); when the chunk is short, the wavefront
+// takes a new one of max(need, FB_CHUNK) slots with one atomic (the rest of
+// the old chunk stays empty).  A single counter taken once per collision
+// phase by every wavefront serialises: same-address atomics cost about as
+// much as a third of the batch.  Returns this lane's first slot.  Every lane
+// must call it; next and end are wavefront-uniform.
+constexpr int32_t FB_CHUNK = 256;
+__device__ __forceinline__ int64_t wave_reserve_chunk(unsigned long long *count, int32_t n, int64_t &next,
+                                                      int64_t &end)
 {
   const int lane = __lane_id();
-  int64_t incl = n;
+  int32_t incl = n;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
-    const int64_t y = __shfl_up(incl, off, 64);
+    const int32_t y = __shfl_up(incl, off, 64);
     if (lane >= off) incl += y;
   }
-  const int64_t total = __shfl(incl, 63, 64);
-  unsigned long long base = 0;
-  if (total > 0 && lane == 0) base = atomicAdd(count, (unsigned long long)total);
-  base = (unsigned long long)__shfl((long long)base, 0, 64);
-  return (int64_t)base + (incl - n);
+  const int32_t total = __shfl(incl, 63, 64);
+  if (total == 0) return 0;
+  if (next + total > end) {
+    const int64_t want = total > FB_CHUNK ? total : FB_CHUNK;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(count, (unsigned long long)want);
+    next = (int64_t)__shfl((long long)base, 0, 64);
+    end = next + want;
+  }
+  const int64_t mine = next + (int64_t)(incl - n);
+  next += total;
+  return mine;
 }
 
 // Per-history keff scores of a history that ended.
@@ -985,6 +1035,9 @@ k_persistent(Params P, SourceBank S)
   bool exhausted = false;
   const int64_t n_hist = P.hist_end - P.hist_begin;
   const int lanes_min = P.lanes_min < 1 ? 1 : (P.lanes_min > 64 ? 64 : P.lanes_min);
+  int64_t fb_next = 0, fb_end = 0;     // eigenvalue: the wavefront's chunk of bank slots
+  int32_t fb_written = 0;              // eigenvalue: this lane's sites written
+  int64_t fb_req = 0;                  // eigenvalue: and asked for
   for (;;) {
     // refill idle lanes together (one atomic per wavefront)
     const bool need = phase == PH_IDLE && !exhausted;
@@ -1033,8 +1086,9 @@ k_persistent(Params P, SourceBank S)
       CollPick cp;
       int64_t nu = 0;
       if (hit) nu = collide_pick<REC, true>(P, st, cp);
-      const int64_t base = wave_reserve(&C->fission_sites, nu);
-      if (hit) ev = collide_finish<REC, true>(P, st, cp, nu, base, s_hash);
+      fb_req += nu;
+      const int64_t base = wave_reserve_chunk(&C->fission_slots, (int32_t)nu, fb_next, fb_end);
+      if (hit) ev = collide_finish<REC, true>(P, st, cp, nu, base, s_hash, &fb_written);
     } else if (hit) {
       ev = collide<REC>(P, st, s_hash);
     }
@@ -1055,6 +1109,14 @@ k_persistent(Params P, SourceBank S)
   }
   if (SERVE) req_flush(P, racc.v != 0ull, racc.req, racc.v);
   flush_counts(C, wc, track);
+  if (EIG) {
+    const unsigned long long w = wave_sum((unsigned long long)fb_written);
+    const unsigned long long r = wave_sum((unsigned long long)fb_req);
+    if (__lane_id() == 0 && r) {
+      atomicAdd(&C->fission_sites, w);
+      atomicAdd(&C->fission_requested, r);
+    }
+  }
   if (P32) {
     const unsigned long long r = wave_sum(rechecks);
     if (__lane_id() == 0 && r) atomicAdd(&C->rechecks, r);
@@ -1062,6 +1124,12 @@ k_persistent(Params P, SourceBank S)
 }
 
 // ------------------------------------------------------------------ utilities
+
+extern "C" __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
 
 extern "C" __global__ void k_fill(double *p, int64_t n, double v)
 {
@@ -1095,6 +1163,24 @@ extern "C" __global__ void k_accumulate(double *value, double *sum, double *sum_
     double v = value[i] / norm;
     sum[i] += v;
     sum_sq[i] += v * v;
+  }
+}
+
+// Direction and energy of the fission sites in slots 0 .. n-1 from their
+// saved stream states (the draws collide_finish stepped over); empty slots
+// (key ~0) are skipped.
+extern "C" __global__ void k_fission_sites(XSView X, FissionView F, int64_t n)
+{
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (F.key[i] == ~0ull) continue;
+    uint64_t s = F.seed[i];
+    double u, v, w, E;
+    sample_fission_site(X, F.nuc[i], s, u, v, w, E);
+    F.u[i] = u;
+    F.v[i] = v;
+    F.w[i] = w;
+    F.E[i] = E;
   }
 }
 
@@ -1276,6 +1362,20 @@ hipError_t launch_gather_scale(hipStream_t s, const double *src, const int32_t *
 hipError_t launch_zero_counters(hipStream_t s, Counters *C)
 {
   hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, int64_t n, unsigned long long v)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fission_sites(hipStream_t s, const XSView &X, const FissionView &F, int64_t n)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fission_sites, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, X, F, n);
   return hipGetLastError();
 }
 

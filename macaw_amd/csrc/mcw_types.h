@@ -8,7 +8,7 @@
 //                        -1 - boundary
 //          elem_mat[e]   int32 material index (-1 void)
 //   xs     energy[]      per-nuclide grids, packed (grid_off[n] .. +n_points[n])
-//          values[]      double4 (total, absorption, scatter, fission) per point
+//          rec[]         double4 (E, total, absorption, fission) per point; scatter apart
 //   bank   SoA particle state (x,y,z,u,v,w,E,sigt f64; seed u64; elem, inface,
 //          mat i32) sized for one chunk of histories
 #pragma once
@@ -92,10 +92,12 @@ struct MeshView {
 struct XSView {
   const int64_t *grid_off;   // per nuclide: first point in rec
   const int32_t *n_points;
-  const double4 *rec;        // per point: (E, total, absorption, scatter)
-  const double *fission;     // per point, nullptr when no nuclide has fission data
+  const double4 *rec;        // per point: (E, total, absorption, fission), fission 0 when absent
+  const double *scatter;     // per point (only scatter scores read it)
   const double4 *fis;        // per nuclide: (nu0, nu1, Watt a, Watt b), nu(E) = nu0 + nu1 E
   const int32_t *fissionable; // per nuclide: 1 when it has fission data
+  const double *watt_t;      // per nuclide: r1 r2 above this means the first Watt attempt is
+                             // below e_max (-a ln(r1 r2) < the w that reaches e_max)
   const double *awr;
   // unionized grid: every nuclide's points merged, sorted, unique
   const double *U;
@@ -203,7 +205,9 @@ struct FissionView {
   double *x, *y, *z, *u, *v, *w, *E;
   int32_t *elem;           // internal element id
   uint64_t *key;           // history << PROGENY_BITS | progeny
-  int64_t cap;             // sites the bank holds
+  uint64_t *seed;          // tracking-stream state before the site's first draw
+  int32_t *nuc;            // nuclide that fissioned
+  int64_t cap;             // slots of the bank: 3 n_particles plus room for partly used chunks
   double *k_hist;          // 3 x n_particles: track-length, collision, absorption scores
   double keff;             // divisor of the fission yield
 };
@@ -215,7 +219,9 @@ struct Counters {
   unsigned long long segments, collisions, absorptions, leaks, cutoffs, lost, reflections, rechecks;
   unsigned long long hex_segments; // segments in HEX8 elements
   unsigned long long trace_count;  // trace records emitted (may exceed the buffer)
-  unsigned long long fission_sites; // sites requested (those beyond the bank are dropped)
+  unsigned long long fission_sites;     // sites written to the bank
+  unsigned long long fission_requested; // sites the collisions asked for
+  unsigned long long fission_slots;     // bank slots handed out in chunks (some stay empty)
   double track;
   unsigned long long next_hist;   // persistent kernel work queue
   unsigned long long q_count[4];  // event queue lengths
