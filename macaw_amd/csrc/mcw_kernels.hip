@@ -127,6 +127,41 @@ __device__ __forceinline__ int64_t wave_append(unsigned long long *count, bool w
 
 __device__ __forceinline__ int wave_count(bool p) { return __popcll(__ballot(p)); }
 
+// Workgroup-aggregated append to two queues at once (256 threads, 4
+// wavefronts; every thread must call it): one atomic per queue per
+// workgroup instead of one per wavefront.  Appends from every wavefront of
+// the event kernels land on the same two counters, and same-address atomics
+// serialise.  Slots are consecutive in thread order within the workgroup.
+__device__ __forceinline__ void block_append2(unsigned long long *count_a, bool want_a, int64_t &slot_a,
+                                              unsigned long long *count_b, bool want_b, int64_t &slot_b)
+{
+  __shared__ unsigned s_cnt[2][4];
+  __shared__ unsigned long long s_base[2];
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const unsigned long long ma = __ballot(want_a), mb = __ballot(want_b);
+  if (lane == 0) {
+    s_cnt[0][wave] = (unsigned)__popcll(ma);
+    s_cnt[1][wave] = (unsigned)__popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned ta = s_cnt[0][0] + s_cnt[0][1] + s_cnt[0][2] + s_cnt[0][3];
+    const unsigned tb = s_cnt[1][0] + s_cnt[1][1] + s_cnt[1][2] + s_cnt[1][3];
+    s_base[0] = ta && count_a ? atomicAdd(count_a, (unsigned long long)ta) : 0ull;
+    s_base[1] = tb && count_b ? atomicAdd(count_b, (unsigned long long)tb) : 0ull;
+  }
+  __syncthreads();
+  unsigned pa = 0, pb = 0;
+  for (int w = 0; w < wave; ++w) {
+    pa += s_cnt[0][w];
+    pb += s_cnt[1][w];
+  }
+  const unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+  slot_a = want_a ? (int64_t)(s_base[0] + pa + (unsigned)__popcll(ma & below)) : -1;
+  slot_b = want_b ? (int64_t)(s_base[1] + pb + (unsigned)__popcll(mb & below)) : -1;
+  __syncthreads(); // s_cnt and s_base are reused by the next call
+}
+
 // ------------------------------------------------------------------ source
 
 __device__ void sample_point_in_tet(const double *p0, const double *p1, const double *p2, const double *p3,
@@ -915,8 +950,8 @@ __global__ void __launch_bounds__(256) k_segment(Params P, BankView in, int qin,
     count_segment(wc, valid, ev, hx);
     const bool to_coll = valid && ev == EV_COLLIDE;
     const bool to_live = valid && (ev == EV_CROSS || ev == EV_REFLECT);
-    const int64_t pc = wave_append(&C->q_count[qcoll], to_coll);
-    const int64_t pl = wave_append(&C->q_count[qout], to_live);
+    int64_t pc, pl;
+    block_append2(&C->q_count[qcoll], to_coll, pc, &C->q_count[qout], to_live, pl);
     if (to_coll) store<REC>(coll, pc, st);
     if (to_live) store<REC>(out, pl, st);
   }
@@ -943,7 +978,8 @@ __global__ void __launch_bounds__(256) k_collide(Params P, BankView coll, int qc
     }
     count_collision(wc, valid, ev);
     const bool to_live = valid && ev == EV_SCATTER;
-    const int64_t pl = wave_append(&C->q_count[qout], to_live);
+    int64_t pl, unused;
+    block_append2(&C->q_count[qout], to_live, pl, nullptr, false, unused);
     if (to_live) store<REC>(out, pl, st);
   }
   flush_counts(C, wc, 0.0);
