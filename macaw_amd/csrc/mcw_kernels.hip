@@ -110,21 +110,6 @@ __device__ void flush_counts(Counters *c, const WaveCount &l, double track)
   }
 }
 
-// Wavefront-aggregated append: every lane of the wavefront must call it.
-// Returns this lane's slot (consecutive in lane order), or -1.
-__device__ __forceinline__ int64_t wave_append(unsigned long long *count, bool want)
-{
-  const unsigned long long mask = __ballot(want);
-  if (mask == 0) return -1;
-  const int lane = __lane_id();
-  const int leader = __ffsll((long long)mask) - 1;
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(mask));
-  base = (unsigned long long)__shfl((long long)base, leader, 64);
-  const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-  return want ? (int64_t)(base + __popcll(below)) : -1;
-}
-
 __device__ __forceinline__ int wave_count(bool p) { return __popcll(__ballot(p)); }
 
 // Workgroup-aggregated append to two queues at once (256 threads, 4
@@ -1072,12 +1057,40 @@ k_persistent(Params P, SourceBank S)
   const int64_t n_hist = P.hist_end - P.hist_begin;
   const int lanes_min = P.lanes_min < 1 ? 1 : (P.lanes_min > 64 ? 64 : P.lanes_min);
   int64_t fb_next = 0, fb_end = 0;     // eigenvalue: the wavefront's chunk of bank slots
+  int64_t h_next = 0, h_end = 0;       // the wavefront's chunk of history slots
   int32_t fb_written = 0;              // eigenvalue: this lane's sites written
   int64_t fb_req = 0;                  // eigenvalue: and asked for
   for (;;) {
-    // refill idle lanes together (one atomic per wavefront)
+    // refill idle lanes together from the wavefront's chunk of history slots
+    // (a new chunk of at least 64 is one atomic on the shared counter; every
+    // wavefront taking its refills from it directly serialised there)
     const bool need = phase == PH_IDLE && !exhausted;
-    const int64_t slot = wave_append(&C->next_hist, need);
+    int64_t slot = -1;
+    {
+      const unsigned long long mask = __ballot(need);
+      if (mask) {
+        const int cnt = __popcll(mask);
+        const int lane = __lane_id();
+        const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+        const int rank = __popcll(below);
+        const int64_t avail = h_end - h_next;
+        int64_t nb = 0, ne = 0;
+        if (avail < cnt) {
+          const int64_t want = (cnt - avail) > 64 ? (cnt - avail) : 64;
+          unsigned long long b0 = 0;
+          if (lane == 0) b0 = atomicAdd(&C->next_hist, (unsigned long long)want);
+          nb = (int64_t)__shfl((long long)b0, 0, 64);
+          ne = nb + want;
+        }
+        if (need) slot = rank < avail ? h_next + rank : nb + (rank - avail);
+        if (avail < cnt) {
+          h_next = nb + (cnt - avail);
+          h_end = ne;
+        } else {
+          h_next += cnt;
+        }
+      }
+    }
     if (need) {
       if (slot < n_hist) {
         load_site(S, slot, st);
