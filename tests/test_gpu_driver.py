@@ -123,3 +123,31 @@ def test_eigen_driver_matches_oracle(tmp_path, oracle_lib):
     assert np.array_equal(read_col(base + ".csv", "keff"), np.array(kf))
     power = read_col(base + "_power.csv", "power")
     assert power.sum() > 0 and close(power, o.tally(1))
+
+
+def test_eigen_driver_ray_output_matches_gold_seeds(tmp_path):
+    """The reference study's own settings -- eigenvalue, 30 particles, 3
+    batches, 1 inactive -- with [Outputs/rays] type = RayTracingExodus: rays
+    60..89 of batch 3, which start from resampled fission sites, carry the
+    gold file's source seeds (they depend only on batch, particle count and
+    id) and a visit sequence that starts at 0."""
+    import json
+    from scipy.io import netcdf_file
+    base = str(tmp_path / "out")
+    args = [build.DRIVER, "-i", os.path.join(INPUTS, "openmc_study_eigen.i"), "Outputs/file_base=" + base,
+            "Outputs/rays/type=RayTracingExodus", "Outputs/rays/study=study", "Outputs/rays/execute_on=final"]
+    r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert r.stdout.count("Keff ") == 3
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "gold_rays.json")))["records"]
+    src_gold = {g["ray_id"]: g["seed_source"] for g in gold if g["intersections"] == 0}
+    f = netcdf_file(base + "_rays.e", "r", mmap=False)
+    names = [b"".join(x).decode().strip("\x00 ") for x in f.variables["name_elem_var"][:]]
+    vals = {n: f.variables[f"vals_elem_var{i + 1}eb1"][0] for i, n in enumerate(names)}
+    ray = vals["ray_id"].astype(int)
+    assert sorted(set(ray)) == list(range(60, 90))
+    for rid in range(60, 90):
+        sel = ray == rid
+        assert list(vals["intersections"][sel]) == list(range(sel.sum()))
+        assert np.all(vals["aux_seed_source"][sel].astype(np.uint64) == np.uint64(src_gold[rid]))
+    f.close()
