@@ -41,7 +41,8 @@ def run_eigen(problem, batches, n_inactive=1, check_oracle=True, **opt):
     n = problem.n_particles
     g = Context(problem, mode=abi.MODE_BATCHED, record_histories=1, run_mode=abi.RUN_EIGENVALUE,
                 n_inactive=n_inactive, **opt)
-    o = Oracle(problem, eigenvalue=True, n_inactive=n_inactive) if check_oracle else None
+    o = (Oracle(problem, eigenvalue=True, n_inactive=n_inactive, seed_roundtrip=bool(opt.get("seed_roundtrip", 0)))
+         if check_oracle else None)
     out = []
     for b in range(1, batches + 1):
         print(f"[eigen] {problem.name} batch {b} n={n} opt={opt}", flush=True)
@@ -107,6 +108,21 @@ def test_eigen_parity_mixed_mesh():
         assert_sites_equal(r["src"], r["o"]["src"])
     for tid in (1, 2):
         assert_tally_close(g.tally(tid), o.tally(tid))
+
+
+def test_eigen_parity_seed_roundtrip_and_trace():
+    """MaCaw's Real-stored seeds (rounded through a double each segment) and
+    a traced final batch, in an eigenvalue run: still bit-identical."""
+    p = small_k2(n_particles=2000)
+    g, o, res = run_eigen(p, 3, n_inactive=1, seed_roundtrip=1, trace_histories=50)
+    for r in res:
+        assert_records_equal(r["hist"], r["o"]["hist"])
+        for k in KEFF_FIELDS:
+            assert r["keff"][k] == r["o"]["keff"][k], k
+        assert_sites_equal(r["bank"], r["o"]["bank"])
+        assert_sites_equal(r["src"], r["o"]["src"])
+    tr = g.trace()
+    assert tr.size > 0 and set(np.unique(tr["history"])) <= set(range(50))
 
 
 @pytest.mark.parametrize("opt", [{"bank_capacity": 1500}, {"sort_cadence": 1}, {"lanes_min": 1},
