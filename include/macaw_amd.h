@@ -137,6 +137,7 @@ typedef struct {
   const double *fission; /* may be NULL (not fissionable) */
   double nu[2];
   double watt[2];
+  double temperature;    /* K, the temperature of this table (0: 293.6) */
 } mcw_nuclide;
 
 /* A material: id (as in materials.xml) and atom densities [1/(b cm)]. */
@@ -232,6 +233,24 @@ int mcw_bc_set(mcw_ctx *ctx, int32_t boundary_id, int32_t kind);
 
 /* Nuclear data (replaces the HDF5 library OpenMC loads in openmc_init). */
 int mcw_xs_set(mcw_ctx *ctx, int32_t n_nuclides, const mcw_nuclide *nuclides);
+
+/* A further table of nuclide `nuclide` (index into mcw_xs_set's list) at
+ * table->temperature, with fission cross sections if and only if the first
+ * table has them; awr, nu and the Watt spectrum stay the first table's (one
+ * set of reaction products per nuclide, as in OpenMC's data files).
+ * mcw_xs_set drops every added table.  A nuclide with several tables is read at each element's
+ * temperature by OpenMC's default "nearest" method: the table whose kT is
+ * closest to the element's kT (CollisionKernel.C:109 sets the particle's
+ * sqrt(kT) from the element's temperature variable; ties go to the lower
+ * temperature). */
+int mcw_xs_add_temperature(mcw_ctx *ctx, int32_t nuclide, const mcw_nuclide *table);
+
+/* Element temperatures [K], caller order (MaCaw's CONSTANT MONOMIAL
+ * temperature variable).  A nuclide whose nearest table is further than
+ * `tolerance` K from an element's temperature is an error at the next batch
+ * (OpenMC settings temperature_tolerance, default 10 K).  Without this call,
+ * every element reads each nuclide's first table. */
+int mcw_temperature_set(mcw_ctx *ctx, int64_t n_elem, const double *kelvin, double tolerance);
 int mcw_materials_set(mcw_ctx *ctx, int32_t n_materials, const mcw_material *materials);
 
 /* CollisionKernel `blocks` / `materials` parameters (CollisionKernel.C:63-69). */

@@ -18,6 +18,7 @@ MODE_PLAIN, MODE_BATCHED = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 FATE_ALIVE, FATE_LEAKED, FATE_ABSORBED, FATE_CUTOFF, FATE_LOST = 0, 1, 2, 3, 4
 RUN_FIXED_SOURCE, RUN_EIGENVALUE = 0, 1
+K_BOLTZMANN = 8.617333262e-5  # eV/K, as OpenMC
 
 SCORE_NAMES = {"flux": SCORE_FLUX, "total": SCORE_TOTAL, "absorption": SCORE_ABSORPTION,
                "scatter": SCORE_SCATTER, "fission": SCORE_FISSION}
@@ -44,7 +45,7 @@ class Options(C.Structure):
 class Nuclide(C.Structure):
     _fields_ = [("n_points", C.c_int32), ("awr", C.c_double), ("energy", _dp), ("total", _dp),
                 ("absorption", _dp), ("scatter", _dp), ("fission", _dp), ("nu", C.c_double * 2),
-                ("watt", C.c_double * 2)]
+                ("watt", C.c_double * 2), ("temperature", C.c_double)]
 
 
 class Material(C.Structure):

@@ -123,7 +123,11 @@ struct HostNuclide {
   std::vector<double> E;
   std::vector<double> t, a, s, f;
   double nu[2] = {0.0, 0.0}, watt[2] = {0.0, 0.0};
+  double temperature = 293.6; // K
 };
+
+// Boltzmann constant [eV/K], as OpenMC's K_BOLTZMANN.
+constexpr double K_BOLTZMANN = 8.617333262e-5;
 
 struct HostMaterial {
   int32_t id;
@@ -159,6 +163,10 @@ struct mcw_ctx {
   bool have_mesh = false;
   std::map<int32_t, int32_t> bc;
   std::vector<HostNuclide> nucs;
+  std::vector<std::vector<HostNuclide>> nuc_temps; // further tables per nuclide
+  std::vector<double> elem_T;                      // element temperatures (caller order), empty: none
+  double T_tol = 10.0;
+  int32_t n_eff_mats = 0;                          // (material, table choice) pairs on the device
   std::vector<HostMaterial> mats;
   std::map<int32_t, int32_t> block_mat;
   bool have_block_map = false;
@@ -171,7 +179,7 @@ struct mcw_ctx {
   // device problem
   DevBuf plane, erec, mtet, mhex, split, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf cell_start, cell_elem;
-  DevBuf grid_off, npts, rec, scatter, fis, fissionable, watt_t, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
+  DevBuf grid_off, npts, rec, scatter, fis, fissionable, watt_t, nuc_of, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
   Counters *h_counters = nullptr; // pinned
   // source bank, live/collision banks of the event-based path, records
@@ -324,6 +332,83 @@ void prepare(mcw_ctx *c, bool need_source = true)
     auto b = c->block_mat.find(m.subdomain[e]);
     if (b != c->block_mat.end()) emat[e] = id_to_idx[b->second];
   }
+  const int32_t n_nuc = (int32_t)c->nucs.size();
+  for (auto &mt : c->mats)
+    for (int32_t q : mt.nuc)
+      require(q >= 0 && q < n_nuc, "material " + std::to_string(mt.id) + " refers to an unknown nuclide");
+
+  // temperatures, OpenMC's "nearest" method: for each element and nuclide
+  // the table whose kT is closest to the element's kT = (sqrt(k T))^2 (ties
+  // to the lower temperature).  Elements with one material and one table
+  // choice share a device material; every (nuclide, table) in use is a
+  // device nuclide ("instance"), instances 0 .. n-1 being the first tables.
+  auto table = [&](int32_t q, int32_t t) -> const HostNuclide & {
+    return t == 0 ? c->nucs[q] : c->nuc_temps[q][t - 1];
+  };
+  std::vector<std::vector<int32_t>> t_order(n_nuc); // tables by ascending temperature
+  for (int32_t q = 0; q < n_nuc; ++q) {
+    const int32_t nt = 1 + (int32_t)c->nuc_temps[q].size();
+    for (int32_t t = 0; t < nt; ++t) t_order[q].push_back(t);
+    std::stable_sort(t_order[q].begin(), t_order[q].end(),
+                     [&](int32_t a, int32_t b) { return table(q, a).temperature < table(q, b).temperature; });
+  }
+  std::vector<std::pair<int32_t, int32_t>> inst; // (nuclide, table)
+  std::map<std::pair<int32_t, int32_t>, int32_t> inst_id;
+  for (int32_t q = 0; q < n_nuc; ++q) {
+    inst_id[{q, 0}] = q;
+    inst.push_back({q, 0});
+  }
+  const bool temps = !c->elem_T.empty();
+  require(!temps || (int64_t)c->elem_T.size() == m.n_elem,
+          "mcw_temperature_set: one temperature per element of the mesh");
+  std::vector<int32_t> eff(m.n_elem, -1); // caller element -> device material
+  std::map<std::vector<int32_t>, int32_t> eff_id;
+  std::vector<std::vector<int32_t>> eff_key; // material, then one instance per nuclide of it
+  for (int64_t e = 0; e < m.n_elem; ++e) {
+    const int32_t mi = emat[e];
+    if (mi < 0) continue;
+    std::vector<int32_t> key{mi};
+    double sq = 0.0, kT = 0.0;
+    if (temps) {
+      sq = std::sqrt(K_BOLTZMANN * c->elem_T[e]);
+      kT = sq * sq;
+    }
+    for (int32_t q : c->mats[mi].nuc) {
+      int32_t best = 0;
+      if (temps) {
+        double dmin = INFINITY;
+        for (int32_t t : t_order[q]) {
+          const double d = std::fabs(K_BOLTZMANN * table(q, t).temperature - kT);
+          if (d < dmin) {
+            dmin = d;
+            best = t;
+          }
+        }
+        require(std::fabs(table(q, best).temperature - c->elem_T[e]) <= c->T_tol,
+                "no cross sections for nuclide " + std::to_string(q) + " at or near " +
+                    std::to_string(c->elem_T[e]) + " K (element " + std::to_string(e) + ")");
+      }
+      auto it = inst_id.find({q, best});
+      int32_t id;
+      if (it == inst_id.end()) {
+        id = (int32_t)inst.size();
+        inst_id[{q, best}] = id;
+        inst.push_back({q, best});
+      } else {
+        id = it->second;
+      }
+      key.push_back(id);
+    }
+    auto f = eff_id.find(key);
+    if (f == eff_id.end()) {
+      eff[e] = (int32_t)eff_key.size();
+      eff_id[key] = eff[e];
+      eff_key.push_back(key);
+    } else {
+      eff[e] = f->second;
+    }
+  }
+  c->n_eff_mats = (int32_t)eff_key.size();
 
   // mesh in internal (Morton) order: planes per side slot, and per element
   // one 32-byte record of neighbour codes, material and side count
@@ -350,7 +435,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
       code[k] = nb < 0 || (nb & 7) == 7 ? nb : m.int_of_orig[nb >> 3] * 8 + (nb & 7);
     }
     erec[2 * i] = make_int4(code[0], code[1], code[2], code[3]);
-    erec[2 * i + 1] = make_int4(code[4], code[5], emat[o], ns);
+    erec[2 * i + 1] = make_int4(code[4], code[5], eff[o], ns);
     if (!m.uniform_sides && i >= m.n_tet) memcpy(&planes[slot + 6], &erec[2 * i], 2 * sizeof(int4));
   }
   c->plane.upload(planes, s);
@@ -424,8 +509,16 @@ void prepare(mcw_ctx *c, bool need_source = true)
   std::vector<double4> recs;
   c->e_min = INFINITY;
   c->e_max = -INFINITY;
+  for (int32_t q = 0; q < n_nuc; ++q) // the grid range over every table loaded
+    for (size_t t = 0; t < t_order[q].size(); ++t) {
+      c->e_min = std::min(c->e_min, table(q, (int32_t)t).E.front());
+      c->e_max = std::max(c->e_max, table(q, (int32_t)t).E.back());
+    }
   std::vector<double> uni;
-  for (auto &n : c->nucs) {
+  std::vector<int32_t> nuc_of;
+  for (auto &iq : inst) {
+    const HostNuclide &n = table(iq.first, iq.second);
+    nuc_of.push_back(iq.first);
     goff.push_back((int64_t)recs.size());
     np.push_back((int32_t)n.E.size());
     aw.push_back(n.awr);
@@ -434,9 +527,8 @@ void prepare(mcw_ctx *c, bool need_source = true)
       sct.push_back(n.s[j]);
     }
     uni.insert(uni.end(), n.E.begin(), n.E.end());
-    c->e_min = std::min(c->e_min, n.E.front());
-    c->e_max = std::max(c->e_max, n.E.back());
   }
+  c->nuc_of.upload(nuc_of, s);
   c->grid_off.upload(goff, s);
   c->npts.upload(np, s);
   c->rec.upload(recs, s);
@@ -445,7 +537,8 @@ void prepare(mcw_ctx *c, bool need_source = true)
   // fission yield and spectrum per nuclide, and which nuclides are fissionable
   std::vector<double4> fpar;
   std::vector<int32_t> fable;
-  for (auto &n : c->nucs) {
+  for (auto &iq : inst) {
+    const HostNuclide &n = table(iq.first, iq.second);
     fpar.push_back(make_double4(n.nu[0], n.nu[1], n.watt[0], n.watt[1]));
     fable.push_back(n.f.empty() ? 0 : 1);
   }
@@ -455,7 +548,8 @@ void prepare(mcw_ctx *c, bool need_source = true)
   // w <= -a ln(r1 r2), so r1 r2 > exp(-w* / a) with w* = (sqrt(e_max) -
   // a sqrt(b) / 2)^2 keeps Ew below e_max; 1e-6 of w* is margin for rounding
   std::vector<double> wt;
-  for (auto &n : c->nucs) {
+  for (auto &iq : inst) {
+    const HostNuclide &n = table(iq.first, iq.second);
     const double a = n.watt[0], b = n.watt[1];
     const double r = std::sqrt(c->e_max) - 0.5 * a * std::sqrt(b);
     wt.push_back(n.f.empty() || !(a > 0.0) || !(r > 0.0) ? 2.0 : std::exp(-(r * r) * (1.0 - 1e-6) / a));
@@ -465,11 +559,11 @@ void prepare(mcw_ctx *c, bool need_source = true)
   std::sort(uni.begin(), uni.end());
   uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
   const int64_t nu = (int64_t)uni.size();
-  const int32_t nn = (int32_t)c->nucs.size();
+  const int32_t nn = (int32_t)inst.size();
   require(nu < ((int64_t)1 << 31) / std::max(1, nn), "unionized grid too large");
   std::vector<int32_t> uidx((size_t)nu * nn);
   for (int32_t q = 0; q < nn; ++q) {
-    const std::vector<double> &E = c->nucs[q].E;
+    const std::vector<double> &E = table(inst[q].first, inst[q].second).E;
     int64_t j = -1;
     for (int64_t k = 0; k < nu; ++k) {
       while (j + 1 < (int64_t)E.size() && E[j + 1] <= uni[k]) ++j;
@@ -491,11 +585,10 @@ void prepare(mcw_ctx *c, bool need_source = true)
   c->uhash.upload(hash, s);
   std::vector<int32_t> moff{0}, mnuc;
   std::vector<double> mden;
-  for (auto &mt : c->mats) {
+  for (auto &key : eff_key) {
+    const HostMaterial &mt = c->mats[key[0]];
     for (size_t q = 0; q < mt.nuc.size(); ++q) {
-      require(mt.nuc[q] >= 0 && mt.nuc[q] < (int32_t)c->nucs.size(),
-              "material " + std::to_string(mt.id) + " refers to an unknown nuclide");
-      mnuc.push_back(mt.nuc[q]);
+      mnuc.push_back(key[1 + q]);
       mden.push_back(mt.dens[q]);
     }
     moff.push_back((int32_t)mnuc.size());
@@ -572,8 +665,9 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P.xs.mat_off = c->mat_off.as<int32_t>();
   P.xs.mat_nuc = c->mat_nuc.as<int32_t>();
   P.xs.mat_dens = c->mat_dens.as<double>();
-  P.xs.n_nuc = (int32_t)c->nucs.size();
-  P.xs.n_mat = (int32_t)c->mats.size();
+  P.xs.n_nuc = (int32_t)inst.size();
+  P.xs.n_mat = c->n_eff_mats;
+  P.xs.nuc_of = c->nuc_of.as<int32_t>();
   P.xs.e_min = c->e_min;
   P.xs.e_max = c->e_max;
   P.src.space = c->src.space;
@@ -777,7 +871,7 @@ void sort_layout(mcw_ctx *c)
   int eb = 1;
   while (eb < 31 && ((int64_t)1 << eb) <= c->mesh.n_elem) ++eb;
   int mb = 0;
-  while (((int64_t)1 << mb) < (int64_t)c->mats.size()) ++mb;
+  while (((int64_t)1 << mb) < (int64_t)c->n_eff_mats) ++mb;
   if (eb + mb <= 32) {
     c->sort_shift = eb;
     c->sort_bits = eb + mb;
@@ -1161,38 +1255,88 @@ int mcw_bc_set(mcw_ctx *c, int32_t boundary_id, int32_t kind)
   });
 }
 
+static HostNuclide read_nuclide(int i, const mcw_nuclide &q)
+{
+  require(q.n_points >= 2 && q.energy && q.total && q.absorption && q.scatter,
+          "nuclide " + std::to_string(i) + ": need >= 2 points and total/absorption/scatter tables");
+  require(q.awr > 0.0, "nuclide " + std::to_string(i) + ": awr must be positive");
+  require(std::isfinite(q.temperature) && q.temperature >= 0.0,
+          "nuclide " + std::to_string(i) + ": temperature must be finite and non-negative");
+  HostNuclide h;
+  h.awr = q.awr;
+  if (q.temperature > 0.0) h.temperature = q.temperature;
+  h.E.assign(q.energy, q.energy + q.n_points);
+  require(h.E[0] > 0.0, "nuclide " + std::to_string(i) + ": energies must be positive");
+  for (int j = 1; j < q.n_points; ++j)
+    require(h.E[j] > h.E[j - 1], "nuclide " + std::to_string(i) + ": energy grid must increase");
+  h.t.assign(q.total, q.total + q.n_points);
+  h.a.assign(q.absorption, q.absorption + q.n_points);
+  h.s.assign(q.scatter, q.scatter + q.n_points);
+  if (q.fission) {
+    h.f.assign(q.fission, q.fission + q.n_points);
+    for (int k = 0; k < 2; ++k) {
+      require(std::isfinite(q.nu[k]) && std::isfinite(q.watt[k]),
+              "nuclide " + std::to_string(i) + ": nu and Watt parameters must be finite");
+      h.nu[k] = q.nu[k];
+      h.watt[k] = q.watt[k];
+    }
+    require(q.watt[0] > 0.0 && q.watt[1] >= 0.0,
+            "nuclide " + std::to_string(i) + ": the fission spectrum needs Watt a > 0 and b >= 0");
+  }
+  return h;
+}
+
 int mcw_xs_set(mcw_ctx *c, int32_t n, const mcw_nuclide *nuc)
 {
   return guard(c, [&] {
     require(n > 0 && nuc, "mcw_xs_set: no nuclides");
-    c->nucs.clear();
-    for (int i = 0; i < n; ++i) {
-      const mcw_nuclide &q = nuc[i];
-      require(q.n_points >= 2 && q.energy && q.total && q.absorption && q.scatter,
-              "nuclide " + std::to_string(i) + ": need >= 2 points and total/absorption/scatter tables");
-      require(q.awr > 0.0, "nuclide " + std::to_string(i) + ": awr must be positive");
-      HostNuclide h;
-      h.awr = q.awr;
-      h.E.assign(q.energy, q.energy + q.n_points);
-      require(h.E[0] > 0.0, "nuclide " + std::to_string(i) + ": energies must be positive");
-      for (int j = 1; j < q.n_points; ++j)
-        require(h.E[j] > h.E[j - 1], "nuclide " + std::to_string(i) + ": energy grid must increase");
-      h.t.assign(q.total, q.total + q.n_points);
-      h.a.assign(q.absorption, q.absorption + q.n_points);
-      h.s.assign(q.scatter, q.scatter + q.n_points);
-      if (q.fission) {
-        h.f.assign(q.fission, q.fission + q.n_points);
-        for (int k = 0; k < 2; ++k) {
-          require(std::isfinite(q.nu[k]) && std::isfinite(q.watt[k]),
-                  "nuclide " + std::to_string(i) + ": nu and Watt parameters must be finite");
-          h.nu[k] = q.nu[k];
-          h.watt[k] = q.watt[k];
-        }
-        require(q.watt[0] > 0.0 && q.watt[1] >= 0.0,
-                "nuclide " + std::to_string(i) + ": the fission spectrum needs Watt a > 0 and b >= 0");
-      }
-      c->nucs.push_back(std::move(h));
+    std::vector<HostNuclide> v;
+    for (int i = 0; i < n; ++i) v.push_back(read_nuclide(i, nuc[i]));
+    c->nucs = std::move(v);
+    c->nuc_temps.assign(c->nucs.size(), {});
+    c->dirty = true;
+  });
+}
+
+int mcw_xs_add_temperature(mcw_ctx *c, int32_t nuclide, const mcw_nuclide *table)
+{
+  return guard(c, [&] {
+    require(table, "mcw_xs_add_temperature: no table");
+    require(nuclide >= 0 && nuclide < (int32_t)c->nucs.size(), "mcw_xs_add_temperature: no such nuclide");
+    HostNuclide h = read_nuclide(nuclide, *table);
+    const HostNuclide &b = c->nucs[nuclide];
+    require(h.f.empty() == b.f.empty(),
+            "nuclide " + std::to_string(nuclide) + ": every temperature needs fission data, or none");
+    require(h.temperature != b.temperature,
+            "nuclide " + std::to_string(nuclide) + ": a table at this temperature is loaded");
+    for (auto &o : c->nuc_temps[nuclide])
+      require(h.temperature != o.temperature,
+              "nuclide " + std::to_string(nuclide) + ": a table at this temperature is loaded");
+    // one set of reaction products per nuclide
+    h.awr = b.awr;
+    for (int k = 0; k < 2; ++k) {
+      h.nu[k] = b.nu[k];
+      h.watt[k] = b.watt[k];
     }
+    c->nuc_temps[nuclide].push_back(std::move(h));
+    c->dirty = true;
+  });
+}
+
+int mcw_temperature_set(mcw_ctx *c, int64_t n_elem, const double *kelvin, double tolerance)
+{
+  return guard(c, [&] {
+    require(std::isfinite(tolerance) && tolerance >= 0.0, "mcw_temperature_set: tolerance must be >= 0");
+    if (n_elem == 0) {
+      c->elem_T.clear(); // back to every nuclide's first table
+    } else {
+      require(n_elem > 0 && kelvin, "mcw_temperature_set: no temperatures");
+      for (int64_t e = 0; e < n_elem; ++e)
+        require(std::isfinite(kelvin[e]) && kelvin[e] >= 0.0,
+                "element " + std::to_string(e) + ": temperature must be finite and non-negative");
+      c->elem_T.assign(kelvin, kelvin + n_elem);
+    }
+    c->T_tol = tolerance;
     c->dirty = true;
   });
 }

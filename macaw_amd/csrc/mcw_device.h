@@ -520,9 +520,9 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
         macro_all(X, mat, k, E, M);
       } else if (mat >= 0) {
         for (int32_t q = X.mat_off[mat]; q < X.mat_off[mat + 1]; ++q)
-          if (X.mat_nuc[q] == nuc) {
+          if (X.nuc_of[X.mat_nuc[q]] == nuc) {
             Micro mu;
-            micro_at(X, nuc, k, E, mu);
+            micro_at(X, X.mat_nuc[q], k, E, mu);
             double dn = X.mat_dens[q];
             M.t = dn * mu.t;
             M.a = dn * mu.a;

@@ -99,12 +99,13 @@ struct XSView {
   const double *watt_t;      // per nuclide: r1 r2 above this means the first Watt attempt is
                              // below e_max (-a ln(r1 r2) < the w that reaches e_max)
   const double *awr;
+  const int32_t *nuc_of;     // per nuclide instance (one temperature's tables): the caller's nuclide
   // unionized grid: every nuclide's points merged, sorted, unique
   const double *U;
   const int32_t *uidx;       // n_union x n_nuc: largest j with E_n[j] <= U[k], -1 if none
   const int32_t *hash;       // n_hash + 1: largest k with U[k] <= lower edge of log bucket b
   int32_t n_union;
-  int32_t n_nuc;
+  int32_t n_nuc;             // nuclide instances: the caller's nuclides first, then other temperatures
   int32_t n_hash;
   int32_t n_mat;
   double logU0, inv_dl;      // bucket b = floor((log E - logU0) * inv_dl)

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 from . import abi
-from .problems import Problem, load_into
+from .problems import Problem, load_into, nuclide_struct
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libmacaw_amd.so")
@@ -23,7 +23,8 @@ EXPORTS = (
     "mcw_tally_shape", "mcw_run_batch", "mcw_tally_get", "mcw_tally_reset", "mcw_history_get",
     "mcw_server_create", "mcw_server_destroy", "mcw_server_last_error", "mcw_submit", "mcw_poll",
     "mcw_result", "mcw_trace_get", "mcw_debug_math_probe", "mcw_debug_xs_probe",
-    "mcw_keff_get", "mcw_fission_bank_get", "mcw_source_bank_get",
+    "mcw_keff_get", "mcw_fission_bank_get", "mcw_source_bank_get", "mcw_xs_add_temperature",
+    "mcw_temperature_set",
 )
 
 
@@ -49,6 +50,8 @@ def lib():
         "mcw_mesh_set": ([vp, i64, dp, i64, u8p, i64p, i32p, i32p, i32p], C.c_int),
         "mcw_bc_set": ([vp, i32, i32], C.c_int),
         "mcw_xs_set": ([vp, i32, C.POINTER(abi.Nuclide)], C.c_int),
+        "mcw_xs_add_temperature": ([vp, i32, C.POINTER(abi.Nuclide)], C.c_int),
+        "mcw_temperature_set": ([vp, i64, dp, C.c_double], C.c_int),
         "mcw_materials_set": ([vp, i32, C.POINTER(abi.Material)], C.c_int),
         "mcw_block_material_map": ([vp, i32, i32p, i32p], C.c_int),
         "mcw_source_set": ([vp, C.POINTER(abi.Source)], C.c_int),
@@ -117,6 +120,8 @@ class Context:
             "mesh_set": lambda *a: self._chk(L.mcw_mesh_set(h, *a)),
             "bc_set": lambda *a: self._chk(L.mcw_bc_set(h, *a)),
             "xs_set": lambda *a: self._chk(L.mcw_xs_set(h, *a)),
+            "xs_add_temperature": lambda *a: self._chk(L.mcw_xs_add_temperature(h, *a)),
+            "temperature_set": lambda *a: self._chk(L.mcw_temperature_set(h, *a)),
             "materials_set": lambda *a: self._chk(L.mcw_materials_set(h, *a)),
             "block_material_map": lambda *a: self._chk(L.mcw_block_material_map(h, *a)),
             "source_set": lambda *a: self._chk(L.mcw_source_set(h, *a)),
@@ -131,6 +136,22 @@ class Context:
         for k, v in kw.items():
             setattr(self.opts, k, v)
         self._chk(self.L.mcw_options_set(self.h, C.byref(self.opts)))
+
+    def set_temperatures(self, kelvin, tolerance=10.0):
+        """Element temperatures [K] in mesh order (None: every element reads
+        each nuclide's first table), mcw_temperature_set."""
+        if kelvin is None:
+            self._chk(self.L.mcw_temperature_set(self.h, 0, None, float(tolerance)))
+            return
+        T = np.ascontiguousarray(kelvin, np.float64)
+        self._chk(self.L.mcw_temperature_set(self.h, T.size, T.ctypes.data_as(C.POINTER(C.c_double)),
+                                             float(tolerance)))
+
+    def add_temperature(self, nuclide, table):
+        """A further table (a NuclideData at table.temperature) of a nuclide."""
+        keep = []
+        n = nuclide_struct(table, keep)
+        self._chk(self.L.mcw_xs_add_temperature(self.h, int(nuclide), C.byref(n)))
 
     def run_batch(self, batch, n_particles):
         st = abi.BatchStats()

@@ -52,6 +52,7 @@ class NuclideData:
     fission: Optional[np.ndarray] = None   # part of absorption; makes the nuclide fissionable
     nu: Tuple[float, float] = (0.0, 0.0)   # nu(E) = nu[0] + nu[1] E
     watt: Tuple[float, float] = (0.988e6, 2.249e-6)  # fission spectrum a [eV], b [1/eV]
+    temperature: float = 0.0               # K of this table (0: 293.6)
 
 
 @dataclasses.dataclass
@@ -82,13 +83,18 @@ class Problem:
     tallies: List[TallySpec]
     n_particles: int
     n_batches: int
+    # further tables per nuclide index (other temperatures), element
+    # temperatures [K] in mesh order, and the nearest-table tolerance [K]
+    nuclide_temps: Dict[int, List[NuclideData]] = dataclasses.field(default_factory=dict)
+    elem_temperature: Optional[np.ndarray] = None
+    temperature_tolerance: float = 10.0
 
 
 # ----------------------------------------------------------------- nuclides
 
 def synth_nuclide(n_points: int, awr: float, seed: int, sigma_a0: float = 1.0,
                   sigma_p: float = 4.7, n_res: int = 50, e_min: float = 1e-5,
-                  e_max: float = 2e7, jitter: float = 0.0) -> NuclideData:
+                  e_max: float = 2e7, jitter: float = 0.0, temperature: float = 0.0) -> NuclideData:
     """A pointwise nuclide: sigma_a = sigma_a0 sqrt(0.0253/E) + SLBW capture,
     sigma_s = sigma_p + SLBW elastic, on a log grid of n_points in [e_min, e_max] eV.
 
@@ -96,7 +102,12 @@ def synth_nuclide(n_points: int, awr: float, seed: int, sigma_a0: float = 1.0,
     [1e-2, 10] eV, peak log-uniform in [10, 1e4] b, split Gamma_n/Gamma_gamma
     by a uniform fraction in [0.2, 0.8].  With jitter > 0 (below 0.5) every
     interior grid point moves by U(-jitter, jitter) log-steps, so nuclides of
-    different seeds have different grids (a union grid of all their points)."""
+    different seeds have different grids (a union grid of all their points).
+
+    temperature > 0 [K] broadens every resonance for the Doppler width
+    Gamma_D = sqrt(4 k T E_r / awr): width sqrt(Gamma^2 + Gamma_D^2), peak
+    scaled to keep Gamma * peak (the same resonances at every temperature
+    for one seed)."""
     rng = np.random.default_rng(seed)
     E = np.geomspace(e_min, e_max, n_points)
     E[0], E[-1] = e_min, e_max
@@ -108,6 +119,11 @@ def synth_nuclide(n_points: int, awr: float, seed: int, sigma_a0: float = 1.0,
         assert jitter < 0.5
         step = (np.log(e_max) - np.log(e_min)) / (n_points - 1)
         E[1:-1] = np.exp(np.log(E[1:-1]) + rng.uniform(-jitter, jitter, n_points - 2) * step)
+    if temperature > 0.0:
+        GD = np.sqrt(4.0 * abi.K_BOLTZMANN * temperature * Er / awr)
+        Geff = np.sqrt(G * G + GD * GD)
+        peak = peak * G / Geff
+        G = Geff
     sa = sigma_a0 * np.sqrt(0.0253 / E)
     ss = np.full_like(E, sigma_p)
     for r in range(n_res):
@@ -116,7 +132,8 @@ def synth_nuclide(n_points: int, awr: float, seed: int, sigma_a0: float = 1.0,
         sa += shape * fg[r] * np.sqrt(Er[r] / E)
         ss += shape * (1.0 - fg[r])
     st = sa + ss
-    return NuclideData(awr=float(awr), energy=E, total=st, absorption=sa, scatter=ss)
+    return NuclideData(awr=float(awr), energy=E, total=st, absorption=sa, scatter=ss,
+                       temperature=float(temperature))
 
 
 def flat_nuclide(awr: float, total: float, absorption: float, fission: float = 0.0,
@@ -426,11 +443,12 @@ C2_NUCLIDES = [  # (A, sigma_p [b], sigma_a0 [b])
 ]
 
 
-def c2_nuclides(n_points=50_000, n_nuclides=4, jitter=0.0):
+def c2_nuclides(n_points=50_000, n_nuclides=4, jitter=0.0, temperature=0.0):
     out = []
     for i in range(n_nuclides):
         A, sp, sa = C2_NUCLIDES[i % len(C2_NUCLIDES)]
-        out.append(synth_nuclide(n_points, A, seed=1000 + i, sigma_a0=sa, sigma_p=sp, jitter=jitter))
+        out.append(synth_nuclide(n_points, A, seed=1000 + i, sigma_a0=sa, sigma_p=sp, jitter=jitter,
+                                 temperature=temperature))
     return out
 
 
@@ -461,12 +479,13 @@ def c2_tet_vacuum(n_lattice=54, n_particles=10_000_000, n_batches=1, n_points=50
                    n_particles=n_particles, n_batches=n_batches)
 
 
-def fissile_nuclide(n_points=50_000, seed=1235, jitter=0.0) -> NuclideData:
+def fissile_nuclide(n_points=50_000, seed=1235, jitter=0.0, temperature=0.0) -> NuclideData:
     """A synthetic heavy fissile nuclide: A = 235, 1/v absorption of 680 b at
     0.0253 eV plus SLBW resonances plus a flat 1.5 b (fast absorption), 85% of
     absorption is fission, nu = 2.43, Watt fission spectrum (a = 0.988 MeV,
     b = 2.249 /MeV)."""
-    n = synth_nuclide(n_points, 235.0, seed=seed, sigma_a0=680.0, sigma_p=10.0, jitter=jitter)
+    n = synth_nuclide(n_points, 235.0, seed=seed, sigma_a0=680.0, sigma_p=10.0, jitter=jitter,
+                      temperature=temperature)
     n = dataclasses.replace(n, absorption=n.absorption + 1.5, total=n.total + 1.5)
     return fissile(n, 0.85, nu=(2.43, 0.0))
 
@@ -488,6 +507,37 @@ def k2_tet_eigen(n_lattice=54, n_particles=10_000_000, n_batches=5, n_points=50_
                    block_material={b: b + 1 for b in range(n_materials)}, bcs={0: abi.BC_VACUUM},
                    source=volume_watt_source(), tallies=flux_tallies(both=False),
                    n_particles=n_particles, n_batches=n_batches)
+
+
+TEMPERATURES = (293.6, 600.0, 900.0, 1200.0)
+
+
+def with_temperatures(p: Problem, temps=TEMPERATURES, t_range=(300.0, 1200.0), n_points=None,
+                      tolerance=200.0, axis=0) -> Problem:
+    """p (built from c2_nuclides, plus fissile_nuclide as a fifth nuclide when
+    present) with every nuclide tabulated at each of `temps` (the first
+    replaces p's own table; each further temperature has a grid 37 points
+    longer, so tables of one nuclide differ in grid too) and element
+    temperatures rising linearly along `axis` over t_range across the mesh."""
+    n_points = n_points or p.nuclides[0].energy.size
+    nn = len(p.nuclides)
+
+    def tables(T, extra):
+        out = c2_nuclides(n_points + extra, min(nn, 4), temperature=T)
+        if nn > 4:
+            out.append(fissile_nuclide(n_points + extra, temperature=T))
+        return out
+
+    base = tables(temps[0], 0)
+    more = {q: [] for q in range(nn)}
+    for k, T in enumerate(temps[1:], 1):
+        for q, t in enumerate(tables(T, 37 * k)):
+            more[q].append(t)
+    c = centroids(p.mesh)[:, axis]
+    lo, hi = c.min(), c.max()
+    T = t_range[0] + (t_range[1] - t_range[0]) * (c - lo) / max(hi - lo, 1e-300)
+    return dataclasses.replace(p, name=p.name + "_temps", nuclides=base, nuclide_temps=more,
+                               elem_temperature=T, temperature_tolerance=float(tolerance))
 
 
 def c3_tet_vacuum(n_lattice=107, n_particles=100_000_000, n_batches=1) -> Problem:
@@ -572,6 +622,16 @@ def c4_requests(n=2000, seed=505, n_meshes=3, n_min=1e4, n_max=1e7, rate=None) -
 
 # ----------------------------------------------------------------- marshalling
 
+def nuclide_struct(n: NuclideData, keep: list) -> abi.Nuclide:
+    """mcw_nuclide view of n; the arrays it points to are appended to keep."""
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (n.energy, n.total, n.absorption, n.scatter)]
+    fis = None if n.fission is None else np.ascontiguousarray(n.fission, np.float64)
+    keep += arrs + [fis]
+    return abi.Nuclide(n.energy.size, n.awr, *[abi.ptr(a, abi.C.c_double) for a in arrs],
+                       abi.ptr(fis, abi.C.c_double), (abi.C.c_double * 2)(*n.nu),
+                       (abi.C.c_double * 2)(*n.watt), float(n.temperature))
+
+
 class Marshal:
     """Holds ctypes views of a Problem's arrays alive while a library copies them."""
 
@@ -580,12 +640,10 @@ class Marshal:
         self.keep = []
         self.nuclides = (abi.Nuclide * len(p.nuclides))()
         for i, n in enumerate(p.nuclides):
-            arrs = [np.ascontiguousarray(a, np.float64) for a in (n.energy, n.total, n.absorption, n.scatter)]
-            fis = None if n.fission is None else np.ascontiguousarray(n.fission, np.float64)
-            self.keep += arrs + [fis]
-            self.nuclides[i] = abi.Nuclide(n.energy.size, n.awr, *[abi.ptr(a, abi.C.c_double) for a in arrs],
-                                           abi.ptr(fis, abi.C.c_double), (abi.C.c_double * 2)(*n.nu),
-                                           (abi.C.c_double * 2)(*n.watt))
+            self.nuclides[i] = self.nuclide(n)
+        self.temps = [(int(q), self.nuclide(t)) for q, ts in sorted(p.nuclide_temps.items()) for t in ts]
+        self.elem_T = (None if p.elem_temperature is None
+                       else np.ascontiguousarray(p.elem_temperature, np.float64))
         self.materials = (abi.Material * len(p.materials))()
         for i, m in enumerate(p.materials):
             nu = np.ascontiguousarray(m.nuclides, np.int32)
@@ -595,6 +653,9 @@ class Marshal:
                                              abi.ptr(de, abi.C.c_double))
         self.blocks = np.array(list(p.block_material.keys()), np.int32)
         self.block_mats = np.array(list(p.block_material.values()), np.int32)
+
+    def nuclide(self, n: NuclideData) -> abi.Nuclide:
+        return nuclide_struct(n, self.keep)
 
     def filters(self, t: TallySpec):
         arr = (abi.Filter * max(1, len(t.filters)))()
@@ -615,9 +676,9 @@ class Marshal:
 def load_into(lib_prefix_calls, p: Problem):
     """Feed a Problem through a set of setter callables with the mcw_* signatures.
 
-    `lib_prefix_calls` maps names ('mesh_set', 'bc_set', 'xs_set', 'materials_set',
-    'block_material_map', 'source_set', 'tally_add') to callables returning 0 on success
-    and raising/returning non-zero on error."""
+    `lib_prefix_calls` maps names ('mesh_set', 'bc_set', 'xs_set', 'xs_add_temperature',
+    'temperature_set', 'materials_set', 'block_material_map', 'source_set', 'tally_add') to
+    callables returning 0 on success and raising/returning non-zero on error."""
     m = p.mesh
     M = Marshal(p)
     c = lib_prefix_calls
@@ -629,6 +690,10 @@ def load_into(lib_prefix_calls, p: Problem):
     for b, k in p.bcs.items():
         c["bc_set"](int(b), int(k))
     c["xs_set"](len(p.nuclides), M.nuclides)
+    for q, t in M.temps:
+        c["xs_add_temperature"](q, abi.C.byref(t))
+    if M.elem_T is not None:
+        c["temperature_set"](M.elem_T.size, abi.ptr(M.elem_T, abi.C.c_double), float(p.temperature_tolerance))
     c["materials_set"](len(p.materials), M.materials)
     c["block_material_map"](len(M.blocks), abi.ptr(M.blocks, abi.C.c_int32), abi.ptr(M.block_mats, abi.C.c_int32))
     c["source_set"](abi.C.byref(p.source))

@@ -25,6 +25,10 @@
  *       (src/userobjects/OpenMCStudy.C:454-607), generation and average keff
  *       (OpenMCStudy.C:404-452, OpenMC calculate_generation_keff /
  *       calculate_average_keff)
+ *   - element temperatures ...................... CollisionKernel.C:109 (sqrt(kT) from the
+ *       element's CONSTANT MONOMIAL temperature), OpenMC Nuclide::calculate_xs with the
+ *       default "nearest" temperature method (the table whose kT is closest to
+ *       sqrt(kT)^2, ties to the lower temperature), looked up per segment
  *
  * Synthetic nuclides are pointwise tables: lin-lin interpolation at the
  * bracketing grid index (largest j with E_j <= E).  Elastic scattering is
@@ -74,7 +78,11 @@ struct Nuclide {
   bool fissionable = false;
   double nu0 = 0.0, nu1 = 0.0;      // nu(E) = nu0 + nu1 E
   double watt_a = 0.0, watt_b = 0.0; // fission spectrum
+  double temperature = 293.6;        // K
 };
+
+// Boltzmann constant [eV/K] (OpenMC K_BOLTZMANN).
+constexpr double K_BOLTZMANN = 8.617333262e-5;
 
 struct Material {
   int id;
@@ -128,6 +136,10 @@ struct Oracle {
   std::map<int, int> bc;           // boundary id -> kind
   // physics
   std::vector<Nuclide> nucs;
+  std::vector<std::vector<Nuclide>> nuc_temps;          // further tables per nuclide
+  std::vector<std::vector<const Nuclide *>> tables;     // every table, ascending temperature
+  std::vector<double> elem_T;                           // K per element; empty: first tables
+  double T_tol = 10.0;
   std::vector<Material> mats;
   std::map<int, int> block_mat;      // block -> material id
   std::vector<int32_t> elem_mat;     // element -> material index (-1 void)
@@ -453,14 +465,40 @@ struct Macro {
   double t, a, s, f;
 };
 
-void macro_xs(const Oracle &o, int mat, double E, Macro *M)
+// sqrt(kT) of an element [sqrt(eV)]; -1 when no temperatures are set.
+inline double elem_sqrtkT(const Oracle &o, int64_t elem)
+{
+  return o.elem_T.empty() ? -1.0 : std::sqrt(K_BOLTZMANN * o.elem_T[elem]);
+}
+
+// The table of nuclide q read at sqrt(kT) = skT: OpenMC's "nearest" method
+// over the nuclide's temperatures in ascending order, strict improvement, so
+// a tie keeps the lower temperature.  skT < 0: the first table given.
+const Nuclide &nuc_at(const Oracle &o, int q, double skT)
+{
+  if (skT < 0.0) return o.nucs[q];
+  const std::vector<const Nuclide *> &tb = o.tables[q];
+  const double kT = skT * skT;
+  size_t it = 0;
+  double dmin = INFINITY;
+  for (size_t t = 0; t < tb.size(); ++t) {
+    const double d = std::fabs(K_BOLTZMANN * tb[t]->temperature - kT);
+    if (d < dmin) {
+      dmin = d;
+      it = t;
+    }
+  }
+  return *tb[it];
+}
+
+void macro_xs(const Oracle &o, int mat, double E, double skT, Macro *M)
 {
   M->t = M->a = M->s = M->f = 0.0;
   if (mat < 0) return;
   const Material &m = o.mats[mat];
   for (size_t i = 0; i < m.nuc.size(); ++i) {
     Micro u;
-    micro_xs(o.nucs[m.nuc[i]], E, &u);
+    micro_xs(nuc_at(o, m.nuc[i], skT), E, &u);
     M->t = M->t + m.dens[i] * u.t;
     M->a = M->a + m.dens[i] * u.a;
     M->s = M->s + m.dens[i] * u.s;
@@ -468,14 +506,14 @@ void macro_xs(const Oracle &o, int mat, double E, Macro *M)
   }
 }
 
-double macro_total(const Oracle &o, int mat, double E)
+double macro_total(const Oracle &o, int mat, double E, double skT)
 {
   if (mat < 0) return 0.0;
   const Material &m = o.mats[mat];
   double t = 0.0;
   for (size_t i = 0; i < m.nuc.size(); ++i) {
     Micro u;
-    micro_xs(o.nucs[m.nuc[i]], E, &u);
+    micro_xs(nuc_at(o, m.nuc[i], skT), E, &u);
     t = t + m.dens[i] * u.t;
   }
   return t;
@@ -641,7 +679,7 @@ struct Scratch {
 };
 
 void score(const Oracle &o, const Tally &t, std::vector<double> &val, int64_t elem, double E, double wfl,
-           int mat)
+           int mat, double skT)
 {
   int64_t fi = 0;
   for (size_t k = 0; k < t.filters.size(); ++k) {
@@ -671,13 +709,13 @@ void score(const Oracle &o, const Tally &t, std::vector<double> &val, int64_t el
     for (int sc : t.scores) need_xs |= sc != MCW_SCORE_FLUX;
     if (need_xs) {
       if (nuc < 0) {
-        macro_xs(o, mat, E, &Ma);
+        macro_xs(o, mat, E, skT, &Ma);
       } else if (mat >= 0) {
         const Material &m = o.mats[mat];
         for (size_t q = 0; q < m.nuc.size(); ++q)
           if (m.nuc[q] == nuc) {
             dens = m.dens[q];
-            micro_xs(o.nucs[nuc], E, &mu);
+            micro_xs(nuc_at(o, nuc, skT), E, &mu);
             Ma.t = dens * mu.t;
             Ma.a = dens * mu.a;
             Ma.s = dens * mu.s;
@@ -724,14 +762,14 @@ struct EigenHistory {
 };
 
 // Macroscopic total and nu-fission at (mat, E).
-double macro_total_nf(const Oracle &o, int mat, double E, double *nusf)
+double macro_total_nf(const Oracle &o, int mat, double E, double skT, double *nusf)
 {
   *nusf = 0.0;
   if (mat < 0) return 0.0;
   const Material &m = o.mats[mat];
   double t = 0.0, nf = 0.0;
   for (size_t i = 0; i < m.nuc.size(); ++i) {
-    const Nuclide &n = o.nucs[m.nuc[i]];
+    const Nuclide &n = nuc_at(o, m.nuc[i], skT);
     Micro u;
     micro_xs(n, E, &u);
     t = t + m.dens[i] * u.t;
@@ -775,7 +813,7 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
   int32_t nseg = 0, ncoll = 0;
   uint64_t h = 0xcbf29ce484222325ULL;
   int last_mat = -2;
-  double last_E = -1.0, sigt = 0.0;
+  double last_E = -1.0, last_skT = -2.0, sigt = 0.0;
   const std::vector<Tally> &T = o.tallies;
 
   if (elem < 0) fate = MCW_FATE_LOST;
@@ -787,10 +825,13 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
     }
     const int64_t seg_elem = elem;
     const int mat = o.elem_mat[elem];
-    if (mat != last_mat || E != last_E) {
-      sigt = eg ? macro_total_nf(o, mat, E, &nusf) : macro_total(o, mat, E);
+    // OpenMC recomputes the material xs when the material, E or sqrt(kT) changed
+    const double skT = elem_sqrtkT(o, elem);
+    if (mat != last_mat || E != last_E || skT != last_skT) {
+      sigt = eg ? macro_total_nf(o, mat, E, skT, &nusf) : macro_total(o, mat, E, skT);
       last_mat = mat;
       last_E = E;
+      last_skT = skT;
     }
     // exit side: smallest non-negative distance over outgoing planes,
     // the incoming side excluded, ties to the lowest side index
@@ -827,7 +868,7 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
     if (eg) eg->k_tl = eg->k_tl + (wgt * track) * nusf;
     if (tally_on)
       for (size_t k = 0; k < T.size(); ++k)
-        if (T[k].estimator == MCW_EST_TRACKLENGTH) score(o, T[k], sc.value[k], elem, E, wgt * track, mat);
+        if (T[k].estimator == MCW_EST_TRACKLENGTH) score(o, T[k], sc.value[k], elem, E, wgt * track, mat, skT);
 
     int ev;
     if (d > L) {
@@ -879,7 +920,7 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
       ++ncoll;
       if (tally_on)
         for (size_t k = 0; k < T.size(); ++k)
-          if (T[k].estimator == MCW_EST_COLLISION) score(o, T[k], sc.value[k], elem, E, wgt / sigt, mat);
+          if (T[k].estimator == MCW_EST_COLLISION) score(o, T[k], sc.value[k], elem, E, wgt / sigt, mat, skT);
       // keff collision estimator (OpenMC Particle::event_collide)
       if (eg) eg->k_col = eg->k_col + (wgt * nusf) / sigt;
       // sample nuclide
@@ -889,14 +930,14 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
       int pick = (int)m.nuc.size() - 1;
       Micro mu;
       for (size_t q = 0; q < m.nuc.size(); ++q) {
-        micro_xs(o.nucs[m.nuc[q]], E, &mu);
+        micro_xs(nuc_at(o, m.nuc[q], skT), E, &mu);
         prob = prob + m.dens[q] * mu.t;
         if (prob >= cutoff) {
           pick = (int)q;
           break;
         }
       }
-      const Nuclide &nuc = o.nucs[m.nuc[pick]];
+      const Nuclide &nuc = nuc_at(o, m.nuc[pick], skT);
       micro_xs(nuc, E, &mu);
       const double nuf = (nuc.nu0 + nuc.nu1 * E) * mu.f;
       if (eg && nuc.fissionable && mu.t > 0.0) {
@@ -1050,12 +1091,32 @@ bool check_ready(Oracle *o)
     if (m == id_to_idx.end()) return fail(o, "Could not find material " + std::to_string(b->second));
     o->elem_mat[e] = m->second;
   }
+  // every nuclide's tables in ascending temperature (OpenMC kTs_ order)
+  o->nuc_temps.resize(o->nucs.size());
+  o->tables.assign(o->nucs.size(), {});
+  for (size_t q = 0; q < o->nucs.size(); ++q) {
+    o->tables[q].push_back(&o->nucs[q]);
+    for (auto &t : o->nuc_temps[q]) o->tables[q].push_back(&t);
+    std::stable_sort(o->tables[q].begin(), o->tables[q].end(),
+                     [](const Nuclide *a, const Nuclide *b) { return a->temperature < b->temperature; });
+  }
+  if (!o->elem_T.empty()) {
+    if ((int64_t)o->elem_T.size() != o->n_elem) return fail(o, "one temperature per element");
+    for (int64_t e = 0; e < o->n_elem; ++e) {
+      if (o->elem_mat[e] < 0) continue;
+      for (int q : o->mats[o->elem_mat[e]].nuc)
+        if (std::fabs(nuc_at(*o, q, elem_sqrtkT(*o, e)).temperature - o->elem_T[e]) > o->T_tol)
+          return fail(o, "no cross sections for nuclide " + std::to_string(q) + " near element " +
+                             std::to_string(e) + "'s temperature");
+    }
+  }
   o->energy_min = INFINITY;
   o->energy_max = -INFINITY;
-  for (auto &n : o->nucs) {
-    o->energy_min = std::min(o->energy_min, n.E.front());
-    o->energy_max = std::max(o->energy_max, n.E.back());
-  }
+  for (auto &tb : o->tables)
+    for (const Nuclide *n : tb) {
+      o->energy_min = std::min(o->energy_min, n->E.front());
+      o->energy_max = std::max(o->energy_max, n->E.back());
+    }
   // a shared grid range is assumed by the energy cutoff
   return true;
 }
@@ -1122,8 +1183,40 @@ int orc_xs_set(void *h, int32_t n, const mcw_nuclide *nuc)
     q.nu1 = nuc[i].nu[1];
     q.watt_a = nuc[i].watt[0];
     q.watt_b = nuc[i].watt[1];
+    if (nuc[i].temperature > 0.0) q.temperature = nuc[i].temperature;
     o->nucs.push_back(std::move(q));
   }
+  o->nuc_temps.assign(o->nucs.size(), {});
+  return 0;
+}
+
+/* A further table of nuclide `q` at t->temperature; awr, nu and the Watt
+ * spectrum stay the first table's. */
+int orc_xs_add_temperature(void *h, int32_t q, const mcw_nuclide *t)
+{
+  Oracle *o = (Oracle *)h;
+  if (q < 0 || q >= (int32_t)o->nucs.size()) return fail(o, "no such nuclide") ? 0 : 1;
+  const int np = t->n_points;
+  if (np < 2) return fail(o, "nuclide grid needs at least 2 points") ? 0 : 1;
+  Nuclide n = o->nucs[q];
+  n.temperature = t->temperature > 0.0 ? t->temperature : 293.6;
+  n.E.assign(t->energy, t->energy + np);
+  n.t.assign(t->total, t->total + np);
+  n.a.assign(t->absorption, t->absorption + np);
+  n.s.assign(t->scatter, t->scatter + np);
+  n.f.clear();
+  if (t->fission) n.f.assign(t->fission, t->fission + np);
+  if (n.f.empty() == o->nucs[q].fissionable) return fail(o, "fission data on some temperatures only") ? 0 : 1;
+  o->nuc_temps[q].push_back(std::move(n));
+  return 0;
+}
+
+/* Element temperatures [K] (n = 0: none, every element reads first tables). */
+int orc_temperature_set(void *h, int64_t n, const double *kelvin, double tolerance)
+{
+  Oracle *o = (Oracle *)h;
+  o->elem_T.assign(kelvin, kelvin + n);
+  o->T_tol = tolerance;
   return 0;
 }
 
@@ -1504,7 +1597,7 @@ int orc_micro_xs(void *h, int32_t nuc, double E, double *out)
   return 0;
 }
 
-double orc_macro_total(void *h, int32_t mat, double E) { return macro_total(*(Oracle *)h, mat, E); }
+double orc_macro_total(void *h, int32_t mat, double E) { return macro_total(*(Oracle *)h, mat, E, -1.0); }
 
 /* Elementary functions and RNG, exported for tests. */
 double orc_log_x(double x) { return orc_log(x); }

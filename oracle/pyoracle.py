@@ -41,6 +41,10 @@ def lib():
         L.orc_mesh_set.argtypes = [vp, i64, dp, i64, u8p, i64p, i32p, i32p, i32p]
         L.orc_bc_set.argtypes = [vp, i32, i32]
         L.orc_xs_set.argtypes = [vp, i32, C.POINTER(abi.Nuclide)]
+        L.orc_xs_add_temperature.restype = C.c_int
+        L.orc_xs_add_temperature.argtypes = [vp, i32, C.POINTER(abi.Nuclide)]
+        L.orc_temperature_set.restype = C.c_int
+        L.orc_temperature_set.argtypes = [vp, i64, dp, C.c_double]
         L.orc_materials_set.argtypes = [vp, i32, C.POINTER(abi.Material)]
         L.orc_block_material_map.argtypes = [vp, i32, i32p, i32p]
         L.orc_source_set.argtypes = [vp, C.POINTER(abi.Source)]
@@ -140,6 +144,8 @@ class Oracle:
             "mesh_set": lambda *a: self._chk(L.orc_mesh_set(h, *a)),
             "bc_set": lambda *a: self._chk(L.orc_bc_set(h, *a)),
             "xs_set": lambda *a: self._chk(L.orc_xs_set(h, *a)),
+            "xs_add_temperature": lambda *a: self._chk(L.orc_xs_add_temperature(h, *a)),
+            "temperature_set": lambda *a: self._chk(L.orc_temperature_set(h, *a)),
             "materials_set": lambda *a: self._chk(L.orc_materials_set(h, *a)),
             "block_material_map": lambda *a: self._chk(L.orc_block_material_map(h, *a)),
             "source_set": lambda *a: self._chk(L.orc_source_set(h, *a)),
