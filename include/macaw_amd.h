@@ -245,11 +245,17 @@ int mcw_xs_set(mcw_ctx *ctx, int32_t n_nuclides, const mcw_nuclide *nuclides);
  * temperature). */
 int mcw_xs_add_temperature(mcw_ctx *ctx, int32_t nuclide, const mcw_nuclide *table);
 
+/* Boltzmann constant [eV/K] (OpenMC K_BOLTZMANN): kT of a table is
+ * MCW_K_BOLTZMANN * temperature. */
+#define MCW_K_BOLTZMANN 8.617333262e-5
+
 /* Element temperatures [K], caller order (MaCaw's CONSTANT MONOMIAL
- * temperature variable).  A nuclide whose nearest table is further than
- * `tolerance` K from an element's temperature is an error at the next batch
- * (OpenMC settings temperature_tolerance, default 10 K).  Without this call,
- * every element reads each nuclide's first table. */
+ * temperature variable); n_elem = 0 clears them.  A nuclide whose nearest
+ * table is further than `tolerance` K from an element's temperature is an
+ * error at the next batch; INFINITY reads the nearest table however far, as
+ * OpenMC does at run time (its temperature_tolerance only decides which
+ * tables are loaded).  Without temperatures, every element reads each
+ * nuclide's first table. */
 int mcw_temperature_set(mcw_ctx *ctx, int64_t n_elem, const double *kelvin, double tolerance);
 int mcw_materials_set(mcw_ctx *ctx, int32_t n_materials, const mcw_material *materials);
 

@@ -126,8 +126,7 @@ struct HostNuclide {
   double temperature = 293.6; // K
 };
 
-// Boltzmann constant [eV/K], as OpenMC's K_BOLTZMANN.
-constexpr double K_BOLTZMANN = 8.617333262e-5;
+constexpr double K_BOLTZMANN = MCW_K_BOLTZMANN;
 
 struct HostMaterial {
   int32_t id;
@@ -1326,7 +1325,7 @@ int mcw_xs_add_temperature(mcw_ctx *c, int32_t nuclide, const mcw_nuclide *table
 int mcw_temperature_set(mcw_ctx *c, int64_t n_elem, const double *kelvin, double tolerance)
 {
   return guard(c, [&] {
-    require(std::isfinite(tolerance) && tolerance >= 0.0, "mcw_temperature_set: tolerance must be >= 0");
+    require(tolerance >= 0.0, "mcw_temperature_set: tolerance must be >= 0 (INFINITY: no limit)");
     if (n_elem == 0) {
       c->elem_T.clear(); // back to every nuclide's first table
     } else {

@@ -194,9 +194,17 @@ MeshData build_mesh(const HitNode &mb)
 
 // ------------------------------------------------------------------ synthetic nuclides
 
+static NuclideData resonance_table(const NuclideData &base, long long np, double e0, double e1, double sa0,
+                                   double sa_flat, double sp, double ff, const std::vector<double> &Er,
+                                   const std::vector<double> &G, const std::vector<double> &pk,
+                                   const std::vector<double> &fg);
+
 // Pointwise nuclide: 1/v absorption plus single-level Breit-Wigner resonances
 // and constant potential scattering on a log grid; resonance parameters are
-// drawn from the OpenMC generator seeded with init_seed(seed, 0).
+// drawn from the OpenMC generator seeded with init_seed(seed, 0).  With
+// `temperatures`, one table per temperature T, every resonance broadened for
+// the Doppler width G_D = sqrt(4 k T E_r / awr): width sqrt(G^2 + G_D^2),
+// peak scaled to keep width x peak; without, one unbroadened table at 293.6 K.
 static NuclideData synthetic_nuclide(const InputParameters &p)
 {
   NuclideData n;
@@ -224,13 +232,51 @@ static NuclideData synthetic_nuclide(const InputParameters &p)
     uint64_t w = ((s >> ((s >> 59u) + 5u)) ^ s) * 12605985483714917081ull;
     return (double)((w >> 43u) ^ w) * 0x1p-64;
   };
-  std::vector<double> Er(nres), G(nres), pk(nres), fg(nres);
+  std::vector<double> Er(nres), G0(nres), pk0(nres), fg(nres);
   for (long long r = 0; r < nres; ++r) {
     Er[r] = std::exp(std::log(1.0) + prn() * (std::log(1e5) - std::log(1.0)));
-    G[r] = std::exp(std::log(1e-2) + prn() * (std::log(10.0) - std::log(1e-2)));
-    pk[r] = std::exp(std::log(10.0) + prn() * (std::log(1e4) - std::log(10.0)));
+    G0[r] = std::exp(std::log(1e-2) + prn() * (std::log(10.0) - std::log(1e-2)));
+    pk0[r] = std::exp(std::log(10.0) + prn() * (std::log(1e4) - std::log(10.0)));
     fg[r] = 0.2 + 0.6 * prn();
   }
+  std::vector<double> temps{0.0};
+  if (p.isParamValid("temperatures")) {
+    temps = p.getReals("temperatures");
+    if (temps.empty()) paramError(p, "temperatures", "give at least one temperature");
+    for (size_t k = 0; k < temps.size(); ++k) {
+      if (!(temps[k] > 0.0 && std::isfinite(temps[k])))
+        paramError(p, "temperatures", "temperatures must be positive (K)");
+      for (size_t j = 0; j < k; ++j)
+        if (temps[j] == temps[k]) paramError(p, "temperatures", "a temperature is given twice");
+    }
+  }
+  std::vector<NuclideData> tables;
+  for (double T : temps) {
+    std::vector<double> G = G0, pk = pk0;
+    if (T > 0.0)
+      for (long long r = 0; r < nres; ++r) {
+        const double gd2 = 4.0 * MCW_K_BOLTZMANN * T * Er[r] / n.awr;
+        const double ge = std::sqrt(G0[r] * G0[r] + gd2);
+        pk[r] = pk0[r] * G0[r] / ge;
+        G[r] = ge;
+      }
+    tables.push_back(resonance_table(n, np, e0, e1, sa0, sa_flat, sp, ff, Er, G, pk, fg));
+    if (T > 0.0) tables.back().temperature = T;
+  }
+  NuclideData out = std::move(tables[0]);
+  for (size_t k = 1; k < tables.size(); ++k) out.temps.push_back(std::move(tables[k]));
+  return out;
+}
+
+// One table of a synthetic nuclide: base's name, awr and fission data, the
+// given resonances.
+static NuclideData resonance_table(const NuclideData &base, long long np, double e0, double e1, double sa0,
+                                   double sa_flat, double sp, double ff, const std::vector<double> &Er,
+                                   const std::vector<double> &G, const std::vector<double> &pk,
+                                   const std::vector<double> &fg)
+{
+  NuclideData n = base;
+  const long long nres = (long long)Er.size();
   n.E.resize(np);
   n.t.resize(np);
   n.a.resize(np);
@@ -262,6 +308,7 @@ CollisionKernel::CollisionKernel(const InputParameters &p, Problem &fe) : _param
 {
   _verbose = p.getBool("verbose", false);
   const std::string T = p.getString("temperature");
+  _temperature = T;
   auto it = fe.aux_vars.find(T);
   if (it == fe.aux_vars.end()) paramError(p, "temperature", "no variable named '" + T + "'");
   // Check that the temperature variable is a constant monomial (CollisionKernel.C:52-57)
@@ -560,8 +607,21 @@ void OpenMCStudy::initialSetup()
   std::vector<mcw_nuclide> nd;
   for (auto &n : fe.nuclides)
     nd.push_back(mcw_nuclide{(int32_t)n.E.size(), n.awr, n.E.data(), n.t.data(), n.a.data(), n.s.data(),
-                             n.f.empty() ? nullptr : n.f.data(), {n.nu[0], n.nu[1]}, {n.watt[0], n.watt[1]}});
+                             n.f.empty() ? nullptr : n.f.data(), {n.nu[0], n.nu[1]}, {n.watt[0], n.watt[1]},
+                             n.temperature});
   check(_ctx, mcw_xs_set(_ctx, (int32_t)nd.size(), nd.data()), "mcw_xs_set");
+  for (size_t q = 0; q < fe.nuclides.size(); ++q)
+    for (auto &n : fe.nuclides[q].temps) {
+      mcw_nuclide t{(int32_t)n.E.size(), n.awr, n.E.data(), n.t.data(), n.a.data(), n.s.data(),
+                    n.f.empty() ? nullptr : n.f.data(), {n.nu[0], n.nu[1]}, {n.watt[0], n.watt[1]}, n.temperature};
+      check(_ctx, mcw_xs_add_temperature(_ctx, (int32_t)q, &t), "mcw_xs_add_temperature");
+    }
+  // CollisionKernel.C:109: sqrt(kT) from the element's temperature variable
+  if (!fe.elem_temperature.empty())
+    check(_ctx,
+          mcw_temperature_set(_ctx, (int64_t)fe.elem_temperature.size(), fe.elem_temperature.data(),
+                              fe.temperature_tolerance),
+          "mcw_temperature_set");
   std::vector<mcw_material> md;
   for (auto &mt : fe.materials)
     md.push_back(mcw_material{mt.id, (int32_t)mt.nuclides.size(), mt.nuclides.data(), mt.densities.data()});
@@ -630,6 +690,7 @@ Problem::Problem(std::unique_ptr<HitNode> root, const std::string &input) : _roo
         std::string order = p.getString("order", "FIRST");
         aux_vars[c->name] = order == "CONSTANT" ? 0 : (order == "FIRST" ? 1 : 2);
         aux_family[c->name] = p.getString("family", "LAGRANGE");
+        aux_initial[c->name] = p.getReal("initial_condition", 0.0);
       }
   HitNode *mc = R.child("MonteCarlo");
   if (!mc) mooseError("the input has no [MonteCarlo] block (settings, source, nuclides, materials)");
@@ -642,6 +703,12 @@ Problem::Problem(std::unique_ptr<HitNode> root, const std::string &input) : _roo
       if (p.getString("type") != "CollisionKernel") paramError(p, "type", "unknown RayKernel type");
       raykernels.push_back(std::make_unique<CollisionKernel>(p, *this));
     }
+  if (!raykernels.empty()) {
+    bool several = false;
+    for (auto &n : nuclides) several |= !n.temps.empty();
+    // one table per nuclide reads the same data at every temperature
+    if (several) elem_temperature = element_values(raykernels[0]->temperatureVariable());
+  }
   if (HitNode *rb = R.child("RayBCs"))
     for (auto &c : rb->children) {
       if (!active(*rb, *c)) continue;
@@ -732,6 +799,10 @@ void Problem::read_monte_carlo(const HitNode &mcb)
   else if (run_mode == "fixed source" || run_mode == "fixed_source") options.run_mode = MCW_RUN_FIXED_SOURCE;
   else paramError(p, "run_mode", "run_mode must be eigenvalue or fixed source");
   options.n_inactive = (int32_t)p.getInt("inactive", 0);
+  // OpenMC settings temperature_tolerance; by default the nearest table is
+  // read however far, as OpenMC does for the sqrt(kT) MaCaw sets per segment
+  temperature_tolerance = p.getReal("temperature_tolerance", INFINITY);
+  if (!(temperature_tolerance >= 0.0)) paramError(p, "temperature_tolerance", "the tolerance must be >= 0 K");
   if (options.n_inactive < 0) paramError(p, "inactive", "the number of inactive batches must not be negative");
   if (options.run_mode == MCW_RUN_EIGENVALUE && options.n_inactive >= n_batches)
     paramError(p, "inactive", "an eigenvalue run needs more batches than inactive batches");
@@ -794,6 +865,28 @@ void Problem::read_monte_carlo(const HitNode &mcb)
     source.energy = MCW_ENERGY_MONO;
     source.E0 = s.getReal("energy", 2e6);
   }
+}
+
+// Per-element values of a CONSTANT MONOMIAL variable: its initial_condition
+// (MOOSE's default 0), then each active [ICs] ConstantIC on it in input
+// order, on its `block`s or everywhere.
+std::vector<double> Problem::element_values(const std::string &var) const
+{
+  std::vector<double> v(mesh.n_elem(), aux_initial.count(var) ? aux_initial.at(var) : 0.0);
+  if (HitNode *ic = _root->child("ICs"))
+    for (auto &c : ic->children) {
+      if (!active(*ic, *c)) continue;
+      InputParameters p(c.get());
+      if (p.getString("variable") != var) continue;
+      if (p.getString("type") != "ConstantIC") paramError(p, "type", "only ConstantIC sets the temperature");
+      const double x = p.getReal("value");
+      std::set<int> blocks;
+      if (p.isParamValid("block"))
+        for (long long b : p.getInts("block")) blocks.insert((int)b);
+      for (int64_t e = 0; e < mesh.n_elem(); ++e)
+        if (blocks.empty() || blocks.count(mesh.subdomain[e])) v[e] = x;
+    }
+  return v;
 }
 
 std::map<int, int> Problem::blockMap() const
@@ -902,7 +995,7 @@ void Problem::dump(const std::string &path) const
   std::ofstream f(path, std::ios::binary);
   auto w64 = [&](int64_t v) { f.write((const char *)&v, 8); };
   auto wd = [&](const void *p, size_t n) { f.write((const char *)p, n); };
-  f.write("MCWD0002", 8);
+  f.write("MCWD0003", 8);
   w64(mesh.n_nodes());
   wd(mesh.xyz.data(), mesh.xyz.size() * 8);
   w64(mesh.n_elem());
@@ -927,8 +1020,7 @@ void Problem::dump(const std::string &path) const
     w64(e.first);
     w64(e.second);
   }
-  w64((int64_t)nuclides.size());
-  for (auto &n : nuclides) {
+  auto table = [&](const NuclideData &n) {
     wd(&n.awr, 8);
     w64((int64_t)n.E.size());
     wd(n.E.data(), n.E.size() * 8);
@@ -939,6 +1031,13 @@ void Problem::dump(const std::string &path) const
     if (!n.f.empty()) wd(n.f.data(), n.f.size() * 8);
     wd(n.nu, 16);
     wd(n.watt, 16);
+    wd(&n.temperature, 8);
+  };
+  w64((int64_t)nuclides.size());
+  for (auto &n : nuclides) {
+    table(n);
+    w64((int64_t)n.temps.size());
+    for (auto &t : n.temps) table(t);
   }
   w64((int64_t)materials.size());
   for (auto &m : materials) {
@@ -954,6 +1053,9 @@ void Problem::dump(const std::string &path) const
   w64((int64_t)options.master_seed);
   w64(options.run_mode);
   w64(options.n_inactive);
+  w64((int64_t)elem_temperature.size());
+  wd(elem_temperature.data(), elem_temperature.size() * 8);
+  wd(&temperature_tolerance, 8);
 }
 
 } // namespace mcwhost

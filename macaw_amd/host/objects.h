@@ -8,6 +8,7 @@
 // CollisionKernel::onSegment once per segment.
 #pragma once
 
+#include <cmath>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -69,6 +70,8 @@ struct NuclideData {
   std::vector<double> f;              // fission (part of absorption); empty: not fissionable
   double nu[2] = {0.0, 0.0};          // nu(E) = nu[0] + nu[1] E
   double watt[2] = {0.988e6, 2.249e-6}; // fission spectrum a [eV], b [1/eV]
+  double temperature = 293.6;         // K
+  std::vector<NuclideData> temps;     // tables at further temperatures
 };
 
 struct MaterialData {
@@ -85,11 +88,13 @@ public:
   CollisionKernel(const InputParameters &p, Problem &fe);
   void initialSetup();
   const std::map<int, int> &blockToMaterial() const { return _block_to_openmc_materials; }
+  const std::string &temperatureVariable() const { return _temperature; }
 
 private:
   InputParameters _params;
   Problem &_fe;
   std::map<int, int> _block_to_openmc_materials;
+  std::string _temperature;
   bool _verbose;
 };
 
@@ -195,6 +200,9 @@ public:
   std::string file_base;
   std::map<std::string, int> aux_vars;   // name -> order (0 = CONSTANT)
   std::map<std::string, std::string> aux_family;
+  std::map<std::string, double> aux_initial;  // AuxVariables initial_condition
+  std::vector<double> elem_temperature;       // K per element: the collision kernel's temperature variable
+  double temperature_tolerance = INFINITY;    // K; [MonteCarlo] temperature_tolerance
   std::vector<std::unique_ptr<OpenMCTally>> tallies;
   std::vector<std::unique_ptr<OpenMCTallyAux>> auxkernels;
   std::vector<std::unique_ptr<CollisionKernel>> raykernels;
@@ -208,6 +216,7 @@ private:
   std::unique_ptr<HitNode> _root;
   std::string _input;
   void read_monte_carlo(const HitNode &mc);
+  std::vector<double> element_values(const std::string &var) const;
   void write_outputs(const std::vector<std::vector<double>> &rows);
 };
 

@@ -710,9 +710,10 @@ def load_dump(path) -> Problem:
     """Problem written by `macaw-amd-opt --dump-problem` (the exact bytes the
     driver handed to the library), for replay on a CPU reference."""
     raw = open(path, "rb").read()
-    if raw[:8] not in (b"MCWD0001", b"MCWD0002"):
+    if raw[:8] not in (b"MCWD0001", b"MCWD0002", b"MCWD0003"):
         raise ValueError(f"{path}: not a problem dump")
-    v2 = raw[:8] == b"MCWD0002"
+    v2 = raw[:8] in (b"MCWD0002", b"MCWD0003")
+    v3 = raw[:8] == b"MCWD0003"
     pos = [8]
 
     def i64():
@@ -743,8 +744,7 @@ def load_dump(path) -> Problem:
     for _ in range(i64()):
         b = i64()
         bm[b] = i64()
-    nucs = []
-    for _ in range(i64()):
+    def table():
         awr = float(arr("<f8", 1)[0])
         npt = i64()
         E, t, a, s = (arr("<f8", npt) for _ in range(4))
@@ -754,7 +754,17 @@ def load_dump(path) -> Problem:
             nu, watt = arr("<f8", 2), arr("<f8", 2)
             n = dataclasses.replace(n, fission=fis, nu=(float(nu[0]), float(nu[1])),
                                     watt=(float(watt[0]), float(watt[1])))
-        nucs.append(n)
+        if v3:
+            n = dataclasses.replace(n, temperature=float(arr("<f8", 1)[0]))
+        return n
+
+    nucs, temps = [], {}
+    for q in range(i64()):
+        nucs.append(table())
+        if v3:
+            more = [table() for _ in range(i64())]
+            if more:
+                temps[q] = more
     mats = []
     for _ in range(i64()):
         mid = i64()
@@ -764,10 +774,16 @@ def load_dump(path) -> Problem:
     pos[0] += abi.C.sizeof(abi.Source)
     particles, batches, roundtrip, seed = i64(), i64(), i64(), i64()
     run_mode, n_inactive = (i64(), i64()) if v2 else (abi.RUN_FIXED_SOURCE, 0)
+    elem_T, tol = None, 10.0
+    if v3:
+        k = i64()
+        elem_T = arr("<f8", k) if k else None
+        tol = float(arr("<f8", 1)[0])
     mesh = Mesh(xyz=np.ascontiguousarray(xyz), elem_type=etype, conn_offsets=off, conn=conn, subdomain=sub,
                 face_boundary=fb if nfb else None)
     p = Problem(name=os.path.basename(path), mesh=mesh, nuclides=nucs, materials=mats, block_material=bm,
-                bcs=bcs, source=src, tallies=[], n_particles=particles, n_batches=batches)
+                bcs=bcs, source=src, tallies=[], n_particles=particles, n_batches=batches,
+                nuclide_temps=temps, elem_temperature=elem_T, temperature_tolerance=tol)
     p.seed_roundtrip = bool(roundtrip)
     p.master_seed = seed
     p.run_mode = run_mode

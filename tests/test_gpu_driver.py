@@ -125,6 +125,23 @@ def test_eigen_driver_matches_oracle(tmp_path, oracle_lib):
     assert power.sum() > 0 and close(power, o.tally(1))
 
 
+def test_temperature_driver_matches_oracle(tmp_path, oracle_lib):
+    """Element temperatures from the temperature variable: per-batch keff and
+    the fission-power aux values equal the oracle's."""
+    base, p = run_driver("openmc_study_temperature.i", tmp_path, "MonteCarlo/particles=400")
+    assert p.elem_temperature is not None and len(p.nuclide_temps) == 2
+    p.tallies = [P.TallySpec(id=1, estimator=abi.EST_TRACKLENGTH, filters=[{"type": abi.FILTER_CELL}],
+                             scores=[abi.SCORE_FISSION])]
+    o = oracle_lib.Oracle(p, eigenvalue=True, n_inactive=p.n_inactive)
+    kg = []
+    for b in (1, 2, 3):
+        o.run_batch(b, p.n_particles, threads=4)
+        kg.append(o.keff()["k_generation"])
+    assert np.array_equal(read_col(base + ".csv", "k_generation"), np.array(kg))
+    power = read_col(base + "_power.csv", "power")
+    assert power.sum() > 0 and close(power, o.tally(1))
+
+
 def test_eigen_driver_ray_output_matches_gold_seeds(tmp_path):
     """The reference study's own settings -- eigenvalue, 30 particles, 3
     batches, 1 inactive -- with [Outputs/rays] type = RayTracingExodus: rays

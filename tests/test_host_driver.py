@@ -26,7 +26,8 @@ def run(args, cwd=None):
                           timeout=120)
 
 
-@pytest.mark.parametrize("name", ["c1_hex8_reflective.i", "openmc_tally.i", "openmc_study_eigen.i"])
+@pytest.mark.parametrize("name", ["c1_hex8_reflective.i", "openmc_tally.i", "openmc_study_eigen.i",
+                                  "openmc_study_temperature.i"])
 def test_inputs_validate(name):
     r = run(["-i", os.path.join(INPUTS, name), "--check-input"])
     assert r.returncode == 0, r.stderr
@@ -53,6 +54,10 @@ ERRORS = [
     ("openmc_study_eigen.i", "MonteCarlo/Nuclides/U235/fission_fraction=1.5", "the fission fraction must be in [0, 1]"),
     ("openmc_study_eigen.i", "MonteCarlo/Nuclides/U235/nu=1 2 3", "nu takes one or two coefficients"),
     ("openmc_study_eigen.i", "MonteCarlo/source/upper_right=0 0", "a box corner needs 3 coordinates"),
+    ("openmc_study_temperature.i", "MonteCarlo/Nuclides/U235/temperatures=600 -1", "temperatures must be positive"),
+    ("openmc_study_temperature.i", "MonteCarlo/Nuclides/O16/temperatures=600 600", "a temperature is given twice"),
+    ("openmc_study_temperature.i", "ICs/centre_temperature/type=FunctionIC", "only ConstantIC sets the temperature"),
+    ("openmc_study_temperature.i", "MonteCarlo/temperature_tolerance=-1", "the tolerance must be >= 0 K"),
 ]
 
 
@@ -137,3 +142,33 @@ def test_dump_replays_on_the_oracle(tmp_path, oracle_lib):
     st = o.run_batch(1, 300, threads=2)
     assert st["lost"] == 0 and st["leaks"] == 0 and st["absorptions"] + st["cutoffs"] == 300
     assert o.tally(1).sum() > 0
+
+
+def test_temperature_dump_replays_on_the_oracle(tmp_path, oracle_lib):
+    """Element temperatures from the temperature variable (initial condition,
+    then a block ConstantIC) and the nuclides' tables reach the library; on
+    the oracle the run equals the same problem with each element's nearest
+    tables made nuclides of their own."""
+    from test_oracle_temperature import assert_same_histories, expand
+    d = tmp_path / "temp.dump"
+    r = run(["-i", os.path.join(INPUTS, "openmc_study_temperature.i"), "MonteCarlo/particles=300",
+             "--dump-problem", str(d)])
+    assert r.returncode == 0, r.stderr
+    p = P.load_dump(str(d))
+    centre = p.mesh.subdomain == 1
+    assert centre.sum() == 1 and np.all(p.elem_temperature[centre] == 1150.0)
+    assert np.all(p.elem_temperature[~centre] == 600.0) and p.temperature_tolerance == np.inf
+    assert [n.temperature for n in p.nuclides] == [293.6] * 3
+    assert {q: [t.temperature for t in ts] for q, ts in p.nuclide_temps.items()} == {
+        0: [600.0, 900.0, 1200.0], 1: [600.0, 900.0, 1200.0]}
+    assert p.nuclide_temps[0][2].fission is not None and p.nuclide_temps[0][2].nu == p.nuclides[0].nu
+    q, n_nuc, _ = expand(p)
+    assert n_nuc == 4  # U235 at 600 K (fuel only), O16 at 600 K and 1200 K, H1
+    runs = []
+    for prob in (p, q):
+        o = oracle_lib.Oracle(prob, eigenvalue=True, n_inactive=p.n_inactive)
+        for b in (1, 2):
+            o.run_batch(b, 300, threads=1, record=True)
+        runs.append((o.keff(), o.histories(300)))
+    assert_same_histories(runs[0][1], runs[1][1])
+    assert runs[0][0]["keff"] == runs[1][0]["keff"]
