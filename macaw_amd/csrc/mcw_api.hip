@@ -17,6 +17,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/macaw_amd.h"
@@ -351,60 +352,73 @@ void prepare(mcw_ctx *c, bool need_source = true)
     std::stable_sort(t_order[q].begin(), t_order[q].end(),
                      [&](int32_t a, int32_t b) { return table(q, a).temperature < table(q, b).temperature; });
   }
+  std::vector<std::vector<double>> t_kT(n_nuc); // k T of each table, ascending
+  for (int32_t q = 0; q < n_nuc; ++q)
+    for (int32_t t : t_order[q]) t_kT[q].push_back(K_BOLTZMANN * table(q, t).temperature);
   std::vector<std::pair<int32_t, int32_t>> inst; // (nuclide, table)
-  std::map<std::pair<int32_t, int32_t>, int32_t> inst_id;
+  std::vector<std::vector<int32_t>> inst_of(n_nuc); // [nuclide][table] -> instance, -1 none yet
   for (int32_t q = 0; q < n_nuc; ++q) {
-    inst_id[{q, 0}] = q;
+    inst_of[q].assign(t_order[q].size(), -1);
+    inst_of[q][0] = q;
     inst.push_back({q, 0});
   }
   const bool temps = !c->elem_T.empty();
   require(!temps || (int64_t)c->elem_T.size() == m.n_elem,
           "mcw_temperature_set: one temperature per element of the mesh");
-  std::vector<int32_t> eff(m.n_elem, -1); // caller element -> device material
-  std::map<std::vector<int32_t>, int32_t> eff_id;
+  std::vector<int32_t> eff(m.n_elem, -1);    // caller element -> device material
   std::vector<std::vector<int32_t>> eff_key; // material, then one instance per nuclide of it
-  for (int64_t e = 0; e < m.n_elem; ++e) {
-    const int32_t mi = emat[e];
-    if (mi < 0) continue;
-    std::vector<int32_t> key{mi};
-    double sq = 0.0, kT = 0.0;
-    if (temps) {
-      sq = std::sqrt(K_BOLTZMANN * c->elem_T[e]);
-      kT = sq * sq;
+  if (!temps) {
+    // every element reads its nuclides' first tables: device material = material
+    for (size_t mi = 0; mi < c->mats.size(); ++mi) {
+      std::vector<int32_t> key{(int32_t)mi};
+      key.insert(key.end(), c->mats[mi].nuc.begin(), c->mats[mi].nuc.end());
+      eff_key.push_back(std::move(key));
     }
-    for (int32_t q : c->mats[mi].nuc) {
-      int32_t best = 0;
-      if (temps) {
+    eff = emat;
+  } else {
+    std::unordered_map<uint64_t, std::vector<int32_t>> by_hash; // key hash -> device materials
+    std::vector<int32_t> key;
+    for (int64_t e = 0; e < m.n_elem; ++e) {
+      const int32_t mi = emat[e];
+      if (mi < 0) continue;
+      const double sq = std::sqrt(K_BOLTZMANN * c->elem_T[e]);
+      const double kT = sq * sq;
+      key.clear();
+      key.push_back(mi);
+      uint64_t h = 0xcbf29ce484222325ull ^ (uint64_t)mi;
+      for (int32_t q : c->mats[mi].nuc) {
+        const std::vector<double> &a = t_kT[q];
+        size_t best = 0;
         double dmin = INFINITY;
-        for (int32_t t : t_order[q]) {
-          const double d = std::fabs(K_BOLTZMANN * table(q, t).temperature - kT);
+        for (size_t k = 0; k < a.size(); ++k) {
+          const double d = std::fabs(a[k] - kT);
           if (d < dmin) {
             dmin = d;
-            best = t;
+            best = k;
           }
         }
-        require(std::fabs(table(q, best).temperature - c->elem_T[e]) <= c->T_tol,
-                "no cross sections for nuclide " + std::to_string(q) + " at or near " +
-                    std::to_string(c->elem_T[e]) + " K (element " + std::to_string(e) + ")");
+        const int32_t t = t_order[q][best];
+        if (!(std::fabs(table(q, t).temperature - c->elem_T[e]) <= c->T_tol))
+          throw McwError{"no cross sections for nuclide " + std::to_string(q) + " at or near " +
+               std::to_string(c->elem_T[e]) + " K (element " + std::to_string(e) + ")"};
+        int32_t &id = inst_of[q][t];
+        if (id < 0) {
+          id = (int32_t)inst.size();
+          inst.push_back({q, t});
+        }
+        key.push_back(id);
+        h = (h ^ (uint64_t)id) * 0x100000001b3ull;
       }
-      auto it = inst_id.find({q, best});
-      int32_t id;
-      if (it == inst_id.end()) {
-        id = (int32_t)inst.size();
-        inst_id[{q, best}] = id;
-        inst.push_back({q, best});
-      } else {
-        id = it->second;
+      std::vector<int32_t> &cand = by_hash[h];
+      int32_t found = -1;
+      for (int32_t k : cand)
+        if (eff_key[k] == key) found = k;
+      if (found < 0) {
+        found = (int32_t)eff_key.size();
+        eff_key.push_back(key);
+        cand.push_back(found);
       }
-      key.push_back(id);
-    }
-    auto f = eff_id.find(key);
-    if (f == eff_id.end()) {
-      eff[e] = (int32_t)eff_key.size();
-      eff_id[key] = eff[e];
-      eff_key.push_back(key);
-    } else {
-      eff[e] = f->second;
+      eff[e] = found;
     }
   }
   c->n_eff_mats = (int32_t)eff_key.size();
