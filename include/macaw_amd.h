@@ -55,7 +55,7 @@ enum { MCW_RESULT_VALUE = 0, MCW_RESULT_SUM = 1, MCW_RESULT_SUM_SQ = 2 };
 /* Tally filters (OpenMCTally.C:125-191) and scores. */
 enum { MCW_FILTER_UNIVERSE = 1, MCW_FILTER_CELL = 2, MCW_FILTER_ENERGY = 3, MCW_FILTER_PARTICLE = 4 };
 enum { MCW_SCORE_FLUX = 0, MCW_SCORE_TOTAL = 1, MCW_SCORE_ABSORPTION = 2, MCW_SCORE_SCATTER = 3,
-       MCW_SCORE_FISSION = 4 };
+       MCW_SCORE_FISSION = 4, MCW_SCORE_KAPPA_FISSION = 5 /* kappa sigma_f: fission energy [eV] */ };
 
 /* Source distributions. */
 enum { MCW_SPACE_POINT = 0, MCW_SPACE_BOX = 1, MCW_SPACE_MESH_VOLUME = 2 };
@@ -138,6 +138,7 @@ typedef struct {
   double nu[2];
   double watt[2];
   double temperature;    /* K, the temperature of this table (0: 293.6) */
+  double kappa;          /* energy released per fission [eV], for kappa-fission scores */
 } mcw_nuclide;
 
 /* A material: id (as in materials.xml) and atom densities [1/(b cm)]. */
@@ -152,7 +153,9 @@ typedef struct {
   int32_t space;  /* MCW_SPACE_* */
   int32_t angle;  /* MCW_ANGLE_* */
   int32_t energy; /* MCW_ENERGY_* */
-  int32_t pad;
+  int32_t only_fissionable; /* BOX: keep only sites in fissionable materials
+                               (OpenMC <space type="fission">); a zero-width
+                               box dimension is allowed */
   double point[3];
   double box_lo[3];
   double box_hi[3];

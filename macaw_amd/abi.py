@@ -10,7 +10,7 @@ BC_VACUUM, BC_REFLECT = 0, 1
 EST_ANALOG, EST_TRACKLENGTH, EST_COLLISION = 0, 1, 2
 RESULT_VALUE, RESULT_SUM, RESULT_SUM_SQ = 0, 1, 2
 FILTER_UNIVERSE, FILTER_CELL, FILTER_ENERGY, FILTER_PARTICLE = 1, 2, 3, 4
-SCORE_FLUX, SCORE_TOTAL, SCORE_ABSORPTION, SCORE_SCATTER, SCORE_FISSION = 0, 1, 2, 3, 4
+SCORE_FLUX, SCORE_TOTAL, SCORE_ABSORPTION, SCORE_SCATTER, SCORE_FISSION, SCORE_KAPPA_FISSION = 0, 1, 2, 3, 4, 5
 SPACE_POINT, SPACE_BOX, SPACE_MESH_VOLUME = 0, 1, 2
 ANGLE_ISOTROPIC, ANGLE_LINEAR_MU = 0, 1
 ENERGY_MONO, ENERGY_WATT = 0, 1
@@ -21,7 +21,7 @@ RUN_FIXED_SOURCE, RUN_EIGENVALUE = 0, 1
 K_BOLTZMANN = 8.617333262e-5  # eV/K, as OpenMC
 
 SCORE_NAMES = {"flux": SCORE_FLUX, "total": SCORE_TOTAL, "absorption": SCORE_ABSORPTION,
-               "scatter": SCORE_SCATTER, "fission": SCORE_FISSION}
+               "scatter": SCORE_SCATTER, "fission": SCORE_FISSION, "kappa-fission": SCORE_KAPPA_FISSION}
 FILTER_NAMES = {"universe": FILTER_UNIVERSE, "cell": FILTER_CELL, "energy": FILTER_ENERGY,
                 "particle": FILTER_PARTICLE}
 ESTIMATOR_NAMES = {"ANALOG": EST_ANALOG, "TRACKLENGTH": EST_TRACKLENGTH,
@@ -45,7 +45,7 @@ class Options(C.Structure):
 class Nuclide(C.Structure):
     _fields_ = [("n_points", C.c_int32), ("awr", C.c_double), ("energy", _dp), ("total", _dp),
                 ("absorption", _dp), ("scatter", _dp), ("fission", _dp), ("nu", C.c_double * 2),
-                ("watt", C.c_double * 2), ("temperature", C.c_double)]
+                ("watt", C.c_double * 2), ("temperature", C.c_double), ("kappa", C.c_double)]
 
 
 class Material(C.Structure):
@@ -55,7 +55,7 @@ class Material(C.Structure):
 
 class Source(C.Structure):
     _fields_ = [("space", C.c_int32), ("angle", C.c_int32), ("energy", C.c_int32),
-                ("pad", C.c_int32), ("point", C.c_double * 3), ("box_lo", C.c_double * 3),
+                ("only_fissionable", C.c_int32), ("point", C.c_double * 3), ("box_lo", C.c_double * 3),
                 ("box_hi", C.c_double * 3), ("axis", C.c_double * 3), ("beta", C.c_double),
                 ("E0", C.c_double), ("watt_a", C.c_double), ("watt_b", C.c_double)]
 

@@ -125,6 +125,7 @@ struct HostNuclide {
   std::vector<double> t, a, s, f;
   double nu[2] = {0.0, 0.0}, watt[2] = {0.0, 0.0};
   double temperature = 293.6; // K
+  double kappa = 0.0;         // eV per fission
 };
 
 constexpr double K_BOLTZMANN = MCW_K_BOLTZMANN;
@@ -179,7 +180,7 @@ struct mcw_ctx {
   // device problem
   DevBuf plane, erec, mtet, mhex, split, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf cell_start, cell_elem;
-  DevBuf grid_off, npts, rec, scatter, fis, fissionable, watt_t, nuc_of, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
+  DevBuf grid_off, npts, rec, scatter, fis, fissionable, watt_t, nuc_of, kappa, mat_fis, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
   Counters *h_counters = nullptr; // pinned
   // source bank, live/collision banks of the event-based path, records
@@ -251,6 +252,8 @@ void require(bool ok, const std::string &m)
 // A source's numbers: finite positions, positive finite energy parameters.
 void check_source_values(const mcw_source &src)
 {
+  require(src.only_fissionable == 0 || (src.only_fissionable == 1 && src.space == MCW_SPACE_BOX),
+          "only_fissionable is 0, or 1 for a box source");
   for (int k = 0; k < 3; ++k) {
     if (src.space == MCW_SPACE_POINT) require(std::isfinite(src.point[k]), "point source coordinates must be finite");
     if (src.space == MCW_SPACE_BOX)
@@ -557,6 +560,9 @@ void prepare(mcw_ctx *c, bool need_source = true)
   }
   c->fis.upload(fpar, s);
   c->fissionable.upload(fable, s);
+  std::vector<double> kap; // per instance: the nuclide's first table's kappa
+  for (auto &iq : inst) kap.push_back(c->nucs[iq.first].kappa);
+  c->kappa.upload(kap, s);
   // Watt first-attempt test: Ew <= (sqrt(w) + a sqrt(b) / 2)^2 and
   // w <= -a ln(r1 r2), so r1 r2 > exp(-w* / a) with w* = (sqrt(e_max) -
   // a sqrt(b) / 2)^2 keeps Ew below e_max; 1e-6 of w* is margin for rounding
@@ -609,6 +615,13 @@ void prepare(mcw_ctx *c, bool need_source = true)
   c->mat_off.upload(moff, s);
   c->mat_nuc.upload(mnuc, s);
   c->mat_dens.upload(mden, s);
+  std::vector<int32_t> mfis; // device material has a fissionable nuclide (OpenMC Material::fissionable_)
+  for (size_t d = 0; d + 1 < moff.size(); ++d) {
+    int32_t f = 0;
+    for (int32_t q = moff[d]; q < moff[d + 1]; ++q) f |= fable[mnuc[q]];
+    mfis.push_back(f);
+  }
+  c->mat_fis.upload(mfis, s);
 
   // point locator for point and box sources (and for the request server)
   build_locator(m);
@@ -681,11 +694,14 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P.xs.n_nuc = (int32_t)inst.size();
   P.xs.n_mat = c->n_eff_mats;
   P.xs.nuc_of = c->nuc_of.as<int32_t>();
+  P.xs.kappa = c->kappa.as<double>();
+  P.xs.mat_fissionable = c->mat_fis.as<int32_t>();
   P.xs.e_min = c->e_min;
   P.xs.e_max = c->e_max;
   P.src.space = c->src.space;
   P.src.angle = c->src.angle;
   P.src.energy = c->src.energy;
+  P.src.only_fissionable = c->src.only_fissionable;
   for (int k = 0; k < 3; ++k) {
     P.src.point[k] = c->src.point[k];
     P.src.box_lo[k] = c->src.box_lo[k];
@@ -1275,8 +1291,10 @@ static HostNuclide read_nuclide(int i, const mcw_nuclide &q)
   require(q.awr > 0.0, "nuclide " + std::to_string(i) + ": awr must be positive");
   require(std::isfinite(q.temperature) && q.temperature >= 0.0,
           "nuclide " + std::to_string(i) + ": temperature must be finite and non-negative");
+  require(std::isfinite(q.kappa) && q.kappa >= 0.0, "nuclide " + std::to_string(i) + ": kappa must be >= 0");
   HostNuclide h;
   h.awr = q.awr;
+  h.kappa = q.kappa;
   if (q.temperature > 0.0) h.temperature = q.temperature;
   h.E.assign(q.energy, q.energy + q.n_points);
   require(h.E[0] > 0.0, "nuclide " + std::to_string(i) + ": energies must be positive");
@@ -1327,6 +1345,7 @@ int mcw_xs_add_temperature(mcw_ctx *c, int32_t nuclide, const mcw_nuclide *table
               "nuclide " + std::to_string(nuclide) + ": a table at this temperature is loaded");
     // one set of reaction products per nuclide
     h.awr = b.awr;
+    h.kappa = b.kappa;
     for (int k = 0; k < 2; ++k) {
       h.nu[k] = b.nu[k];
       h.watt[k] = b.watt[k];
@@ -1418,7 +1437,7 @@ int mcw_tally_add(mcw_ctx *c, int32_t id, int32_t estimator, int32_t n_filters, 
       t->fedges.push_back(edges);
     }
     for (int k = 0; k < n_scores; ++k)
-      require(scores[k] >= MCW_SCORE_FLUX && scores[k] <= MCW_SCORE_FISSION, "unknown score");
+      require(scores[k] >= MCW_SCORE_FLUX && scores[k] <= MCW_SCORE_KAPPA_FISSION, "unknown score");
     t->scores.assign(scores, scores + n_scores);
     if (n_nuclides > 0) t->nuclides.assign(nuclides, nuclides + n_nuclides);
     else t->nuclides.push_back(-1);
@@ -1983,6 +2002,7 @@ bool launch_round(mcw_server *s, int mi, RoundSlot &R)
     r.src.space = src.space;
     r.src.angle = src.angle;
     r.src.energy = src.energy;
+    r.src.only_fissionable = src.only_fissionable;
     for (int k = 0; k < 3; ++k) {
       r.src.point[k] = src.point[k];
       r.src.box_lo[k] = src.box_lo[k];

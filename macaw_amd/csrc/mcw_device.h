@@ -349,22 +349,24 @@ __device__ __forceinline__ double macro_total_nf(const XSView &X, int mat, int32
 }
 
 struct Macro {
-  double t, a, s, f;
+  double t, a, s, f, kf; // kf: kappa-fission, sum of N kappa sigma_f
 };
 
 __device__ __forceinline__ void macro_all(const XSView &X, int mat, int32_t k, double E, Macro &M)
 {
-  M.t = M.a = M.s = M.f = 0.0;
+  M.t = M.a = M.s = M.f = M.kf = 0.0;
   if (mat < 0) return;
   const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
   for (int32_t q = b; q < e; ++q) {
     Micro u;
-    micro_at(X, X.mat_nuc[q], k, E, u);
+    const int32_t n = X.mat_nuc[q];
+    micro_at(X, n, k, E, u);
     double dn = X.mat_dens[q];
     M.t = M.t + dn * u.t;
     M.a = M.a + dn * u.a;
     M.s = M.s + dn * u.s;
     M.f = M.f + dn * u.f;
+    M.kf = M.kf + dn * (X.kappa[n] * u.f);
   }
 }
 
@@ -514,7 +516,7 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
   const int nsc = T.n_scores;
   for (int ni = 0; ni < T.n_nuclides; ++ni) {
     const int nuc = T.nuclides[ni];
-    Macro M = {0.0, 0.0, 0.0, 0.0};
+    Macro M = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (T.needs_xs) {
       if (nuc < 0) {
         macro_all(X, mat, k, E, M);
@@ -528,6 +530,7 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
             M.a = dn * mu.a;
             M.s = dn * mu.s;
             M.f = dn * mu.f;
+            M.kf = dn * (X.kappa[X.mat_nuc[q]] * mu.f);
           }
       }
     }
@@ -538,7 +541,8 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
         case 1: m = M.t; break;
         case 2: m = M.a; break;
         case 3: m = M.s; break;
-        default: m = M.f; break;
+        case 4: m = M.f; break;
+        default: m = M.kf; break;
       }
       atomic_add_dev(row + ni * nsc + si, wfl * m);
     }

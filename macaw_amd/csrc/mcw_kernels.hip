@@ -248,6 +248,11 @@ __device__ void sample_site(const Params &P, const SourceView &S, uint64_t maste
 #pragma unroll
       for (int k = 0; k < 3; ++k) p[k] = S.box_lo[k] + prn(ss) * (S.box_hi[k] - S.box_lo[k]);
       e = locate_point_d(M, p, 1e-12);
+      // OpenMC only_fissionable: a site outside a fissionable material is rejected too
+      if (e >= 0 && S.only_fissionable) {
+        const int32_t mt = M.erec[2 * (int64_t)e + 1].z;
+        if (mt < 0 || !P.xs.mat_fissionable[mt]) e = -1;
+      }
     }
     elem = e; // internal id, -1 when the box misses the mesh
   } else {

@@ -100,6 +100,8 @@ struct XSView {
                              // below e_max (-a ln(r1 r2) < the w that reaches e_max)
   const double *awr;
   const int32_t *nuc_of;     // per nuclide instance (one temperature's tables): the caller's nuclide
+  const double *kappa;       // per nuclide instance: energy released per fission [eV]
+  const int32_t *mat_fissionable; // per device material: 1 when a nuclide of it is fissionable
   // unionized grid: every nuclide's points merged, sorted, unique
   const double *U;
   const int32_t *uidx;       // n_union x n_nuc: largest j with E_n[j] <= U[k], -1 if none
@@ -118,7 +120,7 @@ struct XSView {
 constexpr int MAX_HASH = 8190; // LDS table of MAX_HASH + 1 ints: five 256-thread workgroups fit in 160 KB
 
 struct SourceView {
-  int32_t space, angle, energy, pad;
+  int32_t space, angle, energy, only_fissionable;
   double point[3];
   double box_lo[3], box_hi[3];
   double axis[3];

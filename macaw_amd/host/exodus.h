@@ -20,6 +20,30 @@ namespace mcwhost {
 // OpenMC's seed of particle `id`'s stream (0 tracking, 1 source, 2 URR).
 uint64_t init_seed(uint64_t id, uint64_t stream, uint64_t master_seed);
 
+// An Exodus II mesh as stored: coordinates (z = 0 in 2D), element blocks with
+// 0-based connectivity, side sets of (0-based element, 1-based Exodus side).
+struct ExodusMesh {
+  struct Block {
+    int64_t id = 0;
+    std::string name, elem_type;
+    int nodes_per_elem = 0;
+    std::vector<int64_t> conn;
+  };
+  struct SideSet {
+    int64_t id = 0;
+    std::string name;
+    std::vector<int64_t> elem, side;
+  };
+  int dim = 3;
+  std::vector<double> xyz; // 3 per node
+  std::vector<Block> blocks;
+  std::vector<SideSet> sidesets;
+};
+
+// Read an Exodus II file (NetCDF classic, 64-bit offset or CDF-5).  Throws
+// std::runtime_error with the reason when it cannot.
+ExodusMesh read_exodus(const std::string &path);
+
 // Write the visit records of batch `batch` (ordered by history, visit) to
 // `path`.  Ray ids are (batch - 1) * n_particles + history.  Throws
 // std::runtime_error when the file cannot be written.

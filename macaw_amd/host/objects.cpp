@@ -171,7 +171,158 @@ static void subdomain_bbox(const InputParameters &p, MeshData &m)
   }
 }
 
-MeshData build_mesh(const HitNode &mb)
+// Exodus side (1-based) -> libMesh side, per element type.  HEX8: Exodus
+// faces 1-4 are libMesh sides 1-4, face 5 (nodes 1,4,3,2) is side 0 and face
+// 6 (5,6,7,8) side 5.  TET4: faces 1-3 are sides 1-3, face 4 (1,3,2) is side
+// 0.  QUAD4: edge k joins nodes k, k+1 (edge k-1 here).
+static int exodus_side(uint8_t type, int64_t s)
+{
+  if (type == MCW_HEX8) {
+    static const int H[7] = {-1, 1, 2, 3, 4, 0, 5};
+    return s >= 1 && s <= 6 ? H[s] : -1;
+  }
+  if (type == MCW_TET4) return s >= 1 && s <= 4 ? (int)(s % 4) : -1;
+  return s >= 1 && s <= 4 ? (int)(s - 1) : -1; // QUAD4
+}
+
+static int n_sides(uint8_t type) { return type == MCW_HEX8 ? 6 : 4; }
+
+// MOOSE FileMeshGenerator: an Exodus II mesh of TET4 and HEX8 blocks, or a
+// 2D mesh of QUAD4 blocks for an extruder.  Elements are numbered block by
+// block as stored; a block's id is its subdomain.  A side in several side
+// sets keeps the smallest id.
+static void file_mesh(const InputParameters &p, MeshData &m, const std::string &input_dir)
+{
+  std::string file = p.getString("file");
+  if (!file.empty() && file[0] != '/' && !input_dir.empty()) file = input_dir + "/" + file;
+  ExodusMesh X;
+  try {
+    X = read_exodus(file);
+  } catch (const std::exception &e) {
+    paramError(p, "file", e.what());
+  }
+  m = MeshData();
+  m.dim = X.dim;
+  m.xyz = X.xyz;
+  std::vector<int64_t> slot_off{0};
+  for (auto &B : X.blocks) {
+    std::string t = B.elem_type;
+    for (auto &ch : t) ch = (char)toupper(ch);
+    uint8_t type;
+    int need;
+    if (X.dim == 3 && (t == "TETRA" || t == "TETRA4" || t == "TET4" || t == "TET")) {
+      type = MCW_TET4;
+      need = 4;
+    } else if (X.dim == 3 && (t == "HEX" || t == "HEX8")) {
+      type = MCW_HEX8;
+      need = 8;
+    } else if (X.dim == 2 && (t == "QUAD" || t == "QUAD4" || t == "SHELL4")) {
+      type = MESH_QUAD4;
+      need = 4;
+    } else {
+      paramError(p, "file", "block " + std::to_string(B.id) + ": element type '" + B.elem_type + "' in a " +
+                                std::to_string(X.dim) + "D mesh is not supported (TET4, HEX8; QUAD4 to extrude)");
+    }
+    if (B.nodes_per_elem != need)
+      paramError(p, "file", "block " + std::to_string(B.id) + ": " + std::to_string(B.nodes_per_elem) +
+                                " nodes per element for " + B.elem_type);
+    const int64_t ne = (int64_t)B.conn.size() / need;
+    for (int64_t e = 0; e < ne; ++e) {
+      m.type.push_back(type);
+      for (int q = 0; q < need; ++q) m.conn.push_back((int32_t)B.conn[(size_t)(e * need + q)]);
+      m.conn_off.push_back((int64_t)m.conn.size());
+      m.subdomain.push_back((int32_t)B.id);
+      slot_off.push_back(slot_off.back() + n_sides(type));
+    }
+  }
+  if (m.type.empty()) paramError(p, "file", "the mesh has no elements");
+  m.face_boundary.assign((size_t)slot_off.back(), -1);
+  for (auto &S : X.sidesets) {
+    for (size_t k = 0; k < S.elem.size(); ++k) {
+      const int64_t e = S.elem[k];
+      if (e < 0 || e >= m.n_elem())
+        paramError(p, "file", "side set " + std::to_string(S.id) + " names an element outside the mesh");
+      const int side = exodus_side(m.type[e], S.side[k]);
+      if (side < 0) paramError(p, "file", "side set " + std::to_string(S.id) + " names an unknown side");
+      int32_t &b = m.face_boundary[(size_t)(slot_off[e] + side)];
+      if (b < 0 || S.id < b) b = (int32_t)S.id;
+    }
+    if (!S.name.empty()) m.boundary_names[S.name] = (int32_t)S.id;
+  }
+}
+
+// MOOSE TransformGenerator: TRANSLATE or SCALE by vector_value.
+static void transform_mesh(const InputParameters &p, MeshData &m)
+{
+  const std::string t = p.getString("transform");
+  const std::vector<double> v = p.getReals("vector_value");
+  if (v.size() != 3) paramError(p, "vector_value", "vector_value needs 3 components");
+  for (int64_t i = 0; i < m.n_nodes(); ++i)
+    for (int d = 0; d < 3; ++d) {
+      double &x = m.xyz[3 * i + d];
+      if (t == "TRANSLATE") x = x + v[d];
+      else if (t == "SCALE") x = x * v[d];
+      else paramError(p, "transform", "transform must be TRANSLATE or SCALE");
+    }
+}
+
+// MOOSE FancyExtruderGenerator (AdvancedExtruderGenerator): a QUAD4 mesh swept
+// along `direction` through elevation zones of `heights`, zone i in
+// num_layers[i] equal layers, into HEX8.  Layer j's nodes are the 2D nodes
+// moved by the height below layer j; elements are numbered layer by layer.
+// An edge's boundary ids carry to the side it sweeps; the first layer's
+// bottoms get bottom_boundary and the last layer's tops top_boundary.  A quad
+// wound clockwise about the direction is taken in reverse node order, so the
+// hex has positive volume.
+static void extrude_mesh(const InputParameters &p, MeshData &m)
+{
+  if (m.dim != 2) paramError(p, "input", "the extruder needs a 2D (QUAD4) mesh");
+  const std::vector<double> h = p.getReals("heights");
+  const std::vector<long long> nl = p.getInts("num_layers");
+  if (h.empty() || h.size() != nl.size()) paramError(p, "num_layers", "one number of layers per height");
+  for (size_t i = 0; i < h.size(); ++i)
+    if (!(h[i] > 0.0) || nl[i] < 1) paramError(p, "heights", "heights must be positive, with at least one layer each");
+  std::vector<double> dir = p.isParamValid("direction") ? p.getReals("direction") : std::vector<double>{0, 0, 1};
+  if (dir.size() != 3) paramError(p, "direction", "direction needs 3 components");
+  const double dn = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+  if (!(dn > 0.0)) paramError(p, "direction", "direction must not be zero");
+  for (double &d : dir) d /= dn;
+  const int32_t bottom = (int32_t)p.getInt("bottom_boundary"), top = (int32_t)p.getInt("top_boundary");
+  std::vector<double> off{0.0};
+  for (size_t i = 0; i < h.size(); ++i)
+    for (long long k = 1; k <= nl[i]; ++k) off.push_back(off.back() + h[i] / (double)nl[i]);
+  const int64_t L = (int64_t)off.size() - 1, n2 = m.n_nodes(), e2 = m.n_elem();
+  MeshData o;
+  o.dim = 3;
+  o.boundary_names = m.boundary_names;
+  for (int64_t j = 0; j <= L; ++j)
+    for (int64_t i = 0; i < n2; ++i)
+      for (int d = 0; d < 3; ++d) o.xyz.push_back(m.xyz[3 * i + d] + dir[d] * off[j]);
+  if ((L + 1) * n2 > INT32_MAX) paramError(p, "num_layers", "too many nodes after extrusion");
+  for (int64_t j = 0; j < L; ++j)
+    for (int64_t e = 0; e < e2; ++e) {
+      const int32_t *c = &m.conn[m.conn_off[e]];
+      const double *a = &m.xyz[3 * (int64_t)c[0]], *b = &m.xyz[3 * (int64_t)c[1]], *q = &m.xyz[3 * (int64_t)c[2]];
+      // quad normal (b - a) x (q - a) against the direction
+      const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, w[3] = {q[0] - a[0], q[1] - a[1], q[2] - a[2]};
+      const double nz = (u[1] * w[2] - u[2] * w[1]) * dir[0] + (u[2] * w[0] - u[0] * w[2]) * dir[1] +
+                        (u[0] * w[1] - u[1] * w[0]) * dir[2];
+      const bool rev = nz < 0.0;
+      const int32_t r[4] = {c[0], rev ? c[3] : c[1], c[2], rev ? c[1] : c[3]};
+      o.type.push_back(MCW_HEX8);
+      for (int k = 0; k < 4; ++k) o.conn.push_back((int32_t)(r[k] + j * n2));
+      for (int k = 0; k < 4; ++k) o.conn.push_back((int32_t)(r[k] + (j + 1) * n2));
+      o.conn_off.push_back((int64_t)o.conn.size());
+      o.subdomain.push_back(m.subdomain[e]);
+      const int32_t *fb = &m.face_boundary[(size_t)(4 * e)];
+      o.face_boundary.push_back(j == 0 ? bottom : -1);
+      for (int k = 0; k < 4; ++k) o.face_boundary.push_back(fb[rev ? 3 - k : k]);
+      o.face_boundary.push_back(j == L - 1 ? top : -1);
+    }
+  m = std::move(o);
+}
+
+MeshData build_mesh(const HitNode &mb, const std::string &input_dir)
 {
   MeshData m;
   bool any = false;
@@ -181,14 +332,21 @@ MeshData build_mesh(const HitNode &mb)
     if (type == "GeneratedMeshGenerator") {
       generated_mesh(p, m);
       any = true;
-    } else if (type == "SubdomainBoundingBoxGenerator") {
+    } else if (type == "FileMeshGenerator") {
+      file_mesh(p, m, input_dir);
+      any = true;
+    } else if (type == "SubdomainBoundingBoxGenerator" || type == "TransformGenerator" ||
+               type == "FancyExtruderGenerator" || type == "AdvancedExtruderGenerator") {
       if (!any) paramError(p, "input", "needs a mesh generated before it");
-      subdomain_bbox(p, m);
+      if (type == "SubdomainBoundingBoxGenerator") subdomain_bbox(p, m);
+      else if (type == "TransformGenerator") transform_mesh(p, m);
+      else extrude_mesh(p, m);
     } else {
       paramError(p, "type", "unsupported mesh generator '" + type + "'");
     }
   }
   if (!any) mooseError("[Mesh] does not generate a mesh");
+  if (m.dim != 3) mooseError("[Mesh] ends with a 2D mesh: rays are traced in 3D meshes (extrude it)");
   return m;
 }
 
@@ -226,6 +384,8 @@ static NuclideData synthetic_nuclide(const InputParameters &p)
     n.watt[0] = p.getReal("watt_a", 0.988e6);
     n.watt[1] = p.getReal("watt_b", 2.249e-6);
   }
+  n.kappa = p.getReal("kappa", 0.0); // eV per fission, read by kappa-fission scores
+  if (!(n.kappa >= 0.0)) paramError(p, "kappa", "kappa (eV per fission) must be >= 0");
   uint64_t s = (uint64_t)p.getInt("seed", 1) * 152917ull;
   auto prn = [&]() {
     s = 6364136223846793005ull * s + 1442695040888963407ull;
@@ -309,12 +469,21 @@ CollisionKernel::CollisionKernel(const InputParameters &p, Problem &fe) : _param
   _verbose = p.getBool("verbose", false);
   const std::string T = p.getString("temperature");
   _temperature = T;
-  auto it = fe.aux_vars.find(T);
-  if (it == fe.aux_vars.end()) paramError(p, "temperature", "no variable named '" + T + "'");
-  // Check that the temperature variable is a constant monomial (CollisionKernel.C:52-57)
-  if (it->second != 0)
+  // a coupled variable may be given as a constant (temperature = 300)
+  char *end = nullptr;
+  std::strtod(T.c_str(), &end);
+  const bool constant = !T.empty() && end && *end == '\0';
+  auto it = constant ? fe.aux_vars.end() : fe.aux_vars.find(T);
+  if (constant) {
+    // nothing to check
+  } else if (it == fe.aux_vars.end()) {
+    paramError(p, "temperature", "no variable named '" + T + "'");
+  } else if (it->second != 0) {
+    // Check that the temperature variable is a constant monomial (CollisionKernel.C:52-57)
     paramError(p, "temperature", "Only CONST MONOMIAL temperatures (current order: " + std::to_string(it->second) +
                                      ") are supported.");
+  }
+  if (p.isParamValid("z_coord")) p.getReal("z_coord"); // 2D runs only; a 3D mesh ignores it
   auto blocks = p.getInts("blocks");
   auto materials = p.getInts("materials");
   if (blocks.size() != materials.size())
@@ -459,6 +628,7 @@ void OpenMCTally::initialize()
     else if (s == "absorption") sc.push_back(MCW_SCORE_ABSORPTION);
     else if (s == "scatter" || s == "elastic") sc.push_back(MCW_SCORE_SCATTER);
     else if (s == "fission") sc.push_back(MCW_SCORE_FISSION);
+    else if (s == "kappa-fission") sc.push_back(MCW_SCORE_KAPPA_FISSION);
     else paramError(_params, "scores", "unknown score '" + s + "'");
   }
   // nuclide bins: "all" = every nuclide + total, "total" alone, else the list + total
@@ -501,8 +671,9 @@ OpenMCTallyAux::OpenMCTallyAux(const InputParameters &p, Problem &fe) : _params(
   else if (g == "cell") _granularity = 2;
   else paramError(p, "granularity", "granularity must be one of global, universe, cell");
   _score = p.getString("score");
-  p.getString("estimator");
-  p.getString("particle_type");
+  // required MooseEnums with defaults (OpenMCTallyAux.C:34-43): an input may leave them out
+  p.getString("estimator", "COLLISION");
+  p.getString("particle_type", "neutron");
   _nuclide = p.getString("nuclide", "all");
   _energy_bin = (int)p.getInt("energy_bin", -1);
   _variable = p.getString("variable");
@@ -608,12 +779,13 @@ void OpenMCStudy::initialSetup()
   for (auto &n : fe.nuclides)
     nd.push_back(mcw_nuclide{(int32_t)n.E.size(), n.awr, n.E.data(), n.t.data(), n.a.data(), n.s.data(),
                              n.f.empty() ? nullptr : n.f.data(), {n.nu[0], n.nu[1]}, {n.watt[0], n.watt[1]},
-                             n.temperature});
+                             n.temperature, n.kappa});
   check(_ctx, mcw_xs_set(_ctx, (int32_t)nd.size(), nd.data()), "mcw_xs_set");
   for (size_t q = 0; q < fe.nuclides.size(); ++q)
     for (auto &n : fe.nuclides[q].temps) {
       mcw_nuclide t{(int32_t)n.E.size(), n.awr, n.E.data(), n.t.data(), n.a.data(), n.s.data(),
-                    n.f.empty() ? nullptr : n.f.data(), {n.nu[0], n.nu[1]}, {n.watt[0], n.watt[1]}, n.temperature};
+                    n.f.empty() ? nullptr : n.f.data(), {n.nu[0], n.nu[1]}, {n.watt[0], n.watt[1]}, n.temperature,
+                    n.kappa};
       check(_ctx, mcw_xs_add_temperature(_ctx, (int32_t)q, &t), "mcw_xs_add_temperature");
     }
   // CollisionKernel.C:109: sqrt(kT) from the element's temperature variable
@@ -682,7 +854,8 @@ Problem::Problem(std::unique_ptr<HitNode> root, const std::string &input) : _roo
   HitNode &R = *_root;
   HitNode *mb = R.child("Mesh");
   if (!mb) mooseError("the input has no [Mesh] block");
-  mesh = build_mesh(*mb);
+  const size_t slash = input.rfind('/');
+  mesh = build_mesh(*mb, slash == std::string::npos ? std::string(".") : input.substr(0, slash));
   for (const char *vb : {"Variables", "AuxVariables"})
     if (HitNode *v = R.child(vb))
       for (auto &c : v->children) {
@@ -842,19 +1015,21 @@ void Problem::read_monte_carlo(const HitNode &mcb)
     for (int k = 0; k < 3; ++k) source.point[k] = pt[k];
   } else if (space == "volume") {
     source.space = MCW_SPACE_MESH_VOLUME;
-  } else if (space == "box") {
-    // settings.xml <space type="box"><parameters>lower_left upper_right</parameters>
+  } else if (space == "box" || space == "fission") {
+    // settings.xml <space type="box"> or <space type="fission"> (a box whose
+    // sites must fall in fissionable materials), <parameters>lower_left upper_right</parameters>
     source.space = MCW_SPACE_BOX;
+    source.only_fissionable = space == "fission" ? 1 : 0;
     auto lo = s.getReals("lower_left"), hi = s.getReals("upper_right");
     if (lo.size() != 3) paramError(s, "lower_left", "a box corner needs 3 coordinates");
     if (hi.size() != 3) paramError(s, "upper_right", "a box corner needs 3 coordinates");
     for (int k = 0; k < 3; ++k) {
-      if (!(hi[k] > lo[k])) paramError(s, "upper_right", "the box's upper corner must lie above its lower corner");
+      if (!(hi[k] >= lo[k])) paramError(s, "upper_right", "the box's upper corner must not lie below its lower corner");
       source.box_lo[k] = lo[k];
       source.box_hi[k] = hi[k];
     }
   } else {
-    paramError(s, "space", "space must be point, box or volume");
+    paramError(s, "space", "space must be point, box, fission or volume");
   }
   source.angle = MCW_ANGLE_ISOTROPIC;
   if (s.isParamValid("watt_a")) {
@@ -872,6 +1047,9 @@ void Problem::read_monte_carlo(const HitNode &mcb)
 // order, on its `block`s or everywhere.
 std::vector<double> Problem::element_values(const std::string &var) const
 {
+  char *end = nullptr;
+  const double c = std::strtod(var.c_str(), &end);
+  if (!var.empty() && end && *end == '\0') return std::vector<double>(mesh.n_elem(), c); // a constant
   std::vector<double> v(mesh.n_elem(), aux_initial.count(var) ? aux_initial.at(var) : 0.0);
   if (HitNode *ic = _root->child("ICs"))
     for (auto &c : ic->children) {
@@ -995,7 +1173,7 @@ void Problem::dump(const std::string &path) const
   std::ofstream f(path, std::ios::binary);
   auto w64 = [&](int64_t v) { f.write((const char *)&v, 8); };
   auto wd = [&](const void *p, size_t n) { f.write((const char *)p, n); };
-  f.write("MCWD0003", 8);
+  f.write("MCWD0004", 8);
   w64(mesh.n_nodes());
   wd(mesh.xyz.data(), mesh.xyz.size() * 8);
   w64(mesh.n_elem());
@@ -1032,6 +1210,7 @@ void Problem::dump(const std::string &path) const
     wd(n.nu, 16);
     wd(n.watt, 16);
     wd(&n.temperature, 8);
+    wd(&n.kappa, 8);
   };
   w64((int64_t)nuclides.size());
   for (auto &n : nuclides) {

@@ -55,13 +55,18 @@ struct MeshData {
   std::vector<int64_t> conn_off{0};
   std::vector<int32_t> conn;
   std::vector<int32_t> subdomain;
-  std::vector<int32_t> face_boundary; // per side slot
+  std::vector<int32_t> face_boundary; // per side slot; -1: no boundary id
   std::map<std::string, int32_t> boundary_names;
+  int dim = 3;                        // 2: QUAD4 elements (MESH_QUAD4), to be extruded
   int64_t n_nodes() const { return (int64_t)xyz.size() / 3; }
   int64_t n_elem() const { return (int64_t)type.size(); }
 };
 
-MeshData build_mesh(const HitNode &mesh_block);
+// Driver-side element type of a 2D mesh before extrusion (never reaches the library).
+constexpr uint8_t MESH_QUAD4 = 200;
+
+// `input_dir`: where relative mesh file names are looked up (the input file's directory).
+MeshData build_mesh(const HitNode &mesh_block, const std::string &input_dir);
 
 struct NuclideData {
   std::string name;
@@ -71,6 +76,7 @@ struct NuclideData {
   double nu[2] = {0.0, 0.0};          // nu(E) = nu[0] + nu[1] E
   double watt[2] = {0.988e6, 2.249e-6}; // fission spectrum a [eV], b [1/eV]
   double temperature = 293.6;         // K
+  double kappa = 0.0;                 // eV released per fission (kappa-fission scores)
   std::vector<NuclideData> temps;     // tables at further temperatures
 };
 

@@ -53,6 +53,7 @@ class NuclideData:
     nu: Tuple[float, float] = (0.0, 0.0)   # nu(E) = nu[0] + nu[1] E
     watt: Tuple[float, float] = (0.988e6, 2.249e-6)  # fission spectrum a [eV], b [1/eV]
     temperature: float = 0.0               # K of this table (0: 293.6)
+    kappa: float = 0.0                     # energy released per fission [eV] (kappa-fission scores)
 
 
 @dataclasses.dataclass
@@ -150,10 +151,13 @@ def flat_nuclide(awr: float, total: float, absorption: float, fission: float = 0
                        nu=(float(nu[0]), float(nu[1])), watt=(float(watt[0]), float(watt[1])))
 
 
-def fissile(n: NuclideData, fraction: float, nu=(2.43, 0.0), watt=(0.988e6, 2.249e-6)) -> NuclideData:
-    """The nuclide with a share `fraction` of its absorption made fission."""
+def fissile(n: NuclideData, fraction: float, nu=(2.43, 0.0), watt=(0.988e6, 2.249e-6),
+            kappa: float = 193.4e6) -> NuclideData:
+    """The nuclide with a share `fraction` of its absorption made fission
+    (kappa [eV] released per fission: U-235's 193.4 MeV by default)."""
     return dataclasses.replace(n, fission=fraction * np.asarray(n.absorption, np.float64),
-                               nu=(float(nu[0]), float(nu[1])), watt=(float(watt[0]), float(watt[1])))
+                               nu=(float(nu[0]), float(nu[1])), watt=(float(watt[0]), float(watt[1])),
+                               kappa=float(kappa))
 
 
 # ----------------------------------------------------------------- meshes
@@ -629,7 +633,7 @@ def nuclide_struct(n: NuclideData, keep: list) -> abi.Nuclide:
     keep += arrs + [fis]
     return abi.Nuclide(n.energy.size, n.awr, *[abi.ptr(a, abi.C.c_double) for a in arrs],
                        abi.ptr(fis, abi.C.c_double), (abi.C.c_double * 2)(*n.nu),
-                       (abi.C.c_double * 2)(*n.watt), float(n.temperature))
+                       (abi.C.c_double * 2)(*n.watt), float(n.temperature), float(n.kappa))
 
 
 class Marshal:
@@ -710,10 +714,11 @@ def load_dump(path) -> Problem:
     """Problem written by `macaw-amd-opt --dump-problem` (the exact bytes the
     driver handed to the library), for replay on a CPU reference."""
     raw = open(path, "rb").read()
-    if raw[:8] not in (b"MCWD0001", b"MCWD0002", b"MCWD0003"):
+    if raw[:8] not in (b"MCWD0001", b"MCWD0002", b"MCWD0003", b"MCWD0004"):
         raise ValueError(f"{path}: not a problem dump")
-    v2 = raw[:8] in (b"MCWD0002", b"MCWD0003")
-    v3 = raw[:8] == b"MCWD0003"
+    v2 = raw[:8] in (b"MCWD0002", b"MCWD0003", b"MCWD0004")
+    v3 = raw[:8] in (b"MCWD0003", b"MCWD0004")
+    v4 = raw[:8] == b"MCWD0004"
     pos = [8]
 
     def i64():
@@ -756,6 +761,8 @@ def load_dump(path) -> Problem:
                                     watt=(float(watt[0]), float(watt[1])))
         if v3:
             n = dataclasses.replace(n, temperature=float(arr("<f8", 1)[0]))
+        if v4:
+            n = dataclasses.replace(n, kappa=float(arr("<f8", 1)[0]))
         return n
 
     nucs, temps = [], {}

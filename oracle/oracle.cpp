@@ -79,6 +79,7 @@ struct Nuclide {
   double nu0 = 0.0, nu1 = 0.0;      // nu(E) = nu0 + nu1 E
   double watt_a = 0.0, watt_b = 0.0; // fission spectrum
   double temperature = 293.6;        // K
+  double kappa = 0.0;                // eV released per fission (kappa-fission scores)
 };
 
 // Boltzmann constant [eV/K] (OpenMC K_BOLTZMANN).
@@ -462,7 +463,7 @@ void micro_xs(const Nuclide &n, double E, Micro *m)
 }
 
 struct Macro {
-  double t, a, s, f;
+  double t, a, s, f, kf; // kf: kappa-fission, sum of N kappa sigma_f
 };
 
 // sqrt(kT) of an element [sqrt(eV)]; -1 when no temperatures are set.
@@ -493,12 +494,14 @@ const Nuclide &nuc_at(const Oracle &o, int q, double skT)
 
 void macro_xs(const Oracle &o, int mat, double E, double skT, Macro *M)
 {
-  M->t = M->a = M->s = M->f = 0.0;
+  M->t = M->a = M->s = M->f = M->kf = 0.0;
   if (mat < 0) return;
   const Material &m = o.mats[mat];
   for (size_t i = 0; i < m.nuc.size(); ++i) {
     Micro u;
-    micro_xs(nuc_at(o, m.nuc[i], skT), E, &u);
+    const Nuclide &n = nuc_at(o, m.nuc[i], skT);
+    micro_xs(n, E, &u);
+    M->kf = M->kf + m.dens[i] * (n.kappa * u.f);
     M->t = M->t + m.dens[i] * u.t;
     M->a = M->a + m.dens[i] * u.a;
     M->s = M->s + m.dens[i] * u.s;
@@ -632,6 +635,14 @@ bool sample_source(const Oracle &o, uint64_t *seed, Site *st)
     for (int it = 0; it < 1000 && e < 0; ++it) {
       for (int k = 0; k < 3; ++k) p[k] = S.box_lo[k] + orc_prn(seed) * (S.box_hi[k] - S.box_lo[k]);
       e = locate(o, p, 1e-12);
+      // OpenMC IndependentSource::sample, only_fissionable: the site's
+      // material must hold a fissionable nuclide
+      if (e >= 0 && S.only_fissionable) {
+        bool fis = false;
+        if (o.elem_mat[e] >= 0)
+          for (int q : o.mats[o.elem_mat[e]].nuc) fis |= o.nucs[q].fissionable;
+        if (!fis) e = -1;
+      }
     }
     st->elem = e;
   } else {
@@ -704,7 +715,7 @@ void score(const Oracle &o, const Tally &t, std::vector<double> &val, int64_t el
     int nuc = t.nuclides[ni];
     double dens = 0.0;
     Micro mu = {0, 0, 0, 0};
-    Macro Ma = {0, 0, 0, 0};
+    Macro Ma = {0, 0, 0, 0, 0};
     bool need_xs = false;
     for (int sc : t.scores) need_xs |= sc != MCW_SCORE_FLUX;
     if (need_xs) {
@@ -720,6 +731,7 @@ void score(const Oracle &o, const Tally &t, std::vector<double> &val, int64_t el
             Ma.a = dens * mu.a;
             Ma.s = dens * mu.s;
             Ma.f = dens * mu.f;
+            Ma.kf = dens * (o.nucs[nuc].kappa * mu.f);
           }
       }
     }
@@ -730,7 +742,8 @@ void score(const Oracle &o, const Tally &t, std::vector<double> &val, int64_t el
         case MCW_SCORE_TOTAL: m = Ma.t; break;
         case MCW_SCORE_ABSORPTION: m = Ma.a; break;
         case MCW_SCORE_SCATTER: m = Ma.s; break;
-        default: m = Ma.f; break;
+        case MCW_SCORE_FISSION: m = Ma.f; break;
+        default: m = Ma.kf; break;
       }
       row[ni * nsc + si] += wfl * m;
     }
@@ -1183,6 +1196,7 @@ int orc_xs_set(void *h, int32_t n, const mcw_nuclide *nuc)
     q.nu1 = nuc[i].nu[1];
     q.watt_a = nuc[i].watt[0];
     q.watt_b = nuc[i].watt[1];
+    q.kappa = nuc[i].kappa;
     if (nuc[i].temperature > 0.0) q.temperature = nuc[i].temperature;
     o->nucs.push_back(std::move(q));
   }

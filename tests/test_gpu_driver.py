@@ -142,6 +142,24 @@ def test_temperature_driver_matches_oracle(tmp_path, oracle_lib):
     assert power.sum() > 0 and close(power, o.tally(1))
 
 
+def test_beavrs_assembly_driver_matches_oracle(tmp_path, oracle_lib):
+    """The reference's BEAVRS assembly example (its Exodus mesh extruded to
+    95,504 HEX8, a fission source on z = 100, reflective): per-batch keff and
+    the kappa-fission power aux values equal the oracle's."""
+    base, p = run_driver("beavrs_assembly_3d.i", tmp_path, "MonteCarlo/particles=20000", "MonteCarlo/batches=3")
+    assert p.mesh.n_elem == 95504 and p.source.only_fissionable == 1
+    p.tallies = [P.TallySpec(id=1, estimator=abi.EST_COLLISION, filters=[{"type": abi.FILTER_CELL}],
+                             scores=[abi.SCORE_KAPPA_FISSION])]
+    o = oracle_lib.Oracle(p, eigenvalue=True, n_inactive=p.n_inactive)
+    kg = []
+    for b in (1, 2, 3):
+        o.run_batch(b, p.n_particles, threads=8)
+        kg.append(o.keff()["k_generation"])
+    assert np.array_equal(read_col(base + ".csv", "k_generation"), np.array(kg))
+    power = read_col(base + "_power.csv", "power")
+    assert power.sum() > 0 and close(power, o.tally(1))
+
+
 def test_eigen_driver_ray_output_matches_gold_seeds(tmp_path):
     """The reference study's own settings -- eigenvalue, 30 particles, 3
     batches, 1 inactive -- with [Outputs/rays] type = RayTracingExodus: rays

@@ -175,6 +175,27 @@ def test_eigen_full_bank_drops_sites():
         assert g.source_bank().size == p.n_particles
 
 
+def test_kappa_fission_and_nuclide_bins_match_oracle():
+    """kappa-fission scores (track-length and collision, total and per
+    nuclide) against the oracle, and kappa-fission = kappa x fission for the
+    one fissionable nuclide."""
+    from oracle.pyoracle import Oracle
+    p = small_k2(n_particles=3000)
+    p.tallies = [P.TallySpec(id=1, estimator=abi.EST_TRACKLENGTH, filters=[{"type": abi.FILTER_CELL}],
+                             scores=[abi.SCORE_FISSION, abi.SCORE_KAPPA_FISSION], nuclides=[4, -1]),
+                 P.TallySpec(id=2, estimator=abi.EST_COLLISION, filters=[{"type": abi.FILTER_UNIVERSE}],
+                             scores=[abi.SCORE_KAPPA_FISSION])]
+    g = Context(p, mode=abi.MODE_BATCHED)
+    o = Oracle(p)
+    g.run_batch(1, p.n_particles)
+    o.run_batch(1, p.n_particles, threads=8)
+    for tid in (1, 2):
+        assert_tally_close(g.tally(tid), o.tally(tid))
+    t = g.tally(1).reshape(-1, 2, 2)  # (cell, nuclide bin, score)
+    assert t[:, :, 0].sum() > 0
+    np.testing.assert_allclose(t[:, :, 1], 193.4e6 * t[:, :, 0], rtol=1e-12, atol=0)
+
+
 def test_eigen_errors():
     p = small_k2(n_particles=500)
     g = Context(p, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE)

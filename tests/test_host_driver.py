@@ -27,7 +27,7 @@ def run(args, cwd=None):
 
 
 @pytest.mark.parametrize("name", ["c1_hex8_reflective.i", "openmc_tally.i", "openmc_study_eigen.i",
-                                  "openmc_study_temperature.i"])
+                                  "openmc_study_temperature.i", "beavrs_assembly_3d.i"])
 def test_inputs_validate(name):
     r = run(["-i", os.path.join(INPUTS, name), "--check-input"])
     assert r.returncode == 0, r.stderr
@@ -172,3 +172,78 @@ def test_temperature_dump_replays_on_the_oracle(tmp_path, oracle_lib):
         runs.append((o.keff(), o.histories(300)))
     assert_same_histories(runs[0][1], runs[1][1])
     assert runs[0][0]["keff"] == runs[1][0]["keff"]
+
+
+def test_exodus_mesh_is_read_and_extruded(tmp_path):
+    """FileMeshGenerator + TransformGenerator + FancyExtruderGenerator on the
+    reference example's QUAD4 assembly mesh, checked against the same file
+    read by scipy's NetCDF reader: nodes layer by layer, each hex's nodes
+    from its quad's, blocks, and side-set ids carried to the swept sides."""
+    from scipy.io import netcdf_file
+    d = tmp_path / "bv.dump"
+    r = run(["-i", os.path.join(INPUTS, "beavrs_assembly_3d.i"), "MonteCarlo/particles=100", "--dump-problem", str(d)])
+    assert r.returncode == 0, r.stderr
+    p = P.load_dump(str(d))
+    f = netcdf_file(os.path.join(INPUTS, "31enr_16_in.e"), "r", mmap=False)
+    V = f.variables
+    x, y = V["coordx"][:].copy(), V["coordy"][:].copy()
+    n2 = x.size
+    quads, blocks = [], []
+    for k, bid in enumerate(V["eb_prop1"][:]):
+        c = V[f"connect{k + 1}"][:].astype(np.int64) - 1
+        quads.append(c)
+        blocks.append(np.full(c.shape[0], bid))
+    quads, blocks = np.concatenate(quads), np.concatenate(blocks)
+    e2 = quads.shape[0]
+    m = p.mesh
+    assert m.n_nodes == 5 * n2 and m.n_elem == 4 * e2 and np.all(m.elem_type == abi.HEX8)
+    for j in range(5):
+        lay = m.xyz[j * n2:(j + 1) * n2]
+        assert np.array_equal(lay[:, 0], x) and np.array_equal(lay[:, 1], y)
+        assert np.all(lay[:, 2] == -100.0 + 100.0 * j)
+    conn = m.conn.reshape(-1, 8).astype(np.int64)
+    for j in range(4):
+        h = conn[j * e2:(j + 1) * e2]
+        assert np.array_equal(np.sort(h[:, :4], 1), np.sort(quads, 1) + j * n2)
+        assert np.array_equal(h[:, 4:], h[:, :4] + n2)
+    assert np.array_equal(m.subdomain, np.tile(blocks, 4))
+    # every hex has positive volume: its bottom quad winds counterclockwise about +z
+    b = m.xyz[conn[:, :4]]
+    area2 = ((b[:, 0, 0] - b[:, 2, 0]) * (b[:, 1, 1] - b[:, 3, 1]) - (b[:, 1, 0] - b[:, 3, 0]) * (b[:, 0, 1] - b[:, 2, 1]))
+    assert np.all(area2 > 0)
+    # side-set ids: each 2D side set's sides, once per layer; bottom 8 and top 9 once per column
+    fb = m.face_boundary.reshape(-1, 6)
+    ids, counts = np.unique(fb, return_counts=True)
+    got = dict(zip(ids.tolist(), counts.tolist()))
+    for k, sid in enumerate(V["ss_prop1"][:]):
+        assert got[int(sid)] == 4 * V[f"elem_ss{k + 1}"].shape[0], sid
+    assert got[8] == e2 and got[9] == e2
+    assert np.all(fb[:e2, 0] == 8) and np.all(fb[3 * e2:, 5] == 9) and np.all(fb[e2:, 0] == -1)
+
+
+def test_beavrs_assembly_replays_on_the_oracle(tmp_path, oracle_lib):
+    """The assembly example runs on the oracle: fission-source sites sit on
+    z = 100 in fuel, nothing is lost in the extruded pin-cell mesh, and the
+    kappa-fission score is kappa times the fission score (U235 is the only
+    fissionable nuclide)."""
+    d = tmp_path / "bv.dump"
+    r = run(["-i", os.path.join(INPUTS, "beavrs_assembly_3d.i"), "MonteCarlo/particles=400", "--dump-problem", str(d)])
+    assert r.returncode == 0, r.stderr
+    p = P.load_dump(str(d))
+    assert p.source.only_fissionable == 1 and p.run_mode == abi.RUN_EIGENVALUE
+    kappa = [n.kappa for n in p.nuclides]
+    assert kappa[0] == 193.4e6 and all(k == 0.0 for k in kappa[1:])
+    p.tallies = [P.TallySpec(id=1, estimator=abi.EST_COLLISION, filters=[{"type": abi.FILTER_CELL}],
+                             scores=[abi.SCORE_KAPPA_FISSION, abi.SCORE_FISSION])]
+    o = oracle_lib.Oracle(p, eigenvalue=True, n_inactive=0)
+    src = o.sources(1, 400, 400)
+    elem = src[:, 7].astype(np.int64)
+    assert np.all(src[:, 2] == 100.0) and np.all(p.mesh.subdomain[elem] == 10010)  # fuel 3.1%
+    for b in (1, 2):
+        st = o.run_batch(b, 400, threads=4)
+        assert st["lost"] == 0 and st["leaks"] == 0 and st["absorptions"] + st["cutoffs"] == 400
+    k = o.keff()
+    assert 0.8 < k["k_generation"] < 2.0
+    t = o.tally(1).reshape(-1, 2)
+    assert t[:, 1].sum() > 0
+    np.testing.assert_allclose(t[:, 0], 193.4e6 * t[:, 1], rtol=1e-13, atol=0)
