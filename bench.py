@@ -23,7 +23,10 @@ HIP-event duration), the serving mix (c2 runs only: 2000 requests over
 resident 100k / 1M / 4M-tet meshes, Poisson arrivals at 70% of measured
 capacity, p50/p99 latency), an eigenvalue run (c2 runs only: C2's mesh with
 a fissile nuclide, 1e7 histories per generation, 2 inactive + 3 active
-batches, each with its fission-bank sort and resampling) and a CPU baseline
+batches, each with its fission-bank sort and resampling), the reference's
+BEAVRS assembly example run from its driver input (c2 runs only: the
+example's Exodus mesh extruded to 95,504 HEX8, eigenvalue, 1e7 histories per
+batch) and a CPU baseline
 (the FP64 oracle, std::thread on the host cores, on a bounded sample of the
 same histories).
 
@@ -175,6 +178,45 @@ def bench_eigen(args, tuning):
                            "nu 2.43, Watt spectrum), vacuum, batch 1 from a volumetric Watt source"}
 
 
+def bench_assembly(args):
+    """The reference's BEAVRS 3.1% assembly example as its driver input sets
+    it up (tests/inputs/beavrs_assembly_3d.i: the example's Exodus mesh
+    extruded to 95,504 HEX8, reflective, fission source on z = 100, synthetic
+    stand-ins for its materials), eigenvalue with n histories per batch: one
+    warm batch, then 4 batches timed."""
+    import subprocess
+    import tempfile
+    from macaw_amd import abi, build, problems as P
+    from macaw_amd.lib import Context
+
+    n = args.assembly_histories
+    inp = os.path.join(ROOT, "tests", "inputs", "beavrs_assembly_3d.i")
+    with tempfile.TemporaryDirectory() as d:
+        dump = os.path.join(d, "assembly.dump")
+        subprocess.run([build.DRIVER, "-i", inp, "--dump-problem", dump], check=True, stdout=subprocess.DEVNULL)
+        prob = P.load_dump(dump)
+    prob.tallies = [P.TallySpec(id=1, estimator=abi.EST_COLLISION, filters=[{"type": abi.FILTER_CELL}],
+                                scores=[abi.SCORE_KAPPA_FISSION])]
+    ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=0)
+    ctx.run_batch(1, n)
+    rows = []
+    t = time.perf_counter()
+    for b in range(2, 6):
+        rows.append(ctx.run_batch(b, n))
+    wall = time.perf_counter() - t
+    k = ctx.keff()
+    ctx.close()
+    segs = sum(st["segments"] for st in rows)
+    log(f"assembly: {wall / 4 * 1e3:.1f} ms per batch of {n}, keff {k['keff']:.5f} +- {k['keff_std']:.5f}")
+    return {"histories_per_s": 4 * n / wall, "segments_per_s": segs / wall, "ms_per_batch": wall / 4 * 1e3,
+            "segments_per_history": segs / (4 * n),
+            "collisions_per_history": sum(st["collisions"] for st in rows) / (4 * n),
+            "keff": k["keff"], "keff_std": k["keff_std"], "histories_per_batch": n, "batches_timed": 4,
+            "workload": f"beavrs_assembly_hex8_{prob.mesh.n_elem // 1000}k_eigen",
+            "description": "the reference's examples/beavrs_assembly mesh (17x17 pin cells, QUAD4) extruded to 4 "
+                           "layers of HEX8 over 400 cm, reflective, fission source on z = 100, synthetic nuclides"}
+
+
 WORKLOADS = {
     # name: (generator, default histories per step, description)
     "c2": ("c2_tet_vacuum", 10_000_000, "1M TET4 unit cube, vacuum, 4 materials / 4 nuclides x 50k points"),
@@ -272,6 +314,8 @@ def main():
     ap.add_argument("--serve-bank-gib", type=float, default=4.0)
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
     ap.add_argument("--eigen-histories", type=int, default=10_000_000)
+    ap.add_argument("--no-assembly", action="store_true", help="skip the BEAVRS assembly leg")
+    ap.add_argument("--assembly-histories", type=int, default=10_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
                          "bytes): the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling")
@@ -363,6 +407,10 @@ def main():
     if not args.no_eigen and args.workload == "c2":
         eigen = bench_eigen(args, tuning)
 
+    assembly = None
+    if not args.no_assembly and args.workload == "c2":
+        assembly = bench_assembly(args)
+
     serving = None
     if not args.no_serve and args.workload == "c2":
         serving = bench_serving(args, tuning)
@@ -402,6 +450,7 @@ def main():
         "roofline": roof,
         "serving": serving,
         "eigenvalue": eigen,
+        "beavrs_assembly": assembly,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
