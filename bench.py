@@ -140,6 +140,25 @@ def bench_serving(args, tuning):
     return out
 
 
+def tune_eigen_lanes(prob, n, candidates=(24, 32, 40), log_name="eigen"):
+    """lanes_min for an eigenvalue kernel, picked on the workload itself: each
+    candidate runs batches 1 and 2 (the second from a fission source) on a
+    fresh context; the fastest batch-2 transport wins.  Results do not depend
+    on lanes_min, only the time does."""
+    from macaw_amd import abi
+    from macaw_amd.lib import Context
+    best, best_ms = candidates[0], float("inf")
+    for lm in candidates:
+        ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=0, lanes_min=lm)
+        ctx.run_batch(1, n)
+        ms = ctx.run_batch(2, n)["time_ms"]
+        ctx.close()
+        log(f"{log_name}: lanes_min {lm}: batch 2 transport {ms:.1f} ms")
+        if ms < best_ms:
+            best, best_ms = lm, ms
+    return best
+
+
 def bench_eigen(args, tuning):
     """Eigenvalue leg: C2's mesh and materials with a fissile nuclide added
     (keff near 1), n histories per generation, 2 inactive + 3 active
@@ -151,8 +170,9 @@ def bench_eigen(args, tuning):
     n = args.eigen_histories
     t = time.perf_counter()
     prob = P.k2_tet_eigen(n_particles=n)
+    lanes = args.lanes_min if args.no_autotune and args.lanes_min else tune_eigen_lanes(prob, n)
     ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=2,
-                  lanes_min=tuning.get("lanes_min", 0), block_size=tuning.get("block_size", 0))
+                  lanes_min=lanes, block_size=tuning.get("block_size", 0))
     log(f"eigen: {prob.name} ready ({time.perf_counter() - t:.1f}s)")
     ctx.run_batch(1, n)  # warm: allocations and code objects
     rows = []
@@ -173,6 +193,7 @@ def bench_eigen(args, tuning):
             "keff": k["keff"], "keff_std": k["keff_std"], "k_generation": [kk["k_generation"] for _, kk in rows],
             "fission_sites_per_history": sum(kk["fission_sites"] for _, kk in rows) / (5 * n),
             "segments_per_history": segs / (5 * n), "histories_per_batch": n, "batches": 5, "inactive": 2,
+            "lanes_min": lanes,
             "workload": f"k2_tet4_{prob.mesh.n_elem // 1000}k_eigen",
             "description": "C2 mesh and materials plus a fissile nuclide (A=235, 85% of absorption is fission, "
                            "nu 2.43, Watt spectrum), vacuum, batch 1 from a volumetric Watt source"}
@@ -197,7 +218,8 @@ def bench_assembly(args):
         prob = P.load_dump(dump)
     prob.tallies = [P.TallySpec(id=1, estimator=abi.EST_COLLISION, filters=[{"type": abi.FILTER_CELL}],
                                 scores=[abi.SCORE_KAPPA_FISSION])]
-    ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=0)
+    lanes = args.lanes_min if args.no_autotune and args.lanes_min else tune_eigen_lanes(prob, n, log_name="assembly")
+    ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=0, lanes_min=lanes)
     ctx.run_batch(1, n)
     rows = []
     t = time.perf_counter()
@@ -212,6 +234,7 @@ def bench_assembly(args):
             "segments_per_history": segs / (4 * n),
             "collisions_per_history": sum(st["collisions"] for st in rows) / (4 * n),
             "keff": k["keff"], "keff_std": k["keff_std"], "histories_per_batch": n, "batches_timed": 4,
+            "lanes_min": lanes,
             "workload": f"beavrs_assembly_hex8_{prob.mesh.n_elem // 1000}k_eigen",
             "description": "the reference's examples/beavrs_assembly mesh (17x17 pin cells, QUAD4) extruded to 4 "
                            "layers of HEX8 over 400 cm, reflective, fission source on z = 100, synthetic nuclides"}
