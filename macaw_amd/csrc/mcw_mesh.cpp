@@ -389,14 +389,30 @@ void build_locator(HostMesh &m)
       lo[k] = std::min(lo[k], X[3 * i + k]);
       hi[k] = std::max(hi[k], X[3 * i + k]);
     }
-  double ext[3], vol = 1.0;
-  for (int k = 0; k < 3; ++k) {
-    ext[k] = std::max(hi[k] - lo[k], 1e-300);
-    vol *= ext[k];
+  double ext[3];
+  for (int k = 0; k < 3; ++k) ext[k] = std::max(hi[k] - lo[k], 1e-300);
+  // cells shaped like the elements: per axis, the mean extent of an element's
+  // bounding box, scaled so that there are about as many cells as elements
+  // (an extruded mesh's elements can be 1000 times taller than wide; cubic
+  // cells would then list hundreds of candidates each)
+  double mean_ext[3] = {0.0, 0.0, 0.0};
+  for (int64_t o = 0; o < ne; ++o) {
+    double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t q = m.conn_off[o]; q < m.conn_off[o + 1]; ++q)
+      for (int k = 0; k < 3; ++k) {
+        blo[k] = std::min(blo[k], X[3 * (int64_t)m.conn[q] + k]);
+        bhi[k] = std::max(bhi[k], X[3 * (int64_t)m.conn[q] + k]);
+      }
+    for (int k = 0; k < 3; ++k) mean_ext[k] += (bhi[k] - blo[k]) / (double)ne;
   }
-  const double h = std::cbrt(vol / (double)std::max<int64_t>(1, ne));
+  double ratio = 1.0; // prod over axes of ext / mean element extent
   for (int k = 0; k < 3; ++k) {
-    m.grid_n[k] = (int32_t)std::min(1024.0, std::max(1.0, std::ceil(ext[k] / h)));
+    mean_ext[k] = std::min(std::max(mean_ext[k], 1e-12 * ext[k]), ext[k]);
+    ratio *= ext[k] / mean_ext[k];
+  }
+  const double alpha = std::cbrt(ratio / (double)std::max<int64_t>(1, ne)); // cell side / mean extent
+  for (int k = 0; k < 3; ++k) {
+    m.grid_n[k] = (int32_t)std::min(1024.0, std::max(1.0, std::ceil(ext[k] / (alpha * mean_ext[k]))));
     m.grid_lo[k] = lo[k];
     m.grid_inv[k] = m.grid_n[k] / ext[k];
   }
