@@ -348,6 +348,67 @@ __device__ __forceinline__ double macro_total_nf(const XSView &X, int mat, int32
   return t;
 }
 
+// Macroscopic total (and with NF, nu-fission) as macro_total / macro_total_nf
+// compute them, for a material of at most XS_GATHER nuclides: the nuclide ids and
+// densities, then the grid offsets and union-grid indices, are loaded for
+// all nuclides together (two rounds instead of two per nuclide), then the
+// bracketing records nuclide by nuclide.  Sums run in material order, so the
+// values are the loop's to the bit.  Larger materials take the loop.
+constexpr int XS_GATHER = 3; // materials of up to this many nuclides take the gathered loads
+
+template <bool NF>
+__device__ __forceinline__ double macro_total_g(const XSView &X, int mat, int32_t k, double E, double &nusf)
+{
+  nusf = 0.0;
+  if (mat < 0) return 0.0;
+  const int32_t b = X.mat_off[mat], n = X.mat_off[mat + 1] - b;
+  if (n > XS_GATHER) return NF ? macro_total_nf(X, mat, k, E, nusf) : macro_total(X, mat, k, E);
+  int32_t nuc[XS_GATHER];
+  double dn[XS_GATHER];
+#pragma unroll
+  for (int i = 0; i < XS_GATHER; ++i) {
+    const int32_t q = b + (i < n ? i : 0); // past n: a repeat of the first, unused
+    nuc[i] = X.mat_nuc[q];
+    dn[i] = X.mat_dens[q];
+  }
+  int64_t lo[XS_GATHER];
+  int32_t np[XS_GATHER], j[XS_GATHER];
+#pragma unroll
+  for (int i = 0; i < XS_GATHER; ++i) {
+    lo[i] = X.grid_off[nuc[i]];
+    np[i] = X.n_points[nuc[i]];
+    j[i] = k < 0 ? -1 : X.uidx[(int64_t)k * X.n_nuc + nuc[i]];
+  }
+  double t = 0.0, nf = 0.0;
+#pragma unroll
+  for (int i = 0; i < XS_GATHER; ++i) {
+    if (i < n) {
+      int32_t jj = j[i];
+      double ff;
+      if (jj < 0) {
+        jj = 0;
+        ff = 0.0;
+      } else if (jj >= np[i] - 1) {
+        jj = np[i] - 2;
+        ff = 1.0;
+      } else {
+        ff = -1.0; // from the bracketing energies
+      }
+      const double4 r0 = X.rec[lo[i] + jj];
+      const double4 r1 = X.rec[lo[i] + jj + 1];
+      if (ff < 0.0) ff = (E - r0.x) / (r1.x - r0.x);
+      const double g = 1.0 - ff;
+      t = t + dn[i] * (g * r0.y + ff * r1.y);
+      if (NF && X.fis) {
+        const double4 fp = X.fis[nuc[i]];
+        nf = nf + dn[i] * ((fp.x + fp.y * E) * (g * r0.w + ff * r1.w));
+      }
+    }
+  }
+  nusf = nf;
+  return t;
+}
+
 struct Macro {
   double t, a, s, f, kf; // kf: kappa-fission, sum of N kappa sigma_f
 };

@@ -466,8 +466,11 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
     }
   }
   if (mat != st.mat) {
-    if (EIG) st.sigt = macro_total_nf(P.xs, mat, st.k, st.E, st.nusf);
-    else st.sigt = macro_total(P.xs, mat, st.k, st.E);
+    if (EIG) st.sigt = macro_total_g<true>(P.xs, mat, st.k, st.E, st.nusf);
+    else {
+      double unused;
+      st.sigt = macro_total_g<false>(P.xs, mat, st.k, st.E, unused);
+    }
     st.mat = mat;
   }
   if (fx < 0) {
