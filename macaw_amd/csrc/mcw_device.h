@@ -348,14 +348,66 @@ __device__ __forceinline__ double macro_total_nf(const XSView &X, int mat, int32
   return t;
 }
 
-// Macroscopic total (and with NF, nu-fission) as macro_total / macro_total_nf
-// compute them, for a material of at most XS_GATHER nuclides: the nuclide ids and
-// densities, then the grid offsets and union-grid indices, are loaded for
-// all nuclides together (two rounds instead of two per nuclide), then the
-// bracketing records nuclide by nuclide.  Sums run in material order, so the
-// values are the loop's to the bit.  Larger materials take the loop.
-constexpr int XS_GATHER = 3; // materials of up to this many nuclides take the gathered loads
+constexpr int XS_GATHER = 3; // materials of up to this many nuclides take the gathered lookups
 
+// A gathered lookup for a material of at most XS_GATHER nuclides: its loads
+// run in rounds for all nuclides together -- ids and densities, then grid
+// offsets and union-grid indices -- and only the bracketing records are read
+// nuclide by nuclide, so a lookup waits for two rounds plus one per nuclide
+// instead of four per nuclide.  The arithmetic is micro_at's and the sums
+// run in material order: values equal the per-nuclide loops' to the bit.
+struct MatIdx {
+  int32_t n;
+  int32_t nuc[XS_GATHER], np[XS_GATHER], j[XS_GATHER];
+  double dn[XS_GATHER];
+  int64_t lo[XS_GATHER];
+};
+
+__device__ __forceinline__ void mat_index(const XSView &X, int32_t b, int32_t n, int32_t k, MatIdx &I)
+{
+  I.n = n;
+#pragma unroll
+  for (int i = 0; i < XS_GATHER; ++i) {
+    const int32_t q = b + (i < n ? i : 0); // past n: a repeat of the first, unused
+    I.nuc[i] = X.mat_nuc[q];
+    I.dn[i] = X.mat_dens[q];
+  }
+#pragma unroll
+  for (int i = 0; i < XS_GATHER; ++i) {
+    I.lo[i] = X.grid_off[I.nuc[i]];
+    I.np[i] = X.n_points[I.nuc[i]];
+    I.j[i] = k < 0 ? -1 : X.uidx[(int64_t)k * X.n_nuc + I.nuc[i]];
+  }
+}
+
+// Micro cross sections of slot i of a gathered lookup (scatter only with S).
+template <bool S>
+__device__ __forceinline__ void micro_slot(const XSView &X, const MatIdx &I, int i, double E, Micro &m)
+{
+  int32_t j = I.j[i];
+  double f;
+  if (j < 0) {
+    j = 0;
+    f = 0.0;
+  } else if (j >= I.np[i] - 1) {
+    j = I.np[i] - 2;
+    f = 1.0;
+  } else {
+    f = -1.0; // set below from the bracketing energies
+  }
+  const int64_t base = I.lo[i];
+  const double4 r0 = X.rec[base + j];
+  const double4 r1 = X.rec[base + j + 1];
+  if (f < 0.0) f = (E - r0.x) / (r1.x - r0.x);
+  const double g = 1.0 - f;
+  m.t = g * r0.y + f * r1.y;
+  m.a = g * r0.z + f * r1.z;
+  m.f = g * r0.w + f * r1.w;
+  m.s = S ? g * X.scatter[base + j] + f * X.scatter[base + j + 1] : 0.0;
+}
+
+// macro_total (and with NF, macro_total_nf) with a gathered lookup when the
+// material is small enough.
 template <bool NF>
 __device__ __forceinline__ double macro_total_g(const XSView &X, int mat, int32_t k, double E, double &nusf)
 {
@@ -363,45 +415,18 @@ __device__ __forceinline__ double macro_total_g(const XSView &X, int mat, int32_
   if (mat < 0) return 0.0;
   const int32_t b = X.mat_off[mat], n = X.mat_off[mat + 1] - b;
   if (n > XS_GATHER) return NF ? macro_total_nf(X, mat, k, E, nusf) : macro_total(X, mat, k, E);
-  int32_t nuc[XS_GATHER];
-  double dn[XS_GATHER];
-#pragma unroll
-  for (int i = 0; i < XS_GATHER; ++i) {
-    const int32_t q = b + (i < n ? i : 0); // past n: a repeat of the first, unused
-    nuc[i] = X.mat_nuc[q];
-    dn[i] = X.mat_dens[q];
-  }
-  int64_t lo[XS_GATHER];
-  int32_t np[XS_GATHER], j[XS_GATHER];
-#pragma unroll
-  for (int i = 0; i < XS_GATHER; ++i) {
-    lo[i] = X.grid_off[nuc[i]];
-    np[i] = X.n_points[nuc[i]];
-    j[i] = k < 0 ? -1 : X.uidx[(int64_t)k * X.n_nuc + nuc[i]];
-  }
+  MatIdx I;
+  mat_index(X, b, n, k, I);
   double t = 0.0, nf = 0.0;
 #pragma unroll
   for (int i = 0; i < XS_GATHER; ++i) {
     if (i < n) {
-      int32_t jj = j[i];
-      double ff;
-      if (jj < 0) {
-        jj = 0;
-        ff = 0.0;
-      } else if (jj >= np[i] - 1) {
-        jj = np[i] - 2;
-        ff = 1.0;
-      } else {
-        ff = -1.0; // from the bracketing energies
-      }
-      const double4 r0 = X.rec[lo[i] + jj];
-      const double4 r1 = X.rec[lo[i] + jj + 1];
-      if (ff < 0.0) ff = (E - r0.x) / (r1.x - r0.x);
-      const double g = 1.0 - ff;
-      t = t + dn[i] * (g * r0.y + ff * r1.y);
+      Micro u;
+      micro_slot<false>(X, I, i, E, u);
+      t = t + I.dn[i] * u.t;
       if (NF && X.fis) {
-        const double4 fp = X.fis[nuc[i]];
-        nf = nf + dn[i] * ((fp.x + fp.y * E) * (g * r0.w + ff * r1.w));
+        const double4 fp = X.fis[I.nuc[i]];
+        nf = nf + I.dn[i] * ((fp.x + fp.y * E) * u.f);
       }
     }
   }
@@ -428,6 +453,33 @@ __device__ __forceinline__ void macro_all(const XSView &X, int mat, int32_t k, d
     M.s = M.s + dn * u.s;
     M.f = M.f + dn * u.f;
     M.kf = M.kf + dn * (X.kappa[n] * u.f);
+  }
+}
+
+// macro_all with a gathered lookup when the material is small enough.
+__device__ __forceinline__ void macro_all_g(const XSView &X, int mat, int32_t k, double E, Macro &M)
+{
+  M.t = M.a = M.s = M.f = M.kf = 0.0;
+  if (mat < 0) return;
+  const int32_t b = X.mat_off[mat], n = X.mat_off[mat + 1] - b;
+  if (n > XS_GATHER) {
+    macro_all(X, mat, k, E, M);
+    return;
+  }
+  MatIdx I;
+  mat_index(X, b, n, k, I);
+#pragma unroll
+  for (int i = 0; i < XS_GATHER; ++i) {
+    if (i < n) {
+      Micro u;
+      micro_slot<true>(X, I, i, E, u);
+      const double dn = I.dn[i];
+      M.t = M.t + dn * u.t;
+      M.a = M.a + dn * u.a;
+      M.s = M.s + dn * u.s;
+      M.f = M.f + dn * u.f;
+      M.kf = M.kf + dn * (X.kappa[I.nuc[i]] * u.f);
+    }
   }
 }
 
@@ -580,7 +632,7 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
     Macro M = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (T.needs_xs) {
       if (nuc < 0) {
-        macro_all(X, mat, k, E, M);
+        macro_all_g(X, mat, k, E, M);
       } else if (mat >= 0) {
         for (int32_t q = X.mat_off[mat]; q < X.mat_off[mat + 1]; ++q)
           if (X.nuc_of[X.mat_nuc[q]] == nuc) {

@@ -580,12 +580,30 @@ __device__ __forceinline__ int64_t collide_pick(const Params &P, PState &st, Col
   double prob = 0.0;
   int32_t pick = e - 1;
   Micro mu;
-  for (int32_t q = b; q < e; ++q) {
-    micro_at(X, X.mat_nuc[q], st.k, st.E, mu);
-    prob = prob + X.mat_dens[q] * mu.t;
-    if (prob >= cutoff) {
-      pick = q;
-      break;
+  if (e - b <= XS_GATHER) {
+    // small material: a gathered lookup, the same sampling
+    MatIdx I;
+    mat_index(X, b, e - b, st.k, I);
+    bool done = false;
+#pragma unroll
+    for (int i = 0; i < XS_GATHER; ++i) {
+      if (!done && b + i < e) {
+        micro_slot<false>(X, I, i, st.E, mu);
+        prob = prob + I.dn[i] * mu.t;
+        if (prob >= cutoff) {
+          pick = b + i;
+          done = true;
+        }
+      }
+    }
+  } else {
+    for (int32_t q = b; q < e; ++q) {
+      micro_at(X, X.mat_nuc[q], st.k, st.E, mu);
+      prob = prob + X.mat_dens[q] * mu.t;
+      if (prob >= cutoff) {
+        pick = q;
+        break;
+      }
     }
   }
   cp.nuc = X.mat_nuc[pick];
