@@ -657,7 +657,11 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
         case 4: m = M.f; break;
         default: m = M.kf; break;
       }
-      atomic_add_dev(row + ni * nsc + si, wfl * m);
+      // scores are never negative and VALUE starts at +0, so adding an
+      // exact zero changes nothing: skip the atomic (a kappa-fission
+      // score outside the fuel, for instance)
+      const double v = wfl * m;
+      if (v != 0.0) atomic_add_dev(row + ni * nsc + si, v);
     }
   }
 }
