@@ -247,3 +247,132 @@ def test_beavrs_assembly_replays_on_the_oracle(tmp_path, oracle_lib):
     t = o.tally(1).reshape(-1, 2)
     assert t[:, 1].sum() > 0
     np.testing.assert_allclose(t[:, 0], 193.4e6 * t[:, 1], rtol=1e-13, atol=0)
+
+
+# Exodus II face node lists (1-based node positions within the element), as
+# the Exodus II manual numbers element sides
+EXO_FACES = {
+    "HEX8": [(1, 2, 6, 5), (2, 3, 7, 6), (3, 4, 8, 7), (1, 5, 8, 4), (1, 4, 3, 2), (5, 6, 7, 8)],
+    "TETRA": [(1, 2, 4), (2, 3, 4), (1, 4, 3), (1, 3, 2)],
+}
+# libMesh side node lists (0-based)
+LM_FACES = {
+    "HEX8": [(0, 3, 2, 1), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7), (4, 5, 6, 7)],
+    "TETRA": [(0, 2, 1), (0, 1, 3), (1, 2, 3), (2, 0, 3)],
+}
+
+
+def write_exodus_3d(path):
+    """A unit cube as Exodus II (NetCDF 64-bit offset, written by scipy):
+    block 7 holds the 4 HEX8 of its lower half (2 x 2 x 1), block 9 the 24
+    TET4 of its upper half (Kuhn split, 6 per cube); every element side on
+    the cube's surface is in side set 11 + (face index), with Exodus side
+    numbers."""
+    from scipy.io import netcdf_file
+    nx = 2
+    g = np.linspace(0.0, 1.0, nx + 1)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    xyz = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    nid = lambda i, j, k: (i * (nx + 1) + j) * (nx + 1) + k
+    hexes, tets = [], []
+    kuhn = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+    for k in range(nx):
+        for j in range(nx):
+            for i in range(nx):
+                h = [nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                     nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)]
+                if k == 0:
+                    hexes.append(h)
+                else:
+                    tets += [[h[q] for q in t] for t in kuhn]
+    elems = [("HEX8", e) for e in hexes] + [("TETRA", e) for e in tets]
+    sides = {}
+    for gi, (t, e) in enumerate(elems):
+        for s, face in enumerate(EXO_FACES[t]):
+            c = xyz[[e[q - 1] for q in face]]
+            for ax in range(3):
+                for side, val in ((0, 0.0), (1, 1.0)):
+                    if np.all(c[:, ax] == val):
+                        sides.setdefault(11 + 2 * ax + side, []).append((gi + 1, s + 1))
+    f = netcdf_file(path, "w", version=2)
+    f.title = b"cube"
+    for name, n in (("len_string", 33), ("len_name", 33), ("four", 4), ("num_dim", 3), ("num_nodes", len(xyz)),
+                    ("num_elem", len(elems)), ("num_el_blk", 2), ("num_side_sets", len(sides)),
+                    ("num_el_in_blk1", len(hexes)), ("num_nod_per_el1", 8), ("num_el_in_blk2", len(tets)),
+                    ("num_nod_per_el2", 4)):
+        f.createDimension(name, n)
+    for k, nm in enumerate(("coordx", "coordy", "coordz")):
+        v = f.createVariable(nm, "d", ("num_nodes",))
+        v[:] = xyz[:, k]
+    v = f.createVariable("eb_prop1", "i", ("num_el_blk",))
+    v[:] = [7, 9]
+    v.name = b"ID"
+    c1 = f.createVariable("connect1", "i", ("num_el_in_blk1", "num_nod_per_el1"))
+    c1[:] = np.array(hexes) + 1
+    c1.elem_type = b"HEX8"
+    c2 = f.createVariable("connect2", "i", ("num_el_in_blk2", "num_nod_per_el2"))
+    c2[:] = np.array(tets) + 1
+    c2.elem_type = b"TETRA"
+    ids = sorted(sides)
+    p = f.createVariable("ss_prop1", "i", ("num_side_sets",))
+    p[:] = ids
+    for k, sid in enumerate(ids, 1):
+        f.createDimension(f"num_side_ss{k}", len(sides[sid]))
+        e = f.createVariable(f"elem_ss{k}", "i", (f"num_side_ss{k}",))
+        s = f.createVariable(f"side_ss{k}", "i", (f"num_side_ss{k}",))
+        e[:] = [a for a, _ in sides[sid]]
+        s[:] = [b for _, b in sides[sid]]
+    f.close()
+    return xyz, elems, sides
+
+
+def test_exodus_3d_blocks_and_side_numbers(tmp_path):
+    """A TET4 + HEX8 Exodus mesh written by scipy: nodes, connectivity and
+    blocks come through, and every Exodus side number lands on the libMesh
+    side with the same nodes."""
+    mesh = tmp_path / "cube.e"
+    xyz, elems, sides = write_exodus_3d(str(mesh))
+    inp = open(os.path.join(INPUTS, "c1_hex8_reflective.i")).read()
+    mb = inp[inp.index("[Mesh]"):inp.index("[]\n\n", inp.index("[Mesh]")) + 3]
+    inp = inp.replace(mb, f"[Mesh]\n  [file]\n    type = FileMeshGenerator\n    file = '{mesh}'\n  []\n[]\n")
+    inp = inp.replace('blocks = "0"', 'blocks = "7 9"').replace('materials = "1"', 'materials = "1 1"')
+    inp = inp.replace("boundary = 'back front top right left bottom'", "boundary = '11 12 13 14 15 16'")
+    (tmp_path / "cube.i").write_text(inp)
+    d = tmp_path / "cube.dump"
+    r = run(["-i", str(tmp_path / "cube.i"), "MonteCarlo/particles=200", "--dump-problem", str(d)])
+    assert r.returncode == 0, r.stderr + r.stdout
+    p = P.load_dump(str(d))
+    m = p.mesh
+    assert np.array_equal(m.xyz, xyz) and m.n_elem == len(elems)
+    assert np.array_equal(m.subdomain, [7] * 4 + [9] * 24)
+    off, fb = m.conn_offsets, m.face_boundary
+    slot = np.concatenate([[0], np.cumsum([6 if t == "HEX8" else 4 for t, _ in elems])])
+    for (t, e), c0 in zip(elems, off[:-1]):
+        assert list(m.conn[c0:c0 + len(e)]) == list(e)
+    seen = 0
+    for sid, lst in sides.items():
+        for gi, s in lst:
+            t, e = elems[gi - 1]
+            nodes = {e[q - 1] for q in EXO_FACES[t][s - 1]}
+            lm = [k for k, f in enumerate(LM_FACES[t]) if {e[q] for q in f} == nodes]
+            assert len(lm) == 1
+            assert fb[slot[gi - 1] + lm[0]] == sid
+            seen += 1
+    assert seen == 12 + 24 and np.count_nonzero(fb >= 0) == seen  # 12 quads, 24 triangles on the surface
+    o = __import__("oracle.pyoracle", fromlist=["Oracle"]).Oracle(p)
+    st = o.run_batch(1, 200, threads=2)
+    assert st["lost"] == 0 and st["leaks"] == 0
+
+
+@pytest.mark.parametrize("content,message", [
+    (b"\x89HDF\r\n\x1a\n" + b"\0" * 64, "NetCDF-4 (HDF5)"),
+    (b"not a mesh at all", "not a NetCDF file"),
+    (b"CDF\x02\0\0\0\0\0\0\0\x0a\0\0\0\x05", "truncated NetCDF header"),
+])
+def test_exodus_reader_errors(tmp_path, content, message):
+    bad = tmp_path / "bad.e"
+    bad.write_bytes(content)
+    inp = open(os.path.join(INPUTS, "beavrs_assembly_3d.i")).read().replace("'31enr_16_in.e'", f"'{bad}'")
+    (tmp_path / "bad.i").write_text(inp)
+    r = run(["-i", str(tmp_path / "bad.i"), "--check-input"])
+    assert r.returncode == 1 and message in r.stderr, r.stderr
