@@ -292,13 +292,13 @@ static void extrude_mesh(const InputParameters &p, MeshData &m)
   for (size_t i = 0; i < h.size(); ++i)
     for (long long k = 1; k <= nl[i]; ++k) off.push_back(off.back() + h[i] / (double)nl[i]);
   const int64_t L = (int64_t)off.size() - 1, n2 = m.n_nodes(), e2 = m.n_elem();
+  if ((L + 1) * n2 > INT32_MAX) paramError(p, "num_layers", "too many nodes after extrusion");
   MeshData o;
   o.dim = 3;
   o.boundary_names = m.boundary_names;
   for (int64_t j = 0; j <= L; ++j)
     for (int64_t i = 0; i < n2; ++i)
       for (int d = 0; d < 3; ++d) o.xyz.push_back(m.xyz[3 * i + d] + dir[d] * off[j]);
-  if ((L + 1) * n2 > INT32_MAX) paramError(p, "num_layers", "too many nodes after extrusion");
   for (int64_t j = 0; j < L; ++j)
     for (int64_t e = 0; e < e2; ++e) {
       const int32_t *c = &m.conn[m.conn_off[e]];
