@@ -112,6 +112,14 @@ __device__ void flush_counts(Counters *c, const WaveCount &l, double track)
 
 __device__ __forceinline__ int wave_count(bool p) { return __popcll(__ballot(p)); }
 
+// A value every lane of the wavefront holds alike, kept in scalar registers.
+__device__ __forceinline__ int64_t wave_uniform(int64_t v)
+{
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Workgroup-aggregated append to two queues at once (256 threads, 4
 // wavefronts; every thread must call it): one atomic per queue per
 // workgroup instead of one per wavefront.  Appends from every wavefront of
@@ -1065,7 +1073,11 @@ __device__ __forceinline__ void req_account(const Params &P, bool dying, const P
 }
 
 template <int NS, bool REC, int WPE, bool P32, bool SERVE = false, bool EIG = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1, 8)))
+// Mixed-mesh variants (NS = 0) of the fixed-source and serving kernels are
+// held to 128 VGPRs, four waves per SIMD, which they measured faster at than
+// at three (C5: 150 against 167 ms per batch); WPE > 0 asks for more.
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : (NS == 0 && !EIG ? 4 : 1), 8)))
 k_persistent(Params P, SourceBank S)
 {
   __shared__ int32_t s_hash[MAX_HASH + 1];
@@ -1084,6 +1096,13 @@ k_persistent(Params P, SourceBank S)
   const int lanes_min = P.lanes_min < 1 ? 1 : (P.lanes_min > 64 ? 64 : P.lanes_min);
   int64_t fb_next = 0, fb_end = 0;     // eigenvalue: the wavefront's chunk of bank slots
   int64_t h_next = 0, h_end = 0;       // the wavefront's chunk of history slots
+  int64_t h_seen = 0;                  // start of the last chunk taken
+  // with tail_refill, within one chunk per wavefront of the end, take
+  // exactly what the idle lanes need: a chunk's unstarted histories wait for
+  // its own wavefront's lanes, which stretches the tail of a launch of long
+  // histories (C5 mixed precision: 148 -> 128 ms), while on short ones the
+  // extra atomics cost more than they save (C2: +1.7%)
+  const int64_t tail_zone = P.tail_refill ? 64 * ((int64_t)gridDim.x * blockDim.x / 64) : 0;
   int32_t fb_written = 0;              // eigenvalue: this lane's sites written
   int64_t fb_req = 0;                  // eigenvalue: and asked for
   for (;;) {
@@ -1102,11 +1121,13 @@ k_persistent(Params P, SourceBank S)
         const int64_t avail = h_end - h_next;
         int64_t nb = 0, ne = 0;
         if (avail < cnt) {
-          const int64_t want = (cnt - avail) > 64 ? (cnt - avail) : 64;
+          const int64_t short_by = cnt - avail;
+          const int64_t want = short_by < 64 && h_seen + tail_zone < n_hist ? 64 : short_by;
           unsigned long long b0 = 0;
           if (lane == 0) b0 = atomicAdd(&C->next_hist, (unsigned long long)want);
           nb = (int64_t)__shfl((long long)b0, 0, 64);
           ne = nb + want;
+          h_seen = nb;
         }
         if (need) slot = rank < avail ? h_next + rank : nb + (rank - avail);
         if (avail < cnt) {
@@ -1115,6 +1136,9 @@ k_persistent(Params P, SourceBank S)
         } else {
           h_next += cnt;
         }
+        h_next = wave_uniform(h_next);
+        h_end = wave_uniform(h_end);
+        h_seen = wave_uniform(h_seen);
       }
     }
     if (need) {
