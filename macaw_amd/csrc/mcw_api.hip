@@ -750,7 +750,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P.master_seed = c->opt.master_seed ? c->opt.master_seed : 1;
   P.roundtrip = c->opt.seed_roundtrip;
   P.lanes_min = c->opt.lanes_min > 0 ? std::min(c->opt.lanes_min, 64) : 40;
-  P.tail_refill = m.uniform_sides == 0 ? 1 : 0; // mixed meshes: graded, long histories
+  P.exact_refill = m.uniform_sides == 0 ? 1 : 0; // mixed meshes: graded, long histories
   P.max_seg = c->opt.max_segments > 0 ? c->opt.max_segments : ((int64_t)1 << 20);
   P.cnt = c->counters.as<Counters>();
   HIPCHK(hipStreamSynchronize(s));

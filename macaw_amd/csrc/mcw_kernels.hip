@@ -1096,13 +1096,12 @@ k_persistent(Params P, SourceBank S)
   const int lanes_min = P.lanes_min < 1 ? 1 : (P.lanes_min > 64 ? 64 : P.lanes_min);
   int64_t fb_next = 0, fb_end = 0;     // eigenvalue: the wavefront's chunk of bank slots
   int64_t h_next = 0, h_end = 0;       // the wavefront's chunk of history slots
-  int64_t h_seen = 0;                  // start of the last chunk taken
-  // with tail_refill, within one chunk per wavefront of the end, take
-  // exactly what the idle lanes need: a chunk's unstarted histories wait for
-  // its own wavefront's lanes, which stretches the tail of a launch of long
-  // histories (C5 mixed precision: 148 -> 128 ms), while on short ones the
-  // extra atomics cost more than they save (C2: +1.7%)
-  const int64_t tail_zone = P.tail_refill ? 64 * ((int64_t)gridDim.x * blockDim.x / 64) : 0;
+  // chunks of 64 save atomics on the shared counter when histories are short
+  // (C2: 52.8 -> 52.5 ms); with exact_refill a wavefront takes exactly what
+  // its idle lanes need, since a chunk's unstarted histories wait for their
+  // own wavefront's lanes and stretch the end of a launch of long histories
+  // (C5: FP64 167 -> 157 ms, mixed precision 148 -> 126 ms)
+  const bool chunked = !P.exact_refill;
   int32_t fb_written = 0;              // eigenvalue: this lane's sites written
   int64_t fb_req = 0;                  // eigenvalue: and asked for
   for (;;) {
@@ -1122,12 +1121,11 @@ k_persistent(Params P, SourceBank S)
         int64_t nb = 0, ne = 0;
         if (avail < cnt) {
           const int64_t short_by = cnt - avail;
-          const int64_t want = short_by < 64 && h_seen + tail_zone < n_hist ? 64 : short_by;
+          const int64_t want = chunked && short_by < 64 ? 64 : short_by;
           unsigned long long b0 = 0;
           if (lane == 0) b0 = atomicAdd(&C->next_hist, (unsigned long long)want);
           nb = (int64_t)__shfl((long long)b0, 0, 64);
           ne = nb + want;
-          h_seen = nb;
         }
         if (need) slot = rank < avail ? h_next + rank : nb + (rank - avail);
         if (avail < cnt) {
@@ -1138,7 +1136,6 @@ k_persistent(Params P, SourceBank S)
         }
         h_next = wave_uniform(h_next);
         h_end = wave_uniform(h_end);
-        h_seen = wave_uniform(h_seen);
       }
     }
     if (need) {

@@ -243,7 +243,7 @@ struct Params {
   int32_t lanes_min;       // persistent kernel: keep stepping segments while this many lanes are in flight
   int32_t tl_fast;         // the one track-length tally is a plain flux per element bin (no energy filter)
   int32_t eigen;           // eigenvalue batch: keff scores and fission sites (fb)
-  int32_t tail_refill;     // persistent kernel: exact refills near the end of a launch (mixed meshes)
+  int32_t exact_refill;    // persistent kernel: refills take exactly the idle lanes' count (no chunks)
   int32_t pad_tail;
   int64_t batch;
   int64_t n_particles;     // particles in the batch (seed id stride)
