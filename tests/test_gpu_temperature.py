@@ -150,3 +150,24 @@ def test_temperature_errors():
         g.add_temperature(0, dataclasses.replace(fis, temperature=1500.0))
     g.set_temperatures(p.elem_temperature, tolerance=200.0)
     g.run_batch(1, 500)
+
+
+def test_temperature_random_field_many_device_materials():
+    """Random element temperatures over the four tables of every nuclide:
+    each material appears at each of the four table choices, sixteen device
+    materials scattered element by element; plain, batched and (material,
+    element)-sorted schedules all equal the oracle."""
+    from oracle.pyoracle import Oracle
+    p = tet_temps(n_lattice=10, n_particles=3000)
+    rng = np.random.default_rng(77)
+    p = dataclasses.replace(p, elem_temperature=rng.uniform(250.0, 1300.0, p.mesh.n_elem))
+    o = Oracle(p)
+    o.run_batch(1, p.n_particles, threads=8, record=True)
+    ho = o.histories(p.n_particles)
+    for opt in ({"mode": abi.MODE_PLAIN}, {"mode": abi.MODE_BATCHED}, {"mode": abi.MODE_BATCHED, "sort_cadence": 1}):
+        print(f"[temperature] random field {opt}", flush=True)
+        g = Context(p, record_histories=1, **opt)
+        g.run_batch(1, p.n_particles)
+        assert_records_equal(g.histories(p.n_particles), ho)
+        for t in p.tallies:
+            assert_tally_close(g.tally(t.id), o.tally(t.id))
