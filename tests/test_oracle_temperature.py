@@ -156,3 +156,15 @@ def test_temperature_out_of_tolerance_is_an_error():
     # inside the tolerance of a table everywhere: fine
     T = np.where(np.arange(p.mesh.n_elem) % 2 == 0, 595.0, 1205.0)
     run(dataclasses.replace(p, elem_temperature=T), n=100)
+
+
+def test_infinite_tolerance_reads_the_nearest_table_however_far():
+    # OpenMC at run time: no tolerance check, the nearest table (1200 K) for 5000 K
+    p = P.with_temperatures(small_c2(n_particles=600), tolerance=np.inf)
+    hot = dataclasses.replace(p, elem_temperature=np.full(p.mesh.n_elem, 5000.0))
+    top = dataclasses.replace(p, nuclides=[p.nuclide_temps[q][-1] for q in range(4)], nuclide_temps={},
+                              elem_temperature=None)
+    assert all(t.temperature == 1200.0 for t in top.nuclides)
+    assert_same_histories(run(hot)[1], run(top)[1])
+    with pytest.raises(OracleError, match="no cross sections"):
+        Oracle(dataclasses.replace(hot, temperature_tolerance=1000.0)).run_batch(1, 10, threads=1)
