@@ -744,6 +744,11 @@ void prepare(mcw_ctx *c, bool need_source = true)
     V.value = t.value.as<double>();
   }
   P.tl_fast = (n_tl == 1 && P.tally[0].simple_flux && P.tally[0].n_edges == 0) ? 1 : 0;
+  // up to 1024 bins (8 KB) beside the 32 KB hash table keep four 256-thread
+  // workgroups per CU
+  P.lds_tally = 0;
+  if (P.tl_fast && order[0]->n_filter_bins * order[0]->n_score_bins <= 1024)
+    P.lds_tally = (int32_t)(order[0]->n_filter_bins * order[0]->n_score_bins);
   P.trace = nullptr;
   P.trace_n = 0;
   P.trace_cap = 0;
@@ -947,10 +952,12 @@ SourceBank sort_sites(mcw_ctx *c, const Params &P, const SourceBank &S, int64_t 
 }
 
 // Resident workgroups of the persistent kernel over the whole device.
-int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, bool p32, int block, bool eig = false)
+int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, bool p32, int block, bool eig = false,
+                      size_t lds_bytes = 0)
 {
   int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns, wpe, rec, p32, eig), block, 0));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns, wpe, rec, p32, eig), block,
+                                                      lds_bytes));
   if (per_cu < 1) per_cu = 1;
   return c->n_cu * per_cu;
 }
@@ -1511,6 +1518,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
         P.n_tl = 0;
         P.n_coll = 0;
         P.tl_fast = 0;
+        P.lds_tally = 0;
       }
     }
     P.record = c->opt.record_histories ? 1 : 0;
@@ -1567,7 +1575,8 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
         const int wpe = c->opt.waves_per_eu == 5 ? 5 : 0;
         const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns != 6;
         require(block % 64 == 0 && block <= 256, "block_size must be 64, 128, 192 or 256");
-        int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, p32, block, eig),
+        int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, p32, block, eig,
+                                                             (size_t)P.lds_tally * sizeof(double)),
                                            (h1 - h0 + block - 1) / block);
         if (blocks < 1) blocks = 1;
         // the work counter restarts for every chunk
@@ -2036,6 +2045,7 @@ bool launch_round(mcw_server *s, int mi, RoundSlot &R)
   Params P = c->base;
   P.n_tallies = P.n_tl = P.n_coll = 0;
   P.tl_fast = 0;
+  P.lds_tally = 0;
   P.record = 0;
   P.batch = 1;
   P.hist_begin = 0;

@@ -1081,6 +1081,12 @@ __global__ void __launch_bounds__(256)
 k_persistent(Params P, SourceBank S)
 {
   __shared__ int32_t s_hash[MAX_HASH + 1];
+  // a small per-element flux tally (a few hundred elements) is summed per
+  // workgroup in LDS and added to the device tally once at the end: every
+  // segment's atomic on one of a few hundred lines serialised in the caches
+  extern __shared__ double s_tally[]; // P.lds_tally doubles
+  const int32_t n_lds = SERVE ? 0 : P.lds_tally;
+  for (int i = threadIdx.x; i < n_lds; i += blockDim.x) s_tally[i] = 0.0;
   stage_hash(P.xs, s_hash);
   Counters *C = P.cnt;
   WaveCount wc;
@@ -1170,7 +1176,11 @@ k_persistent(Params P, SourceBank S)
           phase = PH_IDLE;
         }
       }
-      if (SERVE || P.tl_fast) atomic_add_agg(tl_ptr != nullptr, tl_ptr, tl_val);
+      if (!SERVE && n_lds) {
+        if (tl_ptr) atomicAdd(s_tally + (tl_ptr - P.tally[0].value), tl_val);
+      } else if (SERVE || P.tl_fast) {
+        atomic_add_agg(tl_ptr != nullptr, tl_ptr, tl_val);
+      }
       if (SERVE) req_account(P, go && is_death(ev), st, racc);
       count_segment(wc, go, ev, hx);
     } while (wave_count(phase == PH_TRANSPORT) >= lanes_min);
@@ -1216,6 +1226,11 @@ k_persistent(Params P, SourceBank S)
   if (P32) {
     const unsigned long long r = wave_sum(rechecks);
     if (__lane_id() == 0 && r) atomicAdd(&C->rechecks, r);
+  }
+  if (!SERVE && n_lds) { // every wave of the workgroup gets here once
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_lds; i += blockDim.x)
+      if (s_tally[i] != 0.0) atomic_add_dev(P.tally[0].value + i, s_tally[i]);
   }
 }
 
@@ -1405,7 +1420,7 @@ hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int blo
 {
   const void *k = persistent_kernel(ns, wpe, P.record != 0, p32, P.eigen != 0);
   void *args[] = {(void *)&P, (void *)&S};
-  return hipLaunchKernel(k, dim3(blocks), dim3(block), args, 0, s);
+  return hipLaunchKernel(k, dim3(blocks), dim3(block), args, (size_t)P.lds_tally * sizeof(double), s);
 }
 
 hipError_t launch_source_req(hipStream_t s, const Params &P, const SourceBank &S, int32_t n_slices, int64_t n)
