@@ -747,8 +747,14 @@ void prepare(mcw_ctx *c, bool need_source = true)
   // up to 1024 bins (8 KB) beside the 32 KB hash table keep four 256-thread
   // workgroups per CU
   P.lds_tally = 0;
+  P.lds_coll = 0;
   if (P.tl_fast && order[0]->n_filter_bins * order[0]->n_score_bins <= 1024)
     P.lds_tally = (int32_t)(order[0]->n_filter_bins * order[0]->n_score_bins);
+  if (P.n_coll == 1) {
+    const TallyView &T = P.tally[n_tl];
+    const int64_t bins = order[n_tl]->n_filter_bins * order[n_tl]->n_score_bins;
+    if (T.simple_flux && T.n_edges == 0 && P.lds_tally + bins <= 1024) P.lds_coll = (int32_t)bins;
+  }
   P.trace = nullptr;
   P.trace_n = 0;
   P.trace_cap = 0;
@@ -1107,6 +1113,9 @@ void eigen_close(mcw_ctx *c, const Params &P, int64_t batch, int64_t n, bool act
 // groups.
 void run_plain(mcw_ctx *c, Params &P, const SourceBank &S, int64_t n)
 {
+  // the event kernels have no dynamic LDS: tallies go straight to the device
+  P.lds_tally = 0;
+  P.lds_coll = 0;
   hipStream_t s = c->stream;
   const int ns = c->mesh.uniform_sides;
   const int block = 256;
@@ -1519,6 +1528,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
         P.n_coll = 0;
         P.tl_fast = 0;
         P.lds_tally = 0;
+        P.lds_coll = 0;
       }
     }
     P.record = c->opt.record_histories ? 1 : 0;
@@ -1576,7 +1586,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
         const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns != 6;
         require(block % 64 == 0 && block <= 256, "block_size must be 64, 128, 192 or 256");
         int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, p32, block, eig,
-                                                             (size_t)P.lds_tally * sizeof(double)),
+                                                             (size_t)(P.lds_tally + P.lds_coll) * sizeof(double)),
                                            (h1 - h0 + block - 1) / block);
         if (blocks < 1) blocks = 1;
         // the work counter restarts for every chunk
@@ -2046,6 +2056,7 @@ bool launch_round(mcw_server *s, int mi, RoundSlot &R)
   P.n_tallies = P.n_tl = P.n_coll = 0;
   P.tl_fast = 0;
   P.lds_tally = 0;
+  P.lds_coll = 0;
   P.record = 0;
   P.batch = 1;
   P.hist_begin = 0;

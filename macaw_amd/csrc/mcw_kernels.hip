@@ -569,6 +569,10 @@ __device__ __forceinline__ int segment(const Params &P, PState &st, double &trac
 //     the keff absorption estimator) or scatter elastically (isotropic in CM,
 //     target at rest) with an energy cutoff at the grid minimum.  A scatter
 //     looks up the new energy's union-grid index through `hash`.
+// Dynamic LDS of the persistent kernel: small flux tallies summed per
+// workgroup (P.lds_tally track-length bins, then P.lds_coll collision bins).
+extern __shared__ double s_tally[];
+
 struct CollPick {
   int32_t nuc;
   double t, a, nuf; // picked nuclide's total, absorption and nu-fission at E
@@ -580,8 +584,14 @@ __device__ __forceinline__ int64_t collide_pick(const Params &P, PState &st, Col
   const XSView &X = P.xs;
   const int32_t mat = st.mat;
   if (REC) ++st.ncoll;
-  for (int t = P.n_tl; t < P.n_tl + P.n_coll; ++t)
-    tally_score(P.tally[t], X, st.elem, st.E, st.k, 1.0 / st.sigt, mat);
+  if (P.lds_coll) { // the persistent kernel's LDS copy of a small collision flux tally
+    const TallyView &T = P.tally[P.n_tl];
+    const int64_t fi = T.elem_base ? T.elem_base[st.elem] : st.elem;
+    if (fi >= 0) atomicAdd(s_tally + P.lds_tally + fi, 1.0 / st.sigt);
+  } else {
+    for (int t = P.n_tl; t < P.n_tl + P.n_coll; ++t)
+      tally_score(P.tally[t], X, st.elem, st.E, st.k, 1.0 / st.sigt, mat);
+  }
   if (EIG) st.kcol = st.kcol + st.nusf / st.sigt;
   const int32_t b = X.mat_off[mat], e = X.mat_off[mat + 1];
   const double cutoff = prn(st.seed) * st.sigt;
@@ -1084,8 +1094,7 @@ k_persistent(Params P, SourceBank S)
   // a small per-element flux tally (a few hundred elements) is summed per
   // workgroup in LDS and added to the device tally once at the end: every
   // segment's atomic on one of a few hundred lines serialised in the caches
-  extern __shared__ double s_tally[]; // P.lds_tally doubles
-  const int32_t n_lds = SERVE ? 0 : P.lds_tally;
+  const int32_t n_lds = SERVE ? 0 : P.lds_tally + P.lds_coll;
   for (int i = threadIdx.x; i < n_lds; i += blockDim.x) s_tally[i] = 0.0;
   stage_hash(P.xs, s_hash);
   Counters *C = P.cnt;
@@ -1230,7 +1239,10 @@ k_persistent(Params P, SourceBank S)
   if (!SERVE && n_lds) { // every wave of the workgroup gets here once
     __syncthreads();
     for (int i = threadIdx.x; i < n_lds; i += blockDim.x)
-      if (s_tally[i] != 0.0) atomic_add_dev(P.tally[0].value + i, s_tally[i]);
+      if (s_tally[i] != 0.0) {
+        double *dst = i < P.lds_tally ? P.tally[0].value + i : P.tally[P.n_tl].value + (i - P.lds_tally);
+        atomic_add_dev(dst, s_tally[i]);
+      }
   }
 }
 
@@ -1420,7 +1432,7 @@ hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int blo
 {
   const void *k = persistent_kernel(ns, wpe, P.record != 0, p32, P.eigen != 0);
   void *args[] = {(void *)&P, (void *)&S};
-  return hipLaunchKernel(k, dim3(blocks), dim3(block), args, (size_t)P.lds_tally * sizeof(double), s);
+  return hipLaunchKernel(k, dim3(blocks), dim3(block), args, (size_t)(P.lds_tally + P.lds_coll) * sizeof(double), s);
 }
 
 hipError_t launch_source_req(hipStream_t s, const Params &P, const SourceBank &S, int32_t n_slices, int64_t n)

@@ -245,6 +245,8 @@ struct Params {
   int32_t eigen;           // eigenvalue batch: keff scores and fission sites (fb)
   int32_t exact_refill;    // persistent kernel: refills take exactly the idle lanes' count (no chunks)
   int32_t lds_tally;       // persistent kernel: bins of the tl_fast tally summed in LDS first (0: off)
+  int32_t lds_coll;        // and of the one collision tally, a plain flux, after them (0: off)
+  int32_t pad_lds;
   int64_t batch;
   int64_t n_particles;     // particles in the batch (seed id stride)
   int64_t hist_begin;      // first history index handled by this launch
