@@ -19,9 +19,13 @@
  *     {0,2,1},{0,1,3},{1,2,3},{2,0,3}; Hex8: sides {0,3,2,1},{0,1,5,4},
  *     {1,2,6,5},{2,3,7,6},{3,0,4,7},{4,5,6,7}); exit-face ties go to the
  *     lowest side index;
- *   - material indices follow CollisionKernel: block -> material *id*, and
- *     the material used is the one whose id matches
- *     (src/raykernels/CollisionKernel.C:63-69,110);
+ *   - blocks map to material *ids* (CollisionKernel `blocks`/`materials`,
+ *     src/raykernels/CollisionKernel.C:63-69), and every mapped id must exist
+ *     (the check at CollisionKernel.C:85-90).  By default the material used
+ *     is the one with that id.  MaCaw itself uses material *index* id - 1
+ *     into OpenMC's materials list (CollisionKernel.C:110), which is the
+ *     same material only when ids are 1..N in list order;
+ *     options.material_index = MCW_MATERIAL_BY_INDEX reproduces that;
  *   - random numbers follow OpenMC 0.13: 64-bit LCG with PCG-RXS-M-XS output,
  *     per-particle streams (0 tracking, 1 source, 2 URR), particle seed
  *     (batch-1)*n_particles + id, stride 152917;
@@ -81,6 +85,13 @@ enum { MCW_PREC_FP64 = 0, MCW_PREC_MIXED = 1 };
  * estimator; tallies score in active batches only. */
 enum { MCW_RUN_FIXED_SOURCE = 0, MCW_RUN_EIGENVALUE = 1 };
 
+/* How a block's material id selects a material (mcw_options.material_index).
+ * BY_ID: the material whose id matches.  BY_INDEX: MaCaw's
+ * `_block_to_openmc_materials.at(subdomain) - 1` (CollisionKernel.C:110),
+ * the (id-1)-th material of mcw_materials_set's list; an id outside
+ * 1..n_materials is an error at the next batch. */
+enum { MCW_MATERIAL_BY_ID = 0, MCW_MATERIAL_BY_INDEX = 1 };
+
 /* Per-history fates. */
 enum { MCW_FATE_ALIVE = 0, MCW_FATE_LEAKED = 1, MCW_FATE_ABSORBED = 2, MCW_FATE_CUTOFF = 3,
        MCW_FATE_LOST = 4 };
@@ -117,7 +128,7 @@ typedef struct {
   int32_t run_mode;         /* MCW_RUN_* (EIGENVALUE needs mode BATCHED) */
   int32_t n_inactive;       /* EIGENVALUE: batches 1..n_inactive are inactive (settings
                                <inactive>): no tally scores, keff not averaged */
-  int32_t reserved;
+  int32_t material_index;   /* MCW_MATERIAL_* (default BY_ID) */
 } mcw_options;
 
 /* One synthetic continuous-energy nuclide: pointwise lin-lin tables on an

@@ -18,6 +18,7 @@ MODE_PLAIN, MODE_BATCHED = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 FATE_ALIVE, FATE_LEAKED, FATE_ABSORBED, FATE_CUTOFF, FATE_LOST = 0, 1, 2, 3, 4
 RUN_FIXED_SOURCE, RUN_EIGENVALUE = 0, 1
+MATERIAL_BY_ID, MATERIAL_BY_INDEX = 0, 1
 K_BOLTZMANN = 8.617333262e-5  # eV/K, as OpenMC
 
 SCORE_NAMES = {"flux": SCORE_FLUX, "total": SCORE_TOTAL, "absorption": SCORE_ABSORPTION,
@@ -39,7 +40,7 @@ class Options(C.Structure):
                 ("max_segments", C.c_int64), ("master_seed", C.c_uint64),
                 ("sort_cadence", C.c_int32), ("lanes_min", C.c_int32),
                 ("waves_per_eu", C.c_int32), ("use_graph", C.c_int32), ("trace_histories", C.c_int32),
-                ("run_mode", C.c_int32), ("n_inactive", C.c_int32), ("reserved", C.c_int32)]
+                ("run_mode", C.c_int32), ("n_inactive", C.c_int32), ("material_index", C.c_int32)]
 
 
 class Nuclide(C.Structure):

@@ -36,18 +36,22 @@ def _run(cmd, cwd=None):
 
 
 def build_library(verbose=False):
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(os.path.join(LIBDIR, "obj"), exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in sources():
         obj = os.path.join(LIBDIR, "obj", os.path.basename(src) + ".o")
         cmd = [HIPCC] + COMMON + ["-I", os.path.join(ROOT, "include")]
         if src.endswith(".hip"):
             cmd += DEVICE + ["-x", "hip"]
         cmd += ["-c", src, "-o", obj]
-        out = _run(cmd)
-        if verbose and out:
-            print(out)
+        cmds.append(cmd)
         objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out:
+                print(out)
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, LIB)

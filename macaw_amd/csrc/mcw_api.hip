@@ -147,6 +147,7 @@ struct HostTally {
   // device
   DevBuf value, sum, sum_sq, elem_base, edges, d_scores, d_nuclides;
   bool identity = false;
+  int64_t max_base = -1; // largest spatial bin base an element scores into (-1: none)
   int32_t n_edges = 0;
   int64_t energy_stride = 0;
 };
@@ -325,15 +326,26 @@ void prepare(mcw_ctx *c, bool need_source = true)
   }
   c->n_bc = max_bid + 1;
 
-  // materials by id (CollisionKernel: block -> OpenMC material id)
+  // block -> OpenMC material id; every id must exist (CollisionKernel.C:85-90).
+  // BY_ID uses the material with that id; BY_INDEX the (id-1)-th material of
+  // the list, as MaCaw's `material() = id - 1` (CollisionKernel.C:110)
   std::map<int32_t, int32_t> id_to_idx;
   for (size_t i = 0; i < c->mats.size(); ++i) id_to_idx[c->mats[i].id] = (int32_t)i;
-  for (auto &bm : c->block_mat)
+  const bool by_index = c->opt.material_index == MCW_MATERIAL_BY_INDEX;
+  require(c->opt.material_index == MCW_MATERIAL_BY_ID || by_index, "unknown material_index option");
+  std::map<int32_t, int32_t> used; // material id -> list position
+  for (auto &bm : c->block_mat) {
     require(id_to_idx.count(bm.second), "Could not find material " + std::to_string(bm.second) + " in OpenMC materials.");
+    if (by_index)
+      require(bm.second >= 1 && bm.second <= (int32_t)c->mats.size(),
+              "material index " + std::to_string(bm.second) + " - 1 is outside the " +
+                  std::to_string(c->mats.size()) + " materials");
+    used[bm.second] = by_index ? bm.second - 1 : id_to_idx[bm.second];
+  }
   std::vector<int32_t> emat(m.n_elem, -1);
   for (int64_t e = 0; e < m.n_elem; ++e) {
     auto b = c->block_mat.find(m.subdomain[e]);
-    if (b != c->block_mat.end()) emat[e] = id_to_idx[b->second];
+    if (b != c->block_mat.end()) emat[e] = used[b->second];
   }
   const int32_t n_nuc = (int32_t)c->nucs.size();
   for (auto &mt : c->mats)
@@ -755,6 +767,20 @@ void prepare(mcw_ctx *c, bool need_source = true)
     const int64_t bins = order[n_tl]->n_filter_bins * order[n_tl]->n_score_bins;
     if (T.simple_flux && T.n_edges == 0 && P.lds_tally + bins <= 1024) P.lds_coll = (int32_t)bins;
   }
+  // every LDS index a kernel branch can write lies inside that branch's own
+  // range: track-length bins [0, lds_tally), collision bins [lds_tally,
+  // lds_tally + lds_coll)
+  if (P.lds_tally) {
+    const HostTally &t = *order[0];
+    require(P.tl_fast && t.n_score_bins == 1 && t.max_base < P.lds_tally,
+            "internal: track-length LDS tally range does not cover its bins");
+  }
+  if (P.lds_coll) {
+    const HostTally &t = *order[n_tl];
+    require(P.n_coll == 1 && P.tally[n_tl].simple_flux && t.n_score_bins == 1 && t.max_base < P.lds_coll &&
+                P.lds_tally + P.lds_coll <= 1024,
+            "internal: collision LDS tally range does not cover its bins");
+  }
   P.trace = nullptr;
   P.trace_n = 0;
   P.trace_cap = 0;
@@ -846,7 +872,11 @@ void build_tally(mcw_ctx *c, HostTally &t)
   // device order: base of internal element i is that of caller element orig_of_int[i];
   // an identity cell tally is stored by internal id and permuted on read-out
   std::vector<int64_t> base_int(m.n_elem);
-  for (int64_t i = 0; i < m.n_elem; ++i) base_int[i] = base[m.orig_of_int[i]];
+  t.max_base = -1;
+  for (int64_t i = 0; i < m.n_elem; ++i) {
+    base_int[i] = base[m.orig_of_int[i]];
+    t.max_base = std::max(t.max_base, base_int[i]);
+  }
   t.elem_base.upload(base_int, c->stream);
   t.d_scores.upload(t.scores, c->stream);
   t.d_nuclides.upload(t.nuclides, c->stream);
@@ -1257,7 +1287,10 @@ int mcw_options_set(mcw_ctx *c, const mcw_options *o)
     require(o->trace_histories >= 0, "trace_histories must not be negative");
     require(o->run_mode == MCW_RUN_FIXED_SOURCE || o->run_mode == MCW_RUN_EIGENVALUE, "unknown run_mode");
     require(o->n_inactive >= 0, "n_inactive must not be negative");
-    const bool reprep = o->max_segments != c->opt.max_segments || o->lanes_min != c->opt.lanes_min;
+    require(o->material_index == MCW_MATERIAL_BY_ID || o->material_index == MCW_MATERIAL_BY_INDEX,
+            "unknown material_index option");
+    const bool reprep = o->max_segments != c->opt.max_segments || o->lanes_min != c->opt.lanes_min ||
+                        o->material_index != c->opt.material_index;
     c->opt = *o;
     if (c->opt.master_seed == 0) c->opt.master_seed = 1;
     if (reprep) c->dirty = true;

@@ -630,21 +630,25 @@ __device__ __forceinline__ void tally_score(const TallyView &T, const XSView &X,
   for (int ni = 0; ni < T.n_nuclides; ++ni) {
     const int nuc = T.nuclides[ni];
     Macro M = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (T.needs_xs) {
-      if (nuc < 0) {
-        macro_all_g(X, mat, k, E, M);
-      } else if (mat >= 0) {
-        for (int32_t q = X.mat_off[mat]; q < X.mat_off[mat + 1]; ++q)
-          if (X.nuc_of[X.mat_nuc[q]] == nuc) {
-            Micro mu;
-            micro_at(X, X.mat_nuc[q], k, E, mu);
-            double dn = X.mat_dens[q];
-            M.t = dn * mu.t;
-            M.a = dn * mu.a;
-            M.s = dn * mu.s;
-            M.f = dn * mu.f;
-            M.kf = dn * (X.kappa[X.mat_nuc[q]] * mu.f);
-          }
+    if (nuc < 0) {
+      if (T.needs_xs) macro_all_g(X, mat, k, E, M);
+    } else if (mat >= 0) {
+      // a nuclide bin scores nothing, flux included, in a material without
+      // that nuclide (OpenMC 0.13 score_tracklength_tally /
+      // score_collision_tally skip the bin)
+      int32_t q = X.mat_off[mat];
+      const int32_t qe = X.mat_off[mat + 1];
+      while (q < qe && X.nuc_of[X.mat_nuc[q]] != nuc) ++q;
+      if (q == qe) continue;
+      if (T.needs_xs) {
+        Micro mu;
+        micro_at(X, X.mat_nuc[q], k, E, mu);
+        const double dn = X.mat_dens[q];
+        M.t = dn * mu.t;
+        M.a = dn * mu.a;
+        M.s = dn * mu.s;
+        M.f = dn * mu.f;
+        M.kf = dn * (X.kappa[X.mat_nuc[q]] * mu.f);
       }
     }
     for (int si = 0; si < nsc; ++si) {

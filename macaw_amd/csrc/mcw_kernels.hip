@@ -1185,7 +1185,10 @@ k_persistent(Params P, SourceBank S)
           phase = PH_IDLE;
         }
       }
-      if (!SERVE && n_lds) {
+      // the track-length bins are in LDS only when P.lds_tally covers them;
+      // P.lds_coll alone (a large track-length tally beside a small
+      // collision one) keeps the track-length adds on the device
+      if (!SERVE && P.lds_tally) {
         if (tl_ptr) atomicAdd(s_tally + (tl_ptr - P.tally[0].value), tl_val);
       } else if (SERVE || P.tl_fast) {
         atomic_add_agg(tl_ptr != nullptr, tl_ptr, tl_val);

@@ -677,6 +677,14 @@ OpenMCTallyAux::OpenMCTallyAux(const InputParameters &p, Problem &fe) : _params(
   _nuclide = p.getString("nuclide", "all");
   _energy_bin = (int)p.getInt("energy_bin", -1);
   _variable = p.getString("variable");
+  // where (score, nuclide) sits among a tally's score bins: OpenMC's
+  // results_ layout (nuclide-major, the default), or the index MaCaw's aux
+  // computes, score_bin * n_nuclides + nuc_bin (OpenMCTallyAux.C:116-119);
+  // they differ when a tally has several scores and several nuclide bins
+  const std::string layout = p.getString("score_index_layout", "openmc");
+  if (layout != "openmc" && layout != "macaw")
+    paramError(p, "score_index_layout", "score_index_layout must be openmc or macaw");
+  _score_major = layout == "macaw";
 }
 
 double OpenMCTallyAux::computeValue(int64_t elem, const std::vector<double> &sum) const
@@ -699,8 +707,8 @@ double OpenMCTallyAux::computeValue(int64_t elem, const std::vector<double> &sum
   auto it = std::find(t->scores().begin(), t->scores().end(), _score);
   if (it == t->scores().end()) paramError(_params, "score", "The specified score does not exist in given tally");
   const int score_bin = (int)(it - t->scores().begin());
-  // OpenMC results_ layout: nuclide-major score bins
-  const int64_t score_index = (int64_t)nuc_bin * (int64_t)t->scores().size() + score_bin;
+  const int64_t score_index = _score_major ? (int64_t)score_bin * (int64_t)nb.size() + nuc_bin
+                                           : (int64_t)nuc_bin * (int64_t)t->scores().size() + score_bin;
   const int64_t nsb = t->nScoreBins();
 
   int64_t univ_end = 1, univ_stride = 1, energy_start = 0, energy_end = 1, energy_stride = 1, cell_stride = 1;
@@ -972,6 +980,12 @@ void Problem::read_monte_carlo(const HitNode &mcb)
   else if (run_mode == "fixed source" || run_mode == "fixed_source") options.run_mode = MCW_RUN_FIXED_SOURCE;
   else paramError(p, "run_mode", "run_mode must be eigenvalue or fixed source");
   options.n_inactive = (int32_t)p.getInt("inactive", 0);
+  // how a CollisionKernel material id selects a material: by id, or as
+  // MaCaw's `id - 1` index into the material list (CollisionKernel.C:110)
+  const std::string mi = p.getString("material_index", "id");
+  if (mi == "id") options.material_index = MCW_MATERIAL_BY_ID;
+  else if (mi == "index") options.material_index = MCW_MATERIAL_BY_INDEX;
+  else paramError(p, "material_index", "material_index must be id or index");
   // OpenMC settings temperature_tolerance; by default the nearest table is
   // read however far, as OpenMC does for the sqrt(kT) MaCaw sets per segment
   temperature_tolerance = p.getReal("temperature_tolerance", INFINITY);
@@ -1173,7 +1187,7 @@ void Problem::dump(const std::string &path) const
   std::ofstream f(path, std::ios::binary);
   auto w64 = [&](int64_t v) { f.write((const char *)&v, 8); };
   auto wd = [&](const void *p, size_t n) { f.write((const char *)p, n); };
-  f.write("MCWD0004", 8);
+  f.write("MCWD0005", 8);
   w64(mesh.n_nodes());
   wd(mesh.xyz.data(), mesh.xyz.size() * 8);
   w64(mesh.n_elem());
@@ -1235,6 +1249,7 @@ void Problem::dump(const std::string &path) const
   w64((int64_t)elem_temperature.size());
   wd(elem_temperature.data(), elem_temperature.size() * 8);
   wd(&temperature_tolerance, 8);
+  w64(options.material_index);
 }
 
 } // namespace mcwhost

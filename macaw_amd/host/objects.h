@@ -162,6 +162,7 @@ private:
   int _granularity; // 0 global, 1 universe, 2 cell
   std::string _score, _nuclide, _variable;
   int _energy_bin;
+  bool _score_major; // score_index_layout = macaw: score_bin * n_nuclides + nuc_bin (OpenMCTallyAux.C:116-119)
 };
 
 // MaCaw OpenMCStudy (src/userobjects/OpenMCStudy.C:49-402).

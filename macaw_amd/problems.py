@@ -714,11 +714,11 @@ def load_dump(path) -> Problem:
     """Problem written by `macaw-amd-opt --dump-problem` (the exact bytes the
     driver handed to the library), for replay on a CPU reference."""
     raw = open(path, "rb").read()
-    if raw[:8] not in (b"MCWD0001", b"MCWD0002", b"MCWD0003", b"MCWD0004"):
+    versions = (b"MCWD0001", b"MCWD0002", b"MCWD0003", b"MCWD0004", b"MCWD0005")
+    if raw[:8] not in versions:
         raise ValueError(f"{path}: not a problem dump")
-    v2 = raw[:8] in (b"MCWD0002", b"MCWD0003", b"MCWD0004")
-    v3 = raw[:8] in (b"MCWD0003", b"MCWD0004")
-    v4 = raw[:8] == b"MCWD0004"
+    ver = versions.index(raw[:8]) + 1
+    v2, v3, v4, v5 = ver >= 2, ver >= 3, ver >= 4, ver >= 5
     pos = [8]
 
     def i64():
@@ -786,6 +786,7 @@ def load_dump(path) -> Problem:
         k = i64()
         elem_T = arr("<f8", k) if k else None
         tol = float(arr("<f8", 1)[0])
+    material_index = i64() if v5 else abi.MATERIAL_BY_ID
     mesh = Mesh(xyz=np.ascontiguousarray(xyz), elem_type=etype, conn_offsets=off, conn=conn, subdomain=sub,
                 face_boundary=fb if nfb else None)
     p = Problem(name=os.path.basename(path), mesh=mesh, nuclides=nucs, materials=mats, block_material=bm,
@@ -795,4 +796,5 @@ def load_dump(path) -> Problem:
     p.master_seed = seed
     p.run_mode = run_mode
     p.n_inactive = n_inactive
+    p.material_index = material_index
     return p

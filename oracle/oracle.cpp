@@ -522,6 +522,23 @@ double macro_total(const Oracle &o, int mat, double E, double skT)
   return t;
 }
 
+// OpenMC sample_nuclide (physics.cpp): cutoff = xi Sigma_t, then the first
+// nuclide (material order) whose running sum of N_i sigma_t,i reaches it; the
+// last one when rounding leaves the sum short.  Returns the position in the
+// material's list.
+int sample_nuclide(const Oracle &o, const Material &m, double E, double skT, double sigt, uint64_t *seed)
+{
+  const double cutoff = orc_prn(seed) * sigt;
+  double prob = 0.0;
+  for (size_t q = 0; q < m.nuc.size(); ++q) {
+    Micro mu;
+    micro_xs(nuc_at(o, m.nuc[q], skT), E, &mu);
+    prob = prob + m.dens[q] * mu.t;
+    if (prob >= cutoff) return (int)q;
+  }
+  return (int)m.nuc.size() - 1;
+}
+
 // OpenMC rotate_angle with phi = 2 pi xi
 void rotate_angle(double *u, double *v, double *w, double mu, double xi)
 {
@@ -713,26 +730,27 @@ void score(const Oracle &o, const Tally &t, std::vector<double> &val, int64_t el
   const int nsc = (int)t.scores.size();
   for (size_t ni = 0; ni < t.nuclides.size(); ++ni) {
     int nuc = t.nuclides[ni];
-    double dens = 0.0;
-    Micro mu = {0, 0, 0, 0};
     Macro Ma = {0, 0, 0, 0, 0};
     bool need_xs = false;
     for (int sc : t.scores) need_xs |= sc != MCW_SCORE_FLUX;
-    if (need_xs) {
-      if (nuc < 0) {
-        macro_xs(o, mat, E, skT, &Ma);
-      } else if (mat >= 0) {
-        const Material &m = o.mats[mat];
-        for (size_t q = 0; q < m.nuc.size(); ++q)
-          if (m.nuc[q] == nuc) {
-            dens = m.dens[q];
-            micro_xs(nuc_at(o, nuc, skT), E, &mu);
-            Ma.t = dens * mu.t;
-            Ma.a = dens * mu.a;
-            Ma.s = dens * mu.s;
-            Ma.f = dens * mu.f;
-            Ma.kf = dens * (o.nucs[nuc].kappa * mu.f);
-          }
+    if (nuc < 0) {
+      if (need_xs) macro_xs(o, mat, E, skT, &Ma);
+    } else if (mat >= 0) {
+      // OpenMC 0.13 score_tracklength_tally / score_collision_tally: a
+      // nuclide the material does not hold scores nothing in its bin
+      const Material &m = o.mats[mat];
+      size_t q = 0;
+      while (q < m.nuc.size() && m.nuc[q] != nuc) ++q;
+      if (q == m.nuc.size()) continue;
+      if (need_xs) {
+        Micro mu = {0, 0, 0, 0};
+        const double dens = m.dens[q];
+        micro_xs(nuc_at(o, nuc, skT), E, &mu);
+        Ma.t = dens * mu.t;
+        Ma.a = dens * mu.a;
+        Ma.s = dens * mu.s;
+        Ma.f = dens * mu.f;
+        Ma.kf = dens * (o.nucs[nuc].kappa * mu.f);
       }
     }
     for (int si = 0; si < nsc; ++si) {
@@ -936,20 +954,9 @@ void transport_history(const Oracle &o, int64_t batch, int64_t n_particles, int6
           if (T[k].estimator == MCW_EST_COLLISION) score(o, T[k], sc.value[k], elem, E, wgt / sigt, mat, skT);
       // keff collision estimator (OpenMC Particle::event_collide)
       if (eg) eg->k_col = eg->k_col + (wgt * nusf) / sigt;
-      // sample nuclide
       const Material &m = o.mats[mat];
-      double cutoff = orc_prn(&seed) * sigt;
-      double prob = 0.0;
-      int pick = (int)m.nuc.size() - 1;
+      const int pick = sample_nuclide(o, m, E, skT, sigt, &seed);
       Micro mu;
-      for (size_t q = 0; q < m.nuc.size(); ++q) {
-        micro_xs(nuc_at(o, m.nuc[q], skT), E, &mu);
-        prob = prob + m.dens[q] * mu.t;
-        if (prob >= cutoff) {
-          pick = (int)q;
-          break;
-        }
-      }
       const Nuclide &nuc = nuc_at(o, m.nuc[pick], skT);
       micro_xs(nuc, E, &mu);
       const double nuf = (nuc.nu0 + nuc.nu1 * E) * mu.f;
@@ -1093,16 +1100,27 @@ bool check_ready(Oracle *o)
   if (o->n_elem == 0) return fail(o, "no mesh");
   if (o->nucs.empty() || o->mats.empty()) return fail(o, "no materials");
   if (!o->have_source) return fail(o, "no source");
-  // element -> material index through the block map (id - 1 semantics: match by id)
+  // element -> material through the block map.  Every mapped id must name a
+  // material (CollisionKernel.C:85-90); the material used is the one with
+  // that id, or with MCW_MATERIAL_BY_INDEX the list entry id - 1, as
+  // `p->material() = _block_to_openmc_materials.at(subdomain) - 1`
+  // (CollisionKernel.C:110) indexes OpenMC's materials vector
   o->elem_mat.assign(o->n_elem, -1);
   std::map<int, int> id_to_idx;
   for (size_t i = 0; i < o->mats.size(); ++i) id_to_idx[o->mats[i].id] = (int)i;
+  const bool by_index = o->opt.material_index == MCW_MATERIAL_BY_INDEX;
   for (int64_t e = 0; e < o->n_elem; ++e) {
     auto b = o->block_mat.find(o->sub[e]);
     if (b == o->block_mat.end()) continue;
     auto m = id_to_idx.find(b->second);
     if (m == id_to_idx.end()) return fail(o, "Could not find material " + std::to_string(b->second));
-    o->elem_mat[e] = m->second;
+    if (by_index) {
+      if (b->second < 1 || b->second > (int)o->mats.size())
+        return fail(o, "material index " + std::to_string(b->second) + " - 1 is outside the materials");
+      o->elem_mat[e] = b->second - 1;
+    } else {
+      o->elem_mat[e] = m->second;
+    }
   }
   // every nuclide's tables in ascending temperature (OpenMC kTs_ order)
   o->nuc_temps.resize(o->nucs.size());
@@ -1612,6 +1630,16 @@ int orc_micro_xs(void *h, int32_t nuc, double E, double *out)
 }
 
 double orc_macro_total(void *h, int32_t mat, double E) { return macro_total(*(Oracle *)h, mat, E, -1.0); }
+
+/* One nuclide pick of the collision step in material `mat` (list position)
+ * at energy E from *seed (advanced past its draw), for tests: the picked
+ * nuclide's index in the nuclide list. */
+int32_t orc_sample_nuclide_x(void *h, int32_t mat, double E, uint64_t *seed)
+{
+  const Oracle &o = *(Oracle *)h;
+  const Material &m = o.mats[mat];
+  return m.nuc[sample_nuclide(o, m, E, -1.0, macro_total(o, mat, E, -1.0), seed)];
+}
 
 /* Elementary functions and RNG, exported for tests. */
 double orc_log_x(double x) { return orc_log(x); }

@@ -66,6 +66,8 @@ def lib():
         L.orc_locate.argtypes = [vp, dp, C.c_double]
         L.orc_sample_sources.argtypes = [vp, i64, i64, i64, dp]
         L.orc_micro_xs.argtypes = [vp, i32, C.c_double, dp]
+        L.orc_sample_nuclide_x.restype = i32
+        L.orc_sample_nuclide_x.argtypes = [vp, i32, C.c_double, C.POINTER(u64)]
         L.orc_macro_total.restype = C.c_double
         L.orc_macro_total.argtypes = [vp, i32, C.c_double]
         L.orc_log_x.restype = C.c_double
@@ -119,9 +121,10 @@ class Oracle:
     """History-based CPU transport of a Problem."""
 
     def __init__(self, problem: Problem = None, seed_roundtrip=False, max_segments=0, master_seed=1,
-                 eigenvalue=False, n_inactive=0):
+                 eigenvalue=False, n_inactive=0, material_index=abi.MATERIAL_BY_ID):
         L = lib()
         o = abi.Options()
+        o.material_index = material_index
         o.seed_roundtrip = int(seed_roundtrip)
         o.max_segments = max_segments
         o.master_seed = master_seed
@@ -250,6 +253,14 @@ class Oracle:
     def cdf(self):
         out = np.zeros(self.n_elem)
         self.L.orc_mesh_cdf(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def sample_nuclides(self, mat, E, seeds):
+        """The nuclide each seed picks in a collision in material `mat` at E."""
+        out = np.empty(len(seeds), np.int32)
+        for i, sd in enumerate(seeds):
+            s = C.c_uint64(int(sd))
+            out[i] = self.L.orc_sample_nuclide_x(self.h, mat, float(E), C.byref(s))
         return out
 
     def micro_xs(self, nuc, E):

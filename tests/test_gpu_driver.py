@@ -67,6 +67,14 @@ def test_tally_driver_matches_oracle(tmp_path, oracle_lib):
     t2 = o.tally(2).reshape(-1, 3, 2)       # cell, nuclide bin, score
     want2 = t2[:, 0, 1]                     # nuclide H1, score total
     assert close(read_col(base + "_cell_flux.csv", "cell_flux"), want2)
+    # MaCaw's aux index score_bin * n_nuclides + nuc_bin (OpenMCTallyAux.C:116-119):
+    # score total (1) of nuclide H1 (0) reads flat bin 1 * 3 + 0 of the
+    # nuclide-major results, which holds nuclide bin 1 (Fe56), score total
+    base2, _ = run_driver("openmc_tally.i", tmp_path, "MonteCarlo/batches=2",
+                          "AuxKernels/cell_val/score_index_layout=macaw")
+    want2m = o.tally(2).reshape(-1, 6)[:, 1 * 3 + 0]
+    assert close(read_col(base2 + "_cell_flux.csv", "cell_flux"), want2m)
+    assert not close(want2m, want2)
 
 
 def test_driver_ray_output_matches_gold_layout_and_source_seeds(tmp_path):

@@ -77,6 +77,28 @@ def test_eigen_parity_tet():
     assert res[-1]["keff"]["n_active"] == 3
 
 
+def test_eigen_large_tracklength_beside_small_collision_tally():
+    """An eigenvalue batch with the tally pair whose LDS ranges differ: a
+    per-element track-length flux over more than 1024 elements (device
+    atomics) and a 4-bin universe collision flux (summed in LDS)."""
+    p = small_k2(n_particles=3000)
+    assert p.mesh.n_elem > 1024
+    p.tallies = [
+        P.TallySpec(id=1, estimator=abi.EST_TRACKLENGTH, filters=[{"type": abi.FILTER_CELL}],
+                    scores=[abi.SCORE_FLUX]),
+        P.TallySpec(id=2, estimator=abi.EST_COLLISION, filters=[{"type": abi.FILTER_UNIVERSE}],
+                    scores=[abi.SCORE_FLUX]),
+    ]
+    g, o, res = run_eigen(p, 3, n_inactive=1)
+    for r in res:
+        assert_records_equal(r["hist"], r["o"]["hist"])
+        assert r["keff"]["keff"] == r["o"]["keff"]["keff"]
+    for tid in (1, 2):
+        for kind in (abi.RESULT_SUM, abi.RESULT_SUM_SQ):
+            assert_tally_close(g.tally(tid, kind), o.tally(tid, kind))
+    assert np.count_nonzero(g.tally(1)) > 0.9 * p.mesh.n_elem
+
+
 def test_eigen_parity_infinite_medium_hex():
     # reflective HEX8 box: k_absorption is exact per history
     p = P.eigen_box(n_particles=3000)

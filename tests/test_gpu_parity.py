@@ -239,6 +239,61 @@ def test_general_tallies_match_oracle():
             assert_tally_close(a, b)
 
 
+def lds_pair_problem(n_particles=3000):
+    """A per-element track-length flux over more than 1024 elements (too many
+    bins for the persistent kernel's LDS copy) beside a universe-filtered
+    collision flux of 4 bins (summed in LDS): the track-length adds must stay
+    on the device tally while the collision ones go through LDS."""
+    p = small_tet(n_particles=n_particles)
+    assert p.mesh.n_elem > 1024
+    p.tallies = [
+        P.TallySpec(id=1, estimator=abi.EST_TRACKLENGTH, filters=[{"type": abi.FILTER_CELL}],
+                    scores=[abi.SCORE_FLUX]),
+        P.TallySpec(id=2, estimator=abi.EST_COLLISION, filters=[{"type": abi.FILTER_UNIVERSE}],
+                    scores=[abi.SCORE_FLUX]),
+    ]
+    return p
+
+
+@pytest.mark.parametrize("mode", [abi.MODE_PLAIN, abi.MODE_BATCHED])
+def test_large_tracklength_beside_small_collision_tally(mode):
+    p = lds_pair_problem()
+    g, o, res = run_both(p, mode, batches=(1, 2))
+    for sg, so, hg, ho in res:
+        assert_records_equal(hg, ho)
+    t1, t2 = g.tally(1), g.tally(2)
+    assert t1.size == p.mesh.n_elem and t2.size == 4
+    assert np.count_nonzero(t1) > 0.9 * t1.size
+    for tid in (1, 2):
+        for kind in (abi.RESULT_VALUE, abi.RESULT_SUM, abi.RESULT_SUM_SQ):
+            assert_tally_close(g.tally(tid, kind), o.tally(tid, kind))
+    # the track-length flux totals the track length of the last batch
+    sg = res[-1][0]
+    assert abs(g.tally(1, abi.RESULT_VALUE).sum() - sg["track_length"]) <= 1e-10 * sg["track_length"]
+
+
+def test_material_index_by_index_matches_oracle():
+    """MaCaw's material index = id - 1 (CollisionKernel.C:110) on a material
+    list whose ids are out of order, against the oracle in the same mode."""
+    from oracle.pyoracle import Oracle
+    from test_oracle_compat import three_material_tet
+    p = three_material_tet(n_particles=2000)
+    for mode in (abi.MODE_PLAIN, abi.MODE_BATCHED):
+        g = Context(p, mode=mode, record_histories=1, material_index=abi.MATERIAL_BY_INDEX)
+        o = Oracle(p, material_index=abi.MATERIAL_BY_INDEX)
+        g.run_batch(1, p.n_particles)
+        o.run_batch(1, p.n_particles, threads=4, record=True)
+        assert_records_equal(g.histories(p.n_particles), o.histories(p.n_particles))
+        for tid in (1, 2):
+            assert_tally_close(g.tally(tid), o.tally(tid))
+    p.materials[1].id = 7
+    p.block_material = {0: 7, 1: 2, 2: 3}
+    from macaw_amd.lib import McwError
+    g = Context(p, material_index=abi.MATERIAL_BY_INDEX)
+    with pytest.raises(McwError, match="material index 7"):
+        g.run_batch(1, 10)
+
+
 def test_mixed_precision_against_fp64():
     """FP32 exit-side distances with FP64 re-solve of near ties: the exit side
     always equals FP64's choice, so crossed-element sequences agree for all but

@@ -129,6 +129,20 @@ def test_eigen_dump_replays_on_the_oracle(tmp_path, oracle_lib):
     assert k["n_active"] == 2 and k["fission_sites"] > 0 and 0.5 < k["keff"] < 3.0
 
 
+def test_material_index_option_reaches_the_library(tmp_path):
+    """[MonteCarlo] material_index = index (MaCaw's id - 1, CollisionKernel.C:110)
+    is handed to the library; the default is by id; other values are refused."""
+    c1 = os.path.join(INPUTS, "c1_hex8_reflective.i")
+    for value, want in ((None, abi.MATERIAL_BY_ID), ("index", abi.MATERIAL_BY_INDEX), ("id", abi.MATERIAL_BY_ID)):
+        d = tmp_path / f"m_{value}.dump"
+        extra = [f"MonteCarlo/material_index={value}"] if value else []
+        r = run(["-i", c1, "MonteCarlo/particles=10", "--dump-problem", str(d)] + extra)
+        assert r.returncode == 0, r.stderr
+        assert P.load_dump(str(d)).material_index == want
+    r = run(["-i", c1, "MonteCarlo/material_index=position", "--check-input"])
+    assert r.returncode != 0 and "material_index must be id or index" in r.stderr
+
+
 def test_dump_replays_on_the_oracle(tmp_path, oracle_lib):
     d = tmp_path / "c1.dump"
     r = run(["-i", os.path.join(INPUTS, "c1_hex8_reflective.i"), "MonteCarlo/particles=300", "--dump-problem", str(d)])
