@@ -59,11 +59,21 @@ struct NcFile {
     pos += n;
     return v;
   }
-  int64_t count() { return (int64_t)u(version == 5 ? 8 : 4); } // NON_NEG
+  // NON_NEG: a count or length; negative (a 64-bit CDF-5 value above
+  // INT64_MAX) or larger than the file (every counted item takes at least a
+  // byte) is refused before it is used in any size computation
+  int64_t count()
+  {
+    const int64_t v = (int64_t)u(version == 5 ? 8 : 4);
+    if (v < 0 || (uint64_t)v > b.size()) bad("bad count in NetCDF header");
+    return v;
+  }
+  // bytes left in the file from `at`
+  uint64_t left(uint64_t at) const { return at <= b.size() ? b.size() - at : 0; }
   std::string name()
   {
     const int64_t n = count();
-    if (n < 0 || pos + (size_t)n > b.size()) bad("bad name in NetCDF header");
+    if ((uint64_t)n > left(pos)) bad("bad name in NetCDF header");
     std::string s((const char *)&b[pos], (size_t)n);
     pos += (size_t)((n + 3) & ~3);
     return s;
@@ -81,8 +91,8 @@ struct NcFile {
       const int64_t ne = count();
       const int sz = type_size(t);
       if (!sz) bad("unknown attribute type");
+      if ((uint64_t)ne > left(pos) / (uint64_t)sz) bad("truncated attribute");
       const size_t bytes = (size_t)ne * sz;
-      if (pos + bytes > b.size()) bad("truncated attribute");
       if (out && t == NC_CHAR) (*out)[nm] = std::string((const char *)&b[pos], bytes);
       pos += (bytes + 3) & ~(size_t)3;
     }
@@ -138,21 +148,31 @@ struct NcFile {
     auto it = vars.find(nm);
     return it == vars.end() ? nullptr : &it->second;
   }
+  // values of a fixed variable; a product of dimensions that cannot fit in
+  // the file is refused before it can overflow
   int64_t n_values(const Var &v) const
   {
-    int64_t n = 1;
+    uint64_t n = 1;
     for (int64_t d : v.dims) {
-      if (dims[d].second == 0) bad("variable " + v.name + " is a record variable");
-      n *= dims[d].second;
+      const uint64_t len = (uint64_t)dims[d].second;
+      if (len == 0) bad("variable " + v.name + " is a record variable");
+      if (n > b.size() / len) bad("variable " + v.name + " is larger than the file");
+      n *= len;
     }
-    return n;
+    return (int64_t)n;
+  }
+  // the variable's n values of sz bytes lie inside the file
+  void check_extent(const Var &v, int64_t n, int sz) const
+  {
+    if (v.begin > b.size() || (uint64_t)n > left(v.begin) / (uint64_t)sz)
+      bad(v.name + " lies past the end of the file");
   }
   std::vector<double> reals(const Var &v) const
   {
     const int64_t n = n_values(v);
     const int sz = type_size(v.type);
     if (v.type != NC_DOUBLE && v.type != NC_FLOAT) bad(v.name + " is not floating point");
-    if (v.begin + (uint64_t)n * sz > b.size()) bad(v.name + " lies past the end of the file");
+    check_extent(v, n, sz);
     std::vector<double> out((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
       uint64_t w = 0;
@@ -176,7 +196,7 @@ struct NcFile {
     const int sz = type_size(v.type);
     if (v.type != NC_INT && v.type != NC_INT64 && v.type != NC_UINT && v.type != NC_SHORT)
       bad(v.name + " is not integer");
-    if (v.begin + (uint64_t)n * sz > b.size()) bad(v.name + " lies past the end of the file");
+    check_extent(v, n, sz);
     std::vector<int64_t> out((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
       uint64_t w = 0;
@@ -191,7 +211,7 @@ struct NcFile {
   {
     if (v.type != NC_CHAR || v.dims.size() != 2) bad(v.name + " is not a name table");
     const int64_t rows = dims[v.dims[0]].second, width = dims[v.dims[1]].second;
-    if (v.begin + (uint64_t)(rows * width) > b.size()) bad(v.name + " lies past the end of the file");
+    check_extent(v, n_values(v), 1);
     std::vector<std::string> out;
     for (int64_t r = 0; r < rows; ++r) {
       std::string s((const char *)&b[v.begin + (uint64_t)(r * width)], (size_t)width);
@@ -219,6 +239,7 @@ ExodusMesh read_exodus(const std::string &path)
   m.xyz.assign((size_t)(3 * nn), 0.0);
   if (const Var *c = f.var("coord")) { // older single coordinate array, (num_dim, num_nodes)
     const std::vector<double> v = f.reals(*c);
+    if ((int64_t)v.size() < m.dim * nn) f.bad("coord holds fewer than num_dim x num_nodes values");
     for (int d = 0; d < m.dim; ++d)
       for (int64_t i = 0; i < nn; ++i) m.xyz[3 * i + d] = v[(size_t)(d * nn + i)];
   } else {
@@ -227,6 +248,7 @@ ExodusMesh read_exodus(const std::string &path)
       const Var *c = f.var(cn[d]);
       if (!c) f.bad(std::string("no ") + cn[d]);
       const std::vector<double> v = f.reals(*c);
+      if ((int64_t)v.size() < nn) f.bad(std::string(cn[d]) + " holds fewer than num_nodes values");
       for (int64_t i = 0; i < nn; ++i) m.xyz[3 * i + d] = v[(size_t)i];
     }
   }
