@@ -300,6 +300,23 @@ int mcw_tally_shape(mcw_ctx *ctx, int32_t id, int64_t *n_filter_bins, int64_t *n
  * fission bank sorted and resampled into the next batch's source. */
 int mcw_run_batch(mcw_ctx *ctx, int64_t batch, int64_t n_particles, mcw_batch_stats *out);
 
+/* No counterpart in the reference.  mcw_run_batch for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same batch, with the same histories
+ * and tallies, and no hipMalloc, hipFree or allocating library call.  Its
+ * counters and source bank live in a static array of 64 MiB per device that
+ * the library holds until the process ends; histories go through it in
+ * chunks of at most MCW_NOALLOC_CHUNK (an explicit bank_capacity above that
+ * is an error).  Everything else must exist already: the context must have
+ * run mcw_run_batch since its problem or options last changed (set-up
+ * allocates), and history records need a record buffer that an earlier
+ * mcw_run_batch of at least n_particles histories allocated.  Fixed-source
+ * runs in mode BATCHED only, without sorting or visit records.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (from
+ * different threads or contexts) race. */
+#define MCW_NOALLOC_CHUNK 986624
+int mcw_run_batch_noalloc(mcw_ctx *ctx, int64_t batch, int64_t n_particles, mcw_batch_stats *out);
+
 /* ---- eigenvalue runs ------------------------------------------------------ */
 
 /* A fission source site (OpenMC SourceSite): position, direction, energy,

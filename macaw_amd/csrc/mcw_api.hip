@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -221,6 +222,8 @@ struct mcw_ctx {
   mcw_keff_result kres{};
 
   Params base{};
+  // this device's copy of the static scratch array (mcw_run_batch_noalloc)
+  unsigned char *scratch = nullptr;
 };
 
 namespace {
@@ -277,6 +280,42 @@ void check_angle(const mcw_source &src)
   } else {
     require(src.angle == MCW_ANGLE_ISOTROPIC, "source angle type not supported");
   }
+}
+
+// ------------------------------------------------------------------ scratch
+// Device memory of mcw_run_batch_noalloc: the batch's counters and its
+// source bank live in one static array per device instead of context
+// allocations.  Each GPU the library runs on holds its own copy for the life
+// of the process.  Host code never passes `scratch` itself to a kernel or a
+// copy: the name is a host-side stand-in, and a kernel handed it would fault.
+// It uses the device address that hipGetSymbolAddress returns for the device
+// that is current, looked up once per device and kept below.
+constexpr size_t SCRATCH_BYTES = (size_t)64 << 20; // 64 MiB per device
+constexpr size_t SCRATCH_COUNTERS = 256;            // Counters, padded to 256 B
+// source bank: 8 arrays of 8 B (x y z u v w E seed) and 1 of 4 B (element)
+// per history, each array starting on a 256-byte boundary
+constexpr int64_t SCRATCH_CHUNK =
+    (int64_t)((SCRATCH_BYTES - SCRATCH_COUNTERS - 9 * 256) / 68) / 256 * 256;
+static_assert(sizeof(Counters) <= SCRATCH_COUNTERS, "counters fit their scratch slot");
+static_assert(SCRATCH_CHUNK == MCW_NOALLOC_CHUNK, "the header states the scratch bank's size");
+static_assert(SCRATCH_COUNTERS + 8 * 8 * SCRATCH_CHUNK + 4 * SCRATCH_CHUNK <= SCRATCH_BYTES, "the bank fits");
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+std::mutex scratch_mu;
+std::map<int, unsigned char *> scratch_dev; // device ordinal -> device address of `scratch`
+
+// The device address of this device's scratch array (device `dev` must be
+// current); looked up once per device.
+unsigned char *scratch_of(int dev)
+{
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  auto it = scratch_dev.find(dev);
+  if (it != scratch_dev.end()) return it->second;
+  unsigned char *ptr = nullptr;
+  HIPCHK(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+  require(ptr != nullptr, "no device address for the scratch array");
+  scratch_dev[dev] = ptr;
+  return ptr;
 }
 
 // OpenMC init_seed on the host: id * 152917 steps of the LCG from master +
@@ -1244,6 +1283,7 @@ int mcw_create(mcw_ctx **out, const mcw_options *opts)
     require(c->opt.run_mode == MCW_RUN_FIXED_SOURCE || c->opt.run_mode == MCW_RUN_EIGENVALUE, "unknown run_mode");
     require(c->opt.n_inactive >= 0, "n_inactive must not be negative");
     HIPCHK(hipSetDevice(c->opt.device));
+    c->scratch = scratch_of(c->opt.device);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
     hipDeviceProp_t prop;
@@ -1524,11 +1564,46 @@ int mcw_tally_reset(mcw_ctx *c)
   });
 }
 
-int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stats *out)
+// The source bank of a chunk of up to SCRATCH_CHUNK histories inside the
+// device's scratch array, after its counters.
+static SourceBank scratch_bank(mcw_ctx *c)
 {
-  return guard(c, [&] {
+  unsigned char *p = c->scratch + SCRATCH_COUNTERS;
+  auto take = [&](size_t elem_bytes) {
+    unsigned char *q = p;
+    p += ((size_t)SCRATCH_CHUNK * elem_bytes + 255) / 256 * 256;
+    return q;
+  };
+  SourceBank S;
+  S.x = (double *)take(8); S.y = (double *)take(8); S.z = (double *)take(8);
+  S.u = (double *)take(8); S.v = (double *)take(8); S.w = (double *)take(8);
+  S.E = (double *)take(8); S.seed = (uint64_t *)take(8); S.elem = (int32_t *)take(4);
+  return S;
+}
+
+// One batch.  noalloc: the caller cannot allocate (mcw_run_batch_noalloc);
+// the counters and the source bank come from the device's scratch array and
+// every other buffer must exist already.
+static void run_batch_impl(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stats *out, bool noalloc)
+{
+  {
     require(batch >= 1, "batch numbers start at 1");
     require(n_particles >= 1, "No Rays were moved to _rays in defineRays()");
+    if (noalloc) {
+      require(!c->dirty && c->have_src && c->h_counters,
+              "mcw_run_batch_noalloc needs a context already set up: run mcw_run_batch once after "
+              "changing the problem or the options (set-up allocates device memory)");
+      require(c->opt.run_mode == MCW_RUN_FIXED_SOURCE, "mcw_run_batch_noalloc runs fixed-source batches only");
+      require(c->opt.mode == MCW_MODE_BATCHED, "mcw_run_batch_noalloc needs mode BATCHED");
+      require(c->opt.sort_cadence == 0, "mcw_run_batch_noalloc does not sort (sort_cadence must be 0)");
+      require(c->opt.trace_histories == 0, "mcw_run_batch_noalloc keeps no visit records (trace_histories 0)");
+      require(!c->opt.record_histories || c->rec_n >= n_particles,
+              "mcw_run_batch_noalloc: history records for " + std::to_string(n_particles) +
+                  " histories were never allocated; run mcw_run_batch with record_histories at this size first");
+      require(c->opt.bank_capacity <= SCRATCH_CHUNK,
+              "mcw_run_batch_noalloc: bank_capacity " + std::to_string(c->opt.bank_capacity) +
+                  " exceeds the scratch bank of " + std::to_string(SCRATCH_CHUNK) + " histories");
+    }
     prepare(c);
     const bool eig = c->opt.run_mode == MCW_RUN_EIGENVALUE;
     bool active = true;
@@ -1551,6 +1626,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
     Params P = c->base;
     P.batch = batch;
     P.n_particles = n_particles;
+    if (noalloc) P.cnt = (Counters *)c->scratch; // zeroed below by a kernel, as the context's
     if (eig) {
       P.eigen = 1;
       P.fb = fission_view(c, n_particles);
@@ -1593,7 +1669,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
     }
     for (auto &t : c->tallies) HIPCHK(launch_fill(s, t->value.as<double>(), t->n_filter_bins * t->n_score_bins, 0.0));
     HIPCHK(launch_zero_counters(s, P.cnt));
-    const int64_t cap_default = (int64_t)1 << 24;
+    const int64_t cap_default = noalloc ? SCRATCH_CHUNK : (int64_t)1 << 24;
     const int64_t cap = c->opt.bank_capacity > 0 ? c->opt.bank_capacity : cap_default;
     const int ns = c->mesh.uniform_sides;
     float kernel_ms = 0.0f;
@@ -1607,7 +1683,7 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       if (eig && batch > 1) {
         S = gen_bank(c, h0); // resampled fission sites of the previous batch
       } else {
-        S = source_bank(c, h1 - h0);
+        S = noalloc ? scratch_bank(c) : source_bank(c, h1 - h0);
         HIPCHK(launch_source_bank(s, P, S, h1 - h0));
       }
       if (c->opt.mode == MCW_MODE_PLAIN) {
@@ -1667,7 +1743,17 @@ int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stat
       out->hex_segments = (int64_t)k.hex_segments;
       out->trace_records = (int64_t)k.trace_count;
     }
-  });
+  }
+}
+
+int mcw_run_batch(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stats *out)
+{
+  return guard(c, [&] { run_batch_impl(c, batch, n_particles, out, false); });
+}
+
+int mcw_run_batch_noalloc(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_batch_stats *out)
+{
+  return guard(c, [&] { run_batch_impl(c, batch, n_particles, out, true); });
 }
 
 int mcw_keff_get(mcw_ctx *c, mcw_keff_result *out)

@@ -24,7 +24,7 @@ EXPORTS = (
     "mcw_server_create", "mcw_server_destroy", "mcw_server_last_error", "mcw_submit", "mcw_poll",
     "mcw_result", "mcw_trace_get", "mcw_debug_math_probe", "mcw_debug_xs_probe",
     "mcw_keff_get", "mcw_fission_bank_get", "mcw_source_bank_get", "mcw_xs_add_temperature",
-    "mcw_temperature_set",
+    "mcw_temperature_set", "mcw_run_batch_noalloc",
 )
 
 
@@ -58,6 +58,7 @@ def lib():
         "mcw_tally_add": ([vp, i32, i32, i32, C.POINTER(abi.Filter), i32, i32p, i32, i32p], C.c_int),
         "mcw_tally_shape": ([vp, i32, i64p, i64p], C.c_int),
         "mcw_run_batch": ([vp, i64, i64, C.POINTER(abi.BatchStats)], C.c_int),
+        "mcw_run_batch_noalloc": ([vp, i64, i64, C.POINTER(abi.BatchStats)], C.c_int),
         "mcw_tally_get": ([vp, i32, i32, dp, C.c_size_t], C.c_int),
         "mcw_tally_reset": ([vp], C.c_int),
         "mcw_history_get": ([vp, vp, C.c_size_t], C.c_int),
@@ -156,6 +157,13 @@ class Context:
     def run_batch(self, batch, n_particles):
         st = abi.BatchStats()
         self._chk(self.L.mcw_run_batch(self.h, batch, n_particles, C.byref(st)))
+        return st.as_dict()
+
+    def run_batch_noalloc(self, batch, n_particles):
+        """mcw_run_batch_noalloc: the same batch with no device allocation
+        (the context must have run mcw_run_batch since it last changed)."""
+        st = abi.BatchStats()
+        self._chk(self.L.mcw_run_batch_noalloc(self.h, batch, n_particles, C.byref(st)))
         return st.as_dict()
 
     def tally_shape(self, tally_id):
