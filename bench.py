@@ -26,7 +26,9 @@ a fissile nuclide, 1e7 histories per generation, 2 inactive + 3 active
 batches, each with its fission-bank sort and resampling), the reference's
 BEAVRS assembly example run from its driver input (c2 runs only: the
 example's Exodus mesh extruded to 95,504 HEX8, eigenvalue, 1e7 histories per
-batch) and a CPU baseline
+batch), the 8M-tet batching + precision workload (c2 runs only: 16
+materials of 4 of 32 nuclides, 1e8 histories per batch in bank chunks,
+sorted sources; FP64, mixed precision and plain) and a CPU baseline
 (the FP64 oracle, std::thread on the host cores, on a bounded sample of the
 same histories).
 
@@ -106,6 +108,7 @@ def bench_serving(args, tuning):
     ctxs = [Context(p, mode=abi.MODE_BATCHED, lanes_min=tuning.get("lanes_min", 0)) for p in probs]
     bank = int(args.serve_bank_gib * (1 << 30))
     srv = Server(ctxs, max_inflight=args.serve_inflight, bank_bytes=bank)
+    srv.tune(args.serve_round, args.serve_age_ms)
     log(f"server up ({time.perf_counter() - t:.1f}s)")
     warm = P.c4_requests(n=6, seed=504, n_max=1e5)
     for r in warm:
@@ -132,7 +135,8 @@ def bench_serving(args, tuning):
     out = summarize(res, wall)
     out.update({"rate_requests_per_s": rate, "offered_load": args.serve_load, "capacity_histories_per_s": cap,
                 "meshes_tets": [p.mesh.n_elem for p in probs], "max_inflight": args.serve_inflight,
-                "bank_gib": args.serve_bank_gib, "particles": "log-uniform [1e4, 1e7]",
+                "bank_gib": args.serve_bank_gib, "round_histories": args.serve_round,
+                "age_ms": args.serve_age_ms, "particles": "log-uniform [1e4, 1e7]",
                 "sources": "point or box (1:1), Watt", "tally": "per-element track-length flux per request",
                 "arrivals": "Poisson, seed 505", "precision": "fp64"})
     log(f"serving: p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, "
@@ -240,6 +244,58 @@ def bench_assembly(args):
                            "layers of HEX8 over 400 cm, reflective, fission source on z = 100, synthetic nuclides"}
 
 
+def bench_c3(args, tuning):
+    """The batching + precision workload at full size: 8,066,911 TET4 (107^3
+    lattice), 16 materials of 4 of 32 nuclides on a 1.6M-point union grid,
+    1e8 histories per batch in bank chunks of 2^24, sources sorted by
+    (material, element).  One warm batch, then `--c3-steps` timed batches in
+    FP64 and in mixed precision, and one plain (event-based) batch of 1e7
+    histories for the ratio; the FP64 tally error of the mixed form over the
+    same batches."""
+    from macaw_amd import abi, problems as P
+    from macaw_amd.lib import Context
+
+    n = args.c3_histories
+    t = time.perf_counter()
+    prob = P.c3_tet_vacuum(n_particles=n)
+    log(f"c3: {prob.mesh.n_elem} tets, {len(prob.materials)} materials, {len(prob.nuclides)} nuclides "
+        f"({time.perf_counter() - t:.1f}s)")
+    opts = dict(mode=abi.MODE_BATCHED, sort_cadence=1, lanes_min=tuning.get("lanes_min", 0),
+                block_size=tuning.get("block_size", 0))
+    out = {"workload": f"c3_tet4_{prob.mesh.n_elem // 1000}k_vacuum", "n_elem": prob.mesh.n_elem,
+           "histories_per_step": n, "materials": len(prob.materials), "nuclides": len(prob.nuclides),
+           "schedule": "persistent, sorted sources, bank chunks of 2^24"}
+    ctxs = {}
+    for form, prec in (("fp64", abi.PREC_FP64), ("mixed", abi.PREC_MIXED)):
+        ctx = Context(prob, precision=prec, **opts)
+        ctxs[form] = ctx
+        steps, wall = timed_steps(ctx, n, args.c3_steps, 1, f"c3 {form}")
+        segs = sum(s["segments"] for s in steps)
+        roof = roofline_of(steps, n, 4, 276.0 if form == "fp64" else 212.0)
+        roof["kernel"] = "k_persistent<4>" if form == "fp64" else "k_persistent<4, mixed>"
+        out[form] = {"histories_per_s": n * args.c3_steps / wall, "segments_per_s": segs / wall,
+                     "ms_per_step": wall / args.c3_steps * 1e3, "segments_per_history": segs / (n * args.c3_steps),
+                     "kernel_launches_per_step": steps[0]["kernel_launches"], "roofline": roof}
+        log(f"c3 {form}: {out[form]['histories_per_s']:.3e} histories/s, frac {roof['frac']:.3f}")
+    out["mixed"]["tally_error_vs_fp64"] = tally_error(ctxs["fp64"], ctxs["mixed"], args.c3_steps + 1)
+    for c in ctxs.values():
+        c.close()
+    pctx = Context(prob, mode=abi.MODE_PLAIN)
+    pn = min(n, 10_000_000)
+    pctx.run_batch(200, pn)
+    t = time.perf_counter()
+    pst = pctx.run_batch(1, pn)
+    pw = time.perf_counter() - t
+    pctx.close()
+    out["plain_gpu"] = {"histories_per_s": pn / pw, "segments_per_s": pst["segments"] / pw, "histories": pn,
+                        "ms_per_step": pw * 1e3}
+    out["speedup_vs_plain"] = out["fp64"]["histories_per_s"] / out["plain_gpu"]["histories_per_s"]
+    out["mixed_speedup_vs_plain"] = out["mixed"]["histories_per_s"] / out["plain_gpu"]["histories_per_s"]
+    log(f"c3 plain: {out['plain_gpu']['histories_per_s']:.3e} histories/s; batched {out['speedup_vs_plain']:.2f}x, "
+        f"mixed {out['mixed_speedup_vs_plain']:.2f}x")
+    return out
+
+
 WORKLOADS = {
     # name: (generator, default histories per step, description)
     "c2": ("c2_tet_vacuum", 10_000_000, "1M TET4 unit cube, vacuum, 4 materials / 4 nuclides x 50k points"),
@@ -331,13 +387,19 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving mix")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-calibrate", type=int, default=100, help="requests in the saturated capacity run")
+    ap.add_argument("--serve-calibrate", type=int, default=200, help="requests in the saturated capacity run")
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered load as a fraction of capacity")
     ap.add_argument("--serve-inflight", type=int, default=32)
     ap.add_argument("--serve-bank-gib", type=float, default=4.0)
+    ap.add_argument("--serve-round", type=int, default=1 << 18, help="histories per serving round")
+    ap.add_argument("--serve-age-ms", type=float, default=250.0,
+                    help="age past which a request is served before shorter ones")
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue leg")
     ap.add_argument("--eigen-histories", type=int, default=10_000_000)
     ap.add_argument("--no-assembly", action="store_true", help="skip the BEAVRS assembly leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 8M-tet batching + precision leg")
+    ap.add_argument("--c3-histories", type=int, default=100_000_000)
+    ap.add_argument("--c3-steps", type=int, default=2)
     ap.add_argument("--assembly-histories", type=int, default=10_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
@@ -438,6 +500,10 @@ def main():
     if not args.no_serve and args.workload == "c2":
         serving = bench_serving(args, tuning)
 
+    c3 = None
+    if not args.no_c3 and args.workload == "c2":
+        c3 = bench_c3(args, tuning)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(prob, N)
@@ -474,6 +540,7 @@ def main():
         "serving": serving,
         "eigenvalue": eigen,
         "beavrs_assembly": assembly,
+        "c3": c3,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

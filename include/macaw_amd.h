@@ -434,13 +434,19 @@ typedef struct {
  * materials + block map, set as for mcw_run_batch; a context's own source and
  * tallies are not used).  max_inflight bounds concurrently admitted requests;
  * bank_bytes (0: 16 GiB) bounds their memory, each request counting its flux
- * tally plus min(n_particles, 2^21) x 76 B of particle bank.  A request's
+ * tally plus min(n_particles, 2^18) x 76 B of particle bank (see
+ * mcw_server_tune).  A request's
  * histories are those of batch 1 with n_particles histories and master seed
  * request_id + 1, whatever else is in flight. */
 typedef struct mcw_server mcw_server;
 int mcw_server_create(mcw_server **srv, mcw_ctx *const *meshes, int32_t n_meshes,
                       int32_t max_inflight, int64_t bank_bytes);
 int mcw_server_destroy(mcw_server *srv);
+/* No counterpart in the reference.  Scheduling of a server with no request
+ * queued: histories per round (1024 .. 2^24, default 2^18; a request's bank
+ * share counts min(n_particles, round_histories) x 76 B) and the age after
+ * which an admitted request is served before shorter ones (default 250 ms). */
+int mcw_server_tune(mcw_server *srv, int64_t round_histories, double age_ms);
 const char *mcw_server_last_error(const mcw_server *srv);
 int mcw_submit(mcw_server *srv, const mcw_request *req);
 /* Advance the device work; returns the number of finished requests waiting. */

@@ -1952,7 +1952,7 @@ int mcw_debug_xs_probe(mcw_ctx *c, int32_t nuc, const double *E, int64_t n, int3
 // `round_budget` histories, filled from the mesh's active requests by
 // shortest remaining work first (a request admitted more than `age_ms` ago
 // goes first, so none starves).  Few requests per round keep few tallies
-// hot: a round spread over many requests' tallies runs markedly slower.  A mesh has two round slots,
+// hot: a round spread over many requests' tallies runs markedly slower.  A mesh has three round slots,
 // each with its own stream, counters and source bank, so the next round's
 // workgroups fill the compute units that the current round's tail (its last,
 // longest histories) leaves idle; requests that arrive meanwhile join at the
@@ -1982,7 +1982,7 @@ struct ServeReq {
 // A request's device buffer: n_elem flux values (internal order), then its
 // segment and collision counters.
 constexpr int REQ_TAIL = 2;
-constexpr int ROUND_SLOTS = 2;
+constexpr int ROUND_SLOTS = 3;
 
 struct RoundSlot {
   hipStream_t stream = nullptr;
@@ -2046,8 +2046,11 @@ struct mcw_server {
   std::vector<ServeReq *> active;
   int32_t max_inflight = 8;
   int64_t bank_bytes = 0, used = 0;
-  int64_t round_budget = (int64_t)1 << 21;
-  double age_ms = 1000.0; // admitted longer ago than this: served first
+  // rounds of 2^18 histories: a short request waits for at most a short
+  // round, and more, smaller rounds overlap each other's tails (C4 mix:
+  // p50 81 -> 27 ms, saturated 7.4e7 -> 7.9e7 histories/s against 2^21)
+  int64_t round_budget = (int64_t)1 << 18;
+  double age_ms = 250.0; // admitted longer ago than this: served first
   std::chrono::steady_clock::time_point t0;
   double now() const
   {
@@ -2127,7 +2130,12 @@ bool launch_round(mcw_server *s, int mi, RoundSlot &R)
   int32_t n_sl = 0;
   for (size_t j = 0; j < mine.size() && total < s->round_budget; ++j) {
     ServeReq *q = mine[j];
-    const int64_t cnt = std::min(q->r.n_particles - q->scheduled, s->round_budget - total);
+    const int64_t rem = q->r.n_particles - q->scheduled;
+    // a round that holds short requests takes no long request's bulk: the
+    // long one goes to the next round (the mesh's other slot), so the short
+    // ones finish when their own small round does, not with the bulk
+    if (total > 0 && rem > 4 * std::max(total, s->round_budget / 16)) break;
+    const int64_t cnt = std::min(rem, s->round_budget - total);
     ReqSlice &r = R.h_slices[n_sl++];
     memset(&r, 0, sizeof r);
     r.first = total;
@@ -2234,6 +2242,10 @@ int mcw_server_create(mcw_server **out, mcw_ctx *const *meshes, int32_t n_meshes
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &per_cu, serve_kernel(ns, c->opt.precision == MCW_PREC_MIXED && ns != 6), 256, 0));
       m.blocks = c->n_cu * std::max(1, per_cu);
+      // the pools of request tallies and pinned result buffers start at the
+      // in-flight limit, so the loop allocates nothing, and an admission's
+      // cost does not include a pinned allocation
+      for (int k = 0; k < max_inflight; ++k) m.free_slots.push_back(m.acquire());
     }
   });
 }
@@ -2254,6 +2266,20 @@ int mcw_server_destroy(mcw_server *s)
 }
 
 const char *mcw_server_last_error(const mcw_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int mcw_server_tune(mcw_server *s, int64_t round_histories, double age_ms)
+{
+  return sguard(s, [&] {
+    require(round_histories >= 1024 && round_histories <= ((int64_t)1 << 24),
+            "round_histories must be in 1024 .. 2^24");
+    require(age_ms > 0.0 && std::isfinite(age_ms), "age_ms must be positive and finite");
+    for (auto &mp : s->meshes)
+      for (RoundSlot &R : mp->rs) require(!R.busy, "mcw_server_tune: rounds are running; tune before submitting");
+    require(s->active.empty() && s->pending.empty(), "mcw_server_tune: requests are queued; tune before submitting");
+    s->round_budget = round_histories;
+    s->age_ms = age_ms;
+  });
+}
 
 int mcw_submit(mcw_server *s, const mcw_request *req)
 {

@@ -24,7 +24,7 @@ EXPORTS = (
     "mcw_server_create", "mcw_server_destroy", "mcw_server_last_error", "mcw_submit", "mcw_poll",
     "mcw_result", "mcw_trace_get", "mcw_debug_math_probe", "mcw_debug_xs_probe",
     "mcw_keff_get", "mcw_fission_bank_get", "mcw_source_bank_get", "mcw_xs_add_temperature",
-    "mcw_temperature_set", "mcw_run_batch_noalloc",
+    "mcw_temperature_set", "mcw_run_batch_noalloc", "mcw_server_tune",
 )
 
 
@@ -65,6 +65,7 @@ def lib():
         "mcw_server_create": ([C.POINTER(vp), C.POINTER(vp), i32, i32, i64], C.c_int),
         "mcw_server_destroy": ([vp], C.c_int),
         "mcw_server_last_error": ([vp], C.c_char_p),
+        "mcw_server_tune": ([vp, i64, C.c_double], C.c_int),
         "mcw_submit": ([vp, C.POINTER(abi.Request)], C.c_int),
         "mcw_poll": ([vp, i32p], C.c_int),
         "mcw_result": ([vp, C.POINTER(abi.RequestStatus), dp, C.c_size_t], C.c_int),
@@ -272,6 +273,11 @@ class Server:
     def _chk(self, rc):
         if rc != 0:
             raise McwError(self.L.mcw_server_last_error(self.h).decode())
+
+    def tune(self, round_histories=1 << 18, age_ms=250.0):
+        """Histories per round and the age (ms) past which a request is served
+        before shorter ones (mcw_server_tune; before the first submit)."""
+        self._chk(self.L.mcw_server_tune(self.h, int(round_histories), float(age_ms)))
 
     def submit(self, request_id, mesh_slot, n_particles, source):
         r = abi.Request()

@@ -118,8 +118,10 @@ def test_many_requests_share_rounds():
 
 
 def test_shortest_remaining_work_first():
-    """A large request and small ones admitted together: the small ones
-    finish first, the large one still completes with its own result."""
+    """A large request and small ones admitted together: the small ones run
+    in a round of their own (a round holding short requests takes none of a
+    long one's bulk), so they finish well before the large one, which still
+    completes with its own result."""
     probs = meshes()
     ctx = contexts(probs)
     src = P.box_source((0.2, 0.2, 0.2), (0.8, 0.8, 0.8), 1e6)
@@ -130,9 +132,44 @@ def test_shortest_remaining_work_first():
     got = s.run_all()
     s.close()
     big = got[500][0]
-    # a small request never finishes after the large one; when all share the
-    # last round their completion times differ only by the host's polling
-    # order (microseconds), so allow 1% of the large one's time
+    # the small requests' round holds 1e5 histories, the large request's
+    # rounds 3e6: the small ones are done in a fraction of the large one's time
     for i in range(5):
-        assert got[501 + i][0]["t_done_ms"] <= 1.01 * big["t_done_ms"]
+        assert got[501 + i][0]["t_done_ms"] - big["t_admit_ms"] < 0.5 * (big["t_done_ms"] - big["t_admit_ms"]), \
+            (got[501 + i][0], big)
     assert big["histories"] == 3_000_000 and big["segments"] > 0
+    # the large request alone gives the same result
+    a = Server(ctx, max_inflight=1)
+    a.submit(500, 0, 3_000_000, src)
+    st, flux = a.run_all()[500]
+    a.close()
+    assert st["segments"] == big["segments"]
+    rel = np.abs(flux - got[500][1]) / np.maximum(np.abs(flux), 1e-300)
+    assert rel.max() <= 1e-10
+
+
+def test_server_tune():
+    """Round size and aging are set before any request; out-of-range values
+    and tuning a busy server are refused; small rounds give the same results."""
+    from macaw_amd.lib import McwError
+    probs = meshes()
+    ctx = contexts(probs)
+    src = P.box_source((0.2, 0.2, 0.2), (0.8, 0.8, 0.8), 1e6)
+    s = Server(ctx, max_inflight=8)
+    with pytest.raises(McwError, match="round_histories"):
+        s.tune(100, 10.0)
+    with pytest.raises(McwError, match="age_ms"):
+        s.tune(1 << 16, -1.0)
+    s.tune(5000, 50.0)
+    s.submit(1, 0, 23_000, src)
+    with pytest.raises(McwError, match="tune before submitting"):
+        s.tune(1 << 16, 10.0)
+    small = s.run_all()[1]
+    s.close()
+    a = Server(ctx, max_inflight=8)
+    a.submit(1, 0, 23_000, src)
+    ref = a.run_all()[1]
+    a.close()
+    assert small[0]["segments"] == ref[0]["segments"]
+    rel = np.abs(small[1] - ref[1]) / np.maximum(np.abs(ref[1]), 1e-300)
+    assert rel.max() <= 1e-10
