@@ -456,6 +456,18 @@ def main():
     seg_s = segs / wall
     roof = roofline_of(steps, N, nuc_per_mat, 276.0)
     roof.update({"traffic": read_traffic(workload), "kernel": "k_persistent<4>" if all_tet else "k_persistent<0>"})
+    # the per-element flux tally makes one FP64 atomic add per segment, and
+    # the device performs every FP64 atomic at the memory side: the kernel's
+    # other ceiling is the device's atomic rate, measured here on an array
+    # of the tally's size
+    bins = 1 << max(10, (n_elem - 1).bit_length())
+    arate, _ = ctx.atomic_rate(bins)
+    seg_per_launch_s = segs / (sum(s["kernel_ms"] for s in steps) * 1e-3)
+    atomic_roof = {"bound": "memory-side FP64 atomics", "achieved": seg_per_launch_s, "peak": arate,
+                   "unit": "atomic adds/s", "frac": seg_per_launch_s / arate, "array_bins": bins,
+                   "note": "one tally add per segment (kernel time); peak measured by mcw_debug_atomic_rate"}
+    log(f"atomics: {seg_per_launch_s:.3e} adds/s in the kernel against {arate:.3e} measured ceiling "
+        f"({seg_per_launch_s / arate:.2f})")
 
     mixed = None
     if not args.no_mixed:
@@ -537,6 +549,7 @@ def main():
         "mixed_precision": mixed,
         "mixed_speedup_vs_plain": (mixed["histories_per_s"] / plain["histories_per_s"]) if (plain and mixed) else None,
         "roofline": roof,
+        "atomic_roofline": atomic_roof,
         "serving": serving,
         "eigenvalue": eigen,
         "beavrs_assembly": assembly,

@@ -407,6 +407,14 @@ int mcw_trace_get(mcw_ctx *ctx, mcw_trace_record *out, size_t max, int64_t *n_ou
 int mcw_debug_xs_probe(mcw_ctx *ctx, int32_t nuc, const double *E, int64_t n, int32_t *k_out,
                        double *xs_out);
 
+/* Diagnostics (no reference counterpart): the device's FP64 atomic-add rate
+ * (adds per second) for pseudo-random adds, with the tallies' instruction,
+ * into an array of n_bins doubles (a power of two, 2^10 .. 2^28); sum_out,
+ * if not NULL, receives the array's sum (the number of adds made, after a
+ * warm-up launch of 8 adds per thread).  The tally kernels' roofline: every
+ * segment of a per-element flux tally makes one such add. */
+int mcw_debug_atomic_rate(mcw_ctx *ctx, int64_t n_bins, double *adds_per_s, double *sum_out);
+
 /* ---- serving (continuous batching of fixed-source requests) -------------- */
 
 /* A request: a source, a particle count and a per-element flux tally

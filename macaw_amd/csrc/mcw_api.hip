@@ -63,6 +63,7 @@ hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *
                              double *sq, const uint64_t *seeds, double *po, uint64_t *seo);
 hipError_t launch_xs_probe(hipStream_t s, const XSView &X, int32_t nuc, const double *E, int64_t n, int32_t *k,
                            double *xs);
+hipError_t launch_atomic_probe(hipStream_t s, double *a, uint64_t mask, unsigned blocks, int32_t iters);
 hipError_t launch_source_req(hipStream_t s, const Params &P, const SourceBank &S, int32_t n_slices, int64_t n);
 const void *serve_kernel(int ns, bool p32);
 hipError_t sort_temp_bytes(int64_t n, size_t *bytes);
@@ -1919,6 +1920,38 @@ int mcw_debug_math_probe(mcw_ctx *c, const double *x, int64_t n, double *log_out
     HIPCHK(hipMemcpyAsync(prn_out, dp.p, b, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(seed_out, dso.p, b, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+  });
+}
+
+int mcw_debug_atomic_rate(mcw_ctx *c, int64_t n_bins, double *adds_per_s, double *sum_out)
+{
+  return guard(c, [&] {
+    require(n_bins >= 1024 && n_bins <= ((int64_t)1 << 28) && (n_bins & (n_bins - 1)) == 0,
+            "n_bins must be a power of two in 2^10 .. 2^28");
+    require(adds_per_s != nullptr, "null output");
+    hipStream_t s = c->stream;
+    DevBuf a;
+    a.alloc((size_t)n_bins * 8);
+    HIPCHK(launch_fill(s, a.as<double>(), n_bins, 0.0));
+    const unsigned blocks = (unsigned)c->n_cu * 16;
+    const int32_t iters = 256;
+    HIPCHK(launch_atomic_probe(s, a.as<double>(), (uint64_t)(n_bins - 1), blocks, 8)); // warm
+    HIPCHK(hipEventRecord(c->ev[2], s));
+    HIPCHK(launch_atomic_probe(s, a.as<double>(), (uint64_t)(n_bins - 1), blocks, iters));
+    HIPCHK(hipEventRecord(c->ev[3], s));
+    HIPCHK(hipEventSynchronize(c->ev[3]));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    *adds_per_s = (double)blocks * 256.0 * iters / (ms * 1e-3);
+    if (sum_out) {
+      // every add landed: the array sums to the number of adds
+      std::vector<double> h((size_t)n_bins);
+      HIPCHK(hipMemcpyAsync(h.data(), a.p, (size_t)n_bins * 8, hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      double t = 0.0;
+      for (double v : h) t += v;
+      *sum_out = t;
+    }
   });
 }
 

@@ -1328,6 +1328,20 @@ extern "C" __global__ void k_math_probe(const double *x, int64_t n, double *log_
   seed_out[i] = future_seed((uint64_t)i * PRN_STRIDE + 7, seeds[i]);
 }
 
+// The device's FP64 atomic-add rate: every thread adds 1.0 `iters` times at
+// pseudo-random positions of a[0 .. mask] (mask + 1 a power of two), with
+// the same instruction the tallies use.  The tally kernels are measured
+// against this rate: on MI355X every FP64 atomic is performed at the memory
+// side, whatever the array's size.
+extern "C" __global__ void __launch_bounds__(256) k_atomic_probe(double *a, uint64_t mask, int32_t iters)
+{
+  uint64_t s = 0x9E3779B97F4A7C15ull * ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1);
+  for (int32_t i = 0; i < iters; ++i) {
+    s = PRN_MULT * s + PRN_ADD;
+    atomic_add_dev(a + ((s >> 20) & mask), 1.0);
+  }
+}
+
 // Union-grid lookup on the device, for tests: k and (total, absorption,
 // scatter, fission) of nuclide `nuc` at each energy.
 extern "C" __global__ void k_xs_probe(XSView X, int32_t nuc, const double *E, int64_t n, int32_t *k_out,
@@ -1516,6 +1530,12 @@ hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *
                              double *sq, const uint64_t *seeds, double *po, uint64_t *seo)
 {
   hipLaunchKernelGGL(k_math_probe, dim3(grid_for(n, 256, 0)), dim3(256), 0, s, x, n, lo, so, co, sq, seeds, po, seo);
+  return hipGetLastError();
+}
+
+hipError_t launch_atomic_probe(hipStream_t s, double *a, uint64_t mask, unsigned blocks, int32_t iters)
+{
+  hipLaunchKernelGGL(k_atomic_probe, dim3(blocks), dim3(256), 0, s, a, mask, iters);
   return hipGetLastError();
 }
 

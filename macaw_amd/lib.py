@@ -24,7 +24,7 @@ EXPORTS = (
     "mcw_server_create", "mcw_server_destroy", "mcw_server_last_error", "mcw_submit", "mcw_poll",
     "mcw_result", "mcw_trace_get", "mcw_debug_math_probe", "mcw_debug_xs_probe",
     "mcw_keff_get", "mcw_fission_bank_get", "mcw_source_bank_get", "mcw_xs_add_temperature",
-    "mcw_temperature_set", "mcw_run_batch_noalloc", "mcw_server_tune",
+    "mcw_temperature_set", "mcw_run_batch_noalloc", "mcw_server_tune", "mcw_debug_atomic_rate",
 )
 
 
@@ -72,6 +72,7 @@ def lib():
         "mcw_trace_get": ([vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int64)], C.c_int),
         "mcw_debug_math_probe": ([vp, dp, i64, dp, dp, dp, dp, u64p, dp, u64p], C.c_int),
         "mcw_debug_xs_probe": ([vp, i32, dp, i64, i32p, dp], C.c_int),
+        "mcw_debug_atomic_rate": ([vp, i64, dp, dp], C.c_int),
         "mcw_keff_get": ([vp, C.POINTER(abi.KeffResult)], C.c_int),
         "mcw_fission_bank_get": ([vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int64)], C.c_int),
         "mcw_source_bank_get": ([vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int64)], C.c_int),
@@ -238,6 +239,13 @@ class Context:
                                             k.ctypes.data_as(C.POINTER(C.c_int32)),
                                             xs.ctypes.data_as(C.POINTER(C.c_double))))
         return k, xs
+
+    def atomic_rate(self, n_bins=1 << 20):
+        """(FP64 atomic adds per second into n_bins doubles, sum of the array
+        after the probe = adds made) -- mcw_debug_atomic_rate."""
+        r, t = C.c_double(), C.c_double()
+        self._chk(self.L.mcw_debug_atomic_rate(self.h, int(n_bins), C.byref(r), C.byref(t)))
+        return r.value, t.value
 
     def close(self):
         if self.h is not None and self.h.value:

@@ -261,6 +261,8 @@ def delaunay_lattice(xyz: np.ndarray, n_lattice: int, n_slabs: int = 0, halo: in
         import multiprocessing as mp
         with mp.get_context("fork").Pool(workers) as pool:
             parts = pool.map(_delaunay_slab, jobs)
+            pool.close()
+            pool.join()  # every worker reaped before the caller goes on
     else:
         parts = [_delaunay_slab(j) for j in jobs]
     return _canonical_tets(xyz, np.concatenate(parts))

@@ -352,3 +352,15 @@ def test_box_source_missing_the_mesh():
     sg, so, hg, ho = res[0]
     assert sg["lost"] == 100 and so["lost"] == 100 and sg["segments"] == 0
     assert_records_equal(hg, ho)
+
+
+def test_atomic_rate_probe_adds_every_add(ctx_probe):
+    """The atomic-rate probe's adds all land (the array sums to their count)
+    and its rate is positive; bad sizes are refused."""
+    from macaw_amd.lib import McwError
+    for n_bins in (1 << 10, 1 << 20):
+        rate, total = ctx_probe.atomic_rate(n_bins)
+        assert rate > 0 and total > 0 and total == int(total)
+        assert total % (256 * (256 + 8)) == 0  # blocks x 256 threads x (256 + 8 warm-up) adds
+    with pytest.raises(McwError, match="power of two"):
+        ctx_probe.atomic_rate(1000)
