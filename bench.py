@@ -59,8 +59,9 @@ def algorithmic_bytes(stats, n_nuc_per_mat, b_seg=276.0):
 
 
 def read_traffic(workload):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    """Bytes per launch of the dominant kernel between L2 and memory, from the
+    committed rocprofv3 --pmc summary (per-size TCC_EA0 read and write
+    request counts and memory-side atomics, times their request sizes)."""
     path = os.path.join(ROOT, "profiles", f"k_persistent_{workload}_pmc.txt")
     if not os.path.exists(path):
         return None
@@ -186,6 +187,7 @@ def bench_eigen(args, tuning):
         k = ctx.keff()
         rows.append((st, k))
         log(f"eigen batch {b}: {st['time_ms']:.1f} ms transport + {k['close_ms']:.1f} ms close, "
+            f"{st['segments'] / n:.1f} segments and {st['collisions'] / n:.2f} collisions per history, "
             f"{k['fission_sites']} sites, k_gen {k['k_generation']:.5f}, keff {k['keff']:.5f} +- {k['keff_std']:.5f}")
     wall = time.perf_counter() - t
     ctx.close()
@@ -197,6 +199,12 @@ def bench_eigen(args, tuning):
             "keff": k["keff"], "keff_std": k["keff_std"], "k_generation": [kk["k_generation"] for _, kk in rows],
             "fission_sites_per_history": sum(kk["fission_sites"] for _, kk in rows) / (5 * n),
             "segments_per_history": segs / (5 * n), "histories_per_batch": n, "batches": 5, "inactive": 2,
+            # the source moves from the volumetric start toward the fundamental
+            # mode, so the work per history changes from batch to batch
+            "per_batch": [{"transport_ms": st["time_ms"], "kernel_ms": st["kernel_ms"],
+                           "segments_per_history": st["segments"] / n,
+                           "collisions_per_history": st["collisions"] / n,
+                           "ns_per_segment": st["kernel_ms"] * 1e6 / max(st["segments"], 1)} for st, _ in rows],
             "lanes_min": lanes,
             "workload": f"k2_tet4_{prob.mesh.n_elem // 1000}k_eigen",
             "description": "C2 mesh and materials plus a fissile nuclide (A=235, 85% of absorption is fission, "
@@ -327,13 +335,33 @@ def tally_error(fp64_ctx, mixed_ctx, n_batches):
     # near-zero sample variance by chance
     good = ok & (sig < 0.1 * np.abs(mean))
     zg = np.abs(m[good] - f[good]) / n_batches / sig[good]
+    # The sample variance of one element over n batches has n - 1 degrees of
+    # freedom ((n-1) s^2 / sigma^2 ~ chi^2_{n-1}): over millions of elements
+    # a few show s a thousand times below sigma by chance, and their z reads
+    # in the tens whatever the difference.  The pooled figure takes each
+    # element's variance as the median sample variance of the elements with
+    # the nearest mean flux (groups of ~2^15), over the median of
+    # chi^2_{n-1} / (n-1).
+    zp = None
+    if n_batches >= 2 and ok.any():
+        from scipy.stats import chi2
+        order = np.argsort(mean)
+        pooled = np.empty_like(var)
+        for g in np.array_split(order, max(1, f.size >> 15)):
+            pooled[g] = np.median(var[g])
+        pooled /= chi2.ppf(0.5, n_batches - 1) / (n_batches - 1)
+        okp = pooled > 0
+        zp = np.abs(m[okp] - f[okp]) / n_batches / np.sqrt(pooled[okp])
     # one batch gives no standard error: the sigma figures need two or more
     return {"total_rel": float(abs(m.sum() - f.sum()) / f.sum()), "elem_rel_max": float(rel.max()),
             "elem_rel_median": float(np.median(rel)), "elem_rel_p99": float(np.percentile(rel, 99)),
             "max_sigma": float(z.max()) if z.size else None,
             "frac_beyond_3sigma": float((z > 3).mean()) if z.size else None,
             "max_sigma_rse_below_0.1": float(zg.max()) if zg.size else None,
-            "elements_rse_below_0.1": int(good.sum()), "batches": n_batches}
+            "elements_rse_below_0.1": int(good.sum()),
+            "max_sigma_pooled": float(zp.max()) if zp is not None and zp.size else None,
+            "frac_beyond_3sigma_pooled": float((zp > 3).mean()) if zp is not None and zp.size else None,
+            "batches": n_batches}
 
 
 def timed_steps(ctx, N, steps, warmup, label):
@@ -384,6 +412,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--lanes-min", type=int, default=0)
+    ap.add_argument("--sort-cadence", type=int, default=0, help="sort sources by (material, element) without autotune")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving mix")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -404,6 +433,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
                          "bytes): the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling")
+    ap.add_argument("--atomic-sweep", action="store_true",
+                    help="only measure the device's FP64 atomic-add rate (k_atomic_probe) on arrays of "
+                         "2^10 .. 2^25 doubles")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -422,6 +454,15 @@ def main():
         c.math_probe(np.linspace(0.1, 0.9, n), np.arange(n, dtype=np.uint64))
         print(json.dumps({"calibration": "k_math_probe", "n": n, "read_bytes": 16 * n, "write_bytes": 48 * n}))
         return
+    if args.atomic_sweep:
+        c = Context()
+        rows = []
+        for lg in (10, 13, 16, 19, 20, 22, 23, 25):
+            rate, total = c.atomic_rate(1 << lg)
+            rows.append({"bins": 1 << lg, "mbytes": (8 << lg) / 1e6, "adds_per_s": rate, "array_sum": total})
+            log(f"atomic probe: 2^{lg} bins ({(8 << lg) / 1e6:.2f} MB): {rate:.3e} adds/s, array sum {total:.0f}")
+        print(json.dumps({"atomic_sweep": "k_atomic_probe", "rows": rows}))
+        return
 
     gen_name, n_default, desc = WORKLOADS[args.workload]
     N = args.histories or n_default
@@ -439,9 +480,10 @@ def main():
     log(f"{workload}: {n_elem} tets, {prob.mesh.n_nodes} nodes, {n_mat} materials, {n_nuc} nuclides "
         f"({time.perf_counter() - t0:.1f}s)")
 
-    ctx = Context(prob, mode=abi.MODE_BATCHED, block_size=args.block, lanes_min=args.lanes_min)
+    ctx = Context(prob, mode=abi.MODE_BATCHED, block_size=args.block, lanes_min=args.lanes_min,
+                  sort_cadence=args.sort_cadence)
     tuning = {"block_size": ctx.opts.block_size, "lanes_min": ctx.opts.lanes_min,
-              "waves_per_eu": ctx.opts.waves_per_eu}
+              "waves_per_eu": ctx.opts.waves_per_eu, "sort_cadence": ctx.opts.sort_cadence}
     if not args.no_autotune:
         from macaw_amd.autotune import autotune
         t = time.perf_counter()

@@ -181,7 +181,7 @@ struct mcw_ctx {
   bool dirty = true;
 
   // device problem
-  DevBuf plane, erec, mtet, mhex, split, bc_kind, cdf, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
+  DevBuf plane, erec, mtet, mhex, split, bc_kind, cdf, cdf_guide, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf cell_start, cell_elem;
   DevBuf grid_off, npts, rec, scatter, fis, fissionable, watt_t, nuc_of, kappa, mat_fis, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
   DevBuf counters;
@@ -561,6 +561,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
   }
   c->bc_kind.upload(kinds, s);
   c->cdf.upload(m.cdf, s);
+  c->cdf_guide.upload(m.cdf_guide, s);
   c->xyz.upload(m.xyz, s);
   c->conn.upload(m.conn, s);
   c->conn_off.upload(m.conn_off, s);
@@ -708,9 +709,13 @@ void prepare(mcw_ctx *c, bool need_source = true)
   P.mesh.bc_kind = c->bc_kind.as<int8_t>();
   P.mesh.n_elem = m.n_elem;
   P.mesh.cdf = c->cdf.as<double>();
+  P.mesh.cdf_guide = c->cdf_guide.as<int32_t>();
+  P.mesh.n_guide = (int64_t)m.cdf_guide.size() - 1;
+  P.mesh.cdf_guide_scale = m.n_elem > 0 ? (double)P.mesh.n_guide / m.cdf[m.n_elem - 1] : 0.0;
   P.mesh.xyz = c->xyz.as<double>();
   P.mesh.conn = c->conn.as<int32_t>();
   P.mesh.conn_off = c->conn_off.as<int64_t>();
+  P.mesh.conn_off0 = m.conn_off.empty() ? 0 : m.conn_off[0];
   P.mesh.type = c->type.as<uint8_t>();
   P.mesh.int_of_orig = c->int_of_orig.as<int32_t>();
   P.mesh.orig_of_int = c->orig_of_int.as<int32_t>();

@@ -217,24 +217,62 @@ __device__ __forceinline__ double prn(uint64_t &s)
   return (double)out * 0x1p-64;
 }
 
-__device__ __forceinline__ uint64_t future_seed(uint64_t n, uint64_t seed)
+// n steps of the LCG are the affine map s -> g s + c (mod 2^64).  Brown's
+// skip-ahead squares the one-step map once per bit of n and composes the
+// squares where n has a one.  The squares do not depend on n, so they are
+// fixed at compile time: LCG_POW2[k] is the map of 2^k steps, and a jump costs
+// two 64-bit products per set bit of n instead of four per bit.
+struct LcgMap {
+  uint64_t g, c;
+};
+struct LcgPow2 {
+  LcgMap m[64];
+};
+constexpr LcgPow2 lcg_pow2_table()
 {
-  uint64_t g = PRN_MULT, c = PRN_ADD, g_new = 1, c_new = 0;
-  while (n > 0) {
-    if (n & 1u) {
-      g_new *= g;
-      c_new = c_new * g + c;
-    }
+  LcgPow2 t{};
+  uint64_t g = PRN_MULT, c = PRN_ADD;
+  for (int k = 0; k < 64; ++k) {
+    t.m[k] = LcgMap{g, c};
     c = (g + 1) * c;
     g *= g;
-    n >>= 1;
   }
-  return g_new * seed + c_new;
+  return t;
+}
+static __constant__ LcgPow2 LCG_POW2 = lcg_pow2_table();
+
+// The map of n steps; the maps are powers of one map, so they commute.
+__device__ __forceinline__ LcgMap lcg_jump(uint64_t n)
+{
+  uint64_t g = 1, c = 0;
+  for (int k = 0; n != 0; ++k, n >>= 1) {
+    if (n & 1u) {
+      const LcgMap p = LCG_POW2.m[k];
+      c = c * p.g + p.c;
+      g *= p.g;
+    }
+  }
+  return LcgMap{g, c};
+}
+
+__device__ __forceinline__ uint64_t future_seed(uint64_t n, uint64_t seed)
+{
+  const LcgMap j = lcg_jump(n);
+  return j.g * seed + j.c;
 }
 
 __device__ __forceinline__ uint64_t init_seed(int64_t id, int stream, uint64_t master)
 {
   return future_seed((uint64_t)id * PRN_STRIDE, master + (uint64_t)stream);
+}
+
+// The tracking (stream 0) and source (stream 1) seeds of history id: one jump
+// of id * PRN_STRIDE steps applied to both stream seeds.
+__device__ __forceinline__ void init_seeds01(int64_t id, uint64_t master, uint64_t &s0, uint64_t &s1)
+{
+  const LcgMap j = lcg_jump((uint64_t)id * PRN_STRIDE);
+  s0 = j.g * master + j.c;
+  s1 = j.g * (master + 1) + j.c;
 }
 
 // MaCaw stores seeds in Real ray aux data (CollisionKernel.C:132-134,157-159).

@@ -240,8 +240,8 @@ __device__ void sample_site(const Params &P, const SourceView &S, uint64_t maste
                             int64_t hid, double *p, double *dir, double &E, uint64_t &track_seed, int32_t &elem_out)
 {
   const int64_t id = (batch - 1) * n_particles + hid + 1;
-  track_seed = init_seed(id, 0, master);
-  uint64_t ss = init_seed(id, 1, master);
+  uint64_t ss;
+  init_seeds01(id, master, track_seed, ss);
   const MeshView &M = P.mesh;
   int64_t elem;
   if (S.space == 0) {
@@ -265,17 +265,37 @@ __device__ void sample_site(const Params &P, const SourceView &S, uint64_t maste
     elem = e; // internal id, -1 when the box misses the mesh
   } else {
     const int64_t ne = M.n_elem;
-    double target = prn(ss) * M.cdf[ne - 1];
-    int64_t lo = 0, hi = ne; // first e with cdf[e] > target
-    while (lo < hi) {
-      int64_t mid = lo + ((hi - lo) >> 1);
-      if (M.cdf[mid] > target) hi = mid;
-      else lo = mid + 1;
+    const double target = prn(ss) * M.cdf[ne - 1];
+    // first e with cdf[e] > target.  The guide table brackets it: bucket j
+    // of G holds targets in [j, j+1) * total / G and guide[j] is the first e
+    // with cdf[e] > j * total / G, so the answer lies in [guide[j-1],
+    // guide[j+2]] even when rounding moves j by one.  A binary search
+    // finishes; if the two neighbours of its result show that the bracket
+    // missed, the search runs again over the whole CDF.
+    auto first_above = [&](int64_t lo, int64_t hi) {
+      while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (M.cdf[mid] > target) hi = mid;
+        else lo = mid + 1;
+      }
+      return lo;
+    };
+    int64_t lo = 0, hi = ne;
+    if (M.n_guide > 0) {
+      const int64_t G = M.n_guide;
+      int64_t j = (int64_t)(target * M.cdf_guide_scale);
+      j = j < 0 ? 0 : (j > G ? G : j);
+      lo = M.cdf_guide[j > 0 ? j - 1 : 0];
+      hi = M.cdf_guide[j + 2 <= G ? j + 2 : G];
     }
-    elem = lo < ne ? lo : ne - 1;
-    const int32_t *c = M.conn + M.conn_off[elem];
+    int64_t e = first_above(lo, hi);
+    if (!((e == 0 || !(M.cdf[e - 1] > target)) && (e == ne || M.cdf[e] > target))) e = first_above(0, ne);
+    elem = e < ne ? e : ne - 1;
+    // all-TET4 meshes: connectivity at a fixed stride, no offset or type load
+    const bool tet = M.stride == 4 || M.type[elem] == 4;
+    const int32_t *c = M.stride == 4 ? M.conn + (M.conn_off0 + 4 * elem) : M.conn + M.conn_off[elem];
     const double *X = M.xyz;
-    if (M.type[elem] == 4) {
+    if (tet) {
       sample_point_in_tet(X + 3 * (int64_t)c[0], X + 3 * (int64_t)c[1], X + 3 * (int64_t)c[2],
                           X + 3 * (int64_t)c[3], ss, p);
     } else {

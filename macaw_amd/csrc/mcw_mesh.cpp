@@ -321,6 +321,16 @@ std::string derive_mesh(HostMesh &m, const int32_t *face_boundary)
     acc = acc + vol[e];
     m.cdf[e] = acc;
   }
+  {
+    const int64_t G = std::min<int64_t>(ne, (int64_t)1 << 22);
+    m.cdf_guide.assign(G + 1, 0);
+    int64_t e = 0;
+    for (int64_t j = 0; j <= G; ++j) {
+      const double t = acc * (double)j / (double)G;
+      while (e < ne && !(m.cdf[e] > t)) ++e;
+      m.cdf_guide[j] = (int32_t)e;
+    }
+  }
 
   // -- Morton order of element centroids (21 bits per axis), ties by index;
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};

@@ -29,6 +29,9 @@ struct HostMesh {
   std::vector<Split> splits;
   std::set<int32_t> boundary_ids;   // ids of external sides
   std::vector<double> cdf;          // element volume CDF (6x volumes)
+  // guide table of the CDF: cdf_guide[j] = first e with cdf[e] > j * total / G,
+  // j = 0 .. G (cdf_guide[G] = n_elem), G = min(n_elem, 2^22) buckets
+  std::vector<int32_t> cdf_guide;
   // device order: TET4 then HEX8, each along a Morton curve of the centroids
   int64_t n_tet = 0;                // internal ids below n_tet are TET4
   std::vector<int32_t> orig_of_int; // internal position -> caller element

@@ -70,9 +70,13 @@ struct MeshView {
   int64_t n_elem;
   // caller order, for volumetric sources
   const double *cdf;
+  const int32_t *cdf_guide;   // cdf_guide[j] = first e with cdf[e] > j * cdf[n_elem-1] / n_guide
+  int64_t n_guide;
+  double cdf_guide_scale;     // n_guide / cdf[n_elem-1]
   const double *xyz;
   const int32_t *conn;
   const int64_t *conn_off;
+  int64_t conn_off0;          // conn_off[0]; all-TET4 meshes (stride 4) hold element e at conn_off0 + 4e
   const uint8_t *type;
   const int32_t *int_of_orig; // caller element -> internal id
   const int32_t *orig_of_int; // internal id -> caller element
