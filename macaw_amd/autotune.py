@@ -21,7 +21,7 @@ import time
 
 DEFAULT_SPACE = {
     "block_size": [256, 128],
-    "lanes_min": [24, 32, 40, 48, 56],
+    "lanes_min": [16, 24, 32, 40, 48, 56],
     "waves_per_eu": [0, 5],
 }
 BANK_SPACE = {
