@@ -145,7 +145,7 @@ def bench_serving(args, tuning):
     return out
 
 
-def tune_eigen_lanes(prob, n, candidates=(24, 32, 40), log_name="eigen"):
+def tune_eigen_lanes(prob, n, candidates=(16, 24, 32, 40), log_name="eigen"):
     """lanes_min for an eigenvalue kernel, picked on the workload itself: each
     candidate runs batches 1 and 2 (the second from a fission source) on a
     fresh context; the fastest batch-2 transport wins.  Results do not depend
