@@ -28,7 +28,9 @@ BEAVRS assembly example run from its driver input (c2 runs only: the
 example's Exodus mesh extruded to 95,504 HEX8, eigenvalue, 1e7 histories per
 batch), the 8M-tet batching + precision workload (c2 runs only: 16
 materials of 4 of 32 nuclides, 1e8 histories per batch in bank chunks,
-sorted sources; FP64, mixed precision and plain) and a CPU baseline
+sorted sources; FP64, mixed precision and plain), the 20M-element stress
+workload (c2 runs only: mixed TET4/HEX8, 64 nuclides, 1e7 histories per
+batch; FP64, mixed precision and plain) and a CPU baseline
 (the FP64 oracle, std::thread on the host cores, on a bounded sample of the
 same histories).
 
@@ -252,40 +254,57 @@ def bench_assembly(args):
                            "layers of HEX8 over 400 cm, reflective, fission source on z = 100, synthetic nuclides"}
 
 
-def bench_c3(args, tuning):
-    """The batching + precision workload at full size: 8,066,911 TET4 (107^3
-    lattice), 16 materials of 4 of 32 nuclides on a 1.6M-point union grid,
-    1e8 histories per batch in bank chunks of 2^24, sources sorted by
-    (material, element).  One warm batch, then `--c3-steps` timed batches in
-    FP64 and in mixed precision, and one plain (event-based) batch of 1e7
-    histories for the ratio; the FP64 tally error of the mixed form over the
-    same batches."""
+def bench_large(args, tuning, which):
+    """A large workload at full size beside the C2 line, sources sorted by
+    (material, element), one warm batch then timed batches in FP64 and in
+    mixed precision, the mixed form's tally error against FP64 over the same
+    batches, and one plain (event-based) batch of up to 1e7 histories for the
+    ratio.
+      c3: the batching + precision workload, 8,066,911 TET4 (107^3 lattice),
+          16 materials of 4 of 32 nuclides on a 1.6M-point union grid, 1e8
+          histories per batch in bank chunks of 2^24;
+      c5: the stress workload, 19,999,360 mixed TET4/HEX8 elements graded
+          toward a corner, 128 materials of 4 of 64 nuclides on a 6.4M-point
+          union grid, low densities, sources linear in mu, 1e7 histories per
+          batch."""
     from macaw_amd import abi, problems as P
     from macaw_amd.lib import Context
 
-    n = args.c3_histories
+    if which == "c3":
+        n, n_steps, lanes = args.c3_histories, args.c3_steps, tuning.get("lanes_min", 0)
+        gen = P.c3_tet_vacuum
+    else:
+        n, n_steps, lanes = args.c5_histories, args.c5_steps, args.c5_lanes_min
+        gen = P.c5_mixed
     t = time.perf_counter()
-    prob = P.c3_tet_vacuum(n_particles=n)
-    log(f"c3: {prob.mesh.n_elem} tets, {len(prob.materials)} materials, {len(prob.nuclides)} nuclides "
+    prob = gen(n_particles=n)
+    all_tet = bool((prob.mesh.elem_type == abi.TET4).all())
+    log(f"{which}: {prob.mesh.n_elem} elements, {len(prob.materials)} materials, {len(prob.nuclides)} nuclides "
         f"({time.perf_counter() - t:.1f}s)")
-    opts = dict(mode=abi.MODE_BATCHED, sort_cadence=1, lanes_min=tuning.get("lanes_min", 0),
-                block_size=tuning.get("block_size", 0))
-    out = {"workload": f"c3_tet4_{prob.mesh.n_elem // 1000}k_vacuum", "n_elem": prob.mesh.n_elem,
+    opts = dict(mode=abi.MODE_BATCHED, sort_cadence=1, lanes_min=lanes, block_size=tuning.get("block_size", 0))
+    kind = "tet4" if all_tet else "mixed"
+    out = {"workload": f"{which}_{kind}_{prob.mesh.n_elem // 1000}k_vacuum", "n_elem": prob.mesh.n_elem,
            "histories_per_step": n, "materials": len(prob.materials), "nuclides": len(prob.nuclides),
-           "schedule": "persistent, sorted sources, bank chunks of 2^24"}
+           "schedule": f"persistent, sorted sources, lanes_min {lanes or 'default'}, bank chunks of 2^24"}
     ctxs = {}
+    ns = 4 if all_tet else 0
     for form, prec in (("fp64", abi.PREC_FP64), ("mixed", abi.PREC_MIXED)):
         ctx = Context(prob, precision=prec, **opts)
         ctxs[form] = ctx
-        steps, wall = timed_steps(ctx, n, args.c3_steps, 1, f"c3 {form}")
+        steps, wall = timed_steps(ctx, n, n_steps, 1, f"{which} {form}")
         segs = sum(s["segments"] for s in steps)
-        roof = roofline_of(steps, n, 4, 276.0 if form == "fp64" else 212.0)
-        roof["kernel"] = "k_persistent<4>" if form == "fp64" else "k_persistent<4, mixed>"
-        out[form] = {"histories_per_s": n * args.c3_steps / wall, "segments_per_s": segs / wall,
-                     "ms_per_step": wall / args.c3_steps * 1e3, "segments_per_history": segs / (n * args.c3_steps),
+        if form == "fp64":
+            roof = roofline_of(steps, n, 4, 276.0)
+        else:
+            roof = roofline_of(steps, n, 4, 212.0, b_hex=252.0)
+        roof["kernel"] = f"k_persistent<{ns}>" if form == "fp64" else f"k_persistent<{ns}, mixed>"
+        out[form] = {"histories_per_s": n * n_steps / wall, "segments_per_s": segs / wall,
+                     "ms_per_step": wall / n_steps * 1e3, "segments_per_history": segs / (n * n_steps),
+                     "hex_segment_fraction": sum(s["hex_segments"] for s in steps) / max(1, segs),
+                     "fp64_rechecks_per_segment": sum(s["fp64_rechecks"] for s in steps) / max(1, segs),
                      "kernel_launches_per_step": steps[0]["kernel_launches"], "roofline": roof}
-        log(f"c3 {form}: {out[form]['histories_per_s']:.3e} histories/s, frac {roof['frac']:.3f}")
-    out["mixed"]["tally_error_vs_fp64"] = tally_error(ctxs["fp64"], ctxs["mixed"], args.c3_steps + 1)
+        log(f"{which} {form}: {out[form]['histories_per_s']:.3e} histories/s, frac {roof['frac']:.3f}")
+    out["mixed"]["tally_error_vs_fp64"] = tally_error(ctxs["fp64"], ctxs["mixed"], n_steps + 1)
     for c in ctxs.values():
         c.close()
     pctx = Context(prob, mode=abi.MODE_PLAIN)
@@ -299,8 +318,8 @@ def bench_c3(args, tuning):
                         "ms_per_step": pw * 1e3}
     out["speedup_vs_plain"] = out["fp64"]["histories_per_s"] / out["plain_gpu"]["histories_per_s"]
     out["mixed_speedup_vs_plain"] = out["mixed"]["histories_per_s"] / out["plain_gpu"]["histories_per_s"]
-    log(f"c3 plain: {out['plain_gpu']['histories_per_s']:.3e} histories/s; batched {out['speedup_vs_plain']:.2f}x, "
-        f"mixed {out['mixed_speedup_vs_plain']:.2f}x")
+    log(f"{which} plain: {out['plain_gpu']['histories_per_s']:.3e} histories/s; batched "
+        f"{out['speedup_vs_plain']:.2f}x, mixed {out['mixed_speedup_vs_plain']:.2f}x")
     return out
 
 
@@ -429,6 +448,10 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 8M-tet batching + precision leg")
     ap.add_argument("--c3-histories", type=int, default=100_000_000)
     ap.add_argument("--c3-steps", type=int, default=2)
+    ap.add_argument("--no-c5", action="store_true", help="skip the 20M-element stress leg")
+    ap.add_argument("--c5-histories", type=int, default=10_000_000)
+    ap.add_argument("--c5-steps", type=int, default=3)
+    ap.add_argument("--c5-lanes-min", type=int, default=40)
     ap.add_argument("--assembly-histories", type=int, default=10_000_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
@@ -556,7 +579,11 @@ def main():
 
     c3 = None
     if not args.no_c3 and args.workload == "c2":
-        c3 = bench_c3(args, tuning)
+        c3 = bench_large(args, tuning, "c3")
+
+    c5 = None
+    if not args.no_c5 and args.workload == "c2":
+        c5 = bench_large(args, tuning, "c5")
 
     cpu = None
     if not args.no_cpu:
@@ -596,6 +623,7 @@ def main():
         "eigenvalue": eigen,
         "beavrs_assembly": assembly,
         "c3": c3,
+        "c5": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
