@@ -288,22 +288,41 @@ def bench_large(args, tuning, which):
            "schedule": f"persistent, sorted sources, lanes_min {lanes or 'default'}, bank chunks of 2^24"}
     ctxs = {}
     ns = 4 if all_tet else 0
+    # the two other ceilings of these kernels, measured on arrays of the
+    # workload's size: random 128-byte line reads (the element records, about
+    # 160 B per element) and FP64 atomic adds (the flux tally)
+    glines = 1 << min(25, max(16, (prob.mesh.n_elem * 160 // 128 - 1).bit_length()))
+    abins = 1 << min(28, max(10, (prob.mesh.n_elem - 1).bit_length()))
     for form, prec in (("fp64", abi.PREC_FP64), ("mixed", abi.PREC_MIXED)):
         ctx = Context(prob, precision=prec, **opts)
         ctxs[form] = ctx
         steps, wall = timed_steps(ctx, n, n_steps, 1, f"{which} {form}")
         segs = sum(s["segments"] for s in steps)
+        kern_s = sum(s["kernel_ms"] for s in steps) * 1e-3
         if form == "fp64":
             roof = roofline_of(steps, n, 4, 276.0)
+            grate, (arate, _) = ctx.gather_rate(glines), ctx.atomic_rate(abins)
         else:
             roof = roofline_of(steps, n, 4, 212.0, b_hex=252.0)
         roof["kernel"] = f"k_persistent<{ns}>" if form == "fp64" else f"k_persistent<{ns}, mixed>"
+        # element lines a segment reads: FP64 the plane record and the
+        # neighbour record (a hex's 256-byte block), mixed precision one record
+        lps = 2.0 if form == "fp64" else 1.0
         out[form] = {"histories_per_s": n * n_steps / wall, "segments_per_s": segs / wall,
                      "ms_per_step": wall / n_steps * 1e3, "segments_per_history": segs / (n * n_steps),
                      "hex_segment_fraction": sum(s["hex_segments"] for s in steps) / max(1, segs),
                      "fp64_rechecks_per_segment": sum(s["fp64_rechecks"] for s in steps) / max(1, segs),
-                     "kernel_launches_per_step": steps[0]["kernel_launches"], "roofline": roof}
-        log(f"{which} {form}: {out[form]['histories_per_s']:.3e} histories/s, frac {roof['frac']:.3f}")
+                     "kernel_launches_per_step": steps[0]["kernel_launches"], "roofline": roof,
+                     "gather_roofline": {"bound": "random 128-byte line reads", "achieved": lps * segs / kern_s,
+                                         "peak": grate, "unit": "lines/s", "frac": lps * segs / kern_s / grate,
+                                         "lines_per_segment": lps, "array_lines": glines,
+                                         "note": "peak measured by mcw_debug_gather_rate"},
+                     "atomic_roofline": {"bound": "memory-side FP64 atomics", "achieved": segs / kern_s,
+                                         "peak": arate, "unit": "atomic adds/s", "frac": segs / kern_s / arate,
+                                         "array_bins": abins,
+                                         "note": "one tally add per segment; peak measured by mcw_debug_atomic_rate"}}
+        log(f"{which} {form}: {out[form]['histories_per_s']:.3e} histories/s, frac {roof['frac']:.3f}, "
+            f"gather {out[form]['gather_roofline']['frac']:.3f}, atomics {out[form]['atomic_roofline']['frac']:.3f}")
     out["mixed"]["tally_error_vs_fp64"] = tally_error(ctxs["fp64"], ctxs["mixed"], n_steps + 1)
     for c in ctxs.values():
         c.close()
@@ -456,6 +475,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
                     help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
                          "bytes): the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling")
+    ap.add_argument("--gather-sweep", action="store_true",
+                    help="only measure the device's rate of pseudo-random 128-byte line reads (k_gather_probe) "
+                         "on arrays of 2^16 .. 2^25 lines")
     ap.add_argument("--atomic-sweep", action="store_true",
                     help="only measure the device's FP64 atomic-add rate (k_atomic_probe) on arrays of "
                          "2^10 .. 2^25 doubles")
@@ -476,6 +498,16 @@ def main():
         c = Context()
         c.math_probe(np.linspace(0.1, 0.9, n), np.arange(n, dtype=np.uint64))
         print(json.dumps({"calibration": "k_math_probe", "n": n, "read_bytes": 16 * n, "write_bytes": 48 * n}))
+        return
+    if args.gather_sweep:
+        c = Context()
+        rows = []
+        for lg in (16, 19, 20, 21, 22, 23, 24, 25):
+            rate = c.gather_rate(1 << lg)
+            rows.append({"lines": 1 << lg, "mbytes": (128 << lg) / 1e6, "lines_per_s": rate,
+                         "GB_per_s": rate * 128 / 1e9})
+            log(f"gather probe: 2^{lg} lines ({(128 << lg) / 1e6:.0f} MB): {rate:.3e} lines/s, {rate * 128 / 1e9:.0f} GB/s")
+        print(json.dumps({"gather_sweep": "k_gather_probe", "rows": rows}))
         return
     if args.atomic_sweep:
         c = Context()
@@ -533,6 +565,13 @@ def main():
                    "note": "one tally add per segment (kernel time); peak measured by mcw_debug_atomic_rate"}
     log(f"atomics: {seg_per_launch_s:.3e} adds/s in the kernel against {arate:.3e} measured ceiling "
         f"({seg_per_launch_s / arate:.2f})")
+    # and random element-record lines: an FP64 TET4 segment reads the
+    # 128-byte plane record and the line holding its neighbour record
+    glines = 1 << min(25, max(16, (n_elem * 160 // 128 - 1).bit_length()))
+    grate = ctx.gather_rate(glines)
+    gather_roof = {"bound": "random 128-byte line reads", "achieved": 2.0 * seg_per_launch_s, "peak": grate,
+                   "unit": "lines/s", "frac": 2.0 * seg_per_launch_s / grate, "lines_per_segment": 2.0,
+                   "array_lines": glines, "note": "peak measured by mcw_debug_gather_rate"}
 
     mixed = None
     if not args.no_mixed:
@@ -619,6 +658,7 @@ def main():
         "mixed_speedup_vs_plain": (mixed["histories_per_s"] / plain["histories_per_s"]) if (plain and mixed) else None,
         "roofline": roof,
         "atomic_roofline": atomic_roof,
+        "gather_roofline": gather_roof,
         "serving": serving,
         "eigenvalue": eigen,
         "beavrs_assembly": assembly,

@@ -415,6 +415,13 @@ int mcw_debug_xs_probe(mcw_ctx *ctx, int32_t nuc, const double *E, int64_t n, in
  * segment of a per-element flux tally makes one such add. */
 int mcw_debug_atomic_rate(mcw_ctx *ctx, int64_t n_bins, double *adds_per_s, double *sum_out);
 
+/* Diagnostics (no reference counterpart): the device's rate of reads of
+ * pseudo-random 128-byte lines (lines per second) from an array of n_lines
+ * lines (a power of two, 2^10 .. 2^28), eight independent reads in flight
+ * per thread.  The gather kernels' roofline: a segment reads one (mixed
+ * precision) or two (FP64) element lines. */
+int mcw_debug_gather_rate(mcw_ctx *ctx, int64_t n_lines, double *lines_per_s);
+
 /* ---- serving (continuous batching of fixed-source requests) -------------- */
 
 /* A request: a source, a particle count and a per-element flux tally

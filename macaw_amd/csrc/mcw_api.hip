@@ -64,6 +64,8 @@ hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *
 hipError_t launch_xs_probe(hipStream_t s, const XSView &X, int32_t nuc, const double *E, int64_t n, int32_t *k,
                            double *xs);
 hipError_t launch_atomic_probe(hipStream_t s, double *a, uint64_t mask, unsigned blocks, int32_t iters);
+hipError_t launch_gather_probe(hipStream_t s, const double4 *a, uint64_t mask, unsigned blocks, int32_t iters,
+                               double *out);
 hipError_t launch_source_req(hipStream_t s, const Params &P, const SourceBank &S, int32_t n_slices, int64_t n);
 const void *serve_kernel(int ns, bool p32);
 hipError_t sort_temp_bytes(int64_t n, size_t *bytes);
@@ -1957,6 +1959,30 @@ int mcw_debug_atomic_rate(mcw_ctx *c, int64_t n_bins, double *adds_per_s, double
       for (double v : h) t += v;
       *sum_out = t;
     }
+  });
+}
+
+int mcw_debug_gather_rate(mcw_ctx *c, int64_t n_lines, double *lines_per_s)
+{
+  return guard(c, [&] {
+    require(n_lines >= 1024 && n_lines <= ((int64_t)1 << 28) && (n_lines & (n_lines - 1)) == 0,
+            "n_lines must be a power of two in 2^10 .. 2^28");
+    require(lines_per_s != nullptr, "null output");
+    hipStream_t s = c->stream;
+    DevBuf a, o;
+    a.alloc((size_t)n_lines * 128);
+    HIPCHK(launch_fill(s, a.as<double>(), n_lines * 16, 1.0));
+    const unsigned blocks = (unsigned)c->n_cu * 16;
+    o.alloc((size_t)blocks * 256 * 8);
+    const int32_t iters = 256;
+    HIPCHK(launch_gather_probe(s, a.as<double4>(), (uint64_t)(n_lines - 1), blocks, 8, o.as<double>())); // warm
+    HIPCHK(hipEventRecord(c->ev[2], s));
+    HIPCHK(launch_gather_probe(s, a.as<double4>(), (uint64_t)(n_lines - 1), blocks, iters, o.as<double>()));
+    HIPCHK(hipEventRecord(c->ev[3], s));
+    HIPCHK(hipEventSynchronize(c->ev[3]));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    *lines_per_s = (double)blocks * 256.0 * iters / (ms * 1e-3);
   });
 }
 

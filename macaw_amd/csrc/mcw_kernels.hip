@@ -1362,6 +1362,32 @@ extern "C" __global__ void __launch_bounds__(256) k_atomic_probe(double *a, uint
   }
 }
 
+// Random 128-byte line reads: each thread reads 16 bytes from `iters`
+// pseudo-random lines of a (lines x 128 B) array, eight independent reads in
+// flight at a time, and writes one sum so that no read is dropped.  The
+// gather kernels' ceiling: a segment reads one or two such lines.
+extern "C" __global__ void __launch_bounds__(256) k_gather_probe(const double4 *a, uint64_t mask, int32_t iters,
+                                                                 double *out)
+{
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t s = 0x9E3779B97F4A7C15ull * (tid + 1);
+  double acc = 0.0;
+  for (int32_t i = 0; i < iters; i += 8) {
+    uint64_t idx[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      s = PRN_MULT * s + PRN_ADD;
+      idx[k] = (s >> 20) & mask;
+    }
+    double4 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = a[4 * idx[k]];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += v[k].x + v[k].w;
+  }
+  out[tid] = acc;
+}
+
 // Union-grid lookup on the device, for tests: k and (total, absorption,
 // scatter, fission) of nuclide `nuc` at each energy.
 extern "C" __global__ void k_xs_probe(XSView X, int32_t nuc, const double *E, int64_t n, int32_t *k_out,
@@ -1556,6 +1582,13 @@ hipError_t launch_math_probe(hipStream_t s, const double *x, int64_t n, double *
 hipError_t launch_atomic_probe(hipStream_t s, double *a, uint64_t mask, unsigned blocks, int32_t iters)
 {
   hipLaunchKernelGGL(k_atomic_probe, dim3(blocks), dim3(256), 0, s, a, mask, iters);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_probe(hipStream_t s, const double4 *a, uint64_t mask, unsigned blocks, int32_t iters,
+                               double *out)
+{
+  hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, s, a, mask, iters, out);
   return hipGetLastError();
 }
 

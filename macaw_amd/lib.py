@@ -25,6 +25,7 @@ EXPORTS = (
     "mcw_result", "mcw_trace_get", "mcw_debug_math_probe", "mcw_debug_xs_probe",
     "mcw_keff_get", "mcw_fission_bank_get", "mcw_source_bank_get", "mcw_xs_add_temperature",
     "mcw_temperature_set", "mcw_run_batch_noalloc", "mcw_server_tune", "mcw_debug_atomic_rate",
+    "mcw_debug_gather_rate",
 )
 
 
@@ -73,6 +74,7 @@ def lib():
         "mcw_debug_math_probe": ([vp, dp, i64, dp, dp, dp, dp, u64p, dp, u64p], C.c_int),
         "mcw_debug_xs_probe": ([vp, i32, dp, i64, i32p, dp], C.c_int),
         "mcw_debug_atomic_rate": ([vp, i64, dp, dp], C.c_int),
+        "mcw_debug_gather_rate": ([vp, i64, dp], C.c_int),
         "mcw_keff_get": ([vp, C.POINTER(abi.KeffResult)], C.c_int),
         "mcw_fission_bank_get": ([vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int64)], C.c_int),
         "mcw_source_bank_get": ([vp, C.c_void_p, C.c_size_t, C.POINTER(C.c_int64)], C.c_int),
@@ -246,6 +248,13 @@ class Context:
         r, t = C.c_double(), C.c_double()
         self._chk(self.L.mcw_debug_atomic_rate(self.h, int(n_bins), C.byref(r), C.byref(t)))
         return r.value, t.value
+
+    def gather_rate(self, n_lines=1 << 22):
+        """Reads of pseudo-random 128-byte lines per second from an array of
+        n_lines lines -- mcw_debug_gather_rate."""
+        r = C.c_double()
+        self._chk(self.L.mcw_debug_gather_rate(self.h, int(n_lines), C.byref(r)))
+        return r.value
 
     def close(self):
         if self.h is not None and self.h.value:
