@@ -364,3 +364,13 @@ def test_atomic_rate_probe_adds_every_add(ctx_probe):
         assert total % (256 * (256 + 8)) == 0  # blocks x 256 threads x (256 + 8 warm-up) adds
     with pytest.raises(McwError, match="power of two"):
         ctx_probe.atomic_rate(1000)
+
+
+def test_gather_rate_probe(ctx_probe):
+    """The gather-rate probe returns a positive rate for arrays inside and
+    beyond the L2s; bad sizes are refused."""
+    from macaw_amd.lib import McwError
+    for n_lines in (1 << 10, 1 << 20):
+        assert ctx_probe.gather_rate(n_lines) > 0
+    with pytest.raises(McwError, match="power of two"):
+        ctx_probe.gather_rate(3000)
