@@ -472,9 +472,6 @@ def main():
     ap.add_argument("--c5-steps", type=int, default=3)
     ap.add_argument("--c5-lanes-min", type=int, default=40)
     ap.add_argument("--assembly-histories", type=int, default=10_000_000)
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="N",
-                    help="only run one streaming probe kernel over N doubles (reads 16N bytes, writes 48N "
-                         "bytes): the known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE scaling")
     ap.add_argument("--gather-sweep", action="store_true",
                     help="only measure the device's rate of pseudo-random 128-byte line reads (k_gather_probe) "
                          "on arrays of 2^16 .. 2^25 lines")
@@ -492,13 +489,6 @@ def main():
     from macaw_amd import abi, problems as P
     from macaw_amd.lib import Context
 
-    if args.calibrate_copy:
-        import numpy as np
-        n = args.calibrate_copy
-        c = Context()
-        c.math_probe(np.linspace(0.1, 0.9, n), np.arange(n, dtype=np.uint64))
-        print(json.dumps({"calibration": "k_math_probe", "n": n, "read_bytes": 16 * n, "write_bytes": 48 * n}))
-        return
     if args.gather_sweep:
         c = Context()
         rows = []
