@@ -82,3 +82,26 @@ def test_bad_schedule_options_are_rejected():
         c.set_options(use_graph=2)
     with pytest.raises(Exception, match="sort_cadence"):
         c.set_options(use_graph=0, sort_cadence=-1)
+
+
+def test_reused_context_matches_fresh_context():
+    """A context keeps its source banks, chunk buffers and counters from
+    batch to batch.  Batches of another size, a skipped batch number and a
+    changed source must not see any of it: every batch below equals the same
+    batch run alone in a fresh context, record for record."""
+    from macaw_amd import problems as P
+    p = small_tet(n_particles=4000)
+    a = Context(p, mode=abi.MODE_BATCHED, record_histories=1, bank_capacity=1500)
+    runs = [(1, 4000), (2, 4000), (3, 2500), (5, 2500), (6, 2500)]
+    for batch, n in runs:
+        if batch == 6:  # a new source between batches 5 and 6
+            import ctypes as C
+            p.source = P.box_source((0.2, 0.2, 0.2), (0.7, 0.6, 0.5))
+            a._chk(a.L.mcw_source_set(a.h, C.byref(p.source)))
+        print(f"[reuse] batch {batch}, {n} histories", flush=True)
+        a.run_batch(batch, n)
+        fresh = Context(p, mode=abi.MODE_BATCHED, record_histories=1, bank_capacity=1500)
+        fresh.run_batch(batch, n)
+        assert_records_equal(a.histories(n), fresh.histories(n))
+        fresh.close()
+    a.close()
