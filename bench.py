@@ -426,15 +426,23 @@ def roofline_of(steps, N, n_nuc, b_seg, b_hex=348.0):
     hexs = sum(s["hex_segments"] for s in steps)
     colls = sum(s["collisions"] for s in steps)
     kern_ms = sum(s["kernel_ms"] for s in steps)
+    stage_ms = sum(s["stage_ms"] for s in steps)
     launches = sum(s["kernel_launches"] for s in steps)
     b_alg = algorithmic_bytes({"histories": N * len(steps), "segments": segs - hexs, "collisions": colls},
                               n_nuc, b_seg) + b_hex * hexs
     per_launch_bytes = b_alg / launches
     per_launch_s = kern_ms / launches / 1e3
     achieved = per_launch_bytes / per_launch_s / 1e9
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_launch": per_launch_bytes,
-            "launch_ms": per_launch_s * 1e3, "launches_per_step": launches / len(steps)}
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_launch": per_launch_bytes,
+           "launch_ms": per_launch_s * 1e3, "launches_per_step": launches / len(steps)}
+    if stage_ms > 0:
+        # staged flux tally: launch_ms is the transport kernel plus the
+        # kernels that sum the staged records (k_stage_count, k_stage_scatter,
+        # k_stage_reduce); the algorithmic bytes are the same
+        out["transport_ms"] = (kern_ms - stage_ms) / launches
+        out["stage_ms"] = stage_ms / launches
+    return out
 
 
 def main():
@@ -452,6 +460,8 @@ def main():
     ap.add_argument("--lanes-min", type=int, default=0)
     ap.add_argument("--sort-cadence", type=int, default=0, help="sort sources by (material, element) without autotune")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tally-stage", type=int, default=0,
+                    help="tally_stage option of the C2 contexts (0: staged flux tally, -1: device atomics)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving mix")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-calibrate", type=int, default=200, help="requests in the saturated capacity run")
@@ -526,13 +536,15 @@ def main():
         f"({time.perf_counter() - t0:.1f}s)")
 
     ctx = Context(prob, mode=abi.MODE_BATCHED, block_size=args.block, lanes_min=args.lanes_min,
-                  sort_cadence=args.sort_cadence)
+                  sort_cadence=args.sort_cadence, tally_stage=args.tally_stage)
     tuning = {"block_size": ctx.opts.block_size, "lanes_min": ctx.opts.lanes_min,
-              "waves_per_eu": ctx.opts.waves_per_eu, "sort_cadence": ctx.opts.sort_cadence}
+              "waves_per_eu": ctx.opts.waves_per_eu, "sort_cadence": ctx.opts.sort_cadence,
+              "tally_stage": ctx.opts.tally_stage}
     if not args.no_autotune:
         from macaw_amd.autotune import autotune
         t = time.perf_counter()
         tuning, rows = autotune(ctx, min(N, 10_000_000), log=log)
+        tuning["tally_stage"] = args.tally_stage
         ctx.set_options(**tuning)
         log(f"autotune: best {tuning} ({time.perf_counter() - t:.1f}s)")
     ctx.tally_reset()  # tallies then hold exactly the warmup + timed batches
@@ -542,26 +554,36 @@ def main():
     hist_s = N * args.steps / wall
     seg_s = segs / wall
     roof = roofline_of(steps, N, nuc_per_mat, 276.0)
-    roof.update({"traffic": read_traffic(workload), "kernel": "k_persistent<4>" if all_tet else "k_persistent<0>"})
+    kname = "k_persistent<4>" if all_tet else "k_persistent<0>"
+    if "stage_ms" in roof:
+        kname += " + staged-tally sums"
+    roof.update({"traffic": read_traffic(workload), "kernel": kname})
     # the per-element flux tally makes one FP64 atomic add per segment, and
     # the device performs every FP64 atomic at the memory side: the kernel's
     # other ceiling is the device's atomic rate, measured here on an array
     # of the tally's size
-    bins = 1 << max(10, (n_elem - 1).bit_length())
-    arate, _ = ctx.atomic_rate(bins)
-    seg_per_launch_s = segs / (sum(s["kernel_ms"] for s in steps) * 1e-3)
-    atomic_roof = {"bound": "memory-side FP64 atomics", "achieved": seg_per_launch_s, "peak": arate,
-                   "unit": "atomic adds/s", "frac": seg_per_launch_s / arate, "array_bins": bins,
-                   "note": "one tally add per segment (kernel time); peak measured by mcw_debug_atomic_rate"}
-    log(f"atomics: {seg_per_launch_s:.3e} adds/s in the kernel against {arate:.3e} measured ceiling "
-        f"({seg_per_launch_s / arate:.2f})")
+    # (with the staged tally the transport kernel makes no device atomic per
+    # segment: its scores go to LDS and a record pool, summed after it)
+    staged = sum(s["stage_ms"] for s in steps) > 0
+    transport_s = sum(s["kernel_ms"] - s["stage_ms"] for s in steps) * 1e-3
+    seg_per_launch_s = segs / transport_s
+    atomic_roof = None
+    if not staged:
+        bins = 1 << max(10, (n_elem - 1).bit_length())
+        arate, _ = ctx.atomic_rate(bins)
+        atomic_roof = {"bound": "memory-side FP64 atomics", "achieved": seg_per_launch_s, "peak": arate,
+                       "unit": "atomic adds/s", "frac": seg_per_launch_s / arate, "array_bins": bins,
+                       "note": "one tally add per segment (kernel time); peak measured by mcw_debug_atomic_rate"}
+        log(f"atomics: {seg_per_launch_s:.3e} adds/s in the kernel against {arate:.3e} measured ceiling "
+            f"({seg_per_launch_s / arate:.2f})")
     # and random element-record lines: an FP64 TET4 segment reads the
     # 128-byte plane record and the line holding its neighbour record
     glines = 1 << min(25, max(16, (n_elem * 160 // 128 - 1).bit_length()))
     grate = ctx.gather_rate(glines)
     gather_roof = {"bound": "random 128-byte line reads", "achieved": 2.0 * seg_per_launch_s, "peak": grate,
                    "unit": "lines/s", "frac": 2.0 * seg_per_launch_s / grate, "lines_per_segment": 2.0,
-                   "array_lines": glines, "note": "peak measured by mcw_debug_gather_rate"}
+                   "array_lines": glines,
+                   "note": "transport kernel time; peak measured by mcw_debug_gather_rate"}
 
     mixed = None
     if not args.no_mixed:
@@ -569,7 +591,8 @@ def main():
         mctx.set_options(**tuning)
         msteps, mwall = timed_steps(mctx, N, args.steps, args.warmup, "mixed")
         mroof = roofline_of(msteps, N, nuc_per_mat, 212.0, b_hex=252.0)
-        mroof["kernel"] = "k_persistent<4, mixed>" if all_tet else "k_persistent<0, mixed>"
+        mroof["kernel"] = ("k_persistent<4, mixed>" if all_tet else "k_persistent<0, mixed>") + \
+            (" + staged-tally sums" if "stage_ms" in mroof else "")
         mixed = {"histories_per_s": N * args.steps / mwall,
                  "segments_per_s": sum(s["segments"] for s in msteps) / mwall,
                  "ms_per_step": mwall / args.steps * 1e3,

@@ -129,6 +129,12 @@ typedef struct {
   int32_t n_inactive;       /* EIGENVALUE: batches 1..n_inactive are inactive (settings
                                <inactive>): no tally scores, keff not averaged */
   int32_t material_index;   /* MCW_MATERIAL_* (default BY_ID) */
+  int32_t tally_stage;      /* BATCHED fixed-source runs on TET4 meshes whose only track-length
+                               tally is a plain flux of more than 1024 and at most 2^20 bins:
+                               0 = stage its scores in LDS and sum them in a second kernel
+                               (no device atomic per segment), -1 = one device atomic per
+                               segment, n > 0 = stage into a pool of at most n records per
+                               launch (the rest go to device atomics) */
 } mcw_options;
 
 /* One synthetic continuous-energy nuclide: pointwise lin-lin tables on an
@@ -200,6 +206,7 @@ typedef struct {
   int64_t fp64_rechecks; /* precision MIXED: segments whose exit side was re-solved in FP64 */
   int64_t hex_segments;  /* segments in HEX8 elements (the rest are in TET4) */
   int64_t trace_records; /* visit records emitted (trace_histories); beyond the buffer they are dropped */
+  double stage_ms;       /* device time of the staged tally's sums (tally_stage), part of kernel_ms */
 } mcw_batch_stats;
 
 /* Per-history record, for parity checks against a history-based reference. */

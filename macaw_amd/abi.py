@@ -40,7 +40,8 @@ class Options(C.Structure):
                 ("max_segments", C.c_int64), ("master_seed", C.c_uint64),
                 ("sort_cadence", C.c_int32), ("lanes_min", C.c_int32),
                 ("waves_per_eu", C.c_int32), ("use_graph", C.c_int32), ("trace_histories", C.c_int32),
-                ("run_mode", C.c_int32), ("n_inactive", C.c_int32), ("material_index", C.c_int32)]
+                ("run_mode", C.c_int32), ("n_inactive", C.c_int32), ("material_index", C.c_int32),
+                ("tally_stage", C.c_int32)]
 
 
 class Nuclide(C.Structure):
@@ -71,7 +72,7 @@ class BatchStats(C.Structure):
                 ("lost", C.c_int64), ("reflections", C.c_int64), ("track_length", C.c_double),
                 ("time_ms", C.c_double), ("kernel_ms", C.c_double),
                 ("kernel_launches", C.c_int64), ("fp64_rechecks", C.c_int64), ("hex_segments", C.c_int64),
-                ("trace_records", C.c_int64)]
+                ("trace_records", C.c_int64), ("stage_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

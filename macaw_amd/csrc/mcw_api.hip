@@ -34,7 +34,9 @@ hipError_t launch_collide(unsigned blocks, hipStream_t s, const Params &P, const
                           const BankView &out, int qout);
 hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int block, hipStream_t s, const Params &P,
                              const SourceBank &S);
-const void *persistent_kernel(int ns, int wpe, bool rec, bool p32, bool eig);
+const void *persistent_kernel(int ns, int wpe, bool rec, bool p32, bool eig, bool stg);
+size_t persistent_lds_bytes(const Params &P, int block);
+hipError_t launch_stage_reduce(hipStream_t s, const StageView &V, double *tally, int64_t n_bins, int split);
 hipError_t launch_tree_sum(hipStream_t s, const double *in, int64_t n, double *scratch, double *out);
 hipError_t launch_fission_sites(hipStream_t s, const XSView &X, const FissionView &F, int64_t n);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long *p, int64_t n, unsigned long long v);
@@ -162,7 +164,7 @@ struct mcw_ctx {
   mcw_options opt;
   std::string err;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int n_cu = 256;
 
   HostMesh mesh;
@@ -186,8 +188,16 @@ struct mcw_ctx {
   DevBuf plane, erec, mtet, mhex, split, bc_kind, cdf, cdf_guide, xyz, conn, conn_off, type, int_of_orig, orig_of_int;
   DevBuf cell_start, cell_elem;
   DevBuf grid_off, npts, rec, scatter, fis, fissionable, watt_t, nuc_of, kappa, mat_fis, awr, mat_off, mat_nuc, mat_dens, uni, uidx, uhash;
+  DevBuf uhash2; // every other entry of uhash: the table of the staging kernels
   DevBuf counters;
   Counters *h_counters = nullptr; // pinned
+  // staged flux tally: the record pool (scores, bins), per-chunk owner and
+  // count, the chunk list by bucket and the per-bucket counts and cursors;
+  // segments per history of the last staged batch
+  DevBuf stg_sc, stg_el, stg_owner, stg_fill, stg_list, stg_count;
+  int64_t stg_chunks = 0;
+  double stg_seg_per_hist = 0.0;
+  int64_t tl_bins = 0; // bins of the one plain track-length flux tally (tl_fast), else 0
   // source bank, live/collision banks of the event-based path, records
   DevBuf sb[9];
   int64_t sb_cap = 0;
@@ -657,6 +667,10 @@ void prepare(mcw_ctx *c, bool need_source = true)
   c->uni.upload(uni, s);
   c->uidx.upload(uidx, s);
   c->uhash.upload(hash, s);
+  static_assert(MAX_HASH == 2 * STG_HASH, "the staging table keeps every other bucket");
+  std::vector<int32_t> hash2(STG_HASH + 1);
+  for (int32_t b = 0; b <= STG_HASH; ++b) hash2[b] = hash[2 * b];
+  c->uhash2.upload(hash2, s);
   std::vector<int32_t> moff{0}, mnuc;
   std::vector<double> mden;
   for (auto &key : eff_key) {
@@ -803,6 +817,7 @@ void prepare(mcw_ctx *c, bool need_source = true)
     V.value = t.value.as<double>();
   }
   P.tl_fast = (n_tl == 1 && P.tally[0].simple_flux && P.tally[0].n_edges == 0) ? 1 : 0;
+  c->tl_bins = P.tl_fast ? order[0]->n_filter_bins * order[0]->n_score_bins : 0;
   // up to 1024 bins (8 KB) beside the 32 KB hash table keep four 256-thread
   // workgroups per CU
   P.lds_tally = 0;
@@ -1036,13 +1051,48 @@ SourceBank sort_sites(mcw_ctx *c, const Params &P, const SourceBank &S, int64_t 
 
 // Resident workgroups of the persistent kernel over the whole device.
 int persistent_blocks(mcw_ctx *c, int ns, int wpe, bool rec, bool p32, int block, bool eig = false,
-                      size_t lds_bytes = 0)
+                      size_t lds_bytes = 0, bool stg = false)
 {
   int per_cu = 0;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns, wpe, rec, p32, eig), block,
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_kernel(ns, wpe, rec, p32, eig, stg), block,
                                                       lds_bytes));
   if (per_cu < 1) per_cu = 1;
   return c->n_cu * per_cu;
+}
+
+// The staged tally's pool for a launch of n histories over `waves` resident
+// wavefronts and nb buckets: a slice per wavefront with room for its share of
+// the segments per history of the last staged batch (128 before one ran) plus
+// a quarter, and a partly filled chunk per bucket; at most 2^31 records over
+// all (tally_stage > 0: about that many, at least one chunk per wavefront).
+// Records beyond a slice go to device atomics.  The pool only grows.
+StageView stage_view(mcw_ctx *c, int64_t n, int64_t waves, int32_t nb)
+{
+  const double per_hist = std::max(128.0, 1.25 * c->stg_seg_per_hist);
+  int64_t wc = (int64_t)std::ceil(per_hist * (double)n / (double)waves / STG_CH) + nb;
+  if (c->opt.tally_stage > 0) wc = std::min<int64_t>(wc, c->opt.tally_stage / ((int64_t)STG_CH * waves));
+  wc = std::max<int64_t>(1, std::min<int64_t>(wc, (((int64_t)1 << 31) / STG_CH) / waves));
+  const int64_t chunks = wc * waves;
+  if (chunks > c->stg_chunks) {
+    c->stg_sc.alloc((size_t)chunks * STG_CH * sizeof(double));
+    c->stg_el.alloc((size_t)chunks * STG_CH * sizeof(uint16_t));
+    c->stg_owner.alloc((size_t)(chunks + 7) / 8 * 8);
+    c->stg_fill.alloc((size_t)chunks * sizeof(uint16_t));
+    c->stg_list.alloc((size_t)chunks * sizeof(uint32_t));
+    c->stg_chunks = chunks;
+  }
+  c->stg_count.alloc(2 * STG_NB * sizeof(unsigned long long));
+  StageView V{};
+  V.sc = c->stg_sc.as<double>();
+  V.el = c->stg_el.as<uint16_t>();
+  V.owner = c->stg_owner.as<uint8_t>();
+  V.fill = c->stg_fill.as<uint16_t>();
+  V.list = c->stg_list.as<uint32_t>();
+  V.bcount = c->stg_count.as<unsigned long long>();
+  V.wave_chunks = wc;
+  V.n_chunks = chunks;
+  V.n_buckets = nb;
+  return V;
 }
 
 // Slots of the fission bank for batches of n histories: the 3 n sites OpenMC
@@ -1290,6 +1340,7 @@ int mcw_create(mcw_ctx **out, const mcw_options *opts)
     require(c->opt.use_graph == 0 || c->opt.use_graph == 1, "use_graph must be 0 or 1");
     require(c->opt.run_mode == MCW_RUN_FIXED_SOURCE || c->opt.run_mode == MCW_RUN_EIGENVALUE, "unknown run_mode");
     require(c->opt.n_inactive >= 0, "n_inactive must not be negative");
+    require(c->opt.tally_stage >= -1, "tally_stage must be -1, 0 or a positive record count");
     HIPCHK(hipSetDevice(c->opt.device));
     c->scratch = scratch_of(c->opt.device);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1337,6 +1388,7 @@ int mcw_options_set(mcw_ctx *c, const mcw_options *o)
     require(o->n_inactive >= 0, "n_inactive must not be negative");
     require(o->material_index == MCW_MATERIAL_BY_ID || o->material_index == MCW_MATERIAL_BY_INDEX,
             "unknown material_index option");
+    require(o->tally_stage >= -1, "tally_stage must be -1, 0 or a positive record count");
     const bool reprep = o->max_segments != c->opt.max_segments || o->lanes_min != c->opt.lanes_min ||
                         o->material_index != c->opt.material_index;
     c->opt = *o;
@@ -1680,8 +1732,9 @@ static void run_batch_impl(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_b
     const int64_t cap_default = noalloc ? SCRATCH_CHUNK : (int64_t)1 << 24;
     const int64_t cap = c->opt.bank_capacity > 0 ? c->opt.bank_capacity : cap_default;
     const int ns = c->mesh.uniform_sides;
-    float kernel_ms = 0.0f;
+    float kernel_ms = 0.0f, stage_ms = 0.0f;
     int64_t launches = 0;
+    bool staged = false;
     HIPCHK(hipEventRecord(c->ev[0], s));
     for (int64_t h0 = 0; h0 < n_particles; h0 += cap) {
       const int64_t h1 = std::min(n_particles, h0 + cap);
@@ -1702,19 +1755,46 @@ static void run_batch_impl(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_b
         const int wpe = c->opt.waves_per_eu == 5 ? 5 : 0;
         const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns != 6;
         require(block % 64 == 0 && block <= 256, "block_size must be 64, 128, 192 or 256");
+        // the one plain per-element flux tally staged in LDS (no device atomic per segment)
+        const bool stg = c->opt.tally_stage >= 0 && !noalloc && !eig && ns == 4 && wpe == 0 && block == 256 &&
+                         P.tl_fast && P.lds_tally == 0 && P.lds_coll == 0 && c->tl_bins > 1024 &&
+                         c->tl_bins <= ((int64_t)STG_NB << STG_BITS);
+        Params PK = P;
+        if (stg) {
+          PK.stg.n_buckets = (int32_t)((c->tl_bins + (1 << STG_BITS) - 1) >> STG_BITS);
+          PK.xs.hash = c->uhash2.as<int32_t>();
+          PK.xs.n_hash = STG_HASH;
+          PK.xs.inv_dl = P.xs.inv_dl * 0.5;
+        }
         int64_t blocks = std::min<int64_t>(persistent_blocks(c, ns, wpe, P.record != 0, p32, block, eig,
-                                                             (size_t)(P.lds_tally + P.lds_coll) * sizeof(double)),
+                                                             persistent_lds_bytes(PK, block), stg),
                                            (h1 - h0 + block - 1) / block);
         if (blocks < 1) blocks = 1;
+        if (stg) {
+          PK.stg = stage_view(c, h1 - h0, blocks * (block / 64), PK.stg.n_buckets);
+          HIPCHK(launch_fill_u64(s, (unsigned long long *)PK.stg.owner, (PK.stg.n_chunks + 7) / 8, ~0ull));
+          HIPCHK(launch_fill_u64(s, PK.stg.bcount, 2 * STG_NB, 0ull));
+        }
         // the work counter restarts for every chunk
         HIPCHK(launch_reset_work(s, P.cnt));
         HIPCHK(hipEventRecord(c->ev[2], s));
-        HIPCHK(launch_persistent(ns, wpe, p32, (unsigned)blocks, block, s, P, S));
+        HIPCHK(launch_persistent(ns, wpe, p32, (unsigned)blocks, block, s, PK, S));
+        if (stg) {
+          HIPCHK(hipEventRecord(c->ev[4], s));
+          // one reduce workgroup per CU (each holds a bucket's 128 KB FP64 array in LDS)
+          const int split = std::max(1, c->n_cu / PK.stg.n_buckets);
+          HIPCHK(launch_stage_reduce(s, PK.stg, P.tally[0].value, c->tl_bins, split));
+        }
         HIPCHK(hipEventRecord(c->ev[3], s));
         HIPCHK(hipEventSynchronize(c->ev[3]));
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
         kernel_ms += ms;
+        if (stg) {
+          HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[3]));
+          stage_ms += ms;
+          staged = true;
+        }
         ++launches;
       }
     }
@@ -1731,6 +1811,7 @@ static void run_batch_impl(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_b
     HIPCHK(hipEventElapsedTime(&total_ms, c->ev[0], c->ev[1]));
     c->last_batch_n = n_particles;
     c->last_batch = batch;
+    if (staged) c->stg_seg_per_hist = (double)c->h_counters->segments / (double)n_particles;
     c->trace_count = P.trace_n > 0 ? std::min<int64_t>((int64_t)c->h_counters->trace_count, c->trace_cap) : 0;
     if (out) {
       const Counters &k = *c->h_counters;
@@ -1750,6 +1831,7 @@ static void run_batch_impl(mcw_ctx *c, int64_t batch, int64_t n_particles, mcw_b
       out->fp64_rechecks = (int64_t)k.rechecks;
       out->hex_segments = (int64_t)k.hex_segments;
       out->trace_records = (int64_t)k.trace_count;
+      out->stage_ms = stage_ms;
     }
   }
 }

@@ -1102,20 +1102,254 @@ __device__ __forceinline__ void req_account(const Params &P, bool dying, const P
   if (full) a.v = 0ull;
 }
 
-template <int NS, bool REC, int WPE, bool P32, bool SERVE = false, bool EIG = false>
+// ------------------------------------------------------------------ staged flux tally
+
+// One wavefront's LDS stage: up to STG_S records per bucket; per bucket the
+// pool position its next full bucket goes to (a multiple of STG_CH means the
+// current chunk is full, or there is none yet); the next unused chunk of the
+// wavefront's slice.
+struct StageWave {
+  double sc[STG_NB][STG_S];
+  uint16_t el[STG_NB][STG_S];
+  uint32_t cnt[STG_NB];
+  uint32_t pos[STG_NB];
+  uint32_t next;
+  uint32_t pad;
+};
+
+static_assert(STG_NB == 64, "a wavefront clears its stage one bucket per lane");
+static_assert(STG_CH % STG_S == 0 && STG_CH % 64 == 0 && STG_CH < 65536, "chunks hold whole stage writes");
+static_assert(64 % STG_S == 0, "lane groups of STG_S");
+static_assert(4 * sizeof(StageWave) + (STG_HASH + 1) * 4 <= 160 * 1024 / 4, "four workgroups of 256 per CU");
+
+constexpr uint32_t STG_NONE = 0xffffffffu;
+
+// Lane group g = lane / STG_S moves the first nrec records of bucket bb
+// (act: the group has a bucket) from the stage to the pool, one record per
+// lane.  The group's first lane takes the next chunk of the wavefront's slice
+// (wave: the wavefront's index in the launch) when the current one is full;
+// past the slice's end the records go to the tally as device atomics.  A
+// chunk's record count is written when it fills, and by the wavefront's final
+// call (final) when it stays partly filled.  Whole wavefront.
+__device__ __forceinline__ void stage_write(const Params &P, StageWave &W, bool act, uint32_t bb, uint32_t nrec,
+                                            int64_t wave, bool final)
+{
+  const StageView &V = P.stg;
+  const int lane = __lane_id();
+  const uint32_t r = (uint32_t)(lane & (STG_S - 1));
+  uint32_t pos = act ? W.pos[bb] : 0u;
+  const bool need = act && r == 0 && nrec > 0 && pos % STG_CH == 0;
+  const unsigned long long nm = __ballot(need);
+  if (nm) {
+    const uint32_t first = W.next;
+    if (need) {
+      const uint32_t ci = first + (uint32_t)__popcll(nm & ((1ull << lane) - 1ull));
+      if (ci < (uint32_t)V.wave_chunks) {
+        const int64_t chunk = wave * V.wave_chunks + ci;
+        V.owner[chunk] = (uint8_t)bb;
+        pos = (uint32_t)(chunk * STG_CH);
+      } else {
+        pos = STG_NONE;
+      }
+    }
+    if (lane == 0) W.next = first + (uint32_t)__popcll(nm);
+  }
+  pos = (uint32_t)__shfl((int)pos, lane & ~(STG_S - 1), 64);
+  if (act && r < nrec) {
+    const double v = W.sc[bb][r];
+    const uint32_t l = W.el[bb][r];
+    if (pos != STG_NONE) {
+      V.sc[(int64_t)pos + r] = v;
+      V.el[(int64_t)pos + r] = (uint16_t)l;
+    } else {
+      atomic_add_dev(P.tally[0].value + (((int64_t)bb << STG_BITS) | l), v);
+    }
+  }
+  if (act && r == 0) {
+    if (pos != STG_NONE) {
+      const uint32_t np = pos + nrec;
+      W.pos[bb] = np;
+      if (np % STG_CH == 0) {
+        if (nrec > 0) V.fill[np / STG_CH - 1] = (uint16_t)STG_CH;
+      } else if (final) {
+        V.fill[np / STG_CH] = (uint16_t)(np % STG_CH);
+      }
+    }
+    W.cnt[bb] = 0u;
+  }
+}
+
+// Stages lane scores (on: the lane has one for bin e) and writes the buckets
+// they fill, eight per pass; a lane that finds its bucket full tries again
+// after the write.  Whole wavefront.
+__device__ __forceinline__ void stage_add(const Params &P, StageWave &W, bool on, uint32_t e, double v, int64_t wave)
+{
+  const uint32_t b = e >> STG_BITS;
+  const uint32_t loc = e & ((1u << STG_BITS) - 1u);
+  const int g = __lane_id() / STG_S;
+  bool pend = on;
+  for (;;) {
+    uint32_t p = STG_S;
+    if (pend) {
+      p = atomicAdd(&W.cnt[b], 1u);
+      if (p < STG_S) {
+        W.sc[b][p] = v;
+        W.el[b][p] = (uint16_t)loc;
+        pend = false;
+      }
+    }
+    // the lane that took a bucket's last slot names it (one lane per bucket)
+    unsigned long long full = __ballot(p == STG_S - 1);
+    while (full) {
+      unsigned long long m = full;
+      for (int i = 0; i < g; ++i) m &= m - 1ull;
+      const bool act = m != 0ull;
+      const uint32_t bb = (uint32_t)__shfl((int)b, act ? __builtin_ctzll(m) : 0, 64);
+      stage_write(P, W, act, bb, STG_S, wave, false);
+      for (int i = 0; i < 64 / STG_S; ++i) full &= full - 1ull;
+    }
+    if (__ballot(pend) == 0ull) break;
+  }
+}
+
+// The wavefront's last records (fewer than STG_S per bucket) and the counts
+// of its partly filled chunks.  Whole wavefront.
+__device__ __forceinline__ void stage_drain(const Params &P, StageWave &W, int64_t wave)
+{
+  const uint32_t g = (uint32_t)(__lane_id() / STG_S);
+  for (uint32_t j = 0; j < (uint32_t)STG_NB; j += 64 / STG_S) {
+    const uint32_t bb = j + g;
+    const bool act = bb < (uint32_t)P.stg.n_buckets;
+    stage_write(P, W, act, bb, act ? W.cnt[bb] : 0u, wave, true);
+  }
+}
+
+// Chunks per bucket: bcount[b] += chunks whose owner is b.
+extern "C" __global__ void __launch_bounds__(256) k_stage_count(StageView V)
+{
+  __shared__ unsigned h[STG_NB];
+  if (threadIdx.x < STG_NB) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < V.n_chunks; c += stride) {
+    const unsigned o = V.owner[c];
+    if (o < (unsigned)STG_NB) atomicAdd(&h[o], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < STG_NB && h[threadIdx.x]) atomicAdd(V.bcount + threadIdx.x, (unsigned long long)h[threadIdx.x]);
+}
+
+constexpr int STG_TILE = 4; // chunks per thread of k_stage_scatter
+
+// Lists each taken chunk under its bucket: bucket b's chunks fill list[off(b)
+// .. off(b) + bcount[b]), off the exclusive prefix sum of bcount, in any
+// order, as chunk << 9 | records; bcount[STG_NB + b] is the bucket's cursor
+// (zero before the call).
+extern "C" __global__ void __launch_bounds__(256) k_stage_scatter(StageView V)
+{
+  __shared__ unsigned h[STG_NB];
+  __shared__ unsigned long long base[STG_NB];
+  if (threadIdx.x < STG_NB) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const int64_t c0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * STG_TILE;
+  unsigned o[STG_TILE], rank[STG_TILE];
+#pragma unroll
+  for (int q = 0; q < STG_TILE; ++q) {
+    o[q] = c0 + q < V.n_chunks ? (unsigned)V.owner[c0 + q] : (unsigned)STG_FREE;
+    rank[q] = o[q] < (unsigned)STG_NB ? atomicAdd(&h[o[q]], 1u) : 0u;
+  }
+  __syncthreads();
+  if (threadIdx.x < STG_NB) {
+    const int b = threadIdx.x;
+    unsigned long long off = 0;
+    for (int j = 0; j < b; ++j) off += V.bcount[j];
+    base[b] = h[b] ? off + atomicAdd(V.bcount + STG_NB + b, (unsigned long long)h[b]) : 0ull;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < STG_TILE; ++q)
+    if (o[q] < (unsigned)STG_NB) {
+      const uint32_t n = min((uint32_t)V.fill[c0 + q], (uint32_t)STG_CH);
+      V.list[base[o[q]] + rank[q]] = (uint32_t)(c0 + q) << 9 | n;
+    }
+}
+
+constexpr int STG_UNROLL = 4; // chunks in flight per wavefront of k_stage_reduce
+
+// Sums bucket blockIdx.x's records in LDS (workgroups blockIdx.y share its
+// chunks, STG_UNROLL at a time per wavefront) and adds the sums to
+// tally[bucket bins].
+extern "C" __global__ void __launch_bounds__(1024) k_stage_reduce(StageView V, double *tally, int64_t n_bins)
+{
+  static_assert(STG_CH <= 256 && (1 << 23) * (int64_t)STG_CH >= ((int64_t)1 << 31), "list entries: chunk << 9 | count");
+  __shared__ double acc[1 << STG_BITS];
+  const int b = blockIdx.x;
+  for (int i = threadIdx.x; i < (1 << STG_BITS); i += blockDim.x) acc[i] = 0.0;
+  unsigned long long off = 0;
+  for (int j = 0; j < b; ++j) off += V.bcount[j];
+  const int64_t cnt = (int64_t)V.bcount[b];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t waves = (int64_t)gridDim.y * (blockDim.x >> 6);
+  constexpr int Q = STG_CH / 64;
+  constexpr uint32_t MASK = (1u << STG_BITS) - 1u;
+  for (int64_t i = wave; i < cnt; i += STG_UNROLL * waves) {
+    uint32_t ent[STG_UNROLL];
+#pragma unroll
+    for (int u = 0; u < STG_UNROLL; ++u) ent[u] = i + u * waves < cnt ? V.list[off + i + u * waves] : 0u;
+    double v[STG_UNROLL][Q];
+    uint32_t l[STG_UNROLL][Q];
+#pragma unroll
+    for (int u = 0; u < STG_UNROLL; ++u) {
+      const int n = (int)(ent[u] & 511u);
+      const int64_t at = (int64_t)(ent[u] >> 9) * STG_CH;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int k = q * 64 + lane;
+        v[u][q] = k < n ? V.sc[at + k] : 0.0;
+        l[u][q] = k < n ? (uint32_t)V.el[at + k] : 0u;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < STG_UNROLL; ++u) {
+      const int n = (int)(ent[u] & 511u);
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (q * 64 + lane < n) atomicAdd(&acc[l[u][q] & MASK], v[u][q]);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < (1 << STG_BITS); i += blockDim.x) {
+    const int64_t e = ((int64_t)b << STG_BITS) + i;
+    if (e < n_bins && acc[i] != 0.0) atomic_add_dev(tally + e, acc[i]);
+  }
+}
+
+template <int NS, bool REC, int WPE, bool P32, bool SERVE = false, bool EIG = false, bool STG = false>
 // Mixed-mesh variants (NS = 0) of the fixed-source and serving kernels are
 // held to 128 VGPRs, four waves per SIMD, which they measured faster at than
-// at three (C5: 150 against 167 ms per batch); WPE > 0 asks for more.
+// at three (C5: 150 against 167 ms per batch), and so are the staging
+// variants without records; WPE > 0 asks for more.
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : (NS == 0 && !EIG ? 4 : 1), 8)))
+    __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : ((NS == 0 && !EIG) || (STG && !REC) ? 4 : 1), 8)))
 k_persistent(Params P, SourceBank S)
 {
-  __shared__ int32_t s_hash[MAX_HASH + 1];
+  // the staging variants keep every other energy bucket (a 16 KB table) to
+  // leave LDS for four wavefront stages per workgroup
+  __shared__ int32_t s_hash[STG ? STG_HASH + 1 : MAX_HASH + 1];
   // a small per-element flux tally (a few hundred elements) is summed per
   // workgroup in LDS and added to the device tally once at the end: every
   // segment's atomic on one of a few hundred lines serialised in the caches
-  const int32_t n_lds = SERVE ? 0 : P.lds_tally + P.lds_coll;
+  const int32_t n_lds = SERVE || STG ? 0 : P.lds_tally + P.lds_coll;
   for (int i = threadIdx.x; i < n_lds; i += blockDim.x) s_tally[i] = 0.0;
+  StageWave &W = reinterpret_cast<StageWave *>(s_tally)[STG ? threadIdx.x >> 6 : 0];
+  const int64_t wave_ix = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (STG) {
+    W.cnt[__lane_id()] = 0u;
+    W.pos[__lane_id()] = 0u;
+    if (__lane_id() == 0) W.next = 0u;
+  }
   stage_hash(P.xs, s_hash);
   Counters *C = P.cnt;
   WaveCount wc;
@@ -1208,7 +1442,9 @@ k_persistent(Params P, SourceBank S)
       // the track-length bins are in LDS only when P.lds_tally covers them;
       // P.lds_coll alone (a large track-length tally beside a small
       // collision one) keeps the track-length adds on the device
-      if (!SERVE && P.lds_tally) {
+      if (STG) {
+        stage_add(P, W, tl_ptr != nullptr, tl_ptr ? (uint32_t)(tl_ptr - P.tally[0].value) : 0u, tl_val, wave_ix);
+      } else if (!SERVE && P.lds_tally) {
         if (tl_ptr) atomicAdd(s_tally + (tl_ptr - P.tally[0].value), tl_val);
       } else if (SERVE || P.tl_fast) {
         atomic_add_agg(tl_ptr != nullptr, tl_ptr, tl_val);
@@ -1246,6 +1482,7 @@ k_persistent(Params P, SourceBank S)
     count_collision(wc, hit, ev);
   }
   if (SERVE) req_flush(P, racc.v != 0ull, racc.req, racc.v);
+  if (STG) stage_drain(P, W, wave_ix);
   flush_counts(C, wc, track);
   if (EIG) {
     const unsigned long long w = wave_sum((unsigned long long)fb_written);
@@ -1484,18 +1721,50 @@ template <bool REC> const void *eigen_kernel_r(int ns, bool p32)
 
 // The kernel variant for side count, record mode, occupancy hint, precision
 // and run mode.
-const void *persistent_kernel(int ns, int wpe, bool rec, bool p32, bool eig)
+// and, with stg, the staged-flux-tally variant (TET4 fixed-source batches,
+// compiler-chosen occupancy).
+const void *persistent_kernel(int ns, int wpe, bool rec, bool p32, bool eig, bool stg)
 {
+  if (stg && ns == 4 && wpe == 0 && !eig) {
+    if (p32) return rec ? (const void *)k_persistent<4, true, 0, true, false, false, true>
+                        : (const void *)k_persistent<4, false, 0, true, false, false, true>;
+    return rec ? (const void *)k_persistent<4, true, 0, false, false, false, true>
+               : (const void *)k_persistent<4, false, 0, false, false, false, true>;
+  }
+  if (stg) return nullptr;
   if (eig) return rec ? eigen_kernel_r<true>(ns, p32) : eigen_kernel_r<false>(ns, p32);
   return wpe == 5 ? persistent_kernel_w<5>(ns, rec, p32) : persistent_kernel_w<0>(ns, rec, p32);
+}
+
+// Dynamic LDS of a persistent launch: the small LDS tallies, or with a staged
+// tally one StageWave per wavefront.
+size_t persistent_lds_bytes(const Params &P, int block)
+{
+  if (P.stg.n_buckets > 0) return (size_t)(block / 64) * sizeof(StageWave);
+  return (size_t)(P.lds_tally + P.lds_coll) * sizeof(double);
 }
 
 hipError_t launch_persistent(int ns, int wpe, bool p32, unsigned blocks, int block, hipStream_t s, const Params &P,
                              const SourceBank &S)
 {
-  const void *k = persistent_kernel(ns, wpe, P.record != 0, p32, P.eigen != 0);
+  const void *k = persistent_kernel(ns, wpe, P.record != 0, p32, P.eigen != 0, P.stg.n_buckets > 0);
+  if (!k) return hipErrorInvalidValue;
   void *args[] = {(void *)&P, (void *)&S};
-  return hipLaunchKernel(k, dim3(blocks), dim3(block), args, (size_t)(P.lds_tally + P.lds_coll) * sizeof(double), s);
+  return hipLaunchKernel(k, dim3(blocks), dim3(block), args, persistent_lds_bytes(P, block), s);
+}
+
+// Sums a staged flux tally's pool into tally[n_bins] (n_bins <= STG_NB << STG_BITS):
+// chunks per bucket, the chunk list, then the per-bucket sums.  bcount holds
+// 2 STG_NB zeros before the call.
+hipError_t launch_stage_reduce(hipStream_t s, const StageView &V, double *tally, int64_t n_bins, int split)
+{
+  const unsigned gc = (unsigned)std::min<int64_t>((V.n_chunks + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_stage_count, dim3(gc), dim3(256), 0, s, V);
+  const unsigned gs = (unsigned)((V.n_chunks + 256 * STG_TILE - 1) / (256 * STG_TILE));
+  hipLaunchKernelGGL(k_stage_scatter, dim3(gs), dim3(256), 0, s, V);
+  hipLaunchKernelGGL(k_stage_reduce, dim3((unsigned)V.n_buckets, (unsigned)split), dim3(1024), 0, s, V, tally,
+                     n_bins);
+  return hipGetLastError();
 }
 
 hipError_t launch_source_req(hipStream_t s, const Params &P, const SourceBank &S, int32_t n_slices, int64_t n)

@@ -221,6 +221,34 @@ struct FissionView {
 
 constexpr int PROGENY_BITS = 24;
 
+// Staged per-element flux tally.  Instead of one device atomic per segment
+// (every FP64 atomic is performed memory-side, at a fixed rate), a wavefront
+// of the persistent kernel stages its track-length scores in LDS by bucket of
+// 2^STG_BITS consecutive tally bins and writes a full bucket of STG_S records
+// with plain stores to a chunk of its own slice of a device pool.  After the
+// launch the chunks are listed by bucket (k_stage_count, k_stage_scatter) and
+// k_stage_reduce sums each bucket's records in an LDS array of the bucket's
+// bins and adds the sums to the tally.
+constexpr int STG_BITS = 14;           // bins per bucket: 2^14 (the reduce's 128 KB FP64 LDS array)
+constexpr int STG_NB = 64;             // buckets: tallies of up to 2^20 bins
+constexpr int STG_S = 8;               // records per bucket staged in LDS
+constexpr int STG_CH = 256;            // records per pool chunk (a multiple of STG_S)
+constexpr int STG_HASH = MAX_HASH / 2; // energy-bucket LDS table of the staging kernels (every other bucket)
+constexpr uint8_t STG_FREE = 0xff;     // owner of a chunk no wavefront took
+
+struct StageView {
+  double *sc;                  // per record: the score
+  uint16_t *el;                // per record: the bin within its bucket
+  uint8_t *owner;              // per chunk: its bucket, STG_FREE before a wavefront takes it
+  uint16_t *fill;              // per chunk: records written
+  uint32_t *list;              // chunks listed by bucket (k_stage_scatter)
+  unsigned long long *bcount;  // per bucket: its chunks; then per bucket: list cursor
+  int64_t wave_chunks;         // chunks of each wavefront's slice
+  int64_t n_chunks;            // chunks of the pool: wavefronts x wave_chunks
+  int32_t n_buckets;           // 0: staging off
+  int32_t pad;
+};
+
 // Global counters, accumulated with one atomic per wavefront.
 struct Counters {
   unsigned long long segments, collisions, absorptions, leaks, cutoffs, lost, reflections, rechecks;
@@ -264,6 +292,7 @@ struct Params {
   int64_t trace_n;
   int64_t trace_cap;
   FissionView fb;
+  StageView stg;           // staged flux tally (persistent kernel, STG variants)
 };
 
 } // namespace mcw

@@ -30,10 +30,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_sizes_match_header():
     import ctypes as C
-    assert C.sizeof(abi.Options) == 80
+    assert C.sizeof(abi.Options) == 88
     assert C.sizeof(abi.Source) == 144
     assert C.sizeof(abi.History) == 72
-    assert C.sizeof(abi.BatchStats) == 120
+    assert C.sizeof(abi.BatchStats) == 128
     assert C.sizeof(abi.TraceRecord) == 88 == abi.trace_dtype().itemsize
     assert C.sizeof(abi.Site) == 72 == abi.site_dtype().itemsize
 
