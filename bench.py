@@ -147,7 +147,7 @@ def bench_serving(args, tuning):
     return out
 
 
-def tune_eigen_lanes(prob, n, candidates=(16, 24, 32, 40), log_name="eigen"):
+def tune_eigen_lanes(prob, n, candidates=(16, 24, 32, 40), log_name="eigen", tally_stage=0):
     """lanes_min for an eigenvalue kernel, picked on the workload itself: each
     candidate runs batches 1 and 2 (the second from a fission source) on a
     fresh context; the fastest batch-2 transport wins.  Results do not depend
@@ -156,7 +156,8 @@ def tune_eigen_lanes(prob, n, candidates=(16, 24, 32, 40), log_name="eigen"):
     from macaw_amd.lib import Context
     best, best_ms = candidates[0], float("inf")
     for lm in candidates:
-        ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=0, lanes_min=lm)
+        ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=0, lanes_min=lm,
+                      tally_stage=tally_stage)
         ctx.run_batch(1, n)
         ms = ctx.run_batch(2, n)["time_ms"]
         ctx.close()
@@ -177,9 +178,10 @@ def bench_eigen(args, tuning):
     n = args.eigen_histories
     t = time.perf_counter()
     prob = P.k2_tet_eigen(n_particles=n)
-    lanes = args.lanes_min if args.no_autotune and args.lanes_min else tune_eigen_lanes(prob, n)
+    stage = tuning.get("tally_stage", args.tally_stage)
+    lanes = args.lanes_min if args.no_autotune and args.lanes_min else tune_eigen_lanes(prob, n, tally_stage=stage)
     ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=2,
-                  lanes_min=lanes, block_size=tuning.get("block_size", 0))
+                  lanes_min=lanes, block_size=tuning.get("block_size", 0), tally_stage=stage)
     log(f"eigen: {prob.name} ready ({time.perf_counter() - t:.1f}s)")
     ctx.run_batch(1, n)  # warm: allocations and code objects
     rows = []

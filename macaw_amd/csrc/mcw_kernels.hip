@@ -1294,10 +1294,17 @@ extern "C" __global__ void __launch_bounds__(1024) k_stage_reduce(StageView V, d
   const int64_t waves = (int64_t)gridDim.y * (blockDim.x >> 6);
   constexpr int Q = STG_CH / 64;
   constexpr uint32_t MASK = (1u << STG_BITS) - 1u;
+  // every load below is unconditional (indices clamped into the chunk or the
+  // list, results masked after): loads under a branch each waited for their
+  // data before the next was issued
   for (int64_t i = wave; i < cnt; i += STG_UNROLL * waves) {
     uint32_t ent[STG_UNROLL];
 #pragma unroll
-    for (int u = 0; u < STG_UNROLL; ++u) ent[u] = i + u * waves < cnt ? V.list[off + i + u * waves] : 0u;
+    for (int u = 0; u < STG_UNROLL; ++u) {
+      const int64_t j = i + u * waves;
+      const uint32_t e = V.list[off + (j < cnt ? j : cnt - 1)];
+      ent[u] = j < cnt ? e : 0u;
+    }
     double v[STG_UNROLL][Q];
     uint32_t l[STG_UNROLL][Q];
 #pragma unroll
@@ -1307,8 +1314,9 @@ extern "C" __global__ void __launch_bounds__(1024) k_stage_reduce(StageView V, d
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int k = q * 64 + lane;
-        v[u][q] = k < n ? V.sc[at + k] : 0.0;
-        l[u][q] = k < n ? (uint32_t)V.el[at + k] : 0u;
+        const int64_t r = at + (k < n ? k : 0);
+        v[u][q] = V.sc[r];
+        l[u][q] = V.el[r];
       }
     }
 #pragma unroll
@@ -1719,18 +1727,21 @@ template <bool REC> const void *eigen_kernel_r(int ns, bool p32)
              : (const void *)k_persistent<0, REC, 0, false, false, true>;
 }
 
-// The kernel variant for side count, record mode, occupancy hint, precision
-// and run mode.
-// and, with stg, the staged-flux-tally variant (TET4 fixed-source batches,
-// compiler-chosen occupancy).
+// Staged-flux-tally variants (TET4 meshes, fixed-source batches: an
+// eigenvalue kernel with a stage needs 180 VGPRs and measured slower).
+const void *staged_kernel(bool rec, bool p32)
+{
+  if (p32) return rec ? (const void *)k_persistent<4, true, 0, true, false, false, true>
+                      : (const void *)k_persistent<4, false, 0, true, false, false, true>;
+  return rec ? (const void *)k_persistent<4, true, 0, false, false, false, true>
+             : (const void *)k_persistent<4, false, 0, false, false, false, true>;
+}
+
+// The kernel variant for side count, record mode, occupancy hint, precision,
+// run mode and tally staging.
 const void *persistent_kernel(int ns, int wpe, bool rec, bool p32, bool eig, bool stg)
 {
-  if (stg && ns == 4 && wpe == 0 && !eig) {
-    if (p32) return rec ? (const void *)k_persistent<4, true, 0, true, false, false, true>
-                        : (const void *)k_persistent<4, false, 0, true, false, false, true>;
-    return rec ? (const void *)k_persistent<4, true, 0, false, false, false, true>
-               : (const void *)k_persistent<4, false, 0, false, false, false, true>;
-  }
+  if (stg && ns == 4 && wpe == 0 && !eig) return staged_kernel(rec, p32);
   if (stg) return nullptr;
   if (eig) return rec ? eigen_kernel_r<true>(ns, p32) : eigen_kernel_r<false>(ns, p32);
   return wpe == 5 ? persistent_kernel_w<5>(ns, rec, p32) : persistent_kernel_w<0>(ns, rec, p32);
