@@ -7,7 +7,7 @@ Workloads (one "step" = one batch of fixed-source histories):
      materials of 4 synthetic nuclides (50k-point grids), uniform volumetric
      Watt source, 1e7 histories per batch, per-element track-length flux;
   c3 the same generator at 8M tets (107^3 lattice), 16 materials of 4 of 32
-     nuclides on the unionized grid, 1e8 histories per batch in bank chunks;
+     nuclides on the unionized grid, 1e8 histories per batch in one launch;
   c5 a 20M-element mixed TET4/HEX8 mesh graded toward a corner (tets meet
      hexes through split interfaces), 128 materials of 4 of 64 nuclides with
      100k-point grids, low densities, a volumetric Watt source linear in mu,
@@ -27,7 +27,7 @@ batches, each with its fission-bank sort and resampling), the reference's
 BEAVRS assembly example run from its driver input (c2 runs only: the
 example's Exodus mesh extruded to 95,504 HEX8, eigenvalue, 1e7 histories per
 batch), the 8M-tet batching + precision workload (c2 runs only: 16
-materials of 4 of 32 nuclides, 1e8 histories per batch in bank chunks,
+materials of 4 of 32 nuclides, 1e8 histories per batch in one launch,
 sorted sources; FP64, mixed precision and plain), the 20M-element stress
 workload (c2 runs only: mixed TET4/HEX8, 64 nuclides, 1e7 histories per
 batch; FP64, mixed precision and plain) and a CPU baseline
@@ -264,7 +264,7 @@ def bench_large(args, tuning, which):
     ratio.
       c3: the batching + precision workload, 8,066,911 TET4 (107^3 lattice),
           16 materials of 4 of 32 nuclides on a 1.6M-point union grid, 1e8
-          histories per batch in bank chunks of 2^24;
+          histories per batch in one launch;
       c5: the stress workload, 19,999,360 mixed TET4/HEX8 elements graded
           toward a corner, 128 materials of 4 of 64 nuclides on a 6.4M-point
           union grid, low densities, sources linear in mu, 1e7 histories per
@@ -273,7 +273,7 @@ def bench_large(args, tuning, which):
     from macaw_amd.lib import Context
 
     if which == "c3":
-        n, n_steps, lanes = args.c3_histories, args.c3_steps, tuning.get("lanes_min", 0)
+        n, n_steps, lanes = args.c3_histories, args.c3_steps, args.c3_lanes_min
         gen = P.c3_tet_vacuum
     else:
         n, n_steps, lanes = args.c5_histories, args.c5_steps, args.c5_lanes_min
@@ -283,11 +283,18 @@ def bench_large(args, tuning, which):
     all_tet = bool((prob.mesh.elem_type == abi.TET4).all())
     log(f"{which}: {prob.mesh.n_elem} elements, {len(prob.materials)} materials, {len(prob.nuclides)} nuclides "
         f"({time.perf_counter() - t:.1f}s)")
-    opts = dict(mode=abi.MODE_BATCHED, sort_cadence=1, lanes_min=lanes, block_size=tuning.get("block_size", 0))
+    # a whole batch in one launch (up to 2^27 histories): C3's 1e8 histories in
+    # one launch took 1365 ms per batch against 1399 ms in six of 2^24 (mixed
+    # precision 1202 against 1242 ms), since each launch ends on a tail of
+    # long histories
+    bank = min(n, 1 << 27)
+    opts = dict(mode=abi.MODE_BATCHED, sort_cadence=1, lanes_min=lanes, block_size=tuning.get("block_size", 0),
+                bank_capacity=bank)
     kind = "tet4" if all_tet else "mixed"
     out = {"workload": f"{which}_{kind}_{prob.mesh.n_elem // 1000}k_vacuum", "n_elem": prob.mesh.n_elem,
            "histories_per_step": n, "materials": len(prob.materials), "nuclides": len(prob.nuclides),
-           "schedule": f"persistent, sorted sources, lanes_min {lanes or 'default'}, bank chunks of 2^24"}
+           "schedule": f"persistent, sorted sources, lanes_min {lanes or 'default'}, "
+                       f"{-(-n // bank)} launch(es) of up to {bank} histories"}
     ctxs = {}
     ns = 4 if all_tet else 0
     # the two other ceilings of these kernels, measured on arrays of the
@@ -479,6 +486,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 8M-tet batching + precision leg")
     ap.add_argument("--c3-histories", type=int, default=100_000_000)
     ap.add_argument("--c3-steps", type=int, default=2)
+    ap.add_argument("--c3-lanes-min", type=int, default=24)
     ap.add_argument("--no-c5", action="store_true", help="skip the 20M-element stress leg")
     ap.add_argument("--c5-histories", type=int, default=10_000_000)
     ap.add_argument("--c5-steps", type=int, default=3)
