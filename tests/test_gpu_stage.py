@@ -12,7 +12,9 @@ tally.  Only the order of the FP64 additions changes, so:
 - every element's flux agrees with the atomic path within 1e-12 and with the
   oracle within the 1e-10 of the other parity tests;
 - a pool too small for the launch sends the rest to device atomics with the
-  same result, and so do launches split by bank_capacity."""
+  same result, and so do launches split by bank_capacity;
+- a pool sized from a small batch grows for a larger one, and SUM and SUM_SQ
+  over several batches agree with the atomic path."""
 import numpy as np
 import pytest
 
@@ -109,3 +111,21 @@ def test_staged_matches_oracle(c2s):
     assert_records_equal(h, o.histories(N_ORACLE))
     assert_tally_close(t, o.tally(1, abi.RESULT_VALUE))
     assert_tally_close(c, o.tally(2, abi.RESULT_VALUE))
+
+
+def test_staged_pool_grows_and_sums_accumulate(c2s):
+    """Batches of 20,000 then 200,000 then 50,000 histories in one context:
+    the pool sized from the first batch grows for the second (the slices of a
+    launch are sized from the last batch's segments per history), and the
+    SUM over three batches equals the device-atomic context's within 1e-12."""
+    print("[stage] growing batches, staged and atomics", flush=True)
+    sizes = (20_000, 200_000, 50_000)
+    ctx = {ts: Context(c2s, mode=abi.MODE_BATCHED, tally_stage=ts) for ts in (0, -1)}
+    for b, n in enumerate(sizes, start=1):
+        st = {ts: g.run_batch(b, n) for ts, g in ctx.items()}
+        assert st[0]["stage_ms"] > 0.0 and st[-1]["stage_ms"] == 0.0
+        same_counts(st[0], st[-1])
+    for kind in (abi.RESULT_SUM, abi.RESULT_SUM_SQ):
+        assert_tally_close(ctx[0].tally(1, kind), ctx[-1].tally(1, kind), 1e-12)
+    for g in ctx.values():
+        g.close()
