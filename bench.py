@@ -272,11 +272,15 @@ def bench_large(args, tuning, which):
     from macaw_amd import abi, problems as P
     from macaw_amd.lib import Context
 
+    # lanes_min per precision, from sweeps on these workloads (one launch per
+    # batch): C3 FP64 24 (1331 ms; 16-40 within 1.5%), mixed 32 (1154 against
+    # 1168 ms at 24); C5 48 for both (FP64 157.0 against 157.9 ms at 40, mixed
+    # 126.6 against 128.8 ms)
     if which == "c3":
-        n, n_steps, lanes = args.c3_histories, args.c3_steps, args.c3_lanes_min
+        n, n_steps, lanes = args.c3_histories, args.c3_steps, (args.c3_lanes_min, args.c3_lanes_min_mixed)
         gen = P.c3_tet_vacuum
     else:
-        n, n_steps, lanes = args.c5_histories, args.c5_steps, args.c5_lanes_min
+        n, n_steps, lanes = args.c5_histories, args.c5_steps, (args.c5_lanes_min, args.c5_lanes_min)
         gen = P.c5_mixed
     t = time.perf_counter()
     prob = gen(n_particles=n)
@@ -288,12 +292,11 @@ def bench_large(args, tuning, which):
     # precision 1202 against 1242 ms), since each launch ends on a tail of
     # long histories
     bank = min(n, 1 << 27)
-    opts = dict(mode=abi.MODE_BATCHED, sort_cadence=1, lanes_min=lanes, block_size=tuning.get("block_size", 0),
-                bank_capacity=bank)
+    opts = dict(mode=abi.MODE_BATCHED, sort_cadence=1, block_size=tuning.get("block_size", 0), bank_capacity=bank)
     kind = "tet4" if all_tet else "mixed"
     out = {"workload": f"{which}_{kind}_{prob.mesh.n_elem // 1000}k_vacuum", "n_elem": prob.mesh.n_elem,
            "histories_per_step": n, "materials": len(prob.materials), "nuclides": len(prob.nuclides),
-           "schedule": f"persistent, sorted sources, lanes_min {lanes or 'default'}, "
+           "schedule": f"persistent, sorted sources, lanes_min {lanes[0]} (FP64) and {lanes[1]} (mixed), "
                        f"{-(-n // bank)} launch(es) of up to {bank} histories"}
     ctxs = {}
     ns = 4 if all_tet else 0
@@ -302,8 +305,8 @@ def bench_large(args, tuning, which):
     # 160 B per element) and FP64 atomic adds (the flux tally)
     glines = 1 << min(25, max(16, (prob.mesh.n_elem * 160 // 128 - 1).bit_length()))
     abins = 1 << min(28, max(10, (prob.mesh.n_elem - 1).bit_length()))
-    for form, prec in (("fp64", abi.PREC_FP64), ("mixed", abi.PREC_MIXED)):
-        ctx = Context(prob, precision=prec, **opts)
+    for form, prec, lm in (("fp64", abi.PREC_FP64, lanes[0]), ("mixed", abi.PREC_MIXED, lanes[1])):
+        ctx = Context(prob, precision=prec, lanes_min=lm, **opts)
         ctxs[form] = ctx
         steps, wall = timed_steps(ctx, n, n_steps, 1, f"{which} {form}")
         segs = sum(s["segments"] for s in steps)
@@ -487,10 +490,11 @@ def main():
     ap.add_argument("--c3-histories", type=int, default=100_000_000)
     ap.add_argument("--c3-steps", type=int, default=2)
     ap.add_argument("--c3-lanes-min", type=int, default=24)
+    ap.add_argument("--c3-lanes-min-mixed", type=int, default=32)
     ap.add_argument("--no-c5", action="store_true", help="skip the 20M-element stress leg")
     ap.add_argument("--c5-histories", type=int, default=10_000_000)
     ap.add_argument("--c5-steps", type=int, default=3)
-    ap.add_argument("--c5-lanes-min", type=int, default=40)
+    ap.add_argument("--c5-lanes-min", type=int, default=48)
     ap.add_argument("--assembly-histories", type=int, default=10_000_000)
     ap.add_argument("--gather-sweep", action="store_true",
                     help="only measure the device's rate of pseudo-random 128-byte line reads (k_gather_probe) "
