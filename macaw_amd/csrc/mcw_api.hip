@@ -78,6 +78,7 @@ hipError_t launch_sort_bank(hipStream_t s, const MeshView &M, const BankView &in
                             bool rec, Counters *C, int qout, int shift, int bits, uint32_t mat_mask, uint32_t *k0,
                             uint32_t *k1, uint32_t *v0, uint32_t *v1, void *tmp, size_t tmp_bytes);
 hipError_t launch_serve(int ns, bool p32, unsigned blocks, hipStream_t s, const Params &P, const SourceBank &S);
+size_t serve_lds_bytes();
 } // namespace mcw
 
 using namespace mcw;
@@ -2336,6 +2337,10 @@ bool launch_round(mcw_server *s, int mi, RoundSlot &R)
   P.hist_end = total;
   P.slices = R.slices.as<ReqSlice>();
   P.cnt = R.counters.as<Counters>();
+  // the serving kernel keeps every other energy bucket in LDS (room for its tally caches)
+  P.xs.hash = c->uhash2.as<int32_t>();
+  P.xs.n_hash = STG_HASH;
+  P.xs.inv_dl = c->base.xs.inv_dl * 0.5;
   const int ns = c->mesh.uniform_sides;
   const bool p32 = c->opt.precision == MCW_PREC_MIXED && ns != 6;
   HIPCHK(launch_reset_work(st, P.cnt));
@@ -2386,7 +2391,7 @@ int mcw_server_create(mcw_server **out, mcw_ctx *const *meshes, int32_t n_meshes
       int per_cu = 0;
       const int ns = c->mesh.uniform_sides;
       HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, serve_kernel(ns, c->opt.precision == MCW_PREC_MIXED && ns != 6), 256, 0));
+          &per_cu, serve_kernel(ns, c->opt.precision == MCW_PREC_MIXED && ns != 6), 256, serve_lds_bytes()));
       m.blocks = c->n_cu * std::max(1, per_cu);
       // the pools of request tallies and pinned result buffers start at the
       // in-flight limit, so the loop allocates nothing, and an admission's

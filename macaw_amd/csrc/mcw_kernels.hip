@@ -1334,18 +1334,112 @@ extern "C" __global__ void __launch_bounds__(1024) k_stage_reduce(StageView V, d
   }
 }
 
+// ------------------------------------------------------------------ request-server tally cache
+
+// A request's flux tally is its own array over the mesh, so a round's bins
+// are (request slice, element) pairs: too many to stage in LDS.  Point
+// sources send every history's first segments to the same few elements,
+// whose device atomics then serialise.  Each wavefront of the serving kernel
+// keeps the sums of its last SC_N (slice, element) keys in LDS: an add to a
+// cached key stays in LDS, and a key that needs the slot adds the slot's sum
+// to the device tally first.  Every cached sum is added at the end of the
+// launch.
+constexpr int SC_N = 256;
+constexpr uint32_t SC_EMPTY = 0xffffffffu;
+constexpr int SC_ELEM_BITS = 26;
+
+struct ServeCache {
+  double val[SC_N];
+  uint32_t key[SC_N]; // slice << SC_ELEM_BITS | element, SC_EMPTY when free
+  uint8_t claim[SC_N];
+};
+
+static_assert(4 * sizeof(ServeCache) + (STG_HASH + 1) * 4 <= 160 * 1024 / 4, "four serving workgroups per CU");
+
+// Volatile pointers into LDS (address space 3): a plain volatile pointer is a
+// generic one, which turns the accesses into flat instructions.
+template <class T> using lds_volatile = __attribute__((address_space(3))) volatile T *;
+
+__device__ __forceinline__ double *serve_bin(const Params &P, uint32_t key)
+{
+  return P.slices[key >> SC_ELEM_BITS].tally + (key & ((1u << SC_ELEM_BITS) - 1u));
+}
+
+// Adds v to slice `slice`'s bin `elem` (bin: its address) through the
+// wavefront's cache.  Lanes with the same key are summed first; then each
+// round, one lane per slot (the one whose id the slot's claim byte keeps)
+// updates it.  Keys that do not fit (slice >= 63 or elem >= 2^26) go to the
+// device directly.  Whole wavefront.
+__device__ __forceinline__ void serve_cache_add(const Params &P, ServeCache &K, bool on, int32_t slice,
+                                                int32_t elem, double *bin, double v)
+{
+  const int lane = __lane_id();
+  const bool fits = on && slice >= 0 && slice < 63 && (uint32_t)elem < (1u << SC_ELEM_BITS);
+  if (on && !fits) atomic_add_dev(bin, v);
+  const uint32_t key = ((uint32_t)slice << SC_ELEM_BITS) | (uint32_t)elem;
+  bool cand = fits, pend = fits;
+  double vv = v;
+  for (int g = 0; g < 4; ++g) {
+    const unsigned long long m = __ballot(cand);
+    if (m == 0ull) break;
+    const int leader = __builtin_ctzll(m);
+    const uint32_t kl = (uint32_t)__shfl((int)key, leader, 64);
+    const bool same = cand && key == kl;
+    if (__popcll(__ballot(same)) < 2) break;
+    const double sum = wave_sum_dbl(same ? vv : 0.0);
+    if (same) {
+      cand = false;
+      if (lane == leader) vv = sum;
+      else pend = false;
+    }
+  }
+  const uint32_t slot = (key ^ (key >> 7) ^ (key >> SC_ELEM_BITS)) & (SC_N - 1);
+  // volatile: a lane must read back the claim byte the wavefront's store left
+  // (another lane's id when several claimed the slot), not its own store
+  // forwarded by the compiler; and the key and sum other lanes wrote
+  const lds_volatile<uint8_t> claim = (lds_volatile<uint8_t>)K.claim;
+  const lds_volatile<uint32_t> keys = (lds_volatile<uint32_t>)K.key;
+  const lds_volatile<double> vals = (lds_volatile<double>)K.val;
+  while (__ballot(pend)) {
+    if (pend) claim[slot] = (uint8_t)lane;
+    if (pend && claim[slot] == (uint8_t)lane) {
+      const uint32_t k = keys[slot];
+      if (k == key) {
+        vals[slot] = vals[slot] + vv;
+      } else {
+        if (k != SC_EMPTY) atomic_add_dev(serve_bin(P, k), vals[slot]);
+        keys[slot] = key;
+        vals[slot] = vv;
+      }
+      pend = false;
+    }
+  }
+}
+
+// The cached sums, at the end of the launch.  Whole wavefront.
+__device__ __forceinline__ void serve_cache_flush(const Params &P, ServeCache &K)
+{
+  const lds_volatile<uint32_t> keys = (lds_volatile<uint32_t>)K.key;
+  const lds_volatile<double> vals = (lds_volatile<double>)K.val;
+  for (int i = __lane_id(); i < SC_N; i += 64) {
+    const uint32_t k = keys[i];
+    if (k != SC_EMPTY) atomic_add_dev(serve_bin(P, k), vals[i]);
+  }
+}
+
 template <int NS, bool REC, int WPE, bool P32, bool SERVE = false, bool EIG = false, bool STG = false>
 // Mixed-mesh variants (NS = 0) of the fixed-source and serving kernels are
 // held to 128 VGPRs, four waves per SIMD, which they measured faster at than
 // at three (C5: 150 against 167 ms per batch), and so are the staging
-// variants without records; WPE > 0 asks for more.
+// variants without records and the TET4 serving variants; WPE > 0 asks for
+// more.
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : ((NS == 0 && !EIG) || (STG && !REC) ? 4 : 1), 8)))
+    __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : ((NS == 0 && !EIG) || (STG && !REC) || (SERVE && NS == 4) ? 4 : 1), 8)))
 k_persistent(Params P, SourceBank S)
 {
-  // the staging variants keep every other energy bucket (a 16 KB table) to
-  // leave LDS for four wavefront stages per workgroup
-  __shared__ int32_t s_hash[STG ? STG_HASH + 1 : MAX_HASH + 1];
+  // the staging and serving variants keep every other energy bucket (a 16 KB
+  // table) to leave LDS for four wavefront stages or tally caches per workgroup
+  __shared__ int32_t s_hash[STG || SERVE ? STG_HASH + 1 : MAX_HASH + 1];
   // a small per-element flux tally (a few hundred elements) is summed per
   // workgroup in LDS and added to the device tally once at the end: every
   // segment's atomic on one of a few hundred lines serialised in the caches
@@ -1358,6 +1452,9 @@ k_persistent(Params P, SourceBank S)
     W.pos[__lane_id()] = 0u;
     if (__lane_id() == 0) W.next = 0u;
   }
+  ServeCache &K = reinterpret_cast<ServeCache *>(s_tally)[SERVE ? threadIdx.x >> 6 : 0];
+  if (SERVE)
+    for (int i = __lane_id(); i < SC_N; i += 64) K.key[i] = SC_EMPTY;
   stage_hash(P.xs, s_hash);
   Counters *C = P.cnt;
   WaveCount wc;
@@ -1454,7 +1551,10 @@ k_persistent(Params P, SourceBank S)
         stage_add(P, W, tl_ptr != nullptr, tl_ptr ? (uint32_t)(tl_ptr - P.tally[0].value) : 0u, tl_val, wave_ix);
       } else if (!SERVE && P.lds_tally) {
         if (tl_ptr) atomicAdd(s_tally + (tl_ptr - P.tally[0].value), tl_val);
-      } else if (SERVE || P.tl_fast) {
+      } else if (SERVE) {
+        // the segment's element (st.elem has moved on past a crossing)
+        serve_cache_add(P, K, tl_ptr != nullptr, st.req, tl_ptr ? (int32_t)(tl_ptr - st.tv) : 0, tl_ptr, tl_val);
+      } else if (P.tl_fast) {
         atomic_add_agg(tl_ptr != nullptr, tl_ptr, tl_val);
       }
       if (SERVE) req_account(P, go && is_death(ev), st, racc);
@@ -1490,6 +1590,7 @@ k_persistent(Params P, SourceBank S)
     count_collision(wc, hit, ev);
   }
   if (SERVE) req_flush(P, racc.v != 0ull, racc.req, racc.v);
+  if (SERVE) serve_cache_flush(P, K);
   if (STG) stage_drain(P, W, wave_ix);
   flush_counts(C, wc, track);
   if (EIG) {
@@ -1792,10 +1893,13 @@ const void *serve_kernel(int ns, bool p32)
   return p32 ? (const void *)k_persistent<0, false, 0, true, true> : (const void *)k_persistent<0, false, 0, false, true>;
 }
 
+// Dynamic LDS of a serving launch of 256-thread workgroups: a tally cache per wavefront.
+size_t serve_lds_bytes() { return 4 * sizeof(ServeCache); }
+
 hipError_t launch_serve(int ns, bool p32, unsigned blocks, hipStream_t s, const Params &P, const SourceBank &S)
 {
   void *args[] = {(void *)&P, (void *)&S};
-  return hipLaunchKernel(serve_kernel(ns, p32), dim3(blocks), dim3(256), args, 0, s);
+  return hipLaunchKernel(serve_kernel(ns, p32), dim3(blocks), dim3(256), args, serve_lds_bytes(), s);
 }
 
 hipError_t launch_reset_queue(hipStream_t s, Counters *C, int q0, int q1)
