@@ -178,7 +178,7 @@ def bench_eigen(args, tuning):
     n = args.eigen_histories
     t = time.perf_counter()
     prob = P.k2_tet_eigen(n_particles=n)
-    stage = tuning.get("tally_stage", args.tally_stage)
+    stage = tuning.get("tally_stage", args.tally_stage or 0)
     lanes = args.lanes_min if args.no_autotune and args.lanes_min else tune_eigen_lanes(prob, n, tally_stage=stage)
     ctx = Context(prob, mode=abi.MODE_BATCHED, run_mode=abi.RUN_EIGENVALUE, n_inactive=2,
                   lanes_min=lanes, block_size=tuning.get("block_size", 0), tally_stage=stage)
@@ -472,8 +472,9 @@ def main():
     ap.add_argument("--lanes-min", type=int, default=0)
     ap.add_argument("--sort-cadence", type=int, default=0, help="sort sources by (material, element) without autotune")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tally-stage", type=int, default=0,
-                    help="tally_stage option of the C2 contexts (0: staged flux tally, -1: device atomics)")
+    ap.add_argument("--tally-stage", type=int, default=None,
+                    help="tally_stage option of the C2 contexts (0: staged flux tally, -1: device atomics; "
+                         "default: the autotuner's choice, 0 without it)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving mix")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-calibrate", type=int, default=200, help="requests in the saturated capacity run")
@@ -550,7 +551,7 @@ def main():
         f"({time.perf_counter() - t0:.1f}s)")
 
     ctx = Context(prob, mode=abi.MODE_BATCHED, block_size=args.block, lanes_min=args.lanes_min,
-                  sort_cadence=args.sort_cadence, tally_stage=args.tally_stage)
+                  sort_cadence=args.sort_cadence, tally_stage=args.tally_stage or 0)
     tuning = {"block_size": ctx.opts.block_size, "lanes_min": ctx.opts.lanes_min,
               "waves_per_eu": ctx.opts.waves_per_eu, "sort_cadence": ctx.opts.sort_cadence,
               "tally_stage": ctx.opts.tally_stage}
@@ -558,7 +559,8 @@ def main():
         from macaw_amd.autotune import autotune
         t = time.perf_counter()
         tuning, rows = autotune(ctx, min(N, 10_000_000), log=log)
-        tuning["tally_stage"] = args.tally_stage
+        if args.tally_stage is not None:
+            tuning["tally_stage"] = args.tally_stage
         ctx.set_options(**tuning)
         log(f"autotune: best {tuning} ({time.perf_counter() - t:.1f}s)")
     ctx.tally_reset()  # tallies then hold exactly the warmup + timed batches

@@ -5,11 +5,14 @@ returns the fastest.  Knobs: workgroup size, `lanes_min` (how many of a
 wavefront's 64 lanes must still be stepping segments before the wavefront
 pauses to run its collisions and refills), the register/occupancy variant of
 the kernel (`waves_per_eu`), then, with those fixed, the bank size
-(histories resident per launch) and the sort cadence (source sites ordered by
-material and element, or not).  The search is staged: a full grid over the
+(histories resident per launch), the sort cadence (source sites ordered by
+material and element, or not) and the flux tally's form (`tally_stage`:
+staged in LDS and summed by a second kernel, or one device atomic per
+segment; it only applies to some tallies, meshes and kernel variants).  The search is staged: a full grid over the
 first group, then a grid over the second with the first group's best held.
-Every candidate produces bit-identical histories, so tuning never changes
-results, only speed.
+Every candidate produces bit-identical histories, and tallies that differ
+only in the order of FP64 additions, so tuning never changes results, only
+speed.
 
     best, table = autotune(ctx, n_particles)
     ctx.set_options(**best)
@@ -27,6 +30,7 @@ DEFAULT_SPACE = {
 BANK_SPACE = {
     "bank_capacity": [1 << 22, 1 << 23, 1 << 24],
     "sort_cadence": [0, 1],
+    "tally_stage": [0, -1],
 }
 
 
