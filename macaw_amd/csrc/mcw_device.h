@@ -551,7 +551,9 @@ __device__ __forceinline__ int exit_side(const double4 *__restrict__ pl, double 
 // FP32 exit side from NSD FP32 planes, the position given relative to the
 // planes' origin (dx, dy, dz) with S = |dx| + |dy| + |dz|.  Each candidate
 // distance t carries an error bound e (evaluation and rounding of FP32
-// operands: |dt| <= (16u(|c| + S) + 16u|t|)/den + 2u|t|, u = 2^-24).  When
+// operands: |dt| <= (16u(|c| + S) + 16u|t|)/den + 4u|t|, u = 2^-24; the
+// quotients use the hardware reciprocal, within 1 ulp, so t = num * rcp(den)
+// is within 3u|t| of num / den, where an IEEE division was within u).  When
 // the best candidate's interval overlaps another's, or a side is within 1e-5
 // of grazing, `recheck` is set and the caller must redo the selection in
 // FP64; otherwise the chosen side is the one FP64 would choose, and L is the
@@ -570,12 +572,13 @@ __device__ __forceinline__ int exit_side_f32(const float4 *__restrict__ pl, floa
     const float4 q = pl[s];
     const float den = (q.x * ux + q.y * uy) + q.z * uz;
     const float num = q.w - ((q.x * dx + q.y * dy) + q.z * dz);
-    float ts = num / den;
+    const float rd = __builtin_amdgcn_rcpf(den);
+    float ts = num * rd;
     ts = ts > 0.0f ? ts : 0.0f;
     ok[s] = (s != inface) && (den > 0.0f);
     graze |= (s != inface) && (fabsf(den) <= 1e-5f);
     t[s] = ts;
-    e[s] = (16.0f * uu * (fabsf(q.w) + S) + 16.0f * uu * ts) / den + 2.0f * uu * ts;
+    e[s] = (16.0f * uu * (fabsf(q.w) + S) + 16.0f * uu * ts) * rd + 4.0f * uu * ts;
   }
   int fx = -1;
   float best = __builtin_inff();
