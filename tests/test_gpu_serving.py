@@ -117,6 +117,34 @@ def test_many_requests_share_rounds():
         assert st["histories"] == n
 
 
+def test_point_source_requests_beyond_the_cache_keys():
+    """80 point-source requests of 3,000 histories in one round (the round
+    holds up to 2^18): their first segments all add to the source element,
+    which the serving kernel's per-wavefront tally cache absorbs, and the
+    requests in slices 63 and up, whose keys the cache does not encode, add
+    to the device directly.  Every request equals its result alone."""
+    probs = meshes()
+    ctx = contexts(probs)
+    src = P.point_source((0.5, 0.5, 0.5))
+    reqs = [(3000 + i, 0, 3_000, src) for i in range(80)]
+    s = Server(ctx, max_inflight=96, bank_bytes=1 << 40)
+    for r in reqs:
+        s.submit(*r)
+    mixed = s.run_all()
+    s.close()
+    assert len(mixed) == 80
+    a = Server(ctx, max_inflight=1)
+    for rid, slot, n, src_ in reqs:
+        a.submit(rid, slot, n, src_)
+    alone = a.run_all()
+    a.close()
+    for rid, slot, n, src_ in reqs:
+        st, flux = alone[rid]
+        rel = np.abs(flux - mixed[rid][1]) / np.maximum(np.abs(flux), 1e-300)
+        assert rel.max() <= 1e-10, (rid, rel.max())
+        assert st["segments"] == mixed[rid][0]["segments"]
+
+
 def test_shortest_remaining_work_first():
     """A large request and small ones admitted together: the small ones run
     in a round of their own (a round holding short requests takes none of a
