@@ -129,8 +129,10 @@ typedef struct {
   int32_t n_inactive;       /* EIGENVALUE: batches 1..n_inactive are inactive (settings
                                <inactive>): no tally scores, keff not averaged */
   int32_t material_index;   /* MCW_MATERIAL_* (default BY_ID) */
-  int32_t tally_stage;      /* BATCHED fixed-source runs on TET4 meshes whose only track-length
-                               tally is a plain flux of more than 1024 and at most 2^20 bins:
+  int32_t tally_stage;      /* BATCHED fixed-source runs on TET4 meshes (256-thread workgroups,
+                               waves_per_eu 0; not mcw_run_batch_noalloc) whose only
+                               track-length tally is a plain flux of more than 1024 and at most
+                               2^20 bins:
                                0 = stage its scores in LDS and sum them in a second kernel
                                (no device atomic per segment), -1 = one device atomic per
                                segment, n > 0 = stage into a pool of at most n records per
