@@ -1159,8 +1159,10 @@ __device__ __forceinline__ void stage_write(const Params &P, StageWave &W, bool 
     const double v = W.sc[bb][r];
     const uint32_t l = W.el[bb][r];
     if (pos != STG_NONE) {
-      V.sc[(int64_t)pos + r] = v;
-      V.el[(int64_t)pos + r] = (uint16_t)l;
+      // streaming stores: the records are read once, by the sums after the
+      // launch, and should not displace element records from the caches
+      __builtin_nontemporal_store(v, V.sc + ((int64_t)pos + r));
+      __builtin_nontemporal_store((uint16_t)l, V.el + ((int64_t)pos + r));
     } else {
       atomic_add_dev(P.tally[0].value + (((int64_t)bb << STG_BITS) | l), v);
     }
